@@ -1,0 +1,486 @@
+"""ctypes bindings for the parity checker.
+
+TEST INFRASTRUCTURE ONLY.  Only tests/, ``__graft_entry__.smoke()`` and the
+``cpu_baseline`` leg of ``bench.py`` import this module.  It loads
+
+* ``oracle/_build/liborc.so`` -- the C restatement of the reference core
+  (``tdse_oracle.c``), and
+* ``oracle/_ref/libsesref.so`` -- the reference core itself, compiled from its
+  own sources by ``oracle/Makefile`` (present only where it was built).
+
+Complex arrays are numpy ``complex128`` (interleaved re/im, the layout of
+``std::complex<double>`` in ``ses::Field1D/Field3D``, core/src/field.ixx:11-66).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORC_PATH = os.path.join(HERE, "_build", "liborc.so")
+REF_PATH = os.path.join(HERE, "_ref", "libsesref.so")
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+
+def build(ref: bool = False) -> None:
+    """Compile the C restatement (and, with ``ref``, the reference core)."""
+    targets = ["oracle"] + (["ref"] if ref else [])
+    subprocess.run(["make", "-s", "-C", HERE, "-j8", *targets], check=True)
+
+
+def _p(a: np.ndarray):
+    return a.ctypes.data_as(_dp)
+
+
+def _ivec(v):
+    return (C.c_int * 3)(*[int(x) for x in v])
+
+
+def _dvec(v):
+    return (C.c_double * 3)(*[float(x) for x in v])
+
+
+_ORC = None
+_REF = None
+
+
+def orc() -> C.CDLL:
+    global _ORC
+    if _ORC is None:
+        if not os.path.exists(ORC_PATH):
+            build()
+        lib = C.CDLL(ORC_PATH)
+        for name in ("orc_norm_sq", "orc_mean_position_1d", "orc_mean_energy_1d",
+                     "orc_probability_in_range_1d", "orc_mean_energy_3d",
+                     "orc_probability_in_slab_3d", "orc_spacing"):
+            getattr(lib, name).restype = C.c_double
+        _ORC = lib
+    return _ORC
+
+
+def ref_available() -> bool:
+    return os.path.exists(REF_PATH)
+
+
+def ref() -> C.CDLL:
+    global _REF
+    if _REF is None:
+        lib = C.CDLL(REF_PATH)
+        for name in ("ref_norm_sq_1d", "ref_norm_sq_3d", "ref_mean_position_1d",
+                     "ref_mean_energy_1d", "ref_probability_in_range_1d", "ref_mean_energy_3d"):
+            getattr(lib, name).restype = C.c_double
+        _REF = lib
+    return _REF
+
+
+def _c128(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.complex128)
+
+
+# ---- C restatement ---------------------------------------------------------
+
+def wavenumbers(n, xmin, xmax):
+    k = np.empty(n)
+    orc().orc_wavenumbers(C.c_int(n), C.c_double(xmin), C.c_double(xmax), _p(k))
+    return k
+
+
+def fft1d(a):
+    a = _c128(a).copy()
+    if orc().orc_fft1d(_p(a), C.c_size_t(a.size)) != 0:
+        raise ValueError("fft: size must be a power of two")
+    return a
+
+
+def ifft1d(a):
+    a = _c128(a).copy()
+    if orc().orc_ifft1d(_p(a), C.c_size_t(a.size)) != 0:
+        raise ValueError("ifft: size must be a power of two")
+    return a
+
+
+def fft3d(a):
+    """Forward 3D FFT of an array shaped (nz, ny, nx) (x fastest)."""
+    a = _c128(a).copy()
+    nz, ny, nx = a.shape
+    if orc().orc_fft3d(_p(a), nx, ny, nz) != 0:
+        raise ValueError("fft: size must be a power of two")
+    return a
+
+
+def ifft3d(a):
+    a = _c128(a).copy()
+    nz, ny, nx = a.shape
+    if orc().orc_ifft3d(_p(a), nx, ny, nz) != 0:
+        raise ValueError("fft: size must be a power of two")
+    return a
+
+
+def fft_any(a, sign=-1):
+    a = _c128(a).copy()
+    orc().orc_fft_any(_p(a), C.c_size_t(a.size), C.c_int(sign))
+    return a
+
+
+def dft_brute(a, sign=-1):
+    a = _c128(a)
+    out = np.empty_like(a)
+    orc().orc_dft_brute(_p(a), _p(out), C.c_size_t(a.size), C.c_int(sign))
+    return out
+
+
+def fft3d_any(a, sign=-1):
+    a = _c128(a).copy()
+    nz, ny, nx = a.shape
+    orc().orc_fft3d_any(_p(a), nx, ny, nz, C.c_int(sign))
+    return a
+
+
+def so1d_tables(v, xmin, xmax, dt):
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    n = v.size
+    hv = np.empty(n, np.complex128)
+    kin = np.empty(n, np.complex128)
+    orc().orc_so1d_tables(_p(v), n, C.c_double(xmin), C.c_double(xmax), C.c_double(dt),
+                          _p(hv), _p(kin))
+    return hv, kin
+
+
+def so1d_evolve(v, xmin, xmax, dt, psi, nsteps, mask=None):
+    """SplitOperator1D::step over nsteps; with ``mask``, psi *= mask after each step."""
+    hv, kin = so1d_tables(v, xmin, xmax, dt)
+    psi = _c128(psi).copy()
+    n = psi.size
+    if mask is None:
+        if orc().orc_so1d_step(_p(psi), n, _p(hv), _p(kin), nsteps) != 0:
+            raise ValueError("size must be a power of two")
+    else:
+        mask = np.ascontiguousarray(mask, dtype=np.float64)
+        for _ in range(nsteps):
+            if orc().orc_so1d_step(_p(psi), n, _p(hv), _p(kin), 1) != 0:
+                raise ValueError("size must be a power of two")
+            orc().orc_apply_mask(_p(psi), _p(mask), C.c_size_t(n))
+    return psi
+
+
+def so3d_tables(v, dims, lo, hi, dt, mass=1.0):
+    v = np.ascontiguousarray(v, dtype=np.float64).ravel()
+    hv = np.empty(v.size, np.complex128)
+    kin = np.empty(v.size, np.complex128)
+    orc().orc_so3d_tables(_p(v), _ivec(dims), _dvec(lo), _dvec(hi), C.c_double(dt),
+                          C.c_double(mass), _p(hv), _p(kin))
+    return hv, kin
+
+
+def so3d_evolve(v, dims, lo, hi, dt, psi, nsteps, mass=1.0, mask=None, any_size=False):
+    """SplitOperator3D::step; ``any_size`` switches to the mixed-radix FFT."""
+    hv, kin = so3d_tables(v, dims, lo, hi, dt, mass)
+    psi = _c128(psi).copy()
+    shape = psi.shape
+    psi = psi.ravel()
+    d = _ivec(dims)
+    step = orc().orc_so3d_step_any if any_size else orc().orc_so3d_step
+    if mask is None:
+        if step(_p(psi), d, _p(hv), _p(kin), nsteps) != 0:
+            raise ValueError("size must be a power of two")
+    else:
+        mask = np.ascontiguousarray(mask, dtype=np.float64).ravel()
+        for _ in range(nsteps):
+            if step(_p(psi), d, _p(hv), _p(kin), 1) != 0:
+                raise ValueError("size must be a power of two")
+            orc().orc_apply_mask(_p(psi), _p(mask), C.c_size_t(psi.size))
+    return psi.reshape(shape)
+
+
+def norm_sq(psi, cell_volume):
+    psi = _c128(psi)
+    return orc().orc_norm_sq(_p(psi), C.c_size_t(psi.size), C.c_double(cell_volume))
+
+
+def harmonic_1d(xmin, xmax, n, omega, x0=0.0):
+    v = np.empty(n)
+    orc().orc_harmonic_1d(C.c_double(xmin), C.c_double(xmax), n, C.c_double(omega),
+                          C.c_double(x0), _p(v))
+    return v
+
+
+def harmonic_3d(dims, lo, hi, omega, c):
+    v = np.empty(int(np.prod(dims)))
+    orc().orc_harmonic_3d(_ivec(dims), _dvec(lo), _dvec(hi), C.c_double(omega), _dvec(c), _p(v))
+    return v
+
+
+def barrier_1d(xmin, xmax, n, v0, x_lo, x_hi):
+    v = np.empty(n)
+    orc().orc_barrier_1d(C.c_double(xmin), C.c_double(xmax), n, C.c_double(v0),
+                         C.c_double(x_lo), C.c_double(x_hi), _p(v))
+    return v
+
+
+def barrier_3d(dims, lo, hi, v0, x_lo, x_hi):
+    v = np.empty(int(np.prod(dims)))
+    orc().orc_barrier_3d(_ivec(dims), _dvec(lo), _dvec(hi), C.c_double(v0), C.c_double(x_lo),
+                         C.c_double(x_hi), _p(v))
+    return v
+
+
+def regularized_coulomb_3d(dims, lo, hi, z, c):
+    v = np.empty(int(np.prod(dims)))
+    orc().orc_regularized_coulomb_3d(_ivec(dims), _dvec(lo), _dvec(hi), C.c_double(z), _dvec(c),
+                                     _p(v))
+    return v
+
+
+def coulomb_floor_3d(dims, lo, hi, z, c):
+    v = np.empty(int(np.prod(dims)))
+    orc().orc_coulomb_floor_3d(_ivec(dims), _dvec(lo), _dvec(hi), C.c_double(z), _dvec(c), _p(v))
+    return v
+
+
+def absorbing_mask_1d(xmin, xmax, n, width):
+    m = np.empty(n)
+    orc().orc_absorbing_mask_1d(C.c_double(xmin), C.c_double(xmax), n, C.c_double(width), _p(m))
+    return m
+
+
+def absorbing_mask_3d(dims, lo, hi, width):
+    m = np.empty(int(np.prod(dims)))
+    orc().orc_absorbing_mask_3d(_ivec(dims), _dvec(lo), _dvec(hi), C.c_double(width), _p(m))
+    return m
+
+
+def gaussian_1d(xmin, xmax, n, x0, sigma, k0):
+    psi = np.empty(n, np.complex128)
+    orc().orc_gaussian_1d(C.c_double(xmin), C.c_double(xmax), n, C.c_double(x0),
+                          C.c_double(sigma), C.c_double(k0), _p(psi))
+    return psi
+
+
+def gaussian_3d(dims, lo, hi, r0, sigma, k0):
+    psi = np.empty(int(np.prod(dims)), np.complex128)
+    orc().orc_gaussian_3d(_ivec(dims), _dvec(lo), _dvec(hi), _dvec(r0), _dvec(sigma), _dvec(k0),
+                          _p(psi))
+    return psi
+
+
+def mean_position_1d(psi, xmin, xmax):
+    psi = _c128(psi)
+    return orc().orc_mean_position_1d(_p(psi), psi.size, C.c_double(xmin), C.c_double(xmax))
+
+
+def mean_energy_1d(psi, xmin, xmax, v):
+    psi = _c128(psi)
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    return orc().orc_mean_energy_1d(_p(psi), psi.size, C.c_double(xmin), C.c_double(xmax), _p(v))
+
+
+def probability_in_range_1d(psi, xmin, xmax, a, b):
+    psi = _c128(psi)
+    return orc().orc_probability_in_range_1d(_p(psi), psi.size, C.c_double(xmin),
+                                             C.c_double(xmax), C.c_double(a), C.c_double(b))
+
+
+def mean_position_3d(psi, dims, lo, hi):
+    psi = _c128(psi)
+    out = np.empty(3)
+    orc().orc_mean_position_3d(_p(psi), _ivec(dims), _dvec(lo), _dvec(hi), _p(out))
+    return out
+
+
+def mean_energy_3d(psi, dims, lo, hi, v, mass=1.0):
+    psi = _c128(psi)
+    v = np.ascontiguousarray(v, dtype=np.float64).ravel()
+    return orc().orc_mean_energy_3d(_p(psi), _ivec(dims), _dvec(lo), _dvec(hi), _p(v),
+                                    C.c_double(mass))
+
+
+def probability_in_slab_3d(psi, dims, lo, hi, axis, a, b):
+    psi = _c128(psi)
+    return orc().orc_probability_in_slab_3d(_p(psi), _ivec(dims), _dvec(lo), _dvec(hi),
+                                            C.c_int(axis), C.c_double(a), C.c_double(b))
+
+
+# ---- the reference core (oracle/_ref) --------------------------------------
+
+class Ref:
+    """Thin wrappers over ``libsesref.so`` with the same signatures as above."""
+
+    @staticmethod
+    def fft1d(a):
+        a = _c128(a).copy()
+        if ref().ref_fft1d(_p(a), C.c_size_t(a.size)) != 0:
+            raise ValueError("fft: size must be a power of two")
+        return a
+
+    @staticmethod
+    def ifft1d(a):
+        a = _c128(a).copy()
+        if ref().ref_ifft1d(_p(a), C.c_size_t(a.size)) != 0:
+            raise ValueError("ifft: size must be a power of two")
+        return a
+
+    @staticmethod
+    def fft3d(a):
+        a = _c128(a).copy()
+        nz, ny, nx = a.shape
+        if ref().ref_fft3d(_p(a), _ivec((nx, ny, nz))) != 0:
+            raise ValueError("fft: size must be a power of two")
+        return a
+
+    @staticmethod
+    def ifft3d(a):
+        a = _c128(a).copy()
+        nz, ny, nx = a.shape
+        if ref().ref_ifft3d(_p(a), _ivec((nx, ny, nz))) != 0:
+            raise ValueError("fft: size must be a power of two")
+        return a
+
+    @staticmethod
+    def wavenumbers(n, xmin, xmax):
+        k = np.empty(n)
+        ref().ref_wavenumbers(n, C.c_double(xmin), C.c_double(xmax), _p(k))
+        return k
+
+    @staticmethod
+    def so1d_evolve(v, xmin, xmax, dt, psi, nsteps, mask=None):
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        psi = _c128(psi).copy()
+        m = None if mask is None else np.ascontiguousarray(mask, dtype=np.float64)
+        rc = ref().ref_so1d_evolve(_p(v), v.size, C.c_double(xmin), C.c_double(xmax),
+                                   C.c_double(dt), _p(psi), nsteps,
+                                   None if m is None else _p(m))
+        if rc != 0:
+            raise ValueError("size must be a power of two")
+        return psi
+
+    @staticmethod
+    def so3d_tables(v, dims, lo, hi, dt, mass=1.0):
+        v = np.ascontiguousarray(v, dtype=np.float64).ravel()
+        hv = np.empty(v.size, np.complex128)
+        kin = np.empty(v.size, np.complex128)
+        ref().ref_so3d_tables(_p(v), _ivec(dims), _dvec(lo), _dvec(hi), C.c_double(dt),
+                              C.c_double(mass), _p(hv), _p(kin))
+        return hv, kin
+
+    @staticmethod
+    def so3d_evolve(v, dims, lo, hi, dt, psi, nsteps, mass=1.0, mask=None):
+        v = np.ascontiguousarray(v, dtype=np.float64).ravel()
+        psi = _c128(psi).copy()
+        shape = psi.shape
+        psi = psi.ravel()
+        m = None if mask is None else np.ascontiguousarray(mask, dtype=np.float64).ravel()
+        rc = ref().ref_so3d_evolve(_p(v), _ivec(dims), _dvec(lo), _dvec(hi), C.c_double(dt),
+                                   C.c_double(mass), _p(psi), nsteps,
+                                   None if m is None else _p(m))
+        if rc != 0:
+            raise ValueError("size must be a power of two")
+        return psi.reshape(shape)
+
+    @staticmethod
+    def harmonic_1d(xmin, xmax, n, omega, x0=0.0):
+        v = np.empty(n)
+        ref().ref_harmonic_1d(C.c_double(xmin), C.c_double(xmax), n, C.c_double(omega),
+                              C.c_double(x0), _p(v))
+        return v
+
+    @staticmethod
+    def harmonic_3d(dims, lo, hi, omega, c):
+        v = np.empty(int(np.prod(dims)))
+        ref().ref_harmonic_3d(_ivec(dims), _dvec(lo), _dvec(hi), C.c_double(omega), _dvec(c),
+                              _p(v))
+        return v
+
+    @staticmethod
+    def barrier_1d(xmin, xmax, n, v0, x_lo, x_hi):
+        v = np.empty(n)
+        ref().ref_barrier_1d(C.c_double(xmin), C.c_double(xmax), n, C.c_double(v0),
+                             C.c_double(x_lo), C.c_double(x_hi), _p(v))
+        return v
+
+    @staticmethod
+    def barrier_3d(dims, lo, hi, v0, x_lo, x_hi):
+        v = np.empty(int(np.prod(dims)))
+        ref().ref_barrier_3d(_ivec(dims), _dvec(lo), _dvec(hi), C.c_double(v0),
+                             C.c_double(x_lo), C.c_double(x_hi), _p(v))
+        return v
+
+    @staticmethod
+    def regularized_coulomb_3d(dims, lo, hi, z, c):
+        v = np.empty(int(np.prod(dims)))
+        ref().ref_regularized_coulomb_3d(_ivec(dims), _dvec(lo), _dvec(hi), C.c_double(z),
+                                         _dvec(c), _p(v))
+        return v
+
+    @staticmethod
+    def absorbing_mask_1d(xmin, xmax, n, width):
+        m = np.empty(n)
+        ref().ref_absorbing_mask_1d(C.c_double(xmin), C.c_double(xmax), n, C.c_double(width),
+                                    _p(m))
+        return m
+
+    @staticmethod
+    def absorbing_mask_3d(dims, lo, hi, width):
+        m = np.empty(int(np.prod(dims)))
+        ref().ref_absorbing_mask_3d(_ivec(dims), _dvec(lo), _dvec(hi), C.c_double(width), _p(m))
+        return m
+
+    @staticmethod
+    def gaussian_1d(xmin, xmax, n, x0, sigma, k0):
+        psi = np.empty(n, np.complex128)
+        ref().ref_gaussian_1d(C.c_double(xmin), C.c_double(xmax), n, C.c_double(x0),
+                              C.c_double(sigma), C.c_double(k0), _p(psi))
+        return psi
+
+    @staticmethod
+    def gaussian_3d(dims, lo, hi, r0, sigma, k0):
+        psi = np.empty(int(np.prod(dims)), np.complex128)
+        ref().ref_gaussian_3d(_ivec(dims), _dvec(lo), _dvec(hi), _dvec(r0), _dvec(sigma),
+                              _dvec(k0), _p(psi))
+        return psi
+
+    @staticmethod
+    def norm_sq_1d(psi, xmin, xmax):
+        psi = _c128(psi)
+        return ref().ref_norm_sq_1d(_p(psi), psi.size, C.c_double(xmin), C.c_double(xmax))
+
+    @staticmethod
+    def norm_sq_3d(psi, dims, lo, hi):
+        psi = _c128(psi)
+        return ref().ref_norm_sq_3d(_p(psi), _ivec(dims), _dvec(lo), _dvec(hi))
+
+    @staticmethod
+    def mean_position_1d(psi, xmin, xmax):
+        psi = _c128(psi)
+        return ref().ref_mean_position_1d(_p(psi), psi.size, C.c_double(xmin), C.c_double(xmax))
+
+    @staticmethod
+    def mean_energy_1d(psi, xmin, xmax, v):
+        psi = _c128(psi)
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        return ref().ref_mean_energy_1d(_p(psi), psi.size, C.c_double(xmin), C.c_double(xmax),
+                                        _p(v))
+
+    @staticmethod
+    def probability_in_range_1d(psi, xmin, xmax, a, b):
+        psi = _c128(psi)
+        return ref().ref_probability_in_range_1d(_p(psi), psi.size, C.c_double(xmin),
+                                                 C.c_double(xmax), C.c_double(a), C.c_double(b))
+
+    @staticmethod
+    def mean_position_3d(psi, dims, lo, hi):
+        psi = _c128(psi)
+        out = np.empty(3)
+        ref().ref_mean_position_3d(_p(psi), _ivec(dims), _dvec(lo), _dvec(hi), _p(out))
+        return out
+
+    @staticmethod
+    def mean_energy_3d(psi, dims, lo, hi, v, mass=1.0):
+        psi = _c128(psi)
+        v = np.ascontiguousarray(v, dtype=np.float64).ravel()
+        return ref().ref_mean_energy_3d(_p(psi), _ivec(dims), _dvec(lo), _dvec(hi), _p(v),
+                                        C.c_double(mass))

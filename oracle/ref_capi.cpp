@@ -1,0 +1,257 @@
+// ref_capi.cpp -- extern "C" shim over the reference core, compiled from the
+// reference's own module sources (oracle/Makefile, target "ref").
+//
+// TEST INFRASTRUCTURE ONLY.  It lets the tests compare the C restatement in
+// tdse_oracle.c bit for bit with the reference implementation, and lets the
+// benchmark time the reference's own CPU propagator (its thread pool included)
+// as the CPU baseline.  Nothing in the product links it.
+//
+// Complex arrays cross as interleaved (re, im) doubles; the shim copies them
+// into ses::Field1D / ses::Field3D (std::complex<double>, same layout).
+
+#include <complex>
+#include <cstddef>
+#include <cstring>
+#include <stdexcept>
+#include <vector>
+
+import ses.grid;
+import ses.vec;
+import ses.field;
+import ses.spectral;
+import ses.fft;
+import ses.propagator;
+import ses.potential;
+import ses.wavepacket;
+import ses.observables;
+import ses.parallel;
+
+namespace {
+
+using Cd = std::complex<double>;
+
+ses::Grid3D grid3(const int d[3], const double lo[3], const double hi[3]) {
+    return ses::Grid3D{ses::Grid1D{lo[0], hi[0], d[0]}, ses::Grid1D{lo[1], hi[1], d[1]},
+                       ses::Grid1D{lo[2], hi[2], d[2]}};
+}
+
+void load(std::vector<Cd>& dst, const double* src) {
+    std::memcpy(static_cast<void*>(dst.data()), src, dst.size() * sizeof(Cd));
+}
+void store(double* dst, const std::vector<Cd>& src) {
+    std::memcpy(dst, static_cast<const void*>(src.data()), src.size() * sizeof(Cd));
+}
+
+}  // namespace
+
+extern "C" {
+
+int ref_parallel_workers() { return ses::parallel_workers(); }
+
+void ref_wavenumbers(int n, double xmin, double xmax, double* k) {
+    const std::vector<double> v = ses::wavenumbers(ses::Grid1D{xmin, xmax, n});
+    std::memcpy(k, v.data(), v.size() * sizeof(double));
+}
+
+int ref_fft1d(double* a, std::size_t n) {
+    std::vector<Cd> v(n);
+    load(v, a);
+    try {
+        ses::fft(v);
+    } catch (const std::invalid_argument&) {
+        return -1;
+    }
+    store(a, v);
+    return 0;
+}
+
+int ref_ifft1d(double* a, std::size_t n) {
+    std::vector<Cd> v(n);
+    load(v, a);
+    try {
+        ses::ifft(v);
+    } catch (const std::invalid_argument&) {
+        return -1;
+    }
+    store(a, v);
+    return 0;
+}
+
+int ref_fft3d(double* a, const int d[3]) {
+    const double lo[3] = {0, 0, 0}, hi[3] = {1, 1, 1};
+    ses::Field3D f{grid3(d, lo, hi)};
+    load(f.data(), a);
+    try {
+        ses::fft(f);
+    } catch (const std::invalid_argument&) {
+        return -1;
+    }
+    store(a, f.data());
+    return 0;
+}
+
+int ref_ifft3d(double* a, const int d[3]) {
+    const double lo[3] = {0, 0, 0}, hi[3] = {1, 1, 1};
+    ses::Field3D f{grid3(d, lo, hi)};
+    load(f.data(), a);
+    try {
+        ses::ifft(f);
+    } catch (const std::invalid_argument&) {
+        return -1;
+    }
+    store(a, f.data());
+    return 0;
+}
+
+// SplitOperator1D: evolve psi in place by nsteps (core/src/propagator.ixx:21-64).
+int ref_so1d_evolve(const double* v, int n, double xmin, double xmax, double dt, double* psi,
+                    int nsteps, const double* mask) {
+    const ses::Grid1D g{xmin, xmax, n};
+    std::vector<double> pot(v, v + n);
+    ses::SplitOperator1D op{g, pot, dt};
+    ses::Field1D f{g};
+    load(f.data(), psi);
+    try {
+        if (mask == nullptr) {
+            op.step(f, nsteps);
+        } else {
+            for (int s = 0; s < nsteps; ++s) {
+                op.step(f, 1);
+                for (int i = 0; i < n; ++i) f[i] *= mask[i];
+            }
+        }
+    } catch (const std::invalid_argument&) {
+        return -1;
+    }
+    store(psi, f.data());
+    return 0;
+}
+
+// SplitOperator3D tables through the reference accessors (propagator.ixx:107-108).
+void ref_so3d_tables(const double* v, const int d[3], const double lo[3], const double hi[3],
+                     double dt, double mass, double* half_v, double* kinetic) {
+    const ses::Grid3D g = grid3(d, lo, hi);
+    std::vector<double> pot(v, v + g.size());
+    const ses::SplitOperator3D op{g, pot, dt, mass};
+    store(half_v, op.half_potential_phase());
+    store(kinetic, op.kinetic_phase());
+}
+
+// SplitOperator3D::step, with the caller-side mask after each step when given.
+int ref_so3d_evolve(const double* v, const int d[3], const double lo[3], const double hi[3],
+                    double dt, double mass, double* psi, int nsteps, const double* mask) {
+    const ses::Grid3D g = grid3(d, lo, hi);
+    std::vector<double> pot(v, v + g.size());
+    const ses::SplitOperator3D op{g, pot, dt, mass};
+    ses::Field3D f{g};
+    load(f.data(), psi);
+    try {
+        if (mask == nullptr) {
+            op.step(f, nsteps);
+        } else {
+            const std::size_t n = f.data().size();
+            for (int s = 0; s < nsteps; ++s) {
+                op.step(f, 1);
+                for (std::size_t i = 0; i < n; ++i) f.data()[i] *= mask[i];
+            }
+        }
+    } catch (const std::invalid_argument&) {
+        return -1;
+    }
+    store(psi, f.data());
+    return 0;
+}
+
+// Builders (core/src/potential.ixx, core/src/wavepacket.ixx).
+void ref_harmonic_1d(double xmin, double xmax, int n, double omega, double x0, double* v) {
+    const std::vector<double> r = ses::harmonic_potential(ses::Grid1D{xmin, xmax, n}, omega, x0);
+    std::memcpy(v, r.data(), r.size() * sizeof(double));
+}
+void ref_harmonic_3d(const int d[3], const double lo[3], const double hi[3], double omega,
+                     const double c[3], double* v) {
+    const std::vector<double> r =
+        ses::harmonic_potential(grid3(d, lo, hi), omega, ses::Vec3d{c[0], c[1], c[2]});
+    std::memcpy(v, r.data(), r.size() * sizeof(double));
+}
+void ref_barrier_1d(double xmin, double xmax, int n, double v0, double x_lo, double x_hi,
+                    double* v) {
+    const std::vector<double> r = ses::barrier_potential(ses::Grid1D{xmin, xmax, n}, v0, x_lo, x_hi);
+    std::memcpy(v, r.data(), r.size() * sizeof(double));
+}
+void ref_barrier_3d(const int d[3], const double lo[3], const double hi[3], double v0,
+                    double x_lo, double x_hi, double* v) {
+    const std::vector<double> r = ses::barrier_potential(grid3(d, lo, hi), v0, x_lo, x_hi);
+    std::memcpy(v, r.data(), r.size() * sizeof(double));
+}
+void ref_regularized_coulomb_3d(const int d[3], const double lo[3], const double hi[3],
+                                double z, const double c[3], double* v) {
+    const std::vector<double> r =
+        ses::regularized_coulomb_potential(grid3(d, lo, hi), z, ses::Vec3d{c[0], c[1], c[2]});
+    std::memcpy(v, r.data(), r.size() * sizeof(double));
+}
+void ref_absorbing_mask_1d(double xmin, double xmax, int n, double width, double* m) {
+    const std::vector<double> r = ses::absorbing_mask(ses::Grid1D{xmin, xmax, n}, width);
+    std::memcpy(m, r.data(), r.size() * sizeof(double));
+}
+void ref_absorbing_mask_3d(const int d[3], const double lo[3], const double hi[3], double width,
+                           double* m) {
+    const std::vector<double> r = ses::absorbing_mask(grid3(d, lo, hi), width);
+    std::memcpy(m, r.data(), r.size() * sizeof(double));
+}
+void ref_gaussian_1d(double xmin, double xmax, int n, double x0, double sigma, double k0,
+                     double* psi) {
+    const ses::Field1D f = ses::gaussian_wavepacket(ses::Grid1D{xmin, xmax, n}, x0, sigma, k0);
+    store(psi, f.data());
+}
+void ref_gaussian_3d(const int d[3], const double lo[3], const double hi[3], const double r0[3],
+                     const double sigma[3], const double k0[3], double* psi) {
+    const ses::Field3D f = ses::gaussian_wavepacket(grid3(d, lo, hi), ses::Vec3d{r0[0], r0[1], r0[2]},
+                                                    ses::Vec3d{sigma[0], sigma[1], sigma[2]},
+                                                    ses::Vec3d{k0[0], k0[1], k0[2]});
+    store(psi, f.data());
+}
+
+// Reductions and observables (core/src/field.ixx, core/src/observables.ixx).
+double ref_norm_sq_1d(const double* psi, int n, double xmin, double xmax) {
+    ses::Field1D f{ses::Grid1D{xmin, xmax, n}};
+    load(f.data(), psi);
+    return ses::norm_sq(f);
+}
+double ref_norm_sq_3d(const double* psi, const int d[3], const double lo[3], const double hi[3]) {
+    ses::Field3D f{grid3(d, lo, hi)};
+    load(f.data(), psi);
+    return ses::norm_sq(f);
+}
+double ref_mean_position_1d(const double* psi, int n, double xmin, double xmax) {
+    ses::Field1D f{ses::Grid1D{xmin, xmax, n}};
+    load(f.data(), psi);
+    return ses::mean_position(f);
+}
+double ref_mean_energy_1d(const double* psi, int n, double xmin, double xmax, const double* v) {
+    ses::Field1D f{ses::Grid1D{xmin, xmax, n}};
+    load(f.data(), psi);
+    return ses::mean_energy(f, std::vector<double>(v, v + n));
+}
+double ref_probability_in_range_1d(const double* psi, int n, double xmin, double xmax, double a,
+                                   double b) {
+    ses::Field1D f{ses::Grid1D{xmin, xmax, n}};
+    load(f.data(), psi);
+    return ses::probability_in_range(f, a, b);
+}
+void ref_mean_position_3d(const double* psi, const int d[3], const double lo[3],
+                          const double hi[3], double out[3]) {
+    ses::Field3D f{grid3(d, lo, hi)};
+    load(f.data(), psi);
+    const ses::Vec3d m = ses::mean_position(f);
+    out[0] = m.x;
+    out[1] = m.y;
+    out[2] = m.z;
+}
+double ref_mean_energy_3d(const double* psi, const int d[3], const double lo[3],
+                          const double hi[3], const double* v, double mass) {
+    ses::Field3D f{grid3(d, lo, hi)};
+    load(f.data(), psi);
+    return ses::mean_energy(f, std::vector<double>(v, v + f.data().size()), mass);
+}
+
+}  // extern "C"

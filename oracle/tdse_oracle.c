@@ -1,0 +1,709 @@
+/*
+ * tdse_oracle.c -- CPU FP64 restatement of the reference split-operator path.
+ *
+ * TEST INFRASTRUCTURE ONLY: the parity checker for the HIP propagator.  Only
+ * tests/, __graft_entry__.smoke() and the cpu_baseline leg of bench.py load
+ * this library; the product path never does.
+ *
+ * Built with -O2 -ffp-contract=off and no fast-math, the flag contract of the
+ * reference core (CMakeLists.txt:59-112), so that every expression below
+ * rounds exactly like its reference counterpart.  Operand order inside each
+ * expression follows the reference text: IEEE addition and multiplication are
+ * commutative but not associative, so the grouping is what matters.
+ *
+ * Complex numbers are interleaved (re, im) doubles.  A complex product is the
+ * textbook formula (ac - bd, ad + bc): the reference builds with
+ * -fcx-limited-range, which makes std::complex<double>::operator* exactly that
+ * (CMakeLists.txt:83-95).  std::norm(z) in libstdc++ is re*re + im*im.
+ *
+ * Pinning: fft/ifft/step/builders are compared bit for bit against the
+ * reference core compiled from its own sources (oracle/Makefile, target
+ * "ref") and against the closed forms of the reference tests
+ * (tests/fft_test.cpp, tests/fft3_test.cpp, tests/propagator_test.cpp,
+ * tests/potential_test.cpp).  The mixed-radix FFT has no reference
+ * counterpart; it is pinned by the brute-force DFT and numpy.fft.
+ */
+#include "tdse_oracle.h"
+
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define ORC_PI 3.14159265358979323846
+
+static inline void cmul(double ar, double ai, double br, double bi, double* re, double* im) {
+    *re = ar * br - ai * bi;
+    *im = ar * bi + ai * br;
+}
+
+/* ---- grid / spectral ---------------------------------------------------- */
+
+/* core/src/grid.ixx:15-17: spacing = (xmax - xmin) / n, coord = xmin + i*h */
+double orc_spacing(double xmin, double xmax, int n) { return (xmax - xmin) / n; }
+
+static inline double coord(double xmin, double xmax, int n, int i) {
+    return xmin + i * orc_spacing(xmin, xmax, n);
+}
+
+/* core/src/spectral.ixx:14-23: fftfreq order, DC at n = 1 */
+void orc_wavenumbers(int n, double xmin, double xmax, double* k) {
+    const double dk = 2.0 * ORC_PI / (xmax - xmin);
+    for (int j = 0; j < n; ++j) {
+        const int shifted = (2 * j < n) ? j : j - n;
+        k[j] = dk * shifted;
+    }
+}
+
+/* ---- radix-2 FFT (core/src/fft.ixx) ------------------------------------- */
+
+/* core/src/fft.ixx:29-37: direct twiddles w[j] = e^{-2 pi i j / n} */
+void orc_fft_twiddles(size_t n, double* w) {
+    const double ang = -2.0 * ORC_PI / (double)n;
+    for (size_t j = 0; j < n / 2; ++j) {
+        const double th = ang * (double)j;
+        w[2 * j] = cos(th);
+        w[2 * j + 1] = sin(th);
+    }
+}
+
+static int is_pow2(size_t n) { return (n & (n - 1)) == 0; }
+
+/* core/src/fft.ixx:39-73: bit reversal then butterflies, stage len uses w[j*n/len] */
+int orc_fft_line(double* a, size_t n, const double* w) {
+    if (!is_pow2(n)) return -1;
+    if (n < 2) return 0;
+    for (size_t i = 1, j = 0; i < n; ++i) {
+        size_t bit = n >> 1;
+        for (; j & bit; bit >>= 1) j ^= bit;
+        j ^= bit;
+        if (i < j) {
+            double t0 = a[2 * i], t1 = a[2 * i + 1];
+            a[2 * i] = a[2 * j];
+            a[2 * i + 1] = a[2 * j + 1];
+            a[2 * j] = t0;
+            a[2 * j + 1] = t1;
+        }
+    }
+    for (size_t len = 2; len <= n; len <<= 1) {
+        const size_t stride = n / len;
+        for (size_t i = 0; i < n; i += len) {
+            for (size_t j = 0; j < len / 2; ++j) {
+                const size_t p = i + j, q = i + j + len / 2;
+                const double ur = a[2 * p], ui = a[2 * p + 1];
+                double vr, vi;
+                cmul(a[2 * q], a[2 * q + 1], w[2 * (j * stride)], w[2 * (j * stride) + 1], &vr, &vi);
+                a[2 * p] = ur + vr;
+                a[2 * p + 1] = ui + vi;
+                a[2 * q] = ur - vr;
+                a[2 * q + 1] = ui - vi;
+            }
+        }
+    }
+    return 0;
+}
+
+/* core/src/fft.ixx:75-84: twiddles rebuilt on every call */
+int orc_fft1d(double* a, size_t n) {
+    if (!is_pow2(n)) return -1;
+    if (n < 2) return 0;
+    double* w = (double*)malloc(sizeof(double) * n);
+    orc_fft_twiddles(n, w);
+    int rc = orc_fft_line(a, n, w);
+    free(w);
+    return rc;
+}
+
+/* core/src/fft.ixx:89-98: ifft = conj(fft(conj(x))) * (1/N), 1/N multiplied */
+int orc_ifft1d(double* a, size_t n) {
+    if (!is_pow2(n)) return -1;
+    for (size_t i = 0; i < n; ++i) a[2 * i + 1] = -a[2 * i + 1];
+    int rc = orc_fft1d(a, n);
+    const double inv = 1.0 / (double)n;
+    for (size_t i = 0; i < n; ++i) {
+        a[2 * i] = inv * a[2 * i];
+        a[2 * i + 1] = inv * -a[2 * i + 1];
+    }
+    return rc;
+}
+
+/* Per-axis line FFT over a 3D x-fastest array; each line is independent, so
+ * the threaded loop is bitwise identical to the serial one
+ * (core/src/fft.ixx:102-173).  `line_fn` is the 1D transform to apply. */
+typedef int (*line_fft_fn)(double* line, size_t n, const double* w, int sign);
+
+static int pow2_line(double* line, size_t n, const double* w, int sign) {
+    (void)sign;
+    return orc_fft_line(line, n, w);
+}
+
+static int axis_pass(double* a, const int d[3], int axis, line_fft_fn fn, const double* w,
+                     int sign) {
+    const int n = d[axis];
+    if (n < 2) return 0;
+    const long nx = d[0], ny = d[1], nz = d[2];
+    const long stride = axis == 0 ? 1 : (axis == 1 ? nx : nx * ny);
+    const long lines = (nx * ny * nz) / n;
+    int rc = 0;
+#pragma omp parallel
+    {
+        double* line = (double*)malloc(sizeof(double) * 2 * (size_t)n);
+#pragma omp for schedule(static)
+        for (long m = 0; m < lines; ++m) {
+            long base;
+            if (axis == 0) {
+                base = m * nx;
+            } else if (axis == 1) {
+                const long k = m / nx, i = m % nx;
+                base = i + nx * ny * k;
+            } else {
+                base = m; /* m = i + nx*j */
+            }
+            for (int p = 0; p < n; ++p) {
+                line[2 * p] = a[2 * (base + p * stride)];
+                line[2 * p + 1] = a[2 * (base + p * stride) + 1];
+            }
+            if (fn(line, (size_t)n, w, sign) != 0) rc = -1;
+            for (int p = 0; p < n; ++p) {
+                a[2 * (base + p * stride)] = line[2 * p];
+                a[2 * (base + p * stride) + 1] = line[2 * p + 1];
+            }
+        }
+        free(line);
+    }
+    return rc;
+}
+
+/* core/src/fft.ixx:102-173: validate up front, then x, y, z lines */
+int orc_fft3d(double* a, int nx, int ny, int nz) {
+    if (!is_pow2((size_t)nx) || !is_pow2((size_t)ny) || !is_pow2((size_t)nz)) return -1;
+    const int d[3] = {nx, ny, nz};
+    for (int axis = 0; axis < 3; ++axis) {
+        const size_t n = (size_t)d[axis];
+        double* w = (double*)malloc(sizeof(double) * (n + 2));
+        orc_fft_twiddles(n, w);
+        axis_pass(a, d, axis, pow2_line, w, -1);
+        free(w);
+    }
+    return 0;
+}
+
+/* core/src/fft.ixx:177-188: parallel conj, fft, parallel inv*conj */
+int orc_ifft3d(double* a, int nx, int ny, int nz) {
+    if (!is_pow2((size_t)nx) || !is_pow2((size_t)ny) || !is_pow2((size_t)nz)) return -1;
+    const long n = (long)nx * ny * nz;
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < n; ++i) a[2 * i + 1] = -a[2 * i + 1];
+    orc_fft3d(a, nx, ny, nz);
+    const double inv = 1.0 / (double)(int)n;
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < n; ++i) {
+        a[2 * i] = inv * a[2 * i];
+        a[2 * i + 1] = inv * -a[2 * i + 1];
+    }
+    return 0;
+}
+
+/* ---- mixed-radix FFT (no reference counterpart) ------------------------- */
+
+/* O(n^2) DFT with sign convention X_k = sum x_n e^{sign 2 pi i k n / N};
+ * the angle uses the exact integer (k*n mod N). */
+void orc_dft_brute(const double* in, double* out, size_t n, int sign) {
+    for (size_t k = 0; k < n; ++k) {
+        double sr = 0.0, si = 0.0;
+        for (size_t m = 0; m < n; ++m) {
+            const double th = sign * 2.0 * ORC_PI * (double)((k * m) % n) / (double)n;
+            double pr, pi;
+            cmul(in[2 * m], in[2 * m + 1], cos(th), sin(th), &pr, &pi);
+            sr += pr;
+            si += pi;
+        }
+        out[2 * k] = sr;
+        out[2 * k + 1] = si;
+    }
+}
+
+/* Recursive decimation in time by the smallest prime factor p:
+ *   X[k] = sum_q W_n^{q k} FFT_{n/p}(x[q::p])[k mod n/p]. */
+static void fft_rec(const double* in, size_t istride, double* out, size_t n, int sign,
+                    double* scratch) {
+    if (n == 1) {
+        out[0] = in[0];
+        out[1] = in[1];
+        return;
+    }
+    size_t p = 2;
+    while (n % p) ++p;
+    const size_t m = n / p;
+    /* sub-transforms into out[q*m .. q*m+m) */
+    for (size_t q = 0; q < p; ++q)
+        fft_rec(in + 2 * q * istride, istride * p, out + 2 * q * m, m, sign, scratch + 2 * n);
+    for (size_t k = 0; k < n; ++k) {
+        double sr = 0.0, si = 0.0;
+        for (size_t q = 0; q < p; ++q) {
+            const double th = sign * 2.0 * ORC_PI * (double)((q * k) % n) / (double)n;
+            double pr, pi;
+            const double* s = out + 2 * (q * m + (k % m));
+            cmul(s[0], s[1], cos(th), sin(th), &pr, &pi);
+            sr += pr;
+            si += pi;
+        }
+        scratch[2 * k] = sr;
+        scratch[2 * k + 1] = si;
+    }
+    memcpy(out, scratch, sizeof(double) * 2 * n);
+}
+
+int orc_fft_any(double* a, size_t n, int sign) {
+    if (n < 2) return 0;
+    /* scratch depth: sum over recursion levels of 2*n_level <= 4n */
+    double* tmp = (double*)malloc(sizeof(double) * 2 * n);
+    double* scratch = (double*)malloc(sizeof(double) * 8 * n + 16);
+    memcpy(tmp, a, sizeof(double) * 2 * n);
+    fft_rec(tmp, 1, a, n, sign, scratch);
+    free(scratch);
+    free(tmp);
+    return 0;
+}
+
+static int any_line(double* line, size_t n, const double* w, int sign) {
+    (void)w;
+    return orc_fft_any(line, n, sign);
+}
+
+int orc_fft3d_any(double* a, int nx, int ny, int nz, int sign) {
+    const int d[3] = {nx, ny, nz};
+    for (int axis = 0; axis < 3; ++axis) axis_pass(a, d, axis, any_line, NULL, sign);
+    return 0;
+}
+
+/* ---- split-operator propagator (core/src/propagator.ixx) ---------------- */
+
+/* core/src/propagator.ixx:23-40 */
+void orc_so1d_tables(const double* v, int n, double xmin, double xmax, double dt,
+                     double* half_v, double* kinetic) {
+    for (int i = 0; i < n; ++i) {
+        const double th = -0.5 * v[i] * dt;
+        half_v[2 * i] = cos(th);
+        half_v[2 * i + 1] = sin(th);
+    }
+    double* k = (double*)malloc(sizeof(double) * (size_t)n);
+    orc_wavenumbers(n, xmin, xmax, k);
+    for (int j = 0; j < n; ++j) {
+        const double th = -0.5 * k[j] * k[j] * dt;
+        kinetic[2 * j] = cos(th);
+        kinetic[2 * j + 1] = sin(th);
+    }
+    free(k);
+}
+
+/* core/src/propagator.ixx:56-61 and 123-129: a[i] = a[i] * phase[i] */
+static void apply_phase(const double* phase, double* a, long n) {
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < n; ++i) {
+        double r, im;
+        cmul(a[2 * i], a[2 * i + 1], phase[2 * i], phase[2 * i + 1], &r, &im);
+        a[2 * i] = r;
+        a[2 * i + 1] = im;
+    }
+}
+
+/* core/src/propagator.ixx:44-53 */
+int orc_so1d_step(double* psi, int n, const double* half_v, const double* kinetic,
+                  int nsteps) {
+    if (!is_pow2((size_t)n)) return -1;
+    for (int s = 0; s < nsteps; ++s) {
+        apply_phase(half_v, psi, n);
+        orc_fft1d(psi, (size_t)n);
+        apply_phase(kinetic, psi, n);
+        orc_ifft1d(psi, (size_t)n);
+        apply_phase(half_v, psi, n);
+    }
+    return 0;
+}
+
+/* core/src/propagator.ixx:72-101: full N-entry kinetic table */
+void orc_so3d_tables(const double* v, const int d[3], const double lo[3], const double hi[3],
+                     double dt, double mass, double* half_v, double* kinetic) {
+    const long n = (long)d[0] * d[1] * d[2];
+    for (long i = 0; i < n; ++i) {
+        const double th = -0.5 * v[i] * dt;
+        half_v[2 * i] = cos(th);
+        half_v[2 * i + 1] = sin(th);
+    }
+    double* kx = (double*)malloc(sizeof(double) * (size_t)d[0]);
+    double* ky = (double*)malloc(sizeof(double) * (size_t)d[1]);
+    double* kz = (double*)malloc(sizeof(double) * (size_t)d[2]);
+    orc_wavenumbers(d[0], lo[0], hi[0], kx);
+    orc_wavenumbers(d[1], lo[1], hi[1], ky);
+    orc_wavenumbers(d[2], lo[2], hi[2], kz);
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < d[2]; ++k) {
+        for (int j = 0; j < d[1]; ++j) {
+            for (int i = 0; i < d[0]; ++i) {
+                const double kxx = kx[i], kyy = ky[j], kzz = kz[k];
+                const double th = -0.5 * (kxx * kxx + kyy * kyy + kzz * kzz) / mass * dt;
+                const long f = i + (long)d[0] * (j + (long)d[1] * k);
+                kinetic[2 * f] = cos(th);
+                kinetic[2 * f + 1] = sin(th);
+            }
+        }
+    }
+    free(kx);
+    free(ky);
+    free(kz);
+}
+
+/* core/src/propagator.ixx:110-119 */
+int orc_so3d_step(double* psi, const int d[3], const double* half_v, const double* kinetic,
+                  int nsteps) {
+    if (!is_pow2((size_t)d[0]) || !is_pow2((size_t)d[1]) || !is_pow2((size_t)d[2])) return -1;
+    const long n = (long)d[0] * d[1] * d[2];
+    for (int s = 0; s < nsteps; ++s) {
+        apply_phase(half_v, psi, n);
+        orc_fft3d(psi, d[0], d[1], d[2]);
+        apply_phase(kinetic, psi, n);
+        orc_ifft3d(psi, d[0], d[1], d[2]);
+        apply_phase(half_v, psi, n);
+    }
+    return 0;
+}
+
+/* Same Strang step with the mixed-radix FFT; the inverse is the e^{+} transform
+ * times 1/N (equal in exact arithmetic to the conjugation identity). */
+int orc_so3d_step_any(double* psi, const int d[3], const double* half_v,
+                      const double* kinetic, int nsteps) {
+    const long n = (long)d[0] * d[1] * d[2];
+    const double inv = 1.0 / (double)n;
+    for (int s = 0; s < nsteps; ++s) {
+        apply_phase(half_v, psi, n);
+        orc_fft3d_any(psi, d[0], d[1], d[2], -1);
+        apply_phase(kinetic, psi, n);
+        orc_fft3d_any(psi, d[0], d[1], d[2], +1);
+#pragma omp parallel for schedule(static)
+        for (long i = 0; i < n; ++i) {
+            psi[2 * i] = inv * psi[2 * i];
+            psi[2 * i + 1] = inv * psi[2 * i + 1];
+        }
+        apply_phase(half_v, psi, n);
+    }
+    return 0;
+}
+
+/* scenario/src/line1d_director.ixx:205-209: psi[i] *= mask[i] (real mask) */
+void orc_apply_mask(double* psi, const double* mask, size_t n) {
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < (long)n; ++i) {
+        psi[2 * i] = psi[2 * i] * mask[i];
+        psi[2 * i + 1] = psi[2 * i + 1] * mask[i];
+    }
+}
+
+/* ---- field reductions (core/src/field.ixx) ------------------------------ */
+
+/* core/src/field.ixx:33-39 and 70-76: serial sum of |z|^2, times the cell volume */
+double orc_norm_sq(const double* psi, size_t n, double cell_volume) {
+    double acc = 0.0;
+    for (size_t i = 0; i < n; ++i) acc += psi[2 * i] * psi[2 * i] + psi[2 * i + 1] * psi[2 * i + 1];
+    return acc * cell_volume;
+}
+
+/* core/src/field.ixx:78-87 (the 1D variant, field.ixx:41-46, has no zero guard;
+ * for a non-zero field both are the same operation) */
+void orc_normalize(double* psi, size_t n, double cell_volume) {
+    const double n2 = orc_norm_sq(psi, n, cell_volume);
+    if (n2 <= 0.0) return;
+    const double inv = 1.0 / sqrt(n2);
+    for (size_t i = 0; i < n; ++i) {
+        psi[2 * i] = inv * psi[2 * i];
+        psi[2 * i + 1] = inv * psi[2 * i + 1];
+    }
+}
+
+/* ---- builders (core/src/potential.ixx, core/src/wavepacket.ixx) --------- */
+
+static inline double cell_volume3(const double lo[3], const double hi[3], const int d[3]) {
+    return orc_spacing(lo[0], hi[0], d[0]) * orc_spacing(lo[1], hi[1], d[1]) *
+           orc_spacing(lo[2], hi[2], d[2]);
+}
+
+/* core/src/potential.ixx:16-23 */
+void orc_harmonic_1d(double xmin, double xmax, int n, double omega, double x0, double* v) {
+    for (int i = 0; i < n; ++i) {
+        const double dx = coord(xmin, xmax, n, i) - x0;
+        v[i] = 0.5 * omega * omega * dx * dx;
+    }
+}
+
+/* core/src/potential.ixx:34-48 */
+void orc_harmonic_3d(const int d[3], const double lo[3], const double hi[3], double omega,
+                     const double c[3], double* v) {
+    for (int k = 0; k < d[2]; ++k)
+        for (int j = 0; j < d[1]; ++j)
+            for (int i = 0; i < d[0]; ++i) {
+                const double dx = coord(lo[0], hi[0], d[0], i) - c[0];
+                const double dy = coord(lo[1], hi[1], d[1], j) - c[1];
+                const double dz = coord(lo[2], hi[2], d[2], k) - c[2];
+                v[i + (long)d[0] * (j + (long)d[1] * k)] =
+                    0.5 * omega * omega * (dx * dx + dy * dy + dz * dz);
+            }
+}
+
+/* core/src/potential.ixx:102-112: half-open [x_lo, x_hi) */
+void orc_barrier_1d(double xmin, double xmax, int n, double v0, double x_lo, double x_hi,
+                    double* v) {
+    for (int i = 0; i < n; ++i) {
+        const double x = coord(xmin, xmax, n, i);
+        v[i] = (x >= x_lo && x < x_hi) ? v0 : 0.0;
+    }
+}
+
+/* core/src/potential.ixx:114-132 */
+void orc_barrier_3d(const int d[3], const double lo[3], const double hi[3], double v0,
+                    double x_lo, double x_hi, double* v) {
+    const long n = (long)d[0] * d[1] * d[2];
+    for (long f = 0; f < n; ++f) v[f] = 0.0;
+    for (int i = 0; i < d[0]; ++i) {
+        const double x = coord(lo[0], hi[0], d[0], i);
+        if (x < x_lo || x >= x_hi) continue;
+        for (int k = 0; k < d[2]; ++k)
+            for (int j = 0; j < d[1]; ++j) v[i + (long)d[0] * (j + (long)d[1] * k)] = v0;
+    }
+}
+
+/* core/src/potential.ixx:134-154: bare -Z/r, nucleus cell -Z*2.3800774/h */
+void orc_regularized_coulomb_3d(const int d[3], const double lo[3], const double hi[3],
+                                double z, const double c[3], double* v) {
+    const double h = orc_spacing(lo[0], hi[0], d[0]);
+    const double center = -z * 2.3800774 / h;
+    for (int k = 0; k < d[2]; ++k)
+        for (int j = 0; j < d[1]; ++j)
+            for (int i = 0; i < d[0]; ++i) {
+                const double dx = coord(lo[0], hi[0], d[0], i) - c[0];
+                const double dy = coord(lo[1], hi[1], d[1], j) - c[1];
+                const double dz = coord(lo[2], hi[2], d[2], k) - c[2];
+                const double r = sqrt(dx * dx + dy * dy + dz * dz);
+                v[i + (long)d[0] * (j + (long)d[1] * k)] = (r < 1e-6 * h) ? center : -z / r;
+            }
+}
+
+/* No reference counterpart: the hydrogen workload's V = -Z / max(r, h/2).
+ * Pinned by its closed form at the nucleus cell (-2Z/h) and away from it (-Z/r). */
+void orc_coulomb_floor_3d(const int d[3], const double lo[3], const double hi[3], double z,
+                          const double c[3], double* v) {
+    const double h = orc_spacing(lo[0], hi[0], d[0]);
+    const double rmin = 0.5 * h;
+    for (int k = 0; k < d[2]; ++k)
+        for (int j = 0; j < d[1]; ++j)
+            for (int i = 0; i < d[0]; ++i) {
+                const double dx = coord(lo[0], hi[0], d[0], i) - c[0];
+                const double dy = coord(lo[1], hi[1], d[1], j) - c[1];
+                const double dz = coord(lo[2], hi[2], d[2], k) - c[2];
+                const double r = sqrt(dx * dx + dy * dy + dz * dz);
+                v[i + (long)d[0] * (j + (long)d[1] * k)] = -z / (r > rmin ? r : rmin);
+            }
+}
+
+/* core/src/potential.ixx:194-215 */
+void orc_absorbing_mask_1d(double xmin, double xmax, int n, double width, double* m) {
+    if (n == 1) {
+        m[0] = 1.0;
+        return;
+    }
+    for (int i = 0; i < n; ++i) {
+        const double x = coord(xmin, xmax, n, i);
+        const double d_lo = x - xmin;
+        const double d_hi = xmax - x;
+        const double d = d_lo < d_hi ? d_lo : d_hi;
+        if (d >= width) {
+            m[i] = 1.0;
+            continue;
+        }
+        const double t = d / width;
+        const double s = sin(0.5 * 3.14159265358979323846 * t);
+        m[i] = s * s;
+    }
+}
+
+/* core/src/potential.ixx:217-233: separable product mx*my*mz */
+void orc_absorbing_mask_3d(const int d[3], const double lo[3], const double hi[3],
+                           double width, double* m) {
+    double* mx = (double*)malloc(sizeof(double) * (size_t)d[0]);
+    double* my = (double*)malloc(sizeof(double) * (size_t)d[1]);
+    double* mz = (double*)malloc(sizeof(double) * (size_t)d[2]);
+    orc_absorbing_mask_1d(lo[0], hi[0], d[0], width, mx);
+    orc_absorbing_mask_1d(lo[1], hi[1], d[1], width, my);
+    orc_absorbing_mask_1d(lo[2], hi[2], d[2], width, mz);
+    for (int k = 0; k < d[2]; ++k)
+        for (int j = 0; j < d[1]; ++j)
+            for (int i = 0; i < d[0]; ++i)
+                m[i + (long)d[0] * (j + (long)d[1] * k)] = mx[i] * my[j] * mz[k];
+    free(mx);
+    free(my);
+    free(mz);
+}
+
+/* core/src/wavepacket.ixx:16-25 */
+void orc_gaussian_1d(double xmin, double xmax, int n, double x0, double sigma, double k0,
+                     double* psi) {
+    const double amp = pow(2.0 * ORC_PI * sigma * sigma, -0.25);
+    for (int i = 0; i < n; ++i) {
+        const double x = coord(xmin, xmax, n, i);
+        const double envelope = amp * exp(-(x - x0) * (x - x0) / (4.0 * sigma * sigma));
+        psi[2 * i] = envelope * cos(k0 * x);
+        psi[2 * i + 1] = envelope * sin(k0 * x);
+    }
+    /* core/src/field.ixx:41-46 */
+    const double inv = 1.0 / sqrt(orc_norm_sq(psi, (size_t)n, orc_spacing(xmin, xmax, n)));
+    for (int i = 0; i < n; ++i) {
+        psi[2 * i] = inv * psi[2 * i];
+        psi[2 * i + 1] = inv * psi[2 * i + 1];
+    }
+}
+
+static inline double envelope(double u, double u0, double s) {
+    return pow(2.0 * ORC_PI * s * s, -0.25) * exp(-(u - u0) * (u - u0) / (4.0 * s * s));
+}
+
+/* core/src/wavepacket.ixx:27-46 */
+void orc_gaussian_3d(const int d[3], const double lo[3], const double hi[3], const double r0[3],
+                     const double sigma[3], const double k0[3], double* psi) {
+    for (int k = 0; k < d[2]; ++k)
+        for (int j = 0; j < d[1]; ++j)
+            for (int i = 0; i < d[0]; ++i) {
+                const double x = coord(lo[0], hi[0], d[0], i);
+                const double y = coord(lo[1], hi[1], d[1], j);
+                const double z = coord(lo[2], hi[2], d[2], k);
+                const double env = envelope(x, r0[0], sigma[0]) * envelope(y, r0[1], sigma[1]) *
+                                   envelope(z, r0[2], sigma[2]);
+                const double phase = k0[0] * x + k0[1] * y + k0[2] * z;
+                const long f = i + (long)d[0] * (j + (long)d[1] * k);
+                psi[2 * f] = env * cos(phase);
+                psi[2 * f + 1] = env * sin(phase);
+            }
+    orc_normalize(psi, (size_t)d[0] * d[1] * d[2], cell_volume3(lo, hi, d));
+}
+
+/* ---- observables (core/src/observables.ixx) ----------------------------- */
+
+static inline double obs_ratio(double num, double den) { return den > 0.0 ? num / den : 0.0; }
+
+/* core/src/observables.ixx:29-38 */
+double orc_mean_position_1d(const double* psi, int n, double xmin, double xmax) {
+    double num = 0.0, den = 0.0;
+    for (int i = 0; i < n; ++i) {
+        const double w = psi[2 * i] * psi[2 * i] + psi[2 * i + 1] * psi[2 * i + 1];
+        num += coord(xmin, xmax, n, i) * w;
+        den += w;
+    }
+    return obs_ratio(num, den);
+}
+
+/* core/src/observables.ixx:67-90: each term normalised separately */
+double orc_mean_energy_1d(const double* psi, int n, double xmin, double xmax, const double* v) {
+    double num_v = 0.0, den_x = 0.0;
+    for (int i = 0; i < n; ++i) {
+        const double w = psi[2 * i] * psi[2 * i] + psi[2 * i + 1] * psi[2 * i + 1];
+        num_v += v[i] * w;
+        den_x += w;
+    }
+    double* phi = (double*)malloc(sizeof(double) * 2 * (size_t)n);
+    memcpy(phi, psi, sizeof(double) * 2 * (size_t)n);
+    orc_fft1d(phi, (size_t)n);
+    double* k = (double*)malloc(sizeof(double) * (size_t)n);
+    orc_wavenumbers(n, xmin, xmax, k);
+    double num_t = 0.0, den_k = 0.0;
+    for (int j = 0; j < n; ++j) {
+        const double w = phi[2 * j] * phi[2 * j] + phi[2 * j + 1] * phi[2 * j + 1];
+        num_t += 0.5 * k[j] * k[j] * w;
+        den_k += w;
+    }
+    free(k);
+    free(phi);
+    return obs_ratio(num_v, den_x) + obs_ratio(num_t, den_k);
+}
+
+/* core/src/observables.ixx:121-131: not scale invariant, times dx */
+double orc_probability_in_range_1d(const double* psi, int n, double xmin, double xmax, double a,
+                                   double b) {
+    double acc = 0.0;
+    for (int i = 0; i < n; ++i) {
+        const double x = coord(xmin, xmax, n, i);
+        if (x >= a && x < b) acc += psi[2 * i] * psi[2 * i] + psi[2 * i + 1] * psi[2 * i + 1];
+    }
+    return acc * orc_spacing(xmin, xmax, n);
+}
+
+/* core/src/observables.ixx:134-153 */
+void orc_mean_position_3d(const double* psi, const int d[3], const double lo[3],
+                          const double hi[3], double out[3]) {
+    double nx = 0.0, ny = 0.0, nz = 0.0, den = 0.0;
+    for (int k = 0; k < d[2]; ++k)
+        for (int j = 0; j < d[1]; ++j)
+            for (int i = 0; i < d[0]; ++i) {
+                const long f = i + (long)d[0] * (j + (long)d[1] * k);
+                const double w = psi[2 * f] * psi[2 * f] + psi[2 * f + 1] * psi[2 * f + 1];
+                nx += coord(lo[0], hi[0], d[0], i) * w;
+                ny += coord(lo[1], hi[1], d[1], j) * w;
+                nz += coord(lo[2], hi[2], d[2], k) * w;
+                den += w;
+            }
+    out[0] = obs_ratio(nx, den);
+    out[1] = obs_ratio(ny, den);
+    out[2] = obs_ratio(nz, den);
+}
+
+/* core/src/observables.ixx:202-236 (power-of-two grids, like the reference) */
+double orc_mean_energy_3d(const double* psi, const int d[3], const double lo[3],
+                          const double hi[3], const double* v, double mass) {
+    const long n = (long)d[0] * d[1] * d[2];
+    double num_v = 0.0, den_x = 0.0;
+    for (long i = 0; i < n; ++i) {
+        const double w = psi[2 * i] * psi[2 * i] + psi[2 * i + 1] * psi[2 * i + 1];
+        num_v += v[i] * w;
+        den_x += w;
+    }
+    double* phi = (double*)malloc(sizeof(double) * 2 * (size_t)n);
+    memcpy(phi, psi, sizeof(double) * 2 * (size_t)n);
+    if (orc_fft3d(phi, d[0], d[1], d[2]) != 0) orc_fft3d_any(phi, d[0], d[1], d[2], -1);
+    double* kx = (double*)malloc(sizeof(double) * (size_t)d[0]);
+    double* ky = (double*)malloc(sizeof(double) * (size_t)d[1]);
+    double* kz = (double*)malloc(sizeof(double) * (size_t)d[2]);
+    orc_wavenumbers(d[0], lo[0], hi[0], kx);
+    orc_wavenumbers(d[1], lo[1], hi[1], ky);
+    orc_wavenumbers(d[2], lo[2], hi[2], kz);
+    double num_t = 0.0, den_k = 0.0;
+    for (int k = 0; k < d[2]; ++k)
+        for (int j = 0; j < d[1]; ++j)
+            for (int i = 0; i < d[0]; ++i) {
+                const long f = i + (long)d[0] * (j + (long)d[1] * k);
+                const double w = phi[2 * f] * phi[2 * f] + phi[2 * f + 1] * phi[2 * f + 1];
+                const double kxx = kx[i], kyy = ky[j], kzz = kz[k];
+                num_t += 0.5 * (kxx * kxx + kyy * kyy + kzz * kzz) / mass * w;
+                den_k += w;
+            }
+    free(kx);
+    free(ky);
+    free(kz);
+    free(phi);
+    return obs_ratio(num_v, den_x) + obs_ratio(num_t, den_k);
+}
+
+/* No 3D counterpart in the reference (it has the 1D form only,
+ * core/src/observables.ixx:121-131): the same half-open, not scale invariant
+ * sum restricted to a slab a <= coord_axis < b, times the cell volume.
+ * Pinned by the Gaussian closed form (erfc tail) in the oracle tests. */
+double orc_probability_in_slab_3d(const double* psi, const int d[3], const double lo[3],
+                                  const double hi[3], int axis, double a, double b) {
+    double acc = 0.0;
+    for (int k = 0; k < d[2]; ++k)
+        for (int j = 0; j < d[1]; ++j)
+            for (int i = 0; i < d[0]; ++i) {
+                const int idx[3] = {i, j, k};
+                const double x = coord(lo[axis], hi[axis], d[axis], idx[axis]);
+                if (!(x >= a && x < b)) continue;
+                const long f = i + (long)d[0] * (j + (long)d[1] * k);
+                acc += psi[2 * f] * psi[2 * f] + psi[2 * f + 1] * psi[2 * f + 1];
+            }
+    return acc * cell_volume3(lo, hi, d);
+}
