@@ -1,0 +1,168 @@
+/*
+ * tdse.h -- C ABI of the MI355X split-operator TDSE propagator.
+ *
+ * Drop-in for the CPU FP64 propagator of the reference core:
+ *   ses::SplitOperator1D(grid, V, dt)             core/src/propagator.ixx:23-40
+ *   ses::SplitOperator3D(grid, V, dt, mass)       core/src/propagator.ixx:72-101
+ *   step(Field&, int nsteps)                      core/src/propagator.ixx:44-53, 110-119
+ *   WavepacketSimulation::advance(int)            core/src/simulation.ixx:40-43
+ * plus the caller-side absorbing mask applied after every step
+ * (scenario/src/line1d_director.ixx:202-211) and the observables of
+ * core/src/field.ixx:33-85 and core/src/observables.ixx.
+ *
+ * Conventions (identical to the reference):
+ *   - grids are periodic, spacing (hi - lo) / n, x-fastest flat index
+ *     i + nx*(j + ny*k)  (core/src/grid.ixx:10-30);
+ *   - a 2D grid is nx x ny x 1, a 1D grid is n x 1 x 1 (any axis of extent 1
+ *     is dropped), as in the reference's Grid3D usage;
+ *   - host states are interleaved complex128 (re, im) doubles, the layout of
+ *     std::vector<std::complex<double>> (core/src/field.ixx:11-66).
+ *
+ * Differences from the reference interface, all in the direction of a C ABI:
+ *   - every call returns a status (0 = TDSE_OK, < 0 = TDSE_E_*) instead of
+ *     throwing std::invalid_argument (core/src/fft.ixx:42-44, 110-112);
+ *   - the state stays resident on the GPU between calls: upload, step and
+ *     download are separate calls (the reference GPU engine's
+ *     upload_state/step/readback, solver/src/vk_engine.ixx:1083-1085, 2136-2157);
+ *   - tdse_step only enqueues work on the plan's stream; tdse_plan_sync and
+ *     tdse_state_download wait for it.
+ *
+ * Threading: one plan is used by one host thread at a time; distinct plans
+ * may be driven from distinct threads.  Line lengths supported per axis:
+ * 2..1024 powers of two and 96, 160, 192; a 1D grid also takes N = N1*N2
+ * with N1, N2 in that set (e.g. 65536).
+ */
+#ifndef TDSE_H
+#define TDSE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TDSE_OK 0
+#define TDSE_E_INVALID (-1)     /* bad argument (null pointer, n < 1, dt not finite, ...) */
+#define TDSE_E_UNSUPPORTED (-2) /* line length without a compiled FFT plan */
+#define TDSE_E_OOM (-3)         /* device allocation failed or HBM budget exceeded */
+#define TDSE_E_DEVICE (-4)      /* HIP runtime error; the plan's context is unusable */
+#define TDSE_E_TIMEOUT (-5)     /* a bounded wait expired */
+#define TDSE_E_NO_DEVICE (-6)   /* no HIP device with that ordinal */
+#define TDSE_E_STATE (-7)       /* call out of order (e.g. step before set_potential) */
+
+#define TDSE_FP32 0 /* complex64 state, FP64 phase arguments and reductions */
+#define TDSE_FP64 1 /* complex128 state */
+
+typedef struct tdse_ctx tdse_ctx;
+typedef struct tdse_plan tdse_plan;
+
+/* Library version string and human-readable status text. */
+const char* tdse_version(void);
+const char* tdse_status_string(int status);
+/* Text of the last error reported on the calling thread ("" if none). */
+const char* tdse_last_error(void);
+
+/* Device context.  hbm_budget_bytes caps the device memory all plans of this
+ * context may hold (0 = no cap); plan creation fails with TDSE_E_OOM beyond it
+ * (the admission policy of core/src/vram_budget.ixx, applied to bytes). */
+int tdse_ctx_create(int device_ordinal, uint64_t hbm_budget_bytes, tdse_ctx** out);
+int tdse_ctx_destroy(tdse_ctx* ctx);
+/* Bytes currently held by the context's plans. */
+int tdse_ctx_bytes_in_use(const tdse_ctx* ctx, uint64_t* out);
+/* Device memory a plan of this shape and precision would hold (for admission). */
+int tdse_plan_bytes(const int n[3], int precision, int with_mask, uint64_t* out);
+
+/* A propagator for one grid: replaces the SplitOperator1D/3D constructor
+ * (core/src/propagator.ixx:23, 72).  lo/hi are the periodic box bounds per
+ * axis (Grid1D::xmin/xmax), mass the effective mass (default 1.0 in the
+ * reference), flags reserved (pass 0). */
+int tdse_plan_create(tdse_ctx* ctx, const int n[3], const double lo[3], const double hi[3],
+                     double dt, double mass, int precision, unsigned flags, tdse_plan** out);
+int tdse_plan_destroy(tdse_plan* plan);
+
+/* The potential V (n[0]*n[1]*n[2] doubles, x-fastest).  Builds the half-kick
+ * phase -0.5*V*dt per cell (core/src/propagator.ixx:28-31, 78-81); the FP32
+ * path reduces it modulo 2*pi in FP64 before rounding. */
+int tdse_plan_set_potential(tdse_plan* plan, const double* v);
+/* Real absorbing mask multiplied into psi after every step (null clears it).
+ * core/src/potential.ixx:194-233 builds such masks. */
+int tdse_plan_set_mask(tdse_plan* plan, const double* mask);
+
+/* State transfer, interleaved complex128 on the host. */
+int tdse_state_upload(tdse_plan* plan, const double* psi);
+int tdse_state_download(tdse_plan* plan, double* psi);
+/* Copy the state of `src` into `dst` (same shape and precision), on the GPU. */
+int tdse_state_copy(tdse_plan* dst, const tdse_plan* src);
+/* Device address of the resident state (complex64 or complex128 per point). */
+int tdse_state_device_ptr(tdse_plan* plan, void** out);
+
+/* nsteps Strang steps (core/src/propagator.ixx:44-53, 110-119), each followed
+ * by the mask when one is set.  nsteps == 0 is the identity.  Asynchronous. */
+int tdse_step(tdse_plan* plan, int nsteps);
+/* Wait for the plan's queued work; timeout_ms < 0 waits forever. */
+int tdse_plan_sync(tdse_plan* plan, int timeout_ms);
+/* The plan's HIP stream (hipStream_t) for event timing by the caller. */
+int tdse_plan_stream(tdse_plan* plan, void** out);
+
+/* Observables of the resident state, FP64 results, fixed-order reductions.
+ *   TDSE_OBS_NORM      norm = sum |psi|^2 * cell volume      (core/src/field.ixx:70-76)
+ *   TDSE_OBS_POSITION  <x>, <y>, <z> (ratio form)            (core/src/observables.ixx:134-153)
+ *   TDSE_OBS_ENERGY    <V> + <k^2/2m>, each term normalised  (core/src/observables.ixx:202-236)
+ */
+#define TDSE_OBS_NORM 1u
+#define TDSE_OBS_POSITION 2u
+#define TDSE_OBS_ENERGY 4u
+typedef struct tdse_observables {
+    double norm;
+    double sum_abs2;      /* sum |psi|^2 without the cell volume */
+    double position[3];
+    double potential;     /* <V> */
+    double kinetic;       /* <k^2 / 2m> */
+    double energy;        /* potential + kinetic */
+} tdse_observables;
+int tdse_observe(tdse_plan* plan, unsigned what, tdse_observables* out);
+/* Probability lo <= coord_axis < hi times the cell volume, not normalised
+ * (core/src/observables.ixx:121-131, extended to one axis of a 3D grid). */
+int tdse_probability_in_slab(tdse_plan* plan, int axis, double lo, double hi, double* out);
+/* Multiply the state by s on the GPU (renormalisation; solver/shaders/scale.comp.slang). */
+int tdse_state_scale(tdse_plan* plan, double s);
+
+/* Per-launch timing of `nsteps` steps with HIP events around every sweep
+ * kernel on the plan's stream.  ms_per_sweep[i] / launches[i] describe sweep
+ * slot i of the step schedule (slots listed by tdse_plan_sweep_name); at most
+ * max_slots are written; *n_slots receives the number of slots. */
+int tdse_profile_step(tdse_plan* plan, int nsteps, int max_slots, double* ms_per_sweep,
+                      int* launches, int* n_slots);
+const char* tdse_plan_sweep_name(const tdse_plan* plan, int slot);
+/* Algorithmic HBM bytes one steady-state step moves (state in and out per
+ * sweep, kick phase and mask once). */
+int tdse_plan_step_bytes(const tdse_plan* plan, uint64_t* out);
+
+/* Host input builders, FP64, x-fastest, bit-identical to the reference
+ * builders for the same arguments (n x 1 x 1 grids use the 1D forms).
+ *   harmonic            0.5 w^2 |r - c|^2           core/src/potential.ixx:16-48
+ *   barrier             v0 on x_lo <= x < x_hi      core/src/potential.ixx:102-132
+ *   regularized_coulomb -Z/r, nucleus cell -2.3800774 Z/h   core/src/potential.ixx:134-154
+ *   coulomb_floor       -Z / max(r, h/2)            (no reference builder)
+ *   absorbing_mask      separable sin^2 ramp        core/src/potential.ixx:194-233
+ *   gaussian            normalised packet, interleaved complex128 out
+ *                                                   core/src/wavepacket.ixx:16-48 */
+int tdse_build_harmonic(const int n[3], const double lo[3], const double hi[3], double omega,
+                        const double c[3], double* v);
+int tdse_build_barrier(const int n[3], const double lo[3], const double hi[3], double v0,
+                       double x_lo, double x_hi, double* v);
+int tdse_build_regularized_coulomb(const int n[3], const double lo[3], const double hi[3],
+                                   double z, const double c[3], double* v);
+int tdse_build_coulomb_floor(const int n[3], const double lo[3], const double hi[3], double z,
+                             const double c[3], double* v);
+int tdse_build_absorbing_mask(const int n[3], const double lo[3], const double hi[3],
+                              double width, double* m);
+int tdse_build_gaussian(const int n[3], const double lo[3], const double hi[3], const double r0[3],
+                        const double sigma[3], const double k0[3], double* psi);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TDSE_H */

@@ -1,0 +1,423 @@
+"""MI355X split-operator TDSE propagator: host-side mirror of the reference
+core interface over the C ABI (include/tdse.h, libtdse.so).
+
+Reference interface mirrored here (names, argument meaning, error cases):
+
+* ``Grid1D`` / ``Grid3D``                  core/src/grid.ixx:10-30
+* ``wavenumbers``                          core/src/spectral.ixx:14-23
+* ``SplitOperator1D(grid, V, dt)``         core/src/propagator.ixx:21-64
+* ``SplitOperator3D(grid, V, dt, mass)``   core/src/propagator.ixx:66-134
+* ``WavepacketSimulation``                 core/src/simulation.ixx:22-88
+* potential / mask / packet builders       core/src/potential.ixx, wavepacket.ixx
+
+Fields are numpy ``complex128`` arrays: shape ``(n,)`` for a ``Grid1D`` and
+``(nz, ny, nx)`` for a ``Grid3D`` (x fastest, core/src/field.ixx).  ``step``
+updates the array in place, like ``SplitOperator*::step(Field&, int)``.
+Unsupported grid lengths raise ``ValueError`` (the reference raises
+``std::invalid_argument`` for non powers of two, core/src/fft.ixx:42-44).
+
+``Propagator`` is the device-resident form: the state stays in HBM between
+calls, which is what the serving and batching layers use.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import capi
+from .capi import (TDSE_FP32, TDSE_FP64, TDSE_OBS_ENERGY, TDSE_OBS_NORM, TDSE_OBS_POSITION,
+                   TdseError)
+
+__all__ = [
+    "Grid1D", "Grid3D", "wavenumbers", "Context", "Propagator", "SplitOperator1D",
+    "SplitOperator3D", "WavepacketSimulation", "harmonic_potential", "barrier_potential",
+    "regularized_coulomb_potential", "coulomb_floor_potential", "absorbing_mask",
+    "gaussian_wavepacket", "TDSE_FP32", "TDSE_FP64", "TdseError",
+]
+
+PRECISIONS = {"fp32": TDSE_FP32, "fp64": TDSE_FP64, TDSE_FP32: TDSE_FP32, TDSE_FP64: TDSE_FP64}
+
+
+# ---- grids (core/src/grid.ixx) ----------------------------------------------
+
+@dataclass(frozen=True)
+class Grid1D:
+    xmin: float
+    xmax: float
+    n: int
+
+    def size(self) -> int:
+        return self.n
+
+    def spacing(self) -> float:
+        return (self.xmax - self.xmin) / self.n
+
+    def coord(self, i):
+        return self.xmin + i * self.spacing()
+
+
+@dataclass(frozen=True)
+class Grid3D:
+    x: Grid1D
+    y: Grid1D
+    z: Grid1D
+
+    def size(self) -> int:
+        return self.x.n * self.y.n * self.z.n
+
+    def flat(self, i, j, k):
+        return i + self.x.n * (j + self.y.n * k)
+
+    def cell_volume(self) -> float:
+        return self.x.spacing() * self.y.spacing() * self.z.spacing()
+
+    @property
+    def shape(self):
+        return (self.z.n, self.y.n, self.x.n)
+
+
+def _axes(grid):
+    """(n[3], lo[3], hi[3]) of a Grid1D (as n x 1 x 1) or Grid3D."""
+    if isinstance(grid, Grid1D):
+        return (grid.n, 1, 1), (grid.xmin, 0.0, 0.0), (grid.xmax, 1.0, 1.0)
+    return ((grid.x.n, grid.y.n, grid.z.n), (grid.x.xmin, grid.y.xmin, grid.z.xmin),
+            (grid.x.xmax, grid.y.xmax, grid.z.xmax))
+
+
+def _shape(grid):
+    return (grid.n,) if isinstance(grid, Grid1D) else grid.shape
+
+
+def wavenumbers(g: Grid1D) -> np.ndarray:
+    """fftfreq-ordered wavenumbers 2 pi / L * (j or j - n) (core/src/spectral.ixx:14-23)."""
+    dk = 2.0 * math.pi / (g.xmax - g.xmin)
+    j = np.arange(g.n)
+    return dk * np.where(2 * j < g.n, j, j - g.n).astype(np.float64)
+
+
+# ---- device context and resident propagator ---------------------------------
+
+class Context:
+    """A device and an optional HBM budget shared by its propagators."""
+
+    def __init__(self, device: int = 0, hbm_budget_bytes: int = 0):
+        L = capi.lib()
+        h = C.c_void_p()
+        capi.check(L.tdse_ctx_create(device, hbm_budget_bytes, C.byref(h)))
+        self._h = h
+        self.device = device
+        self.budget = hbm_budget_bytes
+
+    @property
+    def handle(self):
+        return self._h
+
+    def bytes_in_use(self) -> int:
+        v = C.c_uint64()
+        capi.check(capi.lib().tdse_ctx_bytes_in_use(self._h, C.byref(v)))
+        return v.value
+
+    def close(self):
+        if self._h:
+            capi.lib().tdse_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_DEFAULT_CTX = None
+
+
+def default_context() -> Context:
+    global _DEFAULT_CTX
+    if _DEFAULT_CTX is None:
+        _DEFAULT_CTX = Context(0)
+    return _DEFAULT_CTX
+
+
+def plan_bytes(grid, precision="fp32", with_mask=False) -> int:
+    n, _, _ = _axes(grid)
+    v = C.c_uint64()
+    capi.check(capi.lib().tdse_plan_bytes(capi.ivec(n), PRECISIONS[precision], int(with_mask),
+                                          C.byref(v)))
+    return v.value
+
+
+def _map_error(e: TdseError):
+    if e.status in (capi.TDSE_E_UNSUPPORTED, capi.TDSE_E_INVALID):
+        return ValueError(str(e))
+    return e
+
+
+class Propagator:
+    """Device-resident split-operator propagator for one grid (one C-ABI plan)."""
+
+    def __init__(self, grid, potential, dt, mass=1.0, precision="fp64", mask=None,
+                 context: Context | None = None):
+        self.grid = grid
+        self.ctx = context or default_context()
+        self.precision = PRECISIONS[precision]
+        self.dt = float(dt)
+        self.mass = float(mass)
+        n, lo, hi = _axes(grid)
+        self.shape = _shape(grid)
+        self.size = int(np.prod(n))
+        L = capi.lib()
+        h = C.c_void_p()
+        try:
+            capi.check(L.tdse_plan_create(self.ctx.handle, capi.ivec(n), capi.dvec(lo),
+                                          capi.dvec(hi), C.c_double(dt), C.c_double(mass),
+                                          self.precision, 0, C.byref(h)))
+        except TdseError as e:
+            raise _map_error(e) from None
+        self._h = h
+        self.set_potential(potential)
+        if mask is not None:
+            self.set_mask(mask)
+
+    @property
+    def handle(self):
+        return self._h
+
+    def _vec(self, a, name):
+        a = np.ascontiguousarray(a, dtype=np.float64).ravel()
+        if a.size != self.size:
+            raise ValueError(f"{name} has {a.size} entries, grid has {self.size}")
+        return a
+
+    def set_potential(self, v):
+        v = self._vec(v, "potential")
+        capi.check(capi.lib().tdse_plan_set_potential(self._h, capi.dptr(v)))
+
+    def set_mask(self, m):
+        if m is None:
+            capi.check(capi.lib().tdse_plan_set_mask(self._h, None))
+            return
+        m = self._vec(m, "mask")
+        capi.check(capi.lib().tdse_plan_set_mask(self._h, capi.dptr(m)))
+
+    def upload(self, psi):
+        a = np.ascontiguousarray(psi, dtype=np.complex128).ravel()
+        if a.size != self.size:
+            raise ValueError(f"state has {a.size} points, grid has {self.size}")
+        capi.check(capi.lib().tdse_state_upload(self._h, capi.dptr(a)))
+
+    def download(self, out=None):
+        if out is None:
+            out = np.empty(self.shape, np.complex128)
+        if not (out.flags.c_contiguous and out.dtype == np.complex128 and out.size == self.size):
+            raise ValueError("output must be a contiguous complex128 array of the grid's size")
+        capi.check(capi.lib().tdse_state_download(self._h, capi.dptr(out)))
+        return out
+
+    def step(self, nsteps=1):
+        capi.check(capi.lib().tdse_step(self._h, int(nsteps)))
+
+    def sync(self, timeout_ms=-1):
+        capi.check(capi.lib().tdse_plan_sync(self._h, int(timeout_ms)))
+
+    def scale(self, s):
+        capi.check(capi.lib().tdse_state_scale(self._h, C.c_double(s)))
+
+    def observe(self, what=TDSE_OBS_NORM | TDSE_OBS_POSITION | TDSE_OBS_ENERGY):
+        o = capi.Observables()
+        capi.check(capi.lib().tdse_observe(self._h, what, C.byref(o)))
+        return {"norm": o.norm, "sum_abs2": o.sum_abs2, "position": tuple(o.position),
+                "potential": o.potential, "kinetic": o.kinetic, "energy": o.energy}
+
+    def probability_in_slab(self, axis, lo, hi):
+        v = C.c_double()
+        capi.check(capi.lib().tdse_probability_in_slab(self._h, axis, lo, hi, C.byref(v)))
+        return v.value
+
+    def stream(self):
+        s = C.c_void_p()
+        capi.check(capi.lib().tdse_plan_stream(self._h, C.byref(s)))
+        return s.value
+
+    def device_ptr(self):
+        s = C.c_void_p()
+        capi.check(capi.lib().tdse_state_device_ptr(self._h, C.byref(s)))
+        return s.value
+
+    def step_bytes(self):
+        v = C.c_uint64()
+        capi.check(capi.lib().tdse_plan_step_bytes(self._h, C.byref(v)))
+        return v.value
+
+    def profile(self, nsteps):
+        """Per-sweep event timing over nsteps: {slot name: (total ms, launches)}."""
+        ms = (C.c_double * 16)()
+        cnt = (C.c_int * 16)()
+        ns = C.c_int()
+        capi.check(capi.lib().tdse_profile_step(self._h, int(nsteps), 16, ms, cnt, C.byref(ns)))
+        L = capi.lib()
+        return {L.tdse_plan_sweep_name(self._h, i).decode(): (ms[i], cnt[i])
+                for i in range(ns.value)}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            capi.lib().tdse_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ---- drop-in propagators (core/src/propagator.ixx) --------------------------
+
+class _SplitOperatorBase:
+    def __init__(self, grid, potential, dt, mass, precision, context):
+        self._grid = grid
+        self._dt = float(dt)
+        self._prop = Propagator(grid, potential, dt, mass=mass, precision=precision,
+                                context=context)
+
+    def dt(self) -> float:
+        return self._dt
+
+    def step(self, psi: np.ndarray, nsteps: int = 1) -> None:
+        """Advance ``psi`` in place by ``nsteps`` Strang steps."""
+        if nsteps < 0:
+            raise ValueError("nsteps must be >= 0")
+        if psi.dtype != np.complex128 or not psi.flags.c_contiguous:
+            raise ValueError("psi must be a contiguous complex128 array")
+        if psi.size != self._prop.size:
+            raise ValueError("psi does not match the grid")
+        if nsteps == 0:
+            return
+        self._prop.upload(psi)
+        self._prop.step(nsteps)
+        self._prop.download(psi.reshape(self._prop.shape))
+
+
+class SplitOperator1D(_SplitOperatorBase):
+    """``ses::SplitOperator1D(grid, V, dt)`` (core/src/propagator.ixx:21-64)."""
+
+    def __init__(self, grid: Grid1D, potential, dt, precision="fp64", context=None):
+        super().__init__(grid, potential, dt, 1.0, precision, context)
+
+
+class SplitOperator3D(_SplitOperatorBase):
+    """``ses::SplitOperator3D(grid, V, dt, mass)`` (core/src/propagator.ixx:66-134)."""
+
+    def __init__(self, grid: Grid3D, potential, dt, mass=1.0, precision="fp64", context=None):
+        super().__init__(grid, potential, dt, mass, precision, context)
+
+
+# ---- builders (core/src/potential.ixx, core/src/wavepacket.ixx) -------------
+
+def _build(fn, grid, *scalars_and_vecs, out_complex=False):
+    n, lo, hi = _axes(grid)
+    out = np.empty(int(np.prod(n)) * (2 if out_complex else 1), np.float64)
+    capi.check(fn(capi.ivec(n), capi.dvec(lo), capi.dvec(hi), *scalars_and_vecs, capi.dptr(out)))
+    return out.view(np.complex128).reshape(_shape(grid)) if out_complex else out
+
+
+def _center(grid, c):
+    if isinstance(grid, Grid1D):
+        return capi.dvec((float(c), 0.0, 0.0))
+    return capi.dvec(c)
+
+
+def harmonic_potential(grid, omega, c=0.0):
+    """0.5 w^2 |r - c|^2 (core/src/potential.ixx:16-48); c is x0 for a Grid1D."""
+    return _build(capi.lib().tdse_build_harmonic, grid, C.c_double(omega), _center(grid, c))
+
+
+def barrier_potential(grid, v0, x_lo, x_hi):
+    """v0 on x_lo <= x < x_hi along x (core/src/potential.ixx:102-132)."""
+    return _build(capi.lib().tdse_build_barrier, grid, C.c_double(v0), C.c_double(x_lo),
+                  C.c_double(x_hi))
+
+
+def regularized_coulomb_potential(grid: Grid3D, Z, c):
+    """-Z/r with the nucleus cell's analytic average (core/src/potential.ixx:134-154)."""
+    return _build(capi.lib().tdse_build_regularized_coulomb, grid, C.c_double(Z), capi.dvec(c))
+
+
+def coulomb_floor_potential(grid: Grid3D, Z, c):
+    """-Z / max(r, h/2): the hydrogen-like workload's Coulomb (no reference builder)."""
+    return _build(capi.lib().tdse_build_coulomb_floor, grid, C.c_double(Z), capi.dvec(c))
+
+
+def absorbing_mask(grid, width):
+    """sin^2 ramp to 0 within ``width`` of each wall (core/src/potential.ixx:194-233)."""
+    return _build(capi.lib().tdse_build_absorbing_mask, grid, C.c_double(width))
+
+
+def gaussian_wavepacket(grid, r0, sigma, k0):
+    """Normalised Gaussian packet (core/src/wavepacket.ixx:16-48); scalars for a Grid1D."""
+    if isinstance(grid, Grid1D):
+        r0, sigma, k0 = (r0, 1.0, 1.0), (sigma, 1.0, 1.0), (k0, 0.0, 0.0)
+    return _build(capi.lib().tdse_build_gaussian, grid, capi.dvec(r0), capi.dvec(sigma),
+                  capi.dvec(k0), out_complex=True)
+
+
+# ---- evolve wrapper (core/src/simulation.ixx) --------------------------------
+
+@dataclass
+class SimulationConfig:
+    grid: Grid3D
+    potential: np.ndarray
+    r0: tuple
+    sigma: tuple
+    k0: tuple
+    dt: float
+    precision: str = "fp64"
+
+
+class WavepacketSimulation:
+    """``ses::WavepacketSimulation`` (core/src/simulation.ixx:22-88): owns the
+    grid, potential, propagator and state; ``advance(n)`` steps it.  The state
+    stays on the GPU between ``advance`` calls and is read back by ``psi()``."""
+
+    Config = SimulationConfig
+
+    def __init__(self, cfg: SimulationConfig, context=None):
+        self._grid = cfg.grid
+        self._dt = float(cfg.dt)
+        self._potential = np.ascontiguousarray(cfg.potential, dtype=np.float64)
+        self._prop = Propagator(cfg.grid, self._potential, cfg.dt, precision=cfg.precision,
+                                context=context)
+        self._prop.upload(gaussian_wavepacket(cfg.grid, cfg.r0, cfg.sigma, cfg.k0))
+        self._steps = 0
+
+    def advance(self, nsteps: int) -> None:
+        self._prop.step(nsteps)
+        self._steps += nsteps
+
+    def time(self) -> float:
+        return self._steps * self._dt
+
+    def dt(self) -> float:
+        return self._dt
+
+    def grid(self):
+        return self._grid
+
+    def potential(self):
+        return self._potential
+
+    def psi(self) -> np.ndarray:
+        return self._prop.download()
+
+    def set_psi(self, psi) -> None:
+        self._prop.upload(psi)
+
+    def density(self) -> np.ndarray:
+        return np.abs(self.psi()) ** 2
+
+    @property
+    def propagator(self) -> Propagator:
+        return self._prop

@@ -1,0 +1,89 @@
+// kernels_impl.hpp -- explicit instantiation of every (line length, mode)
+// sweep for one precision.  Included by kernels_f32.hip / kernels_f64.hip
+// with TDSE_REAL and TDSE_SUFFIX defined.
+#pragma once
+
+#include "launch.hpp"
+
+namespace tdse {
+namespace {
+
+template <typename T, int N, int MODE>
+hipError_t launch_sweep(const SweepArgs& a, int max_blocks, hipStream_t stream) {
+    using S = Shape<T, N>;
+    long long tiles;
+    int block;
+    if constexpr (MODE <= C_SINGLE || MODE == C_FWD) {
+        tiles = (a.nlines + S::LPB - 1) / S::LPB;
+        block = S::BLOCK_C;
+    } else {
+        tiles = static_cast<long long>((a.nx + S::TX - 1) / S::TX) * a.nw;
+        block = S::BLOCK_S;
+    }
+    if (tiles <= 0) return hipSuccess;
+    const int grid = static_cast<int>(tiles < max_blocks ? tiles : max_blocks);
+    if constexpr (MODE <= C_SINGLE || MODE == C_FWD) {
+        hipLaunchKernelGGL((contig_sweep<T, N, MODE>), dim3(grid), dim3(block), 0, stream, a);
+    } else {
+        hipLaunchKernelGGL((strided_sweep<T, N, MODE>), dim3(grid), dim3(block), 0, stream, a);
+    }
+    return hipGetLastError();
+}
+
+template <typename T, int N>
+LaunchFn pick_mode(int mode) {
+    switch (mode) {
+        case C_KICK_FIRST: return &launch_sweep<T, N, C_KICK_FIRST>;
+        case C_KICK_MID: return &launch_sweep<T, N, C_KICK_MID>;
+        case C_KICK_LAST: return &launch_sweep<T, N, C_KICK_LAST>;
+        case C_KIN_TW: return &launch_sweep<T, N, C_KIN_TW>;
+        case C_SINGLE: return &launch_sweep<T, N, C_SINGLE>;
+        case C_FWD: return &launch_sweep<T, N, C_FWD>;
+        case S_FWD: return &launch_sweep<T, N, S_FWD>;
+        case S_INV: return &launch_sweep<T, N, S_INV>;
+        case S_KIN: return &launch_sweep<T, N, S_KIN>;
+        case S_KICK_FIRST_TW: return &launch_sweep<T, N, S_KICK_FIRST_TW>;
+        case S_KICK_MID_TW: return &launch_sweep<T, N, S_KICK_MID_TW>;
+        case S_KICK_LAST: return &launch_sweep<T, N, S_KICK_LAST>;
+        case S_FWD_TW: return &launch_sweep<T, N, S_FWD_TW>;
+        default: return nullptr;
+    }
+}
+
+template <typename T, int N>
+SweepShape shape_of(int mode) {
+    using S = Shape<T, N>;
+    if (mode_is_contiguous(mode)) return {S::BLOCK_C, S::LPB};
+    return {S::BLOCK_S, S::TX};
+}
+
+}  // namespace
+
+#define TDSE_CAT2(a, b) a##b
+#define TDSE_CAT(a, b) TDSE_CAT2(a, b)
+
+LaunchFn TDSE_CAT(find_launcher_, TDSE_SUFFIX)(int n, int mode) {
+    switch (n) {
+#define TDSE_CASE(N_) \
+    case N_:          \
+        return pick_mode<TDSE_REAL, N_>(mode);
+        TDSE_FOR_EACH_N(TDSE_CASE)
+#undef TDSE_CASE
+        default:
+            return nullptr;
+    }
+}
+
+SweepShape TDSE_CAT(sweep_shape_, TDSE_SUFFIX)(int n, int mode) {
+    switch (n) {
+#define TDSE_CASE(N_) \
+    case N_:          \
+        return shape_of<TDSE_REAL, N_>(mode);
+        TDSE_FOR_EACH_N(TDSE_CASE)
+#undef TDSE_CASE
+        default:
+            return {0, 0};
+    }
+}
+
+}  // namespace tdse

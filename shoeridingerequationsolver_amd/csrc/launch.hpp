@@ -1,0 +1,26 @@
+// launch.hpp -- host-side view of the compiled sweep kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "sweeps.hpp"
+
+namespace tdse {
+
+// Launch one sweep over the whole array described by `a` on `stream`.
+using LaunchFn = hipError_t (*)(const SweepArgs& a, int max_blocks, hipStream_t stream);
+
+struct SweepShape {
+    int block;      // threads per block
+    int per_block;  // lines (contiguous) or x positions (strided) per block
+};
+
+// Lookup of the compiled kernel for (line length, mode); null if not compiled.
+LaunchFn find_launcher_f32(int n, int mode);
+LaunchFn find_launcher_f64(int n, int mode);
+SweepShape sweep_shape_f32(int n, int mode);
+SweepShape sweep_shape_f64(int n, int mode);
+
+inline bool mode_is_contiguous(int mode) { return mode <= C_SINGLE || mode == C_FWD; }
+
+}  // namespace tdse

@@ -1,0 +1,173 @@
+// observe.hip -- deterministic FP64 reductions over the resident state.
+//
+// Every block walks a fixed, grid-stride set of points in a fixed order,
+// accumulates in FP64, reduces inside each wave64 with a fixed xor-shuffle
+// tree and across waves through LDS in wave order, and writes one partial per
+// block.  A single-block pass then sums the partials in index order.  For a
+// given shape the result is bitwise reproducible from run to run (no atomics),
+// which the reference's deterministic pool guarantees on the CPU
+// (core/src/parallel.ixx:14-16, 211-230).
+#include <hip/hip_runtime.h>
+
+#include "observe.hpp"
+
+namespace tdse {
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kWaves = kBlock / 64;
+
+template <int Q>
+__device__ __forceinline__ void block_reduce_store(double (&acc)[Q], double* out) {
+    __shared__ double part[kWaves][Q];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        double v = acc[q];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+        acc[q] = v;
+    }
+    if (lane == 0) {
+#pragma unroll
+        for (int q = 0; q < Q; ++q) part[wave][q] = acc[q];
+    }
+    __syncthreads();
+    if (threadIdx.x < Q) {
+        double s = 0.0;
+#pragma unroll
+        for (int w = 0; w < kWaves; ++w) s += part[w][threadIdx.x];
+        out[threadIdx.x] = s;
+    }
+}
+
+__device__ __forceinline__ int shift_k(int j, int n) { return (2 * j < n) ? j : j - n; }
+
+// x-space sums: |psi|^2, x|psi|^2, y|psi|^2, z|psi|^2, V|psi|^2, slab |psi|^2
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+reduce_x(const ReduceArgs a, double* partials) {
+    const cx<T>* psi = static_cast<const cx<T>*>(a.psi);
+    const T* v = static_cast<const T*>(a.v);
+    double acc[kReduceQ] = {0, 0, 0, 0, 0, 0};
+    const long long stride = static_cast<long long>(gridDim.x) * kBlock;
+    for (long long f = blockIdx.x * static_cast<long long>(kBlock) + threadIdx.x; f < a.total;
+         f += stride) {
+        const cx<T> z = psi[f];
+        const double w = static_cast<double>(z.x) * z.x + static_cast<double>(z.y) * z.y;
+        const int ix = static_cast<int>(f % a.n[0]);
+        const long long r = f / a.n[0];
+        const int iy = static_cast<int>(r % a.n[1]);
+        const int iz = static_cast<int>(r / a.n[1]);
+        const double cxv = a.lo[0] + ix * a.h[0];
+        const double cyv = a.lo[1] + iy * a.h[1];
+        const double czv = a.lo[2] + iz * a.h[2];
+        acc[0] += w;
+        acc[1] += cxv * w;
+        acc[2] += cyv * w;
+        acc[3] += czv * w;
+        if (v) acc[4] += static_cast<double>(v[f]) * w;
+        const double c = a.slab_axis == 0 ? cxv : (a.slab_axis == 1 ? cyv : czv);
+        if (c >= a.slab_lo && c < a.slab_hi) acc[5] += w;
+    }
+    block_reduce_store<kReduceQ>(acc, partials + blockIdx.x * kReduceQ);
+}
+
+// k-space sums of the forward transform: |phi|^2 and 0.5 |k|^2 / m |phi|^2.
+// Natural order on the squeezed axes, or the four-step order of a 1D grid
+// (address k1 + N1*k2 holds bin k2 + N2*k1).
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+reduce_k(const ReduceArgs a, double* partials) {
+    const cx<T>* phi = static_cast<const cx<T>*>(a.psi);
+    double acc[kReduceQ] = {0, 0, 0, 0, 0, 0};
+    const long long stride = static_cast<long long>(gridDim.x) * kBlock;
+    for (long long f = blockIdx.x * static_cast<long long>(kBlock) + threadIdx.x; f < a.total;
+         f += stride) {
+        const cx<T> z = phi[f];
+        const double w = static_cast<double>(z.x) * z.x + static_cast<double>(z.y) * z.y;
+        double k2sum;
+        if (a.four_step_rows > 0) {
+            const long long k1 = f % a.sq[0];
+            const long long k2 = f / a.sq[0];
+            const int j = static_cast<int>(k2 + a.four_step_rows * k1);
+            const double k = a.dk[0] * shift_k(j, static_cast<int>(a.total));
+            k2sum = k * k;
+        } else {
+            const int i0 = static_cast<int>(f % a.sq[0]);
+            const long long r = f / a.sq[0];
+            const int i1 = static_cast<int>(r % a.sq[1]);
+            const int i2 = static_cast<int>(r / a.sq[1]);
+            const double k0 = a.dk[0] * shift_k(i0, a.sq[0]);
+            const double k1 = a.dk[1] * shift_k(i1, a.sq[1]);
+            const double k2 = a.dk[2] * shift_k(i2, a.sq[2]);
+            k2sum = k0 * k0 + k1 * k1 + k2 * k2;
+        }
+        acc[0] += w;
+        acc[1] += 0.5 * k2sum / a.mass * w;
+    }
+    block_reduce_store<kReduceQ>(acc, partials + blockIdx.x * kReduceQ);
+}
+
+__global__ void __launch_bounds__(kBlock)
+reduce_final(const double* partials, int nparts, double* out) {
+    double acc[kReduceQ] = {0, 0, 0, 0, 0, 0};
+    for (int p = threadIdx.x; p < nparts; p += kBlock) {
+#pragma unroll
+        for (int q = 0; q < kReduceQ; ++q) acc[q] += partials[p * kReduceQ + q];
+    }
+    block_reduce_store<kReduceQ>(acc, out);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+scale_state(void* psi_v, long long total, double s) {
+    cx<T>* psi = static_cast<cx<T>*>(psi_v);
+    const T ss = static_cast<T>(s);
+    const long long stride = static_cast<long long>(gridDim.x) * kBlock;
+    for (long long f = blockIdx.x * static_cast<long long>(kBlock) + threadIdx.x; f < total;
+         f += stride) {
+        cx<T> z = psi[f];
+        psi[f] = {z.x * ss, z.y * ss};
+    }
+}
+
+}  // namespace
+
+int reduce_blocks(long long total) {
+    long long b = (total + kBlock * 8 - 1) / (kBlock * 8);
+    if (b < 1) b = 1;
+    if (b > kMaxReduceBlocks) b = kMaxReduceBlocks;
+    return static_cast<int>(b);
+}
+
+hipError_t launch_reduce(bool fp64, bool kspace, const ReduceArgs& a, double* partials,
+                         double* out, hipStream_t stream) {
+    const int blocks = reduce_blocks(a.total);
+    if (kspace) {
+        if (fp64)
+            hipLaunchKernelGGL(reduce_k<double>, dim3(blocks), dim3(kBlock), 0, stream, a, partials);
+        else
+            hipLaunchKernelGGL(reduce_k<float>, dim3(blocks), dim3(kBlock), 0, stream, a, partials);
+    } else {
+        if (fp64)
+            hipLaunchKernelGGL(reduce_x<double>, dim3(blocks), dim3(kBlock), 0, stream, a, partials);
+        else
+            hipLaunchKernelGGL(reduce_x<float>, dim3(blocks), dim3(kBlock), 0, stream, a, partials);
+    }
+    hipLaunchKernelGGL(reduce_final, dim3(1), dim3(kBlock), 0, stream, partials, blocks, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_scale(bool fp64, void* psi, long long total, double s, hipStream_t stream) {
+    long long b = (total + kBlock * 4 - 1) / (kBlock * 4);
+    if (b < 1) b = 1;
+    if (b > 8192) b = 8192;
+    if (fp64)
+        hipLaunchKernelGGL(scale_state<double>, dim3(b), dim3(kBlock), 0, stream, psi, total, s);
+    else
+        hipLaunchKernelGGL(scale_state<float>, dim3(b), dim3(kBlock), 0, stream, psi, total, s);
+    return hipGetLastError();
+}
+
+}  // namespace tdse

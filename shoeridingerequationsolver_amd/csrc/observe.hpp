@@ -1,0 +1,33 @@
+// observe.hpp -- host view of the reduction kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "fft_engine.hpp"
+
+namespace tdse {
+
+constexpr int kReduceQ = 6;
+constexpr int kMaxReduceBlocks = 2048;
+
+struct ReduceArgs {
+    const void* psi;   // state (x space) or its forward transform (k space)
+    const void* v;     // potential in path precision, or null
+    long long total;   // points
+    int n[3];          // original dims (x space coordinates)
+    double lo[3], h[3];
+    int slab_axis;
+    double slab_lo, slab_hi;
+    int sq[3];         // squeezed dims (k space)
+    double dk[3];      // 2 pi / L per squeezed axis
+    double mass;
+    long long four_step_rows;  // N2 of a four-step 1D grid, else 0
+};
+
+int reduce_blocks(long long total);
+// partials: reduce_blocks(total) * kReduceQ doubles; out: kReduceQ doubles (device)
+hipError_t launch_reduce(bool fp64, bool kspace, const ReduceArgs& a, double* partials,
+                         double* out, hipStream_t stream);
+hipError_t launch_scale(bool fp64, void* psi, long long total, double s, hipStream_t stream);
+
+}  // namespace tdse

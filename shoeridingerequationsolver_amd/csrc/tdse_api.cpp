@@ -1,0 +1,847 @@
+// tdse_api.cpp -- the C ABI of include/tdse.h: contexts, plans, tables,
+// the sweep schedule of a Strang step, observables and timing.
+//
+// Replaces ses::SplitOperator1D/3D (core/src/propagator.ixx:21-134) for a
+// caller that owns psi (core/src/field.ixx layout) and wants it stepped on the
+// GPU.  Everything that runs per step is a HIP kernel; the host only builds
+// the O(N) tables once (as the reference constructor does) and enqueues.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/tdse.h"
+#include "launch.hpp"
+#include "observe.hpp"
+
+using tdse::SweepArgs;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int status, const std::string& what) {
+    g_last_error = what;
+    return status;
+}
+
+#define TDSE_HIP(call)                                                                     \
+    do {                                                                                   \
+        hipError_t e_ = (call);                                                            \
+        if (e_ != hipSuccess)                                                              \
+            return fail(TDSE_E_DEVICE, std::string(#call ": ") + hipGetErrorString(e_));   \
+    } while (0)
+
+constexpr double kTwoPiHi = 6.28318530717958623200e+00;
+constexpr double kTwoPiLo = 2.44929359829470635445e-16;
+
+// FP64 reduction of an angle into [-pi, pi] (two-term 2 pi, exact n * hi).
+double reduce_2pi_host(double th) {
+    const double n = std::nearbyint(th / (2.0 * 3.14159265358979323846));
+    return (th - n * kTwoPiHi) - n * kTwoPiLo;
+}
+
+struct Axis {
+    int n;
+    double lo, hi;
+    double length() const { return hi - lo; }
+};
+
+enum Kind { kKind3D, kKind2D, kKindSingle, kKindFourStep };
+
+struct Launch {
+    int mode;
+    int n;          // line length
+    SweepArgs args;
+    tdse::LaunchFn fn;
+    int slot;       // profiling slot
+};
+
+}  // namespace
+
+struct tdse_ctx {
+    int device = 0;
+    uint64_t budget = 0;
+    uint64_t used = 0;
+    int max_blocks = 1 << 20;
+    std::mutex mu;
+};
+
+struct tdse_plan {
+    tdse_ctx* ctx = nullptr;
+    int precision = TDSE_FP32;
+    int n[3] = {1, 1, 1};
+    double lo[3] = {0, 0, 0}, hi[3] = {1, 1, 1};
+    double dt = 0, mass = 1;
+    long long total = 0;
+    std::vector<Axis> sq;  // axes with n > 1, x-fastest order
+    Kind kind = kKind3D;
+    int n1 = 0, n2 = 0;    // four-step split: rows of n1 contiguous points, n2 rows
+    hipStream_t stream = nullptr;
+    void* psi = nullptr;
+    void* phi = nullptr;
+    void* v = nullptr;
+    void* mask = nullptr;
+    void* scratch = nullptr;  // forward transform for <T>
+    double* partials = nullptr;
+    double* red_out = nullptr;
+    std::map<int, void*> tw;  // line length -> W_N table
+    bool has_potential = false;
+    uint64_t bytes = 0;
+
+    size_t csize() const { return precision == TDSE_FP64 ? 16 : 8; }
+    size_t rsize() const { return precision == TDSE_FP64 ? 8 : 4; }
+    bool fp64() const { return precision == TDSE_FP64; }
+};
+
+namespace {
+
+tdse::LaunchFn launcher(const tdse_plan* p, int n, int mode) {
+    return p->fp64() ? tdse::find_launcher_f64(n, mode) : tdse::find_launcher_f32(n, mode);
+}
+
+int set_device(const tdse_ctx* ctx) {
+    TDSE_HIP(hipSetDevice(ctx->device));
+    return TDSE_OK;
+}
+
+// Four-step split N = n1 * n2 (n1 contiguous row length, n2 rows), both
+// compiled lengths; prefer the most balanced split, then the longer rows.
+bool four_step_split(long long n, int* n1, int* n2) {
+    int best1 = 0, best2 = 0;
+    for (int a = 2; a <= 1024; ++a) {
+        if (!tdse::line_supported(a) || n % a) continue;
+        const long long b = n / a;
+        if (b > 1024 || !tdse::line_supported(static_cast<int>(b))) continue;
+        const int mn = std::min<long long>(a, b), bmn = std::min(best1, best2);
+        if (mn > bmn || (mn == bmn && a > best1)) {
+            best1 = a;
+            best2 = static_cast<int>(b);
+        }
+    }
+    if (!best1) return false;
+    *n1 = best1;
+    *n2 = best2;
+    return true;
+}
+
+uint64_t plan_bytes_for(long long total, const std::vector<int>& lines, bool fp64, bool mask) {
+    const uint64_t cs = fp64 ? 16 : 8, rs = fp64 ? 8 : 4;
+    uint64_t b = total * cs + 2 * total * rs + (mask ? total * rs : 0);
+    for (int n : lines) b += static_cast<uint64_t>(n) * cs;
+    b += static_cast<uint64_t>(tdse::kMaxReduceBlocks) * tdse::kReduceQ * 8 + 64 * 8;
+    return b;
+}
+
+int classify(const int n[3], std::vector<Axis>& sq, const double lo[3], const double hi[3],
+             Kind* kind, int* n1, int* n2, std::vector<int>* lines) {
+    sq.clear();
+    for (int a = 0; a < 3; ++a)
+        if (n[a] > 1) sq.push_back({n[a], lo ? lo[a] : 0.0, hi ? hi[a] : 1.0});
+    lines->clear();
+    if (sq.empty()) return fail(TDSE_E_UNSUPPORTED, "grid of one point has no FFT axis");
+    if (sq.size() == 1) {
+        const int N = sq[0].n;
+        if (N <= 1024 && tdse::line_supported(N)) {
+            *kind = kKindSingle;
+            lines->push_back(N);
+            return TDSE_OK;
+        }
+        if (!four_step_split(N, n1, n2))
+            return fail(TDSE_E_UNSUPPORTED, "1D length " + std::to_string(N) +
+                                                " is not a product of two compiled line lengths");
+        *kind = kKindFourStep;
+        lines->push_back(*n1);
+        if (*n2 != *n1) lines->push_back(*n2);
+        return TDSE_OK;
+    }
+    for (const Axis& ax : sq) {
+        if (!tdse::line_supported(ax.n))
+            return fail(TDSE_E_UNSUPPORTED, "line length " + std::to_string(ax.n) +
+                                                " has no compiled FFT plan");
+        if (std::find(lines->begin(), lines->end(), ax.n) == lines->end()) lines->push_back(ax.n);
+    }
+    *kind = sq.size() == 2 ? kKind2D : kKind3D;
+    return TDSE_OK;
+}
+
+template <typename T>
+std::vector<T> to_real(const double* src, long long n, bool reduce) {
+    std::vector<T> out(static_cast<size_t>(n));
+    for (long long i = 0; i < n; ++i)
+        out[i] = static_cast<T>(reduce ? reduce_2pi_host(src[i]) : src[i]);
+    return out;
+}
+
+int upload_real(tdse_plan* p, void* dst, const double* src, bool reduce) {
+    if (p->fp64()) {
+        std::vector<double> h = to_real<double>(src, p->total, reduce);
+        TDSE_HIP(hipMemcpy(dst, h.data(), h.size() * 8, hipMemcpyHostToDevice));
+    } else {
+        std::vector<float> h = to_real<float>(src, p->total, reduce);
+        TDSE_HIP(hipMemcpy(dst, h.data(), h.size() * 4, hipMemcpyHostToDevice));
+    }
+    return TDSE_OK;
+}
+
+int alloc(tdse_plan* p, void** out, uint64_t bytes) {
+    {
+        std::lock_guard<std::mutex> lk(p->ctx->mu);
+        if (p->ctx->budget && p->ctx->used + bytes > p->ctx->budget)
+            return fail(TDSE_E_OOM, "HBM budget exceeded");
+        p->ctx->used += bytes;
+    }
+    if (hipMalloc(out, bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        std::lock_guard<std::mutex> lk(p->ctx->mu);
+        p->ctx->used -= bytes;
+        *out = nullptr;
+        return fail(TDSE_E_OOM, "hipMalloc of " + std::to_string(bytes) + " bytes failed");
+    }
+    p->bytes += bytes;
+    return TDSE_OK;
+}
+
+void release(tdse_plan* p, void*& ptr, uint64_t bytes) {
+    if (!ptr) return;
+    (void)hipFree(ptr);
+    ptr = nullptr;
+    std::lock_guard<std::mutex> lk(p->ctx->mu);
+    p->ctx->used -= bytes;
+    p->bytes -= bytes;
+}
+
+// Twiddle table W_N^m = exp(-2 pi i m / N), m < N, FP64 then rounded.
+int make_twiddles(tdse_plan* p, int N) {
+    if (p->tw.count(N)) return TDSE_OK;
+    void* d = nullptr;
+    int rc = alloc(p, &d, static_cast<uint64_t>(N) * p->csize());
+    if (rc) return rc;
+    std::vector<double> w(2 * static_cast<size_t>(N));
+    for (int m = 0; m < N; ++m) {
+        const double th = -2.0 * 3.14159265358979323846 * static_cast<double>(m) / N;
+        w[2 * m] = std::cos(th);
+        w[2 * m + 1] = std::sin(th);
+    }
+    if (p->fp64()) {
+        TDSE_HIP(hipMemcpy(d, w.data(), w.size() * 8, hipMemcpyHostToDevice));
+    } else {
+        std::vector<float> f(w.begin(), w.end());
+        TDSE_HIP(hipMemcpy(d, f.data(), f.size() * 4, hipMemcpyHostToDevice));
+    }
+    p->tw[N] = d;
+    return TDSE_OK;
+}
+
+SweepArgs base_args(const tdse_plan* p) {
+    SweepArgs a{};
+    a.psi = p->psi;
+    a.phi = p->phi;
+    a.mask = p->mask;
+    a.dt = p->dt;
+    a.mass = p->mass;
+    a.scale = 1.0 / static_cast<double>(p->total);
+    for (int i = 0; i < 3; ++i) {
+        a.dk[i] = 0.0;
+        a.kn[i] = 1;
+    }
+    return a;
+}
+
+double dk_of(const Axis& ax) { return 2.0 * 3.14159265358979323846 / ax.length(); }
+
+// Sweep descriptors of the plan's geometry.  `target` is the array swept.
+struct Geometry {
+    SweepArgs x, yf, yi, z;     // 3D / 2D (y carries K in 2D)
+    SweepArgs col, row;         // four-step 1D
+    SweepArgs single;           // 1D in one block
+};
+
+Geometry geometry(const tdse_plan* p, void* target) {
+    Geometry g{};
+    SweepArgs a = base_args(p);
+    a.psi = target;
+    if (p->kind == kKind3D) {
+        const int n0 = p->sq[0].n, n1 = p->sq[1].n, n2 = p->sq[2].n;
+        g.x = a;
+        g.x.nlines = static_cast<long long>(n1) * n2;
+        g.x.tw = p->tw.at(n0);
+        g.yf = a;
+        g.yf.nx = n0;
+        g.yf.nw = n2;
+        g.yf.wstride = static_cast<long long>(n0) * n1;
+        g.yf.estride = n0;
+        g.yf.tw = p->tw.at(n1);
+        g.yi = g.yf;
+        g.z = a;
+        g.z.nx = n0;
+        g.z.nw = n1;
+        g.z.wstride = n0;
+        g.z.estride = static_cast<long long>(n0) * n1;
+        g.z.tw = p->tw.at(n2);
+        g.z.dk[0] = dk_of(p->sq[0]);
+        g.z.kn[0] = n0;
+        g.z.dk[1] = dk_of(p->sq[1]);
+        g.z.kn[1] = n1;
+        g.z.dk[2] = dk_of(p->sq[2]);
+        g.z.kn[2] = n2;
+    } else if (p->kind == kKind2D) {
+        const int n0 = p->sq[0].n, n1 = p->sq[1].n;
+        g.x = a;
+        g.x.nlines = n1;
+        g.x.tw = p->tw.at(n0);
+        g.yf = a;
+        g.yf.nx = n0;
+        g.yf.nw = 1;
+        g.yf.wstride = 0;
+        g.yf.estride = n0;
+        g.yf.tw = p->tw.at(n1);
+        g.yf.dk[0] = dk_of(p->sq[0]);
+        g.yf.kn[0] = n0;
+        g.yf.dk[2] = dk_of(p->sq[1]);
+        g.yf.kn[2] = n1;
+        g.yi = g.yf;
+    } else if (p->kind == kKindSingle) {
+        g.single = a;
+        g.single.nlines = 1;
+        g.single.tw = p->tw.at(p->sq[0].n);
+        g.single.dk[2] = dk_of(p->sq[0]);
+        g.single.kn[2] = p->sq[0].n;
+    } else {
+        const int N = p->sq[0].n;
+        g.col = a;
+        g.col.nx = p->n1;
+        g.col.nw = 1;
+        g.col.wstride = 0;
+        g.col.estride = p->n1;
+        g.col.tw = p->tw.at(p->n2);
+        g.col.n_total = N;
+        g.row = a;
+        g.row.nlines = p->n2;
+        g.row.tw = p->tw.at(p->n1);
+        g.row.dk[2] = dk_of(p->sq[0]);
+        g.row.kn[2] = N;
+        g.row.n_total = N;
+        g.row.n_rows = p->n2;
+    }
+    return g;
+}
+
+// The launch list of `nsteps` steps (nsteps >= 1).
+std::vector<Launch> schedule(const tdse_plan* p, int nsteps) {
+    std::vector<Launch> L;
+    Geometry g = geometry(p, p->psi);
+    auto add = [&](int mode, int n, const SweepArgs& a, int slot) {
+        L.push_back({mode, n, a, launcher(p, n, mode), slot});
+    };
+    if (p->kind == kKind3D) {
+        const int n0 = p->sq[0].n, n1 = p->sq[1].n, n2 = p->sq[2].n;
+        add(tdse::C_KICK_FIRST, n0, g.x, 0);
+        for (int s = 0; s < nsteps; ++s) {
+            add(tdse::S_FWD, n1, g.yf, 1);
+            add(tdse::S_KIN, n2, g.z, 2);
+            add(tdse::S_INV, n1, g.yi, 3);
+            if (s + 1 < nsteps) add(tdse::C_KICK_MID, n0, g.x, 4);
+            else add(tdse::C_KICK_LAST, n0, g.x, 5);
+        }
+    } else if (p->kind == kKind2D) {
+        const int n0 = p->sq[0].n, n1 = p->sq[1].n;
+        add(tdse::C_KICK_FIRST, n0, g.x, 0);
+        for (int s = 0; s < nsteps; ++s) {
+            add(tdse::S_KIN, n1, g.yf, 1);
+            if (s + 1 < nsteps) add(tdse::C_KICK_MID, n0, g.x, 2);
+            else add(tdse::C_KICK_LAST, n0, g.x, 3);
+        }
+    } else if (p->kind == kKindSingle) {
+        SweepArgs a = g.single;
+        a.nsteps = nsteps;
+        add(tdse::C_SINGLE, p->sq[0].n, a, 0);
+    } else {
+        add(tdse::S_KICK_FIRST_TW, p->n2, g.col, 0);
+        for (int s = 0; s < nsteps; ++s) {
+            add(tdse::C_KIN_TW, p->n1, g.row, 1);
+            if (s + 1 < nsteps) add(tdse::S_KICK_MID_TW, p->n2, g.col, 2);
+            else add(tdse::S_KICK_LAST, p->n2, g.col, 3);
+        }
+    }
+    return L;
+}
+
+const char* const kSlots3D[] = {"x_first", "y_fwd", "z_kin", "y_inv", "x_mid", "x_last"};
+const char* const kSlots2D[] = {"x_first", "y_kin", "x_mid", "x_last"};
+const char* const kSlotsSingle[] = {"line_all_steps"};
+const char* const kSlotsFour[] = {"col_first", "row_kin", "col_mid", "col_last"};
+
+int slot_count(const tdse_plan* p) {
+    switch (p->kind) {
+        case kKind3D: return 6;
+        case kKind2D: return 4;
+        case kKindSingle: return 1;
+        default: return 4;
+    }
+}
+
+int run_launches(tdse_plan* p, const std::vector<Launch>& L) {
+    for (const Launch& l : L) {
+        if (!l.fn) return fail(TDSE_E_UNSUPPORTED, "no kernel for line length " + std::to_string(l.n));
+        TDSE_HIP(l.fn(l.args, p->ctx->max_blocks, p->stream));
+    }
+    return TDSE_OK;
+}
+
+int check_plan(const tdse_plan* p) {
+    if (!p) return fail(TDSE_E_INVALID, "null plan");
+    return TDSE_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* tdse_version(void) { return "tdse-mi355x 0.1 (gfx950)"; }
+
+const char* tdse_status_string(int status) {
+    switch (status) {
+        case TDSE_OK: return "ok";
+        case TDSE_E_INVALID: return "invalid argument";
+        case TDSE_E_UNSUPPORTED: return "unsupported shape";
+        case TDSE_E_OOM: return "out of device memory or budget";
+        case TDSE_E_DEVICE: return "device error";
+        case TDSE_E_TIMEOUT: return "timeout";
+        case TDSE_E_NO_DEVICE: return "no such device";
+        case TDSE_E_STATE: return "call out of order";
+        default: return "unknown status";
+    }
+}
+
+const char* tdse_last_error(void) { return g_last_error.c_str(); }
+
+int tdse_ctx_create(int device_ordinal, uint64_t hbm_budget_bytes, tdse_ctx** out) {
+    if (!out) return fail(TDSE_E_INVALID, "null out");
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || device_ordinal < 0 || device_ordinal >= count) {
+        (void)hipGetLastError();
+        return fail(TDSE_E_NO_DEVICE, "no HIP device " + std::to_string(device_ordinal));
+    }
+    tdse_ctx* c = new tdse_ctx();
+    c->device = device_ordinal;
+    c->budget = hbm_budget_bytes;
+    *out = c;
+    return TDSE_OK;
+}
+
+int tdse_ctx_destroy(tdse_ctx* ctx) {
+    if (!ctx) return fail(TDSE_E_INVALID, "null ctx");
+    delete ctx;
+    return TDSE_OK;
+}
+
+int tdse_ctx_bytes_in_use(const tdse_ctx* ctx, uint64_t* out) {
+    if (!ctx || !out) return fail(TDSE_E_INVALID, "null argument");
+    std::lock_guard<std::mutex> lk(const_cast<tdse_ctx*>(ctx)->mu);
+    *out = ctx->used;
+    return TDSE_OK;
+}
+
+int tdse_plan_bytes(const int n[3], int precision, int with_mask, uint64_t* out) {
+    if (!n || !out) return fail(TDSE_E_INVALID, "null argument");
+    if (precision != TDSE_FP32 && precision != TDSE_FP64) return fail(TDSE_E_INVALID, "precision");
+    std::vector<Axis> sq;
+    Kind kind;
+    int n1 = 0, n2 = 0;
+    std::vector<int> lines;
+    for (int a = 0; a < 3; ++a)
+        if (n[a] < 1) return fail(TDSE_E_INVALID, "n < 1");
+    int rc = classify(n, sq, nullptr, nullptr, &kind, &n1, &n2, &lines);
+    if (rc) return rc;
+    const long long total = static_cast<long long>(n[0]) * n[1] * n[2];
+    *out = plan_bytes_for(total, lines, precision == TDSE_FP64, with_mask != 0);
+    return TDSE_OK;
+}
+
+int tdse_plan_create(tdse_ctx* ctx, const int n[3], const double lo[3], const double hi[3],
+                     double dt, double mass, int precision, unsigned flags, tdse_plan** out) {
+    (void)flags;
+    if (!ctx || !n || !lo || !hi || !out) return fail(TDSE_E_INVALID, "null argument");
+    *out = nullptr;
+    if (precision != TDSE_FP32 && precision != TDSE_FP64)
+        return fail(TDSE_E_INVALID, "precision must be TDSE_FP32 or TDSE_FP64");
+    if (!std::isfinite(dt) || !std::isfinite(mass) || mass == 0.0)
+        return fail(TDSE_E_INVALID, "dt and mass must be finite, mass non-zero");
+    for (int a = 0; a < 3; ++a) {
+        if (n[a] < 1) return fail(TDSE_E_INVALID, "grid extent < 1");
+        if (!(hi[a] > lo[a]) && n[a] > 1) return fail(TDSE_E_INVALID, "hi must exceed lo");
+    }
+    const long long total = static_cast<long long>(n[0]) * n[1] * n[2];
+    if (total > (1LL << 31) - 1) return fail(TDSE_E_UNSUPPORTED, "more than 2^31-1 points");
+    int rc = set_device(ctx);
+    if (rc) return rc;
+
+    tdse_plan* p = new tdse_plan();
+    p->ctx = ctx;
+    p->precision = precision;
+    for (int a = 0; a < 3; ++a) {
+        p->n[a] = n[a];
+        p->lo[a] = lo[a];
+        p->hi[a] = hi[a];
+    }
+    p->dt = dt;
+    p->mass = mass;
+    p->total = total;
+    std::vector<int> lines;
+    rc = classify(n, p->sq, lo, hi, &p->kind, &p->n1, &p->n2, &lines);
+    if (rc) {
+        delete p;
+        return rc;
+    }
+    auto bail = [&](int code) {
+        tdse_plan_destroy(p);
+        return code;
+    };
+    if (hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess) {
+        (void)hipGetLastError();
+        delete p;
+        return fail(TDSE_E_DEVICE, "hipStreamCreate failed");
+    }
+    if ((rc = alloc(p, &p->psi, total * p->csize()))) return bail(rc);
+    if ((rc = alloc(p, &p->phi, total * p->rsize()))) return bail(rc);
+    if ((rc = alloc(p, &p->v, total * p->rsize()))) return bail(rc);
+    void* part = nullptr;
+    if ((rc = alloc(p, &part, static_cast<uint64_t>(tdse::kMaxReduceBlocks) * tdse::kReduceQ * 8 +
+                                  64 * 8)))
+        return bail(rc);
+    p->partials = static_cast<double*>(part);
+    p->red_out = p->partials + tdse::kMaxReduceBlocks * tdse::kReduceQ;
+    for (int N : lines)
+        if ((rc = make_twiddles(p, N))) return bail(rc);
+    if (hipMemset(p->psi, 0, total * p->csize()) != hipSuccess ||
+        hipMemset(p->phi, 0, total * p->rsize()) != hipSuccess ||
+        hipMemset(p->v, 0, total * p->rsize()) != hipSuccess) {
+        (void)hipGetLastError();
+        return bail(fail(TDSE_E_DEVICE, "hipMemset failed"));
+    }
+    *out = p;
+    return TDSE_OK;
+}
+
+int tdse_plan_destroy(tdse_plan* p) {
+    if (!p) return fail(TDSE_E_INVALID, "null plan");
+    (void)hipSetDevice(p->ctx->device);
+    if (p->stream) (void)hipStreamSynchronize(p->stream);
+    const uint64_t cs = p->csize(), rs = p->rsize();
+    release(p, p->psi, p->total * cs);
+    release(p, p->phi, p->total * rs);
+    release(p, p->v, p->total * rs);
+    release(p, p->mask, p->total * rs);
+    release(p, p->scratch, p->total * cs);
+    void* part = p->partials;
+    release(p, part, static_cast<uint64_t>(tdse::kMaxReduceBlocks) * tdse::kReduceQ * 8 + 64 * 8);
+    for (auto& kv : p->tw) release(p, kv.second, static_cast<uint64_t>(kv.first) * cs);
+    if (p->stream) (void)hipStreamDestroy(p->stream);
+    delete p;
+    return TDSE_OK;
+}
+
+int tdse_plan_set_potential(tdse_plan* p, const double* v) {
+    int rc = check_plan(p);
+    if (rc) return rc;
+    if (!v) return fail(TDSE_E_INVALID, "null potential");
+    if ((rc = set_device(p->ctx))) return rc;
+    TDSE_HIP(hipStreamSynchronize(p->stream));
+    std::vector<double> th(static_cast<size_t>(p->total));
+    for (long long i = 0; i < p->total; ++i) {
+        if (!std::isfinite(v[i])) return fail(TDSE_E_INVALID, "potential not finite");
+        th[i] = -0.5 * v[i] * p->dt;  // core/src/propagator.ixx:29, 79
+    }
+    if ((rc = upload_real(p, p->phi, th.data(), !p->fp64()))) return rc;
+    if ((rc = upload_real(p, p->v, v, false))) return rc;
+    p->has_potential = true;
+    return TDSE_OK;
+}
+
+int tdse_plan_set_mask(tdse_plan* p, const double* mask) {
+    int rc = check_plan(p);
+    if (rc) return rc;
+    if ((rc = set_device(p->ctx))) return rc;
+    TDSE_HIP(hipStreamSynchronize(p->stream));
+    if (!mask) {
+        release(p, p->mask, p->total * p->rsize());
+        return TDSE_OK;
+    }
+    if (!p->mask && (rc = alloc(p, &p->mask, p->total * p->rsize()))) return rc;
+    return upload_real(p, p->mask, mask, false);
+}
+
+int tdse_state_upload(tdse_plan* p, const double* psi) {
+    int rc = check_plan(p);
+    if (rc) return rc;
+    if (!psi) return fail(TDSE_E_INVALID, "null state");
+    if ((rc = set_device(p->ctx))) return rc;
+    TDSE_HIP(hipStreamSynchronize(p->stream));
+    if (p->fp64()) {
+        TDSE_HIP(hipMemcpy(p->psi, psi, p->total * 16, hipMemcpyHostToDevice));
+    } else {
+        std::vector<float> f(2 * static_cast<size_t>(p->total));
+        for (long long i = 0; i < 2 * p->total; ++i) f[i] = static_cast<float>(psi[i]);
+        TDSE_HIP(hipMemcpy(p->psi, f.data(), f.size() * 4, hipMemcpyHostToDevice));
+    }
+    return TDSE_OK;
+}
+
+int tdse_state_download(tdse_plan* p, double* psi) {
+    int rc = check_plan(p);
+    if (rc) return rc;
+    if (!psi) return fail(TDSE_E_INVALID, "null state");
+    if ((rc = set_device(p->ctx))) return rc;
+    TDSE_HIP(hipStreamSynchronize(p->stream));
+    if (p->fp64()) {
+        TDSE_HIP(hipMemcpy(psi, p->psi, p->total * 16, hipMemcpyDeviceToHost));
+    } else {
+        std::vector<float> f(2 * static_cast<size_t>(p->total));
+        TDSE_HIP(hipMemcpy(f.data(), p->psi, f.size() * 4, hipMemcpyDeviceToHost));
+        for (long long i = 0; i < 2 * p->total; ++i) psi[i] = f[i];
+    }
+    return TDSE_OK;
+}
+
+int tdse_state_copy(tdse_plan* dst, const tdse_plan* src) {
+    if (!dst || !src) return fail(TDSE_E_INVALID, "null plan");
+    if (dst->total != src->total || dst->precision != src->precision)
+        return fail(TDSE_E_INVALID, "shape or precision mismatch");
+    int rc = set_device(dst->ctx);
+    if (rc) return rc;
+    TDSE_HIP(hipStreamSynchronize(src->stream));
+    TDSE_HIP(hipMemcpyAsync(dst->psi, src->psi, dst->total * dst->csize(), hipMemcpyDeviceToDevice,
+                            dst->stream));
+    return TDSE_OK;
+}
+
+int tdse_state_device_ptr(tdse_plan* p, void** out) {
+    int rc = check_plan(p);
+    if (rc) return rc;
+    if (!out) return fail(TDSE_E_INVALID, "null out");
+    *out = p->psi;
+    return TDSE_OK;
+}
+
+int tdse_step(tdse_plan* p, int nsteps) {
+    int rc = check_plan(p);
+    if (rc) return rc;
+    if (nsteps < 0) return fail(TDSE_E_INVALID, "nsteps < 0");
+    if (nsteps == 0) return TDSE_OK;
+    if (!p->has_potential) return fail(TDSE_E_STATE, "tdse_plan_set_potential not called");
+    if ((rc = set_device(p->ctx))) return rc;
+    return run_launches(p, schedule(p, nsteps));
+}
+
+int tdse_plan_sync(tdse_plan* p, int timeout_ms) {
+    int rc = check_plan(p);
+    if (rc) return rc;
+    if ((rc = set_device(p->ctx))) return rc;
+    if (timeout_ms < 0) {
+        TDSE_HIP(hipStreamSynchronize(p->stream));
+        return TDSE_OK;
+    }
+    const auto t0 = std::chrono::steady_clock::now();
+    for (;;) {
+        hipError_t e = hipStreamQuery(p->stream);
+        if (e == hipSuccess) return TDSE_OK;
+        if (e != hipErrorNotReady) return fail(TDSE_E_DEVICE, hipGetErrorString(e));
+        const auto ms = std::chrono::duration_cast<std::chrono::milliseconds>(
+                            std::chrono::steady_clock::now() - t0)
+                            .count();
+        if (ms >= timeout_ms) return fail(TDSE_E_TIMEOUT, "stream wait timed out");
+        std::this_thread::yield();
+    }
+}
+
+int tdse_plan_stream(tdse_plan* p, void** out) {
+    int rc = check_plan(p);
+    if (rc) return rc;
+    if (!out) return fail(TDSE_E_INVALID, "null out");
+    *out = p->stream;
+    return TDSE_OK;
+}
+
+int tdse_observe(tdse_plan* p, unsigned what, tdse_observables* out) {
+    int rc = check_plan(p);
+    if (rc) return rc;
+    if (!out) return fail(TDSE_E_INVALID, "null out");
+    if ((rc = set_device(p->ctx))) return rc;
+    std::memset(out, 0, sizeof(*out));
+    tdse::ReduceArgs r{};
+    r.psi = p->psi;
+    r.v = p->has_potential ? p->v : nullptr;
+    r.total = p->total;
+    for (int a = 0; a < 3; ++a) {
+        r.n[a] = p->n[a];
+        r.lo[a] = p->lo[a];
+        r.h[a] = (p->hi[a] - p->lo[a]) / p->n[a];
+        r.sq[a] = a < static_cast<int>(p->sq.size()) ? p->sq[a].n : 1;
+        r.dk[a] = a < static_cast<int>(p->sq.size()) ? dk_of(p->sq[a]) : 0.0;
+    }
+    r.slab_axis = 0;
+    r.slab_lo = 1.0;
+    r.slab_hi = 0.0;  // empty slab
+    r.mass = p->mass;
+    double host[tdse::kReduceQ];
+    TDSE_HIP(tdse::launch_reduce(p->fp64(), false, r, p->partials, p->red_out, p->stream));
+    TDSE_HIP(hipMemcpyAsync(host, p->red_out, sizeof(host), hipMemcpyDeviceToHost, p->stream));
+    TDSE_HIP(hipStreamSynchronize(p->stream));
+    const double cell = r.h[0] * r.h[1] * r.h[2];
+    auto ratio = [](double num, double den) { return den > 0.0 ? num / den : 0.0; };
+    out->sum_abs2 = host[0];
+    out->norm = host[0] * cell;
+    out->position[0] = ratio(host[1], host[0]);
+    out->position[1] = ratio(host[2], host[0]);
+    out->position[2] = ratio(host[3], host[0]);
+    out->potential = ratio(host[4], host[0]);
+    if (what & TDSE_OBS_ENERGY) {
+        if (!p->scratch && (rc = alloc(p, &p->scratch, p->total * p->csize()))) return rc;
+        TDSE_HIP(hipMemcpyAsync(p->scratch, p->psi, p->total * p->csize(), hipMemcpyDeviceToDevice,
+                                p->stream));
+        Geometry g = geometry(p, p->scratch);
+        std::vector<Launch> L;
+        auto add = [&](int mode, int n, const SweepArgs& a) {
+            L.push_back({mode, n, a, launcher(p, n, mode), 0});
+        };
+        if (p->kind == kKind3D) {
+            add(tdse::C_FWD, p->sq[0].n, g.x);
+            add(tdse::S_FWD, p->sq[1].n, g.yf);
+            add(tdse::S_FWD, p->sq[2].n, g.z);
+        } else if (p->kind == kKind2D) {
+            add(tdse::C_FWD, p->sq[0].n, g.x);
+            add(tdse::S_FWD, p->sq[1].n, g.yf);
+        } else if (p->kind == kKindSingle) {
+            add(tdse::C_FWD, p->sq[0].n, g.single);
+        } else {
+            add(tdse::S_FWD_TW, p->n2, g.col);
+            add(tdse::C_FWD, p->n1, g.row);
+        }
+        if ((rc = run_launches(p, L))) return rc;
+        tdse::ReduceArgs rk = r;
+        rk.psi = p->scratch;
+        rk.v = nullptr;
+        if (p->kind == kKindFourStep) {
+            rk.sq[0] = p->n1;
+            rk.sq[1] = p->n2;
+            rk.four_step_rows = p->n2;
+        }
+        double hk[tdse::kReduceQ];
+        TDSE_HIP(tdse::launch_reduce(p->fp64(), true, rk, p->partials, p->red_out, p->stream));
+        TDSE_HIP(hipMemcpyAsync(hk, p->red_out, sizeof(hk), hipMemcpyDeviceToHost, p->stream));
+        TDSE_HIP(hipStreamSynchronize(p->stream));
+        out->kinetic = ratio(hk[1], hk[0]);
+        out->energy = out->potential + out->kinetic;
+    }
+    return TDSE_OK;
+}
+
+int tdse_probability_in_slab(tdse_plan* p, int axis, double lo, double hi, double* out) {
+    int rc = check_plan(p);
+    if (rc) return rc;
+    if (!out || axis < 0 || axis > 2) return fail(TDSE_E_INVALID, "bad axis or null out");
+    if ((rc = set_device(p->ctx))) return rc;
+    tdse::ReduceArgs r{};
+    r.psi = p->psi;
+    r.v = nullptr;
+    r.total = p->total;
+    for (int a = 0; a < 3; ++a) {
+        r.n[a] = p->n[a];
+        r.lo[a] = p->lo[a];
+        r.h[a] = (p->hi[a] - p->lo[a]) / p->n[a];
+    }
+    r.slab_axis = axis;
+    r.slab_lo = lo;
+    r.slab_hi = hi;
+    r.mass = p->mass;
+    double host[tdse::kReduceQ];
+    TDSE_HIP(tdse::launch_reduce(p->fp64(), false, r, p->partials, p->red_out, p->stream));
+    TDSE_HIP(hipMemcpyAsync(host, p->red_out, sizeof(host), hipMemcpyDeviceToHost, p->stream));
+    TDSE_HIP(hipStreamSynchronize(p->stream));
+    *out = host[5] * (r.h[0] * r.h[1] * r.h[2]);
+    return TDSE_OK;
+}
+
+int tdse_state_scale(tdse_plan* p, double s) {
+    int rc = check_plan(p);
+    if (rc) return rc;
+    if ((rc = set_device(p->ctx))) return rc;
+    TDSE_HIP(tdse::launch_scale(p->fp64(), p->psi, p->total, s, p->stream));
+    return TDSE_OK;
+}
+
+int tdse_profile_step(tdse_plan* p, int nsteps, int max_slots, double* ms_per_sweep, int* launches,
+                      int* n_slots) {
+    int rc = check_plan(p);
+    if (rc) return rc;
+    if (nsteps < 1 || !ms_per_sweep || !launches || !n_slots)
+        return fail(TDSE_E_INVALID, "bad profiling arguments");
+    if (!p->has_potential) return fail(TDSE_E_STATE, "tdse_plan_set_potential not called");
+    if ((rc = set_device(p->ctx))) return rc;
+    const std::vector<Launch> L = schedule(p, nsteps);
+    std::vector<hipEvent_t> ev(2 * L.size());
+    for (auto& e : ev) TDSE_HIP(hipEventCreate(&e));
+    int status = TDSE_OK;
+    for (size_t i = 0; i < L.size() && status == TDSE_OK; ++i) {
+        if (!L[i].fn) {
+            status = fail(TDSE_E_UNSUPPORTED, "no kernel");
+            break;
+        }
+        if (hipEventRecord(ev[2 * i], p->stream) != hipSuccess ||
+            L[i].fn(L[i].args, p->ctx->max_blocks, p->stream) != hipSuccess ||
+            hipEventRecord(ev[2 * i + 1], p->stream) != hipSuccess)
+            status = fail(TDSE_E_DEVICE, "launch failed while profiling");
+    }
+    if (status == TDSE_OK && hipStreamSynchronize(p->stream) != hipSuccess)
+        status = fail(TDSE_E_DEVICE, "sync failed while profiling");
+    const int slots = slot_count(p);
+    *n_slots = slots;
+    for (int s = 0; s < std::min(slots, max_slots); ++s) {
+        ms_per_sweep[s] = 0.0;
+        launches[s] = 0;
+    }
+    if (status == TDSE_OK) {
+        for (size_t i = 0; i < L.size(); ++i) {
+            float ms = 0.f;
+            if (hipEventElapsedTime(&ms, ev[2 * i], ev[2 * i + 1]) != hipSuccess) continue;
+            if (L[i].slot < max_slots) {
+                ms_per_sweep[L[i].slot] += ms;
+                launches[L[i].slot] += 1;
+            }
+        }
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    return status;
+}
+
+const char* tdse_plan_sweep_name(const tdse_plan* p, int slot) {
+    if (!p || slot < 0 || slot >= slot_count(p)) return "";
+    switch (p->kind) {
+        case kKind3D: return kSlots3D[slot];
+        case kKind2D: return kSlots2D[slot];
+        case kKindSingle: return kSlotsSingle[slot];
+        default: return kSlotsFour[slot];
+    }
+}
+
+int tdse_plan_step_bytes(const tdse_plan* p, uint64_t* out) {
+    if (!p || !out) return fail(TDSE_E_INVALID, "null argument");
+    int sweeps = 1;
+    if (p->kind == kKind3D) sweeps = 4;
+    else if (p->kind == kKind2D || p->kind == kKindFourStep) sweeps = 2;
+    const uint64_t N = static_cast<uint64_t>(p->total);
+    *out = N * (sweeps * 2 * p->csize() + p->rsize() + (p->mask ? p->rsize() : 0));
+    return TDSE_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,183 @@
+"""GPU parity: the HIP propagator (through the C ABI) against the oracle.
+
+The oracle is the C restatement of the reference core (oracle/tdse_oracle.c),
+itself pinned bit for bit to the reference and to the closed forms of the
+reference tests (tests/test_oracle.py).  Both sides take the same FP64 inputs.
+
+Tolerances (max |psi_gpu - psi_oracle| / max |psi_oracle| after the run):
+  FP64 state:  1e-11  (the reference parity target for the FP64 path)
+  FP32 state:  2e-5 at these short runs (FP32 round-off of the FFT passes).
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-11
+FP32_TOL = 2e-5
+
+
+def grid3(t, n, lo, hi):
+    return t.Grid3D(t.Grid1D(lo[0], hi[0], n[0]), t.Grid1D(lo[1], hi[1], n[1]),
+                    t.Grid1D(lo[2], hi[2], n[2]))
+
+
+def rel(a, b):
+    return float(np.abs(a - b).max() / np.abs(b).max())
+
+
+def packet3(orc, dims, lo, hi, r0, sigma, k0):
+    return orc.gaussian_3d(dims, lo, hi, r0, sigma, k0).reshape(dims[::-1])
+
+
+CASES_3D = [
+    # dims, lo, hi, potential kind, dt, steps, mass
+    ((16, 16, 16), (-8, -8, -8), (8, 8, 8), "harmonic", 0.05, 20, 1.0),
+    ((8, 16, 4), (-4, -6, -3), (4, 6, 3), "coulomb", 0.01, 15, 1.0),
+    ((32, 16, 8), (-10, -5, -4), (10, 5, 4), "barrier", 0.02, 12, 0.38),
+    ((64, 32, 1), (-16, -8, 0), (16, 8, 1), "harmonic", 0.05, 25, 1.0),
+    ((128, 128, 128), (-20, -20, -20), (20, 20, 20), "coulomb", 0.005, 5, 1.0),
+    ((256, 16, 8), (-40, -4, -2), (40, 4, 2), "barrier", 0.01, 6, 1.0),
+    ((512, 32, 1), (-40, -8, 0), (40, 8, 1), "harmonic", 0.02, 6, 1.0),
+    ((32, 1024, 1), (-8, -60, 0), (8, 60, 1), "harmonic", 0.02, 4, 1.0),
+]
+
+
+def potential(orc, kind, dims, lo, hi):
+    if kind == "harmonic":
+        return orc.harmonic_3d(dims, lo, hi, 0.3, (0.2, -0.1, 0.0))
+    if kind == "coulomb":
+        return orc.regularized_coulomb_3d(dims, lo, hi, 1.0, (0.0, 0.0, 0.0))
+    return orc.barrier_3d(dims, lo, hi, 1.2, 0.0, 1.0)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("case", CASES_3D, ids=lambda c: "x".join(map(str, c[0])) + "-" + c[3])
+def test_step_3d_matches_oracle(orc, tdse, case, prec):
+    dims, lo, hi, kind, dt, steps, mass = case
+    print(f"case dims={dims} kind={kind} prec={prec}", flush=True)
+    v = potential(orc, kind, dims, lo, hi)
+    psi0 = packet3(orc, dims, lo, hi, (0.5, -0.3, 0.2), (1.3, 1.1, 0.9), (0.7, -0.4, 0.3))
+    want = orc.so3d_evolve(v, dims, lo, hi, dt, psi0, steps, mass=mass)
+    g = grid3(tdse, dims, lo, hi)
+    op = tdse.SplitOperator3D(g, v, dt, mass=mass, precision=prec)
+    got = psi0.copy()
+    op.step(got, steps)
+    err = rel(got, want)
+    print(f"  rel err {err:.3e}", flush=True)
+    assert err < (FP64_TOL if prec == "fp64" else FP32_TOL)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_step_3d_with_mask_matches_oracle(orc, tdse, prec):
+    dims, lo, hi = (32, 16, 16), (-12, -6, -6), (12, 6, 6)
+    v = orc.barrier_3d(dims, lo, hi, 1.5, 0.0, 2.0)
+    m = orc.absorbing_mask_3d(dims, lo, hi, 3.0)
+    psi0 = packet3(orc, dims, lo, hi, (-6.0, 0.0, 0.0), (1.5, 1.5, 1.5), (2.5, 0.0, 0.0))
+    want = orc.so3d_evolve(v, dims, lo, hi, 0.02, psi0, 30, mask=m)
+    p = tdse.Propagator(grid3(tdse, dims, lo, hi), v, 0.02, precision=prec, mask=m)
+    p.upload(psi0)
+    p.step(30)
+    err = rel(p.download(), want)
+    print(f"mask {prec} rel err {err:.3e}", flush=True)
+    assert err < (FP64_TOL if prec == "fp64" else FP32_TOL)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("n,steps", [(1024, 200), (256, 50), (65536, 6), (96 * 512, 4)])
+def test_step_1d_matches_oracle(orc, tdse, n, steps, prec):
+    lo, hi = (-40.0, 40.0) if n <= 1024 else (-100.0, 100.0)
+    print(f"case 1d n={n} prec={prec}", flush=True)
+    v = orc.harmonic_1d(lo, hi, n, 0.1)
+    psi0 = orc.gaussian_1d(lo, hi, n, -15.0, 1.0, 2.0)
+    m = orc.absorbing_mask_1d(lo, hi, n, 8.0)
+    if n & (n - 1) == 0:
+        want = orc.so1d_evolve(v, lo, hi, 0.05, psi0, steps, mask=m)
+    else:
+        want = orc.so3d_evolve(v, (n, 1, 1), (lo, 0, 0), (hi, 1, 1), 0.05,
+                               psi0.reshape(1, 1, n), steps, mask=m, any_size=True).ravel()
+    p = tdse.Propagator(tdse.Grid1D(lo, hi, n), v, 0.05, precision=prec, mask=m)
+    p.upload(psi0)
+    p.step(steps)
+    err = rel(p.download(), want)
+    print(f"  rel err {err:.3e}", flush=True)
+    assert err < (FP64_TOL if prec == "fp64" else FP32_TOL)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("dims", [(96, 8, 4), (8, 160, 4), (4, 8, 192), (96, 96, 1), (160, 32, 16)])
+def test_step_non_pow2_matches_oracle(orc, tdse, dims, prec):
+    lo = tuple(-0.25 * d for d in dims)
+    hi = tuple(0.25 * d for d in dims)
+    v = orc.harmonic_3d(dims, lo, hi, 0.4, (0.0, 0.0, 0.0))
+    psi0 = packet3(orc, dims, lo, hi, (0.3, 0.1, -0.2), (1.0, 1.0, 1.0), (0.5, 0.2, -0.1))
+    want = orc.so3d_evolve(v, dims, lo, hi, 0.02, psi0, 10, any_size=True)
+    op = tdse.SplitOperator3D(grid3(tdse, dims, lo, hi), v, 0.02, precision=prec)
+    got = psi0.copy()
+    op.step(got, 10)
+    err = rel(got, want)
+    print(f"non-pow2 {dims} {prec} rel err {err:.3e}", flush=True)
+    assert err < (FP64_TOL if prec == "fp64" else FP32_TOL)
+
+
+def test_zero_steps_is_identity(orc, tdse):
+    dims, lo, hi = (16, 8, 8), (-4, -2, -2), (4, 2, 2)
+    psi0 = packet3(orc, dims, lo, hi, (0, 0, 0), (1, 1, 1), (0, 0, 0))
+    op = tdse.SplitOperator3D(grid3(tdse, dims, lo, hi), np.zeros(16 * 8 * 8), 0.1)
+    got = psi0.copy()
+    op.step(got, 0)
+    assert np.array_equal(got, psi0)
+
+
+def test_split_calls_match_one_call(orc, tdse):
+    """advance(a); advance(b) == advance(a + b) (simulation_test.cpp pattern)."""
+    dims, lo, hi = (32, 32, 16), (-8, -8, -4), (8, 8, 4)
+    v = orc.harmonic_3d(dims, lo, hi, 0.5, (0, 0, 0))
+    psi0 = packet3(orc, dims, lo, hi, (1, 0, 0), (1, 1, 1), (0, 0.5, 0))
+    a = tdse.Propagator(grid3(tdse, dims, lo, hi), v, 0.03)
+    b = tdse.Propagator(grid3(tdse, dims, lo, hi), v, 0.03)
+    a.upload(psi0)
+    b.upload(psi0)
+    a.step(7)
+    a.step(5)
+    b.step(12)
+    assert rel(a.download(), b.download()) < 1e-13
+
+
+def test_observables_match_oracle(orc, tdse):
+    dims, lo, hi = (32, 16, 16), (-8, -4, -4), (8, 4, 4)
+    v = orc.harmonic_3d(dims, lo, hi, 0.7, (0.5, 0, 0))
+    psi0 = packet3(orc, dims, lo, hi, (1.0, 0.5, -0.3), (1.2, 0.8, 0.9), (0.6, 0.0, 0.2))
+    psi = orc.so3d_evolve(v, dims, lo, hi, 0.02, psi0, 10)
+    p = tdse.Propagator(grid3(tdse, dims, lo, hi), v, 0.02)
+    p.upload(psi0)
+    p.step(10)
+    o = p.observe()
+    cell = (16 / 32) * (8 / 16) * (8 / 16)
+    assert abs(o["norm"] - orc.norm_sq(psi, cell)) < 1e-12
+    np.testing.assert_allclose(o["position"], orc.mean_position_3d(psi, dims, lo, hi),
+                               rtol=0, atol=1e-11)
+    assert abs(o["energy"] - orc.mean_energy_3d(psi, dims, lo, hi, v)) < 1e-10
+    slab = p.probability_in_slab(0, 0.0, 3.0)
+    assert abs(slab - orc.probability_in_slab_3d(psi, dims, lo, hi, 0, 0.0, 3.0)) < 1e-12
+
+
+def test_observables_1d_match_oracle(orc, tdse):
+    for n, (lo, hi) in [(1024, (-40.0, 40.0)), (65536, (-100.0, 100.0))]:
+        v = orc.harmonic_1d(lo, hi, n, 0.1)
+        psi0 = orc.gaussian_1d(lo, hi, n, 5.0, 2.0, 0.5)
+        p = tdse.Propagator(tdse.Grid1D(lo, hi, n), v, 0.01)
+        p.upload(psi0)
+        o = p.observe()
+        assert abs(o["norm"] - orc.norm_sq(psi0, (hi - lo) / n)) < 1e-12
+        assert abs(o["position"][0] - orc.mean_position_1d(psi0, lo, hi)) < 1e-11
+        assert abs(o["energy"] - orc.mean_energy_1d(psi0, lo, hi, v)) < 1e-10
+
+
+def test_unsupported_lengths_raise(tdse):
+    for dims in [(7, 8, 8), (2048, 4, 4), (8, 8, 13)]:
+        g = grid3(tdse, dims, (-1, -1, -1), (1, 1, 1))
+        with pytest.raises(ValueError):
+            tdse.SplitOperator3D(g, np.zeros(int(np.prod(dims))), 0.1)
+    with pytest.raises(ValueError):
+        tdse.SplitOperator1D(tdse.Grid1D(-1, 1, 1031), np.zeros(1031), 0.1)
