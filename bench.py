@@ -1,0 +1,235 @@
+#!/usr/bin/env python3
+"""Throughput of the split-operator propagator on one MI355X.
+
+Workload (the headline configuration): 3D barrier tunnelling, 256^3 grid on
+[-40, 40)^3, complex64 state (FP64 phase-argument reduction and FP64
+reductions), rectangular barrier V0 ~ U(0.5, 2) on 0 <= x < 2, sin^2 absorbing
+mask over the outer 8 a.u., Gaussian packet sigma = 3 at (-15, 0, 0) with
+kx ~ U(0.8, 1.5), dt = 0.01, seed 3.  Inputs are built on the host, uploaded
+once, and stay resident in HBM; the timed region is `steps` Strang steps
+(one tdse_step call) bracketed by HIP events on the plan's stream.
+
+Printed JSON line (stdout, one line):
+  value           steps/s of the FP32 path (higher is better)
+  fp64_steps_per_s  the same workload with a complex128 state
+  accuracy        FP32 vs FP64 after the full 2000-step run: max|dpsi|, norm
+                  drift, transmitted probability error (the FP64 GPU path is
+                  parity-tested against the FP64 CPU oracle)
+  roofline        the dominant sweep kernel: algorithmic bytes per launch over
+                  its average launch time (HIP events), vs 8 TB/s
+  cpu_baseline    the reference's own CPU FP64 propagator (oracle/_ref) on a
+                  bounded sample of the same workload, on the host's cores
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_barrier256_fp32.json")
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+class Hip:
+    """Minimal HIP event API over ctypes (timing on the plan's own stream)."""
+
+    def __init__(self):
+        self.lib = C.CDLL("libamdhip64.so")
+        self.lib.hipEventCreate.argtypes = [C.POINTER(C.c_void_p)]
+        self.lib.hipEventRecord.argtypes = [C.c_void_p, C.c_void_p]
+        self.lib.hipEventSynchronize.argtypes = [C.c_void_p]
+        self.lib.hipEventElapsedTime.argtypes = [C.POINTER(C.c_float), C.c_void_p, C.c_void_p]
+        self.lib.hipEventDestroy.argtypes = [C.c_void_p]
+
+    def event(self):
+        e = C.c_void_p()
+        assert self.lib.hipEventCreate(C.byref(e)) == 0
+        return e
+
+    def record(self, e, stream):
+        assert self.lib.hipEventRecord(e, C.c_void_p(stream)) == 0
+
+    def elapsed_ms(self, a, b):
+        assert self.lib.hipEventSynchronize(b) == 0
+        ms = C.c_float()
+        assert self.lib.hipEventElapsedTime(C.byref(ms), a, b) == 0
+        return ms.value
+
+
+def time_steps(hip, prop, steps, warmup):
+    prop.step(warmup)
+    prop.sync()
+    s = prop.stream()
+    e0, e1 = hip.event(), hip.event()
+    t0 = time.perf_counter()
+    hip.record(e0, s)
+    prop.step(steps)
+    hip.record(e1, s)
+    prop.sync()
+    wall = time.perf_counter() - t0
+    return hip.elapsed_ms(e0, e1) / 1e3, wall
+
+
+def cpu_baseline(solve, v, mask, psi0, sample_steps, threads):
+    """Reference CPU FP64 propagator (oracle/_ref) on `sample_steps` steps."""
+    os.environ.setdefault("SES_PARALLEL_WORKERS", str(threads))
+    from oracle import oracle as orc
+    kind = "reference" if orc.ref_available() else "port"
+    dims, lo, hi = solve.n, solve.lo, solve.hi
+    psi = np.ascontiguousarray(psi0).reshape(dims[::-1])
+    if kind == "reference":
+        lib = orc.ref()
+        workers = lib.ref_parallel_workers()
+        run = lambda p, k: orc.Ref.so3d_evolve(v, dims, lo, hi, solve.dt, p, k, mask=mask)
+    else:
+        workers = int(os.environ.get("OMP_NUM_THREADS", threads))
+        run = lambda p, k: orc.so3d_evolve(v, dims, lo, hi, solve.dt, p, k, mask=mask)
+    # each call builds the FP64 tables (once per solve in the reference), then
+    # steps: time one warm-up step and the sampled steps, subtract table time
+    t0 = time.perf_counter()
+    run(psi, 1)
+    t1 = time.perf_counter()
+    run(psi, 1 + sample_steps)
+    t2 = time.perf_counter()
+    per_step = ((t2 - t1) - (t1 - t0)) / sample_steps
+    return {"value": 1.0 / per_step, "unit": "steps/s", "cores": workers, "kind": kind,
+            "sample": f"{sample_steps} FP64 Strang steps (+ per-step mask) of the same 256^3 "
+                      f"barrier workload after 1 warm-up step; table build subtracted"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--seed", type=int, default=3)
+    ap.add_argument("--no-accuracy", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=30)
+    ap.add_argument("--cpu-threads", type=int, default=16)
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(independent solves shard across GPUs with no exchange; see DESIGN.md)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import shoeridingerequationsolver_amd as t
+    from shoeridingerequationsolver_amd import workloads
+
+    hip = Hip()
+    solve = workloads.barrier(seed=args.seed, n=args.n)
+    log(f"workload {solve.name} n={solve.n} V0={solve.params['V0']:.6f} "
+        f"kx={solve.k0[0]:.6f} dt={solve.dt} mask={solve.mask_width}")
+    grid = solve.grid()
+    v = solve.build_potential()
+    mask = solve.build_mask()
+    psi0 = solve.build_psi0()
+
+    result = {}
+    props = {}
+    for prec in (["fp32"] if args.no_fp64 else ["fp32", "fp64"]):
+        p = t.Propagator(grid, v, solve.dt, precision=prec, mask=mask)
+        p.upload(psi0)
+        gpu_s, wall_s = time_steps(hip, p, args.steps, args.warmup)
+        result[prec] = {"steps_per_s": args.steps / gpu_s, "ms_per_step": 1e3 * gpu_s / args.steps,
+                        "wall_steps_per_s": args.steps / wall_s}
+        log(f"{prec}: {args.steps / gpu_s:.1f} steps/s ({1e3 * gpu_s / args.steps:.4f} ms/step, "
+            f"wall {args.steps / wall_s:.1f} steps/s)")
+        props[prec] = p
+
+    # roofline of the dominant sweep (the x sweep: IFFT, kick*mask, FFT)
+    p32 = props["fp32"]
+    prof = p32.profile(max(8, min(args.steps, 200)))
+    log("per-sweep ms: " + ", ".join(f"{k}={ms / max(c, 1):.4f}x{c}" for k, (ms, c) in prof.items()))
+    N = solve.points
+    csz, rsz = 8, 4
+    sweep_bytes = {"x_mid": N * (2 * csz + 2 * rsz), "y_fwd": N * 2 * csz, "z_kin": N * 2 * csz,
+                   "y_inv": N * 2 * csz}
+    dom = "x_mid"
+    ms, cnt = prof[dom]
+    avg_s = ms / cnt / 1e3
+    achieved = sweep_bytes[dom] / avg_s / 1e9
+    traffic = None
+    if os.path.exists(TRAFFIC_FILE):
+        with open(TRAFFIC_FILE) as f:
+            traffic = json.load(f).get(dom)
+    roofline = {"bound": "hbm", "kernel": "contig_sweep<float,256,C_KICK_MID> (x sweep)",
+                "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "bytes_per_launch": sweep_bytes[dom], "avg_launch_ms": round(avg_s * 1e3, 5),
+                "step_bytes": p32.step_bytes(),
+                "step_frac": round(p32.step_bytes() * result["fp32"]["steps_per_s"] / 1e9
+                                   / HBM_PEAK_GBS, 4),
+                "sweeps_ms": {k: round(ms_ / max(c, 1), 5) for k, (ms_, c) in prof.items()}}
+
+    accuracy = None
+    if not args.no_accuracy and "fp64" in props:
+        full = solve.steps
+        outs = {}
+        for prec in ("fp32", "fp64"):
+            p = props[prec]
+            p.upload(psi0)
+            p.step(full)
+            obs = p.observe(t.TDSE_OBS_NORM)
+            outs[prec] = (p.download(), obs["norm"], p.probability_in_slab(0, 2.0, 1e300))
+        d = np.abs(outs["fp32"][0] - outs["fp64"][0]).max()
+        accuracy = {"steps": full, "max_abs_dpsi_fp32_vs_fp64": float(d),
+                    "max_abs_psi": float(np.abs(outs["fp64"][0]).max()),
+                    "norm_fp64": outs["fp64"][1], "norm_fp32": outs["fp32"][1],
+                    "norm_drift_fp32_vs_fp64": outs["fp32"][1] - outs["fp64"][1],
+                    "transmitted_fp64": outs["fp64"][2],
+                    "transmitted_err_fp32": outs["fp32"][2] - outs["fp64"][2]}
+        log(f"accuracy after {full} steps: max|dpsi| {d:.3e}, transmitted {outs['fp64'][2]:.6f}")
+
+    cpu = None
+    if not args.no_cpu:
+        try:
+            cpu = cpu_baseline(solve, v, mask, psi0, args.cpu_steps, args.cpu_threads)
+            log(f"cpu baseline: {cpu}")
+        except Exception as e:  # the baseline is reported, never required
+            log(f"cpu baseline unavailable: {e}")
+
+    line = {
+        "metric": "256^3 split-op steps/s FP32 (FP64 beside it) @ max|dpsi|<=1e-5 vs FP64",
+        "value": round(result["fp32"]["steps_per_s"], 2),
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(result["fp32"]["ms_per_step"], 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"barrier-tunnelling {args.n}^3 complex64 + absorbing mask",
+                   "grid": list(solve.n), "box": [solve.lo[0], solve.hi[0]], "dt": solve.dt,
+                   "V0": solve.params["V0"], "kx": solve.k0[0], "mask_width": solve.mask_width,
+                   "seed": args.seed},
+        "fp64_steps_per_s": round(result["fp64"]["steps_per_s"], 2) if "fp64" in result else None,
+        "wall_steps_per_s": round(result["fp32"]["wall_steps_per_s"], 2),
+        "accuracy": accuracy,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

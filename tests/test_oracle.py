@@ -1,0 +1,323 @@
+"""Pins the oracle (oracle/tdse_oracle.c) before it is trusted as the checker.
+
+Three kinds of pins, chosen so that a plausible slip (wrong kernel sign, a
+transposed axis, a dropped 1/N, a wrong table formula, a swapped mask order)
+fails at least one:
+
+1. The closed forms of the reference's own tests (tests/fft_test.cpp,
+   tests/fft3_test.cpp, tests/propagator_test.cpp,
+   tests/propagator_tables_test.cpp, tests/potential_test.cpp,
+   tests/harmonic_dynamics_test.cpp), restated here with the same tolerances.
+2. Bit-for-bit equality with the reference core compiled from its own sources
+   (oracle/_ref, built by oracle/Makefile) where that library is present.
+3. Golden vectors written by tests/golden/make_golden.py from that reference
+   build (tests/golden/*.npz), which travel with the repository.
+Mixed-radix transforms (no reference counterpart) are pinned by a brute-force
+DFT, numpy.fft and Parseval/inverse/linearity.
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def det_signal(n):
+    """tests/fft_test.cpp deterministic_signal (no RNG)."""
+    x = np.arange(n, dtype=np.float64)
+    return np.sin(1.3 * x) + 0.1 * x + 1j * (np.cos(2.7 * x) - 0.05 * x)
+
+
+# ---- FFT closed forms (tests/fft_test.cpp, tests/fft3_test.cpp) -------------
+
+def test_fft_size_one_is_identity(orc):
+    assert orc.fft1d(np.array([3 - 4j]))[0] == 3 - 4j
+
+
+def test_fft_dc_and_delta(orc):
+    a = orc.fft1d(np.ones(8, complex))
+    assert abs(a[0] - 8) < 1e-12 and np.abs(a[1:]).max() < 1e-12
+    d = np.zeros(8, complex)
+    d[0] = 1
+    assert np.abs(orc.fft1d(d) - 1).max() < 1e-12
+
+
+def test_fft_kernel_sign(orc):
+    n, k0 = 16, 3
+    i = np.arange(n)
+    a = orc.fft1d(np.exp(2j * np.pi * k0 * i / n))
+    want = np.zeros(n, complex)
+    want[k0] = 16
+    assert np.abs(a - want).max() < 1e-12  # e^{+} kernel would spike at 13
+
+
+def test_fft_cosine_pair(orc):
+    n, k0 = 16, 3
+    a = orc.fft1d(np.cos(2 * np.pi * k0 * np.arange(n) / n) + 0j)
+    want = np.zeros(n, complex)
+    want[k0] = want[n - k0] = 8
+    assert np.abs(a - want).max() < 1e-12
+
+
+def test_fft_inverse_linear_parseval(orc):
+    x = det_signal(16)
+    assert np.abs(orc.ifft1d(orc.fft1d(x)) - x).max() < 1e-12
+    x8 = det_signal(8)
+    y8 = 0.5 * np.arange(8) - 1j
+    assert np.abs(orc.fft1d(2 * x8 + y8) - (2 * orc.fft1d(x8) + orc.fft1d(y8))).max() < 1e-12
+    assert abs(np.sum(abs(x) ** 2) - np.sum(abs(orc.fft1d(x)) ** 2) / 16) < 1e-10
+
+
+def test_fft_rejects_non_pow2(orc):
+    with pytest.raises(ValueError):
+        orc.fft1d(np.ones(12, complex))
+
+
+def test_fft_matches_numpy(orc):
+    rng = np.random.default_rng(5)
+    for n in (2, 4, 32, 1024, 65536):
+        x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+        assert np.abs(orc.fft1d(x) - np.fft.fft(x)).max() < 1e-11 * max(1, math.log2(n))
+        assert np.abs(orc.ifft1d(x) - np.fft.ifft(x)).max() < 1e-13 * max(1, math.log2(n))
+
+
+def test_fft3_dc_plane_wave_inverse_parseval(orc):
+    f = orc.fft3d(np.ones((2, 4, 8), complex))
+    assert abs(f[0, 0, 0] - 64) < 1e-12 and abs(f[0, 0, 1]) < 1e-12 and abs(f[1, 0, 0]) < 1e-12
+    nx, ny, nz = 8, 16, 4
+    k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    th = 2 * np.pi * (3.0 * i / nx + 5.0 * j / ny + 1.0 * k / nz)
+    f = orc.fft3d(np.exp(1j * th))
+    want = np.zeros((nz, ny, nx), complex)
+    want[1, 5, 3] = 512  # (i, j, k) = (3, 5, 1): a transposed axis would miss
+    assert np.abs(f - want).max() < 1e-9
+    t = np.arange(64, dtype=float)
+    x = (np.sin(1.3 * t) + 0.1 * t + 1j * np.cos(2.7 * t)).reshape(2, 4, 8)
+    assert np.abs(orc.ifft3d(orc.fft3d(x)) - x).max() < 1e-12
+    y = (np.cos(0.9 * t) + 0.2j * np.sin(1.7 * t)).reshape(2, 8, 4)
+    assert abs(np.sum(abs(y) ** 2) - np.sum(abs(orc.fft3d(y)) ** 2) / 64) < 1e-10
+    assert np.abs(orc.fft3d(y) - np.fft.fftn(y)).max() < 1e-12
+
+
+# ---- mixed radix (no reference counterpart) ----------------------------------
+
+@pytest.mark.parametrize("n", [3, 5, 6, 10, 12, 48, 96, 160, 192, 7, 14])
+def test_mixed_radix_matches_brute_and_numpy(orc, n):
+    rng = np.random.default_rng(n)
+    x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    brute = orc.dft_brute(x)
+    assert np.abs(brute - np.fft.fft(x)).max() < 1e-12 * n
+    assert np.abs(orc.fft_any(x) - brute).max() < 1e-12 * n
+    assert np.abs(orc.fft_any(orc.fft_any(x), +1) / n - x).max() < 1e-13 * n
+
+
+def test_mixed_radix_3d_matches_numpy(orc):
+    rng = np.random.default_rng(9)
+    x = rng.standard_normal((4, 160, 96)) + 1j * rng.standard_normal((4, 160, 96))
+    assert np.abs(orc.fft3d_any(x) - np.fft.fftn(x)).max() < 1e-10
+    assert np.abs(orc.fft3d_any(x, +1) - np.fft.ifftn(x) * x.size).max() < 1e-10
+
+
+# ---- tables (tests/propagator_tables_test.cpp) --------------------------------
+
+def test_tables_equal_defining_formula(orc):
+    dims, lo, hi = (8, 4, 2), (-2.0, -1.0, -0.5), (2.0, 1.0, 0.5)
+    v = 0.01 * np.arange(64, dtype=float)
+    hv, kin = orc.so3d_tables(v, dims, lo, hi, 0.05, 1.0)
+    assert np.array_equal(hv, np.cos(-0.5 * v * 0.05) + 1j * np.sin(-0.5 * v * 0.05))
+    kx = orc.wavenumbers(8, -2.0, 2.0)
+    ky = orc.wavenumbers(4, -1.0, 1.0)
+    kz = orc.wavenumbers(2, -0.5, 0.5)
+    k, j, i = np.meshgrid(np.arange(2), np.arange(4), np.arange(8), indexing="ij")
+    th = -0.5 * (kx[i] ** 2 + ky[j] ** 2 + kz[k] ** 2) / 1.0 * 0.05
+    assert np.array_equal(kin.reshape(2, 4, 8), np.cos(th) + 1j * np.sin(th))
+
+
+def test_wavenumbers_fftfreq_layout(orc):
+    """tests/spectral_test.cpp: fftfreq order, DC at n = 1."""
+    k = orc.wavenumbers(8, 0.0, 2 * np.pi)
+    assert np.array_equal(k, np.array([0, 1, 2, 3, -4, -3, -2, -1], float))
+    assert orc.wavenumbers(1, 0.0, 1.0)[0] == 0.0
+    assert np.allclose(orc.wavenumbers(16, -3, 5), 2 * np.pi * np.fft.fftfreq(16, 8 / 16))
+
+
+# ---- propagation (tests/propagator_test.cpp, harmonic_dynamics_test.cpp) -----
+
+GRID = (-40.0, 40.0, 1024)
+
+
+def free_run(orc, steps=200, dt=0.05, x0=-15.0, s0=1.0, k0=2.0):
+    lo, hi, n = GRID
+    psi = orc.gaussian_1d(lo, hi, n, x0, s0, k0)
+    return orc.so1d_evolve(np.zeros(n), lo, hi, dt, psi, steps)
+
+
+def test_free_packet_norm_velocity_dispersion_momentum(orc):
+    lo, hi, n = GRID
+    psi = free_run(orc)
+    h = (hi - lo) / n
+    x = lo + np.arange(n) * h
+    w = abs(psi) ** 2
+    assert abs(orc.norm_sq(psi, h) - 1.0) < 1e-12
+    t = 0.05 * 200
+    mean = orc.mean_position_1d(psi, lo, hi)
+    assert abs(mean - (-15.0 + 2.0 * t)) < 1e-6
+    sigma = math.sqrt(max(0.0, np.sum(x * x * w) / np.sum(w) - mean ** 2))
+    assert abs(sigma - math.sqrt(1 + (t / 2) ** 2)) < 1e-6
+    phi = orc.fft1d(psi)
+    k = orc.wavenumbers(n, lo, hi)
+    assert abs(np.sum(k * abs(phi) ** 2) / np.sum(abs(phi) ** 2) - 2.0) < 1e-8
+
+
+def test_zero_steps_identity(orc):
+    lo, hi, n = GRID
+    psi = orc.gaussian_1d(lo, hi, n, -15.0, 1.0, 2.0)
+    assert np.array_equal(orc.so1d_evolve(np.zeros(n), lo, hi, 0.05, psi, 0), psi)
+
+
+def test_harmonic_ehrenfest_and_strang_order(orc):
+    """Quadratic V: <x> follows the exact Stormer-Verlet map of the split step;
+    the Strang error is O(dt^2) (harmonic_dynamics_test.cpp:75-92)."""
+    lo, hi, n = GRID
+    w = 0.5
+    v = orc.harmonic_1d(lo, hi, n, w)
+    psi0 = orc.gaussian_1d(lo, hi, n, 3.0, 1 / math.sqrt(2 * w), 0.0)
+    dt, steps = 0.05, 100
+    psi = orc.so1d_evolve(v, lo, hi, dt, psi0, steps)
+    # half kick, drift, half kick on (x, p) is linear; iterate it exactly
+    x, p = 3.0, 0.0
+    for _ in range(steps):
+        p -= 0.5 * dt * w * w * x
+        x += dt * p
+        p -= 0.5 * dt * w * w * x
+    assert abs(orc.mean_position_1d(psi, lo, hi) - x) < 1e-8
+    T = 2 * math.pi / w
+    errs = []
+    for m in (200, 400):
+        d = T / m
+        ps = orc.so1d_evolve(v, lo, hi, d, psi0, m)
+        errs.append(abs(orc.mean_position_1d(ps, lo, hi) - 3.0))
+    assert 3.4 < errs[0] / errs[1] < 4.6
+
+
+def test_3d_free_dispersion_exact_at_dt1(orc):
+    """dynamics3_test.cpp: V = 0 makes the split step exact at any dt."""
+    dims, lo, hi = (32, 32, 16), (-16, -16, -8), (16, 16, 8)
+    psi0 = orc.gaussian_3d(dims, lo, hi, (0, 0, 0), (1.0, 1.5, 0.8), (0.5, 0, 0)).reshape(16, 32, 32)
+    one = orc.so3d_evolve(np.zeros(32 * 32 * 16), dims, lo, hi, 1.0, psi0, 1)
+    many = orc.so3d_evolve(np.zeros(32 * 32 * 16), dims, lo, hi, 0.1, psi0, 10)
+    assert np.abs(one - many).max() < 1e-12
+
+
+def test_mass_drift(orc):
+    """MassParameter.FreePacketDriftsAtK0OverM (propagator_test.cpp)."""
+    dims, lo, hi = (128, 1, 1), (-32.0, -1.0, -1.0), (32.0, 1.0, 1.0)
+    psi0 = orc.gaussian_3d(dims, lo, hi, (-8, 0, 0), (2.0, 0.5, 0.5), (1.0, 0, 0)).reshape(1, 1, 128)
+    psi = orc.so3d_evolve(np.zeros(128), dims, lo, hi, 4.0 / 200, psi0, 200, mass=2.0)
+    assert abs(orc.mean_position_3d(psi, dims, lo, hi)[0] - (-8 + 0.5 * 4.0)) < 0.05
+
+
+def test_mask_formula_and_order(orc):
+    """potential_test.cpp mask formula; the mask multiplies after each step."""
+    lo, hi, n = -10.0, 10.0, 64
+    m = orc.absorbing_mask_1d(lo, hi, n, 4.0)
+    x = lo + np.arange(n) * (hi - lo) / n
+    d = np.minimum(x - lo, hi - x)
+    want = np.where(d >= 4.0, 1.0, np.sin(0.5 * np.pi * d / 4.0) ** 2)
+    assert np.abs(m - want).max() < 1e-15 and m[0] == 0.0
+    assert orc.absorbing_mask_1d(lo, hi, 1, 4.0)[0] == 1.0
+    v = orc.harmonic_1d(lo, hi, n, 0.3)
+    psi0 = orc.gaussian_1d(lo, hi, n, 2.0, 1.0, 1.0)
+    one = orc.so1d_evolve(v, lo, hi, 0.1, psi0, 1) * m
+    assert np.array_equal(orc.so1d_evolve(v, lo, hi, 0.1, psi0, 1, mask=m), one)
+
+
+def test_barrier_half_open_and_coulomb(orc):
+    lo, hi, n = -2.0, 2.0, 8  # coords -2, -1.5, ..., 1.5
+    v = orc.barrier_1d(lo, hi, n, 1.5, -1.0, 0.5)
+    assert list(v) == [0, 0, 1.5, 1.5, 1.5, 0, 0, 0]
+    dims, l3, h3 = (8, 8, 8), (-2.0,) * 3, (2.0,) * 3
+    c = orc.regularized_coulomb_3d(dims, l3, h3, 1.0, (0, 0, 0))
+    f = orc.coulomb_floor_3d(dims, l3, h3, 1.0, (0, 0, 0))
+    center = 4 + 8 * (4 + 8 * 4)
+    assert c[center] == -1.0 * 2.3800774 / 0.5 and f[center] == -1.0 / 0.25
+    assert c[center + 1] == f[center + 1] == -1.0 / 0.5
+
+
+def test_slab_probability_gaussian_tail(orc):
+    """P(x >= a) of a normalised Gaussian is erfc((a - x0) / (sqrt(2) sigma)) / 2."""
+    dims, lo, hi = (256, 8, 8), (-20.0, -8.0, -8.0), (20.0, 8.0, 8.0)
+    psi = orc.gaussian_3d(dims, lo, hi, (0.3, 0, 0), (1.2, 1.0, 1.0), (0, 0, 0))
+    p = orc.probability_in_slab_3d(psi, dims, lo, hi, 0, 1.0, 1e300)
+    assert abs(p - 0.5 * math.erfc((1.0 - 0.3) / (math.sqrt(2) * 1.2))) < 2e-3
+
+
+# ---- bitwise against the reference build -------------------------------------
+
+needs_ref = pytest.mark.skipif(
+    not os.path.exists(os.path.join(os.path.dirname(__file__), "..", "oracle", "_ref",
+                                    "libsesref.so")),
+    reason="reference core not built here (oracle/Makefile ref needs the reference sources)")
+
+
+@needs_ref
+def test_restatement_bitwise_equals_reference(orc):
+    R = orc.Ref
+    rng = np.random.default_rng(1)
+    for n in (1, 2, 8, 1024, 65536):
+        a = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+        assert np.array_equal(orc.fft1d(a), R.fft1d(a))
+        assert np.array_equal(orc.ifft1d(a), R.ifft1d(a))
+    a = rng.standard_normal((4, 16, 8)) + 1j * rng.standard_normal((4, 16, 8))
+    assert np.array_equal(orc.fft3d(a), R.fft3d(a))
+    assert np.array_equal(orc.ifft3d(a), R.ifft3d(a))
+    dims, lo, hi = (16, 8, 32), (-5.0, -4.0, -8.0), (5.0, 4.0, 8.0)
+    v = orc.regularized_coulomb_3d(dims, lo, hi, 1.0, (0, 0, 0))
+    assert np.array_equal(v, R.regularized_coulomb_3d(dims, lo, hi, 1.0, (0, 0, 0)))
+    m = orc.absorbing_mask_3d(dims, lo, hi, 2.0)
+    assert np.array_equal(m, R.absorbing_mask_3d(dims, lo, hi, 2.0))
+    psi = orc.gaussian_3d(dims, lo, hi, (0.5, 0, 1), (1.0, 1.2, 2), (0.7, 0, -0.3))
+    assert np.array_equal(psi, R.gaussian_3d(dims, lo, hi, (0.5, 0, 1), (1.0, 1.2, 2), (0.7, 0, -0.3)))
+    h1, k1 = orc.so3d_tables(v, dims, lo, hi, 0.01, 0.38)
+    h2, k2 = R.so3d_tables(v, dims, lo, hi, 0.01, 0.38)
+    assert np.array_equal(h1, h2) and np.array_equal(k1, k2)
+    psi = psi.reshape(dims[::-1])
+    assert np.array_equal(orc.so3d_evolve(v, dims, lo, hi, 0.01, psi, 7, mask=m),
+                          R.so3d_evolve(v, dims, lo, hi, 0.01, psi, 7, mask=m))
+    lo1, hi1, n1 = GRID
+    v1 = orc.harmonic_1d(lo1, hi1, n1, 0.1)
+    p1 = orc.gaussian_1d(lo1, hi1, n1, -15.0, 1.0, 2.0)
+    assert np.array_equal(orc.so1d_evolve(v1, lo1, hi1, 0.05, p1, 20),
+                          R.so1d_evolve(v1, lo1, hi1, 0.05, p1, 20))
+    assert orc.mean_energy_1d(p1, lo1, hi1, v1) == R.mean_energy_1d(p1, lo1, hi1, v1)
+    assert orc.mean_energy_3d(psi, dims, lo, hi, v, 0.38) == R.mean_energy_3d(psi, dims, lo, hi, v, 0.38)
+
+
+# ---- golden vectors from the reference build ---------------------------------
+
+def golden(name):
+    path = os.path.join(GOLDEN, name)
+    if not os.path.exists(path):
+        pytest.skip(f"{name} not generated")
+    return np.load(path, allow_pickle=False)
+
+
+def test_golden_fft(orc):
+    g = golden("fft.npz")
+    assert np.array_equal(orc.fft1d(g["x1024"]), g["fft1024"])
+    assert np.array_equal(orc.ifft1d(g["x1024"]), g["ifft1024"])
+    assert np.array_equal(orc.fft3d(g["x3"]), g["fft3"])
+    assert np.array_equal(orc.ifft3d(g["x3"]), g["ifft3"])
+
+
+def test_golden_step_small(orc):
+    g = golden("step_small.npz")
+    dims, lo, hi = tuple(g["dims"]), tuple(g["lo"]), tuple(g["hi"])
+    out = orc.so3d_evolve(g["v"], dims, lo, hi, float(g["dt"]), g["psi0"], int(g["steps"]),
+                          mass=float(g["mass"]), mask=g["mask"])
+    assert np.array_equal(out, g["psi"])
+    o1 = orc.so1d_evolve(g["v1"], float(g["lo1"]), float(g["hi1"]), float(g["dt1"]), g["psi1_0"],
+                         int(g["steps1"]), mask=g["mask1"])
+    assert np.array_equal(o1, g["psi1"])
