@@ -82,8 +82,8 @@ int tdse_plan_create(tdse_ctx* ctx, const int n[3], const double lo[3], const do
 int tdse_plan_destroy(tdse_plan* plan);
 
 /* The potential V (n[0]*n[1]*n[2] doubles, x-fastest).  Builds the half-kick
- * phase -0.5*V*dt per cell (core/src/propagator.ixx:28-31, 78-81); the FP32
- * path reduces it modulo 2*pi in FP64 before rounding. */
+ * phase -0.5*V*dt per cell (core/src/propagator.ixx:28-31, 78-81), reduced
+ * modulo 2*pi in FP64 (then rounded to float on the FP32 path). */
 int tdse_plan_set_potential(tdse_plan* plan, const double* v);
 /* Real absorbing mask multiplied into psi after every step (null clears it).
  * core/src/potential.ixx:194-233 builds such masks. */

@@ -52,6 +52,17 @@ __device__ __forceinline__ cx<T> mul_i(cx<T> a) {
     else return {-a.y, a.x};
 }
 
+// x * K for a compile-time constant K.  In FP32 the constant is split into
+// hi + lo (hi = K rounded to float, lo = the rounding error) so the product
+// carries K to about 48 bits: a rounded constant multiplied into the same
+// fraction of the state every step would otherwise bias its norm coherently.
+__device__ __forceinline__ double mulk(double x, double k) { return x * k; }
+__device__ __forceinline__ float mulk(float x, double k) {
+    const float hi = static_cast<float>(k);
+    const float lo = static_cast<float>(k - static_cast<double>(hi));
+    return fmaf(x, hi, x * lo);
+}
+
 // ---- compile-time sin/cos of 2*pi*m/R --------------------------------------
 
 constexpr double kPi = 3.14159265358979323846;
@@ -115,11 +126,12 @@ constexpr SinCos sincos_2pi(long long num, long long den) {
     return SinCos{c, s};
 }
 
-// W_R^m with sign DIR: exp(DIR * 2 pi i m / R)
+// a * W_R^m with sign DIR: a * exp(DIR * 2 pi i m / R), constant folded
 template <typename T, int DIR, int M, int R>
-__device__ __forceinline__ constexpr cx<T> cw() {
+__device__ __forceinline__ cx<T> mul_cw(cx<T> a) {
     constexpr SinCos sc = sincos_2pi(M, R);
-    return cx<T>{static_cast<T>(sc.c), static_cast<T>(DIR * sc.s)};
+    constexpr double c = sc.c, s = DIR * sc.s;
+    return {mulk(a.x, c) - mulk(a.y, s), mulk(a.x, s) + mulk(a.y, c)};
 }
 
 template <typename T, int DIR, int M, int R>
@@ -134,7 +146,7 @@ __device__ __forceinline__ cx<T> twiddle_const(cx<T> a) {
     } else if constexpr (4 * m == 3 * R) {
         return mul_i<-DIR>(a);
     } else {
-        return cmul(a, cw<T, DIR, m, R>());
+        return mul_cw<T, DIR, m, R>(a);
     }
 }
 
@@ -163,10 +175,10 @@ template <int DIR>
 struct Dft<3, DIR> {
     template <typename T>
     __device__ __forceinline__ static void run(cx<T>* v) {
-        constexpr T s = static_cast<T>(DIR * 0.86602540378443864676);
+        constexpr double s = DIR * 0.86602540378443864676;
         const cx<T> t1 = v[1] + v[2], t2 = v[1] - v[2];
         const cx<T> m = v[0] - cscale(t1, static_cast<T>(0.5));
-        const cx<T> is = {-s * t2.y, s * t2.x};  // i*s*t2
+        const cx<T> is = {-mulk(t2.y, s), mulk(t2.x, s)};  // i*s*t2
         v[0] = v[0] + t1;
         v[1] = m + is;
         v[2] = m - is;
@@ -190,16 +202,18 @@ template <int DIR>
 struct Dft<5, DIR> {
     template <typename T>
     __device__ __forceinline__ static void run(cx<T>* v) {
-        constexpr T c1 = static_cast<T>(0.30901699437494742410);   // cos(2pi/5)
-        constexpr T c2 = static_cast<T>(-0.80901699437494742410);  // cos(4pi/5)
-        constexpr T s1 = static_cast<T>(DIR * 0.95105651629515357212);
-        constexpr T s2 = static_cast<T>(DIR * 0.58778525229247312917);
+        constexpr double c1 = 0.30901699437494742410;   // cos(2pi/5)
+        constexpr double c2 = -0.80901699437494742410;  // cos(4pi/5)
+        constexpr double s1 = DIR * 0.95105651629515357212;
+        constexpr double s2 = DIR * 0.58778525229247312917;
         const cx<T> t1 = v[1] + v[4], t2 = v[2] + v[3];
         const cx<T> t3 = v[1] - v[4], t4 = v[2] - v[3];
-        const cx<T> m1 = {v[0].x + c1 * t1.x + c2 * t2.x, v[0].y + c1 * t1.y + c2 * t2.y};
-        const cx<T> m2 = {v[0].x + c2 * t1.x + c1 * t2.x, v[0].y + c2 * t1.y + c1 * t2.y};
-        const cx<T> n1 = {s1 * t3.x + s2 * t4.x, s1 * t3.y + s2 * t4.y};
-        const cx<T> n2 = {s2 * t3.x - s1 * t4.x, s2 * t3.y - s1 * t4.y};
+        const cx<T> m1 = {v[0].x + mulk(t1.x, c1) + mulk(t2.x, c2),
+                          v[0].y + mulk(t1.y, c1) + mulk(t2.y, c2)};
+        const cx<T> m2 = {v[0].x + mulk(t1.x, c2) + mulk(t2.x, c1),
+                          v[0].y + mulk(t1.y, c2) + mulk(t2.y, c1)};
+        const cx<T> n1 = {mulk(t3.x, s1) + mulk(t4.x, s2), mulk(t3.y, s1) + mulk(t4.y, s2)};
+        const cx<T> n2 = {mulk(t3.x, s2) - mulk(t4.x, s1), mulk(t3.y, s2) - mulk(t4.y, s1)};
         v[0] = v[0] + t1 + t2;
         v[1] = {m1.x - n1.y, m1.y + n1.x};
         v[4] = {m1.x + n1.y, m1.y - n1.x};
@@ -294,12 +308,26 @@ struct Last<Radices<R, R2, Rest...>> {
     static constexpr int value = Last<Radices<R2, Rest...>>::value;
 };
 
+// ---- FP64 phase application ------------------------------------------------------
+
+// a * k with a widened to FP64 and the product rounded once to T.
+template <typename T>
+__device__ __forceinline__ cx<T> mul_d(cx<T> a, cx<double> k) {
+    const double x = static_cast<double>(a.x), y = static_cast<double>(a.y);
+    return {static_cast<T>(x * k.x - y * k.y), static_cast<T>(x * k.y + y * k.x)};
+}
+
 // ---- one Stockham pass in registers -------------------------------------------
 
 // Twiddle then DFT for every butterfly this thread owns.  `tw` holds
-// W_N^m = exp(-2 pi i m / N) for m < N (forward sign).
+// W_N^m = exp(-2 pi i m / N), m < N, in FP64 (forward sign).  The butterfly's
+// twiddles W^{r k} (r = 1..R-1) are generated in FP64 from the one table
+// entry W^k by repeated multiplication (relative error below 2e-15 for
+// R <= 16) and applied in FP64, so no float-rounded twiddle ever scales the
+// state.
 template <typename T, int N, int E, int R, int NS, int DIR>
-__device__ __forceinline__ void pass_compute(cx<T> (&v)[E], int t, const cx<T>* __restrict__ tw) {
+__device__ __forceinline__ void pass_compute(cx<T> (&v)[E], int t,
+                                             const cx<double>* __restrict__ tw) {
     constexpr int TT = N / E;
 #pragma unroll
     for (int b = 0; b < E / R; ++b) {
@@ -307,11 +335,13 @@ __device__ __forceinline__ void pass_compute(cx<T> (&v)[E], int t, const cx<T>* 
             const int j = t + b * TT;
             const int k = j % NS;
             constexpr int step = N / (NS * R);
+            cx<double> w1 = tw[k * step];
+            if constexpr (DIR > 0) w1.y = -w1.y;
+            cx<double> w = w1;
 #pragma unroll
             for (int r = 1; r < R; ++r) {
-                cx<T> w = tw[r * k * step];
-                if constexpr (DIR > 0) w.y = -w.y;
-                v[b * R + r] = cmul(v[b * R + r], w);
+                v[b * R + r] = mul_d(v[b * R + r], w);
+                if (r + 1 < R) w = cmul(w, w1);
             }
         }
         Dft<R, DIR>::run(&v[b * R]);
@@ -343,8 +373,8 @@ __device__ __forceinline__ void pass_gather(cx<T> (&v)[E], int t, Get&& get) {
     }
 }
 
-// Visit the elements a thread holds in the register pattern of radix R:
-// fn(register slot, line index of that point).
+// Visit the points a thread holds in the register pattern of radix R:
+// fn(register slot, line index of that point, j of its butterfly, r).
 template <int N, int E, int R, class Fn>
 __device__ __forceinline__ void for_each_point(int t, Fn&& fn) {
     constexpr int TT = N / E;
@@ -352,7 +382,7 @@ __device__ __forceinline__ void for_each_point(int t, Fn&& fn) {
     for (int b = 0; b < E / R; ++b) {
         const int j = t + b * TT;
 #pragma unroll
-        for (int r = 0; r < R; ++r) fn(b * R + r, j + r * (N / R));
+        for (int r = 0; r < R; ++r) fn(b * R + r, j + r * (N / R), j, r * (N / R));
     }
 }
 
@@ -364,7 +394,7 @@ struct Transform;
 template <typename T, int N, int E, int DIR, int NS, int R>
 struct Transform<T, N, E, DIR, NS, Radices<R>> {
     template <class X>
-    __device__ __forceinline__ static void run(cx<T> (&v)[E], int t, const cx<T>* tw, X& x) {
+    __device__ __forceinline__ static void run(cx<T> (&v)[E], int t, const cx<double>* tw, X& x) {
         pass_compute<T, N, E, R, NS, DIR>(v, t, tw);
     }
 };
@@ -372,7 +402,7 @@ struct Transform<T, N, E, DIR, NS, Radices<R>> {
 template <typename T, int N, int E, int DIR, int NS, int R, int R2, int... Rest>
 struct Transform<T, N, E, DIR, NS, Radices<R, R2, Rest...>> {
     template <class X>
-    __device__ __forceinline__ static void run(cx<T> (&v)[E], int t, const cx<T>* tw, X& x) {
+    __device__ __forceinline__ static void run(cx<T> (&v)[E], int t, const cx<double>* tw, X& x) {
         pass_compute<T, N, E, R, NS, DIR>(v, t, tw);
         x.sync();
         pass_scatter<T, N, E, R, NS>(v, t, [&](int i, cx<T> z) { x.put(i, z); });

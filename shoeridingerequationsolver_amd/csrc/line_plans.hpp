@@ -80,23 +80,22 @@ struct Shape {
     static constexpr bool EXCHANGE = PASSES > 1;
     static constexpr int CSZ = static_cast<int>(sizeof(cx<T>));
     static constexpr int LSTR = lds_len(N);
-    static constexpr int TW_SLOTS = EXCHANGE ? N : 0;
+    // FP64 tables staged in LDS: twiddles and (strided kinetic sweep) K_p
+    static constexpr int DATA_CAP = (kLdsCap - 2 * N * 16) / CSZ;
 
     // contiguous sweeps
     static constexpr int LPB_WANT = cmax(1, 256 / TT);
-    static constexpr int LPB_LDS =
-        EXCHANGE ? cmax(1, (kLdsCap / CSZ - TW_SLOTS) / LSTR) : LPB_WANT;
+    static constexpr int LPB_LDS = EXCHANGE ? cmax(1, DATA_CAP / LSTR) : LPB_WANT;
     static constexpr int LPB = pow2_floor(cmin(LPB_WANT, LPB_LDS));
     static constexpr int BLOCK_C = LPB * TT;
-    static constexpr int LDS_C = EXCHANGE ? LPB * LSTR + TW_SLOTS : 1;
+    static constexpr int LDS_C = EXCHANGE ? LPB * LSTR : 1;
 
     // strided sweeps
     static constexpr int TX_WANT = cmax(1, cmin(64, 256 / TT));
-    static constexpr int TX_LDS =
-        EXCHANGE ? cmax(1, (kLdsCap / CSZ - TW_SLOTS) / LSTR) : TX_WANT;
+    static constexpr int TX_LDS = EXCHANGE ? cmax(1, DATA_CAP / LSTR) : TX_WANT;
     static constexpr int TX = pow2_floor(cmin(TX_WANT, TX_LDS));
     static constexpr int BLOCK_S = TX * TT;
-    static constexpr int LDS_S = EXCHANGE ? TX * LSTR + TW_SLOTS : 1;
+    static constexpr int LDS_S = EXCHANGE ? TX * LSTR : 1;
 };
 
 }  // namespace tdse

@@ -16,13 +16,29 @@
 // mask of scenario/src/line1d_director.ixx:202-211).  The inverse's 1/N is
 // folded into K.  Between calls psi is kept in position space, natural order.
 //
-// Phases:  the kick reads phi = -V dt / 2 per cell (FP32 path: reduced modulo
-// 2 pi in FP64 on the host), the kinetic angle -0.5 |k|^2 / m dt is formed in
-// FP64 from the wavenumber index (FP32 path: reduced modulo 2 pi in FP64 on
-// the device before the single-precision sincos).
+// Phases are applied in FP64 for both state precisions: the kick is
+// m * exp(i f phi) with phi = -V dt / 2 stored per cell, reduced modulo 2 pi
+// in FP64 on the host (and rounded to float on the FP32 path), and the kinetic factor is
+// the product of per-axis FP64 tables exp(-i k_a^2 dt / 2m) (the reference
+// builds the same phase from the summed angle, core/src/propagator.ixx:83-98;
+// the two agree to rounding).  The state point is widened to FP64, multiplied
+// and rounded once, so a phase never enters with a float-rounded magnitude.
+//
+// Global memory is addressed through buffer resources: one SGPR descriptor
+// per array and 32-bit byte offsets split into a per-lane part (VGPR) and a
+// per-radix-slot part that is uniform across the wave (SGPR).  Arrays are
+// therefore limited to 4 GiB each.  Reads beyond the descriptor's range
+// return zero and writes beyond it are dropped by the hardware.
 #pragma once
 
+#include <stdint.h>
+
 #include "line_plans.hpp"
+
+// Minimum waves per SIMD the compiler must fit (register cap); 1 = no cap.
+#ifndef TDSE_MIN_WAVES
+#define TDSE_MIN_WAVES 1
+#endif
 
 namespace tdse {
 
@@ -45,86 +61,120 @@ enum SweepMode : int {
     kNumModes = 13
 };
 
+constexpr bool mode_contiguous(int m) { return m <= C_SINGLE || m == C_FWD; }
+
 struct SweepArgs {
     void* psi;            // cx<T>*: the state, x-fastest
     const void* phi;      // T*: half-kick angle -V dt/2 per cell
     const void* mask;     // T*: per-step absorbing mask, or null
-    const void* tw;       // cx<T>*: W_N^m = exp(-2 pi i m / N), m < N, for this line length
+    const void* tw;       // cx<double>*: W_N^m = exp(-2 pi i m / N), m < N, this line length
+    uint32_t psi_bytes;   // byte extent of psi (buffer range)
+    uint32_t real_bytes;  // byte extent of phi / mask
     long long nlines;     // contiguous: number of lines
-    long long wstride;    // strided: element stride of the outer index w
-    long long estride;    // strided: element stride along the line
+    uint32_t wstride;     // strided: element stride of the outer index w
+    uint32_t estride;     // strided: element stride along the line
     int nx;               // strided: extent of the unit-stride index x
     int nw;               // strided: extent of the outer index w
-    // kinetic angle: k_a = dk[a] * shifted(idx_a, kn[a]) for slots (x, w, p)
-    double dk[3];
-    int kn[3];
-    double dt;
-    double mass;
+    // kinetic factor: product of per-axis FP64 tables exp(-i k^2 dt / 2m),
+    // indexed by slot (x, w, p); a null slot contributes 1
+    const void* kt[3];
     double scale;         // 1 / total points (inverse normalisation)
     int n_total;          // four-step twiddle modulus (1D), else 0
     int n_rows;           // four-step: number of rows N2 (contiguous K mapping)
     int nsteps;           // C_SINGLE: steps to run
 };
 
+// ---- buffer access -----------------------------------------------------------
+
+constexpr int32_t kBufferConfig = 0x00020000;  // raw 32-bit data format, gfx9 family
+
+template <typename V>
+struct Buf {
+    __amdgpu_buffer_rsrc_t r;
+    __device__ __forceinline__ Buf(const void* p, uint32_t bytes)
+        : r(__builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), static_cast<short>(0), bytes,
+                                              kBufferConfig)) {}
+    __device__ __forceinline__ V ld(uint32_t voff, uint32_t soff) const {
+        if constexpr (sizeof(V) == 4) {
+            return __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+        } else if constexpr (sizeof(V) == 8) {
+            return __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+        } else {
+            return __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0));
+        }
+    }
+    __device__ __forceinline__ void st(V v, uint32_t voff, uint32_t soff) const {
+        if constexpr (sizeof(V) == 4) {
+            __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, voff, soff, 0);
+        } else if constexpr (sizeof(V) == 8) {
+            using U = __attribute__((ext_vector_type(2))) uint32_t;
+            __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(U, v), r, voff, soff, 0);
+        } else {
+            using U = __attribute__((ext_vector_type(4))) uint32_t;
+            __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(U, v), r, voff, soff, 0);
+        }
+    }
+};
+
 // ---- phase helpers ------------------------------------------------------------
 
-__device__ __forceinline__ int shifted(int j, int n) { return (2 * j < n) ? j : j - n; }
-
-__device__ __forceinline__ double kin_angle(const SweepArgs& a, int ix, int iw, int ip) {
-    const double kx = a.dk[0] * static_cast<double>(shifted(ix, a.kn[0]));
-    const double kw = a.dk[1] * static_cast<double>(shifted(iw, a.kn[1]));
-    const double kp = a.dk[2] * static_cast<double>(shifted(ip, a.kn[2]));
-    return -0.5 * (kx * kx + kw * kw + kp * kp) / a.mass * a.dt;
-}
-
-// 2 pi split for FP64 argument reduction (hi has 33 significant bits, so
-// n * hi is exact for |n| < 2^20).
-constexpr double kTwoPiHi = 6.28318530717958623200e+00;
-constexpr double kTwoPiLo = 2.44929359829470635445e-16;
-constexpr double kInvTwoPi = 0.15915494309189533577;
-
-__device__ __forceinline__ double reduce_2pi(double th) {
-    const double n = rint(th * kInvTwoPi);
-    return (th - n * kTwoPiHi) - n * kTwoPiLo;
-}
-
-template <typename T>
-__device__ __forceinline__ cx<T> cis_scaled(double th, T scale);
-
-template <>
-__device__ __forceinline__ cx<double> cis_scaled<double>(double th, double scale) {
-    double s, c;
-    sincos(th, &s, &c);
-    return {c * scale, s * scale};
-}
-
-template <>
-__device__ __forceinline__ cx<float> cis_scaled<float>(double th, float scale) {
-    const float r = static_cast<float>(reduce_2pi(th));
-    float s, c;
-    sincosf(r, &s, &c);
-    return {c * scale, s * scale};
+// sin and cos of x for |x| <= 8 (kick angles are stored reduced to
+// [-pi, pi] and doubled at most).  Cody-Waite reduction by pi/2 with a
+// two-part constant (A = pi/2 rounded to 33 bits so q*A is exact, B = the
+// rest rounded to double; the split is 3.5e-27 short of pi/2), then the
+// Taylor series of sin and cos on |r| <= pi/4 to degree 17 / 18 (truncation
+// below 1e-19), Horner in FP64.  About 30 FP64 operations, no table, no branch.
+__device__ __forceinline__ void sincos_small(double x, double* s, double* c) {
+    constexpr double kTwoOverPi = 0.6366197723675814;
+    constexpr double kPio2A = 1.5707963267341256;   // 0x1.921fb544p+0
+    constexpr double kPio2B = 6.077100506506192e-11;
+    const double q = rint(x * kTwoOverPi);
+    const double r = (x - q * kPio2A) - q * kPio2B;
+    const double r2 = r * r;
+    // sin r = r + r^3 P(r^2), P = sum_{k>=1} (-1)^k r^{2k-2} / (2k+1)!
+    double p = 1.0 / 355687428096000.0;      //  1/17!
+    p = fma(p, r2, -1.0 / 1307674368000.0);  // -1/15!
+    p = fma(p, r2, 1.0 / 6227020800.0);      //  1/13!
+    p = fma(p, r2, -1.0 / 39916800.0);       // -1/11!
+    p = fma(p, r2, 1.0 / 362880.0);          //  1/9!
+    p = fma(p, r2, -1.0 / 5040.0);           // -1/7!
+    p = fma(p, r2, 1.0 / 120.0);             //  1/5!
+    p = fma(p, r2, -1.0 / 6.0);              // -1/3!
+    const double sr = fma(r * r2, p, r);
+    // cos r = 1 + r^2 Q(r^2), Q = sum_{k>=1} (-1)^k r^{2k-2} / (2k)!
+    double qc = -1.0 / 6402373705728000.0;   // -1/18!
+    qc = fma(qc, r2, 1.0 / 20922789888000.0);  //  1/16!
+    qc = fma(qc, r2, -1.0 / 87178291200.0);    // -1/14!
+    qc = fma(qc, r2, 1.0 / 479001600.0);       //  1/12!
+    qc = fma(qc, r2, -1.0 / 3628800.0);        // -1/10!
+    qc = fma(qc, r2, 1.0 / 40320.0);           //  1/8!
+    qc = fma(qc, r2, -1.0 / 720.0);            // -1/6!
+    qc = fma(qc, r2, 1.0 / 24.0);              //  1/4!
+    qc = fma(qc, r2, -0.5);                    // -1/2!
+    const double cr = fma(r2, qc, 1.0);
+    // x = q pi/2 + r: sin = (sr, cr, -sr, -cr), cos = (cr, -sr, -cr, sr) for q mod 4
+    const int qi = static_cast<int>(q) & 3;
+    const double ss = (qi & 1) ? cr : sr;
+    const double cc = (qi & 1) ? sr : cr;
+    *s = (qi & 2) ? -ss : ss;
+    *c = ((qi + 1) & 2) ? -cc : cc;
 }
 
 // kick factor m * exp(i * f * phi); f = 1 (half kick) or 2 (merged full kick)
 template <typename T>
-__device__ __forceinline__ cx<T> kick_factor(T phi, T f, T m) {
-    T s, c;
-    if constexpr (sizeof(T) == 8) {
-        sincos(f * phi, &s, &c);
-    } else {
-        sincosf(f * phi, &s, &c);
-    }
+__device__ __forceinline__ cx<double> kick_factor(T phi, double f, double m) {
+    double s, c;
+    sincos_small(f * static_cast<double>(phi), &s, &c);
     return {c * m, s * m};
 }
 
-// four-step twiddle exp(DIR * 2 pi i (a*b mod n) / n), FP64 then rounded
-template <typename T, int DIR>
-__device__ __forceinline__ cx<T> step_twiddle(long long a, long long b, int n) {
+// four-step twiddle exp(DIR * 2 pi i (a*b mod n) / n) in FP64
+template <int DIR>
+__device__ __forceinline__ cx<double> step_twiddle(long long a, long long b, int n) {
     const long long m = (a * b) % n;
     double s, c;
     sincospi(2.0 * static_cast<double>(m) / static_cast<double>(n), &s, &c);
-    return {static_cast<T>(c), static_cast<T>(DIR * s)};
+    return {c, DIR * s};
 }
 
 // ---- LDS exchange buffers -------------------------------------------------------
@@ -145,123 +195,145 @@ struct StridedX {
     __device__ __forceinline__ cx<T> get(int i) const { return base[lds_pad(i) * TX]; }
 };
 
-template <typename T, int N, int SLOTS>
-__device__ __forceinline__ const cx<T>* stage_twiddles(cx<T>* lds_tw, const void* tw_global,
-                                                       int tid, int nthreads) {
-    if constexpr (SLOTS > 0) {
-        const cx<T>* g = static_cast<const cx<T>*>(tw_global);
-        for (int i = tid; i < N; i += nthreads) lds_tw[i] = g[i];
-        __syncthreads();
-    }
-    return lds_tw;
+template <int N>
+__device__ __forceinline__ void stage_table(cx<double>* lds, const void* g, int tid, int nthreads) {
+    const cx<double>* src = static_cast<const cx<double>*>(g);
+    for (int i = tid; i < N; i += nthreads) lds[i] = src[i];
 }
+
+// ---- the per-line work of every mode ------------------------------------------------
+
+// One line (or one lane's share of it) of a sweep.  Point (b, r) of the
+// thread sits at element  base + (j_b + r*N/R) * es  of psi/phi/mask.
+template <typename T, int N, class X>
+struct Line {
+    using S = Shape<T, N>;
+    using Fwd = typename S::P::Fwd;
+    using Inv = Reverse<Fwd>;
+    static constexpr int E = S::E;
+    static constexpr int R1 = First<Fwd>::value, RL = Last<Fwd>::value;
+    static constexpr uint32_t CS = sizeof(cx<T>), RS = sizeof(T);
+
+    const SweepArgs& a;
+    const Buf<cx<T>>& P;
+    const Buf<T>& F;
+    const Buf<T>& M;
+    X& x;
+    const cx<double>* tw;
+    int t;
+    uint32_t base, es;
+    bool valid;
+    cx<T> v[E];
+
+    template <int R>
+    __device__ __forceinline__ void load() {
+        for_each_point<N, E, R>(t, [&](int s, int, int j, int ro) {
+            v[s] = valid ? P.ld((base + j * es) * CS, ro * es * CS) : cx<T>{0, 0};
+        });
+    }
+    template <int R>
+    __device__ __forceinline__ void store() {
+        for_each_point<N, E, R>(t, [&](int s, int, int j, int ro) {
+            if (valid) P.st(v[s], (base + j * es) * CS, ro * es * CS);
+        });
+    }
+    // kick in pattern R1 (position space after IFFT / before FFT)
+    __device__ __forceinline__ void kick(double f, bool with_mask) {
+        for_each_point<N, E, R1>(t, [&](int s, int, int j, int ro) {
+            if (!valid) return;
+            const uint32_t vo = (base + j * es) * RS, so = ro * es * RS;
+            const T p = F.ld(vo, so);
+            const T m = (with_mask && a.mask) ? M.ld(vo, so) : T(1);
+            if (__any(p != T(0))) {
+                v[s] = mul_d(v[s], kick_factor<T>(p, f, static_cast<double>(m)));
+            } else if (with_mask && a.mask) {
+                v[s] = cscale(v[s], m);  // real factor: exact per component
+            }
+        });
+    }
+    __device__ __forceinline__ void fft() { Transform<T, N, E, -1, 1, Fwd>::run(v, t, tw, x); }
+    __device__ __forceinline__ void ifft() { Transform<T, N, E, +1, 1, Inv>::run(v, t, tw, x); }
+    // four-step twiddle after the column FFT: exp(-2 pi i n1 k2 / N), k2 = position
+    __device__ __forceinline__ void twiddle_out(int n1) {
+        for_each_point<N, E, RL>(t, [&](int s, int i, int, int) {
+            v[s] = mul_d(v[s], step_twiddle<-1>(n1, i, a.n_total));
+        });
+    }
+};
 
 // ---- contiguous sweep -------------------------------------------------------------
 
 template <typename T, int N, int MODE>
-__global__ void __launch_bounds__((Shape<T, N>::BLOCK_C))
+__global__ void __launch_bounds__((Shape<T, N>::BLOCK_C), TDSE_MIN_WAVES)
 contig_sweep(SweepArgs a) {
     using S = Shape<T, N>;
-    using Fwd = typename S::P::Fwd;
-    using Inv = Reverse<Fwd>;
-    constexpr int E = S::E, TT = S::TT, LPB = S::LPB;
-    constexpr int R1 = First<Fwd>::value, RL = Last<Fwd>::value;
+    using LT = Line<T, N, ContigX<T>>;
+    constexpr int TT = S::TT, LPB = S::LPB, E = S::E;
 
     __shared__ cx<T> lds[S::LDS_C];
+    __shared__ cx<double> lds_tw[S::EXCHANGE ? N : 1];
     const int tid = threadIdx.x;
     const int l = tid / TT, t = tid % TT;
-    const cx<T>* tw = stage_twiddles<T, N, S::TW_SLOTS>(lds + LPB * S::LSTR, a.tw, tid, S::BLOCK_C);
+    if constexpr (S::EXCHANGE) {
+        stage_table<N>(lds_tw, a.tw, tid, S::BLOCK_C);
+        __syncthreads();
+    }
     ContigX<T> x{lds + l * S::LSTR};
-
-    cx<T>* psi = static_cast<cx<T>*>(a.psi);
-    const T* phi = static_cast<const T*>(a.phi);
-    const T* mask = static_cast<const T*>(a.mask);
+    const Buf<cx<T>> P(a.psi, a.psi_bytes);
+    const Buf<T> F(a.phi, a.real_bytes);
+    const Buf<T> M(a.mask, a.mask ? a.real_bytes : 0u);
     const long long ntiles = (a.nlines + LPB - 1) / LPB;
 
     for (long long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
         const long long line = tile * LPB + l;
-        const bool valid = line < a.nlines;
-        const long long base = line * N;
-        cx<T> v[E];
-        auto load = [&](int i) { return valid ? psi[base + i] : cx<T>{0, 0}; };
-        auto store = [&](int s, int i) {
-            if (valid) psi[base + i] = v[s];
-        };
-        auto kick = [&](int s, int i, T f, bool with_mask) {
-            if (!valid) return;
-            const T m = (with_mask && mask) ? mask[base + i] : T(1);
-            v[s] = cmul(v[s], kick_factor<T>(phi[base + i], f, m));
-        };
-
+        LT L{a, P, F, M, x, lds_tw, t, static_cast<uint32_t>(line * N), 1u, line < a.nlines};
         if constexpr (MODE == C_KICK_FIRST) {
-            pass_gather<T, N, E, R1>(v, t, load);
-            for_each_point<N, E, R1>(t, [&](int s, int i) { kick(s, i, T(1), false); });
-            Transform<T, N, E, -1, 1, Fwd>::run(v, t, tw, x);
-            for_each_point<N, E, RL>(t, store);
+            L.template load<LT::R1>();
+            L.kick(1.0, false);
+            L.fft();
+            L.template store<LT::RL>();
         } else if constexpr (MODE == C_KICK_MID) {
-            pass_gather<T, N, E, RL>(v, t, load);
-            Transform<T, N, E, +1, 1, Inv>::run(v, t, tw, x);
-            for_each_point<N, E, R1>(t, [&](int s, int i) { kick(s, i, T(2), true); });
-            Transform<T, N, E, -1, 1, Fwd>::run(v, t, tw, x);
-            for_each_point<N, E, RL>(t, store);
+            L.template load<LT::RL>();
+            L.ifft();
+            L.kick(2.0, true);
+            L.fft();
+            L.template store<LT::RL>();
         } else if constexpr (MODE == C_KICK_LAST) {
-            pass_gather<T, N, E, RL>(v, t, load);
-            Transform<T, N, E, +1, 1, Inv>::run(v, t, tw, x);
-            for_each_point<N, E, R1>(t, [&](int s, int i) {
-                kick(s, i, T(1), true);
-                store(s, i);
-            });
-        } else if constexpr (MODE == C_KIN_TW) {
-            // row k2 = line, position k1 holds X[k2 + N2*k1]
-            pass_gather<T, N, E, R1>(v, t, load);
-            Transform<T, N, E, -1, 1, Fwd>::run(v, t, tw, x);
-            const T scale = static_cast<T>(a.scale);
-            for_each_point<N, E, RL>(t, [&](int s, int i) {
-                const int j = static_cast<int>(line) + a.n_rows * i;
-                v[s] = cmul(v[s], cis_scaled<T>(kin_angle(a, 0, 0, j), scale));
-            });
-            Transform<T, N, E, +1, 1, Inv>::run(v, t, tw, x);
-            for_each_point<N, E, R1>(t, [&](int s, int i) {
-                v[s] = cmul(v[s], step_twiddle<T, +1>(i, line, a.n_total));
-                store(s, i);
-            });
+            L.template load<LT::RL>();
+            L.ifft();
+            L.kick(1.0, true);
+            L.template store<LT::R1>();
         } else if constexpr (MODE == C_FWD) {
-            pass_gather<T, N, E, R1>(v, t, load);
-            Transform<T, N, E, -1, 1, Fwd>::run(v, t, tw, x);
-            for_each_point<N, E, RL>(t, store);
+            L.template load<LT::R1>();
+            L.fft();
+            L.template store<LT::RL>();
+        } else if constexpr (MODE == C_KIN_TW) {
+            // row k2 = line; after the row FFT position k1 holds bin k2 + N2*k1
+            const cx<double>* kt = static_cast<const cx<double>*>(a.kt[2]);
+            L.template load<LT::R1>();
+            L.fft();
+            for_each_point<N, E, LT::RL>(t, [&](int s, int i, int, int) {
+                const int j = static_cast<int>(line) + a.n_rows * i;
+                L.v[s] = mul_d(L.v[s], cscale(kt[j], a.scale));
+            });
+            L.ifft();
+            for_each_point<N, E, LT::R1>(t, [&](int s, int i, int, int) {
+                L.v[s] = mul_d(L.v[s], step_twiddle<+1>(i, line, a.n_total));
+            });
+            L.template store<LT::R1>();
         } else if constexpr (MODE == C_SINGLE) {
-            pass_gather<T, N, E, R1>(v, t, load);
-            // per-thread phases and mask stay in registers for the whole run
-            cx<T> half[E], full[E];
-            const T scale = static_cast<T>(a.scale);
-            for_each_point<N, E, R1>(t, [&](int s, int i) {
-                const T p = valid ? phi[base + i] : T(0);
-                const T m = (valid && mask) ? mask[base + i] : T(1);
-                half[s] = kick_factor<T>(p, T(1), T(1));
-                full[s] = kick_factor<T>(p, T(2), m);
-            });
-            cx<T> kin[E];
-            for_each_point<N, E, RL>(t, [&](int s, int i) {
-                kin[s] = cis_scaled<T>(kin_angle(a, 0, 0, i), scale);
-            });
-#pragma unroll
-            for (int s = 0; s < E; ++s) v[s] = cmul(v[s], half[s]);
+            const cx<double>* kt = static_cast<const cx<double>*>(a.kt[2]);
+            L.template load<LT::R1>();
+            L.kick(1.0, false);
             for (int step = 0; step < a.nsteps; ++step) {
-                Transform<T, N, E, -1, 1, Fwd>::run(v, t, tw, x);
-#pragma unroll
-                for (int s = 0; s < E; ++s) v[s] = cmul(v[s], kin[s]);
-                Transform<T, N, E, +1, 1, Inv>::run(v, t, tw, x);
-                if (step + 1 < a.nsteps) {
-#pragma unroll
-                    for (int s = 0; s < E; ++s) v[s] = cmul(v[s], full[s]);
-                } else {
-                    for_each_point<N, E, R1>(t, [&](int s, int i) {
-                        const T m = (valid && mask) ? mask[base + i] : T(1);
-                        v[s] = cscale(cmul(v[s], half[s]), m);
-                    });
-                }
+                L.fft();
+                for_each_point<N, E, LT::RL>(t, [&](int s, int i, int, int) {
+                    L.v[s] = mul_d(L.v[s], cscale(kt[i], a.scale));
+                });
+                L.ifft();
+                L.kick(step + 1 == a.nsteps ? 1.0 : 2.0, true);
             }
-            for_each_point<N, E, R1>(t, store);
+            L.template store<LT::R1>();
         }
     }
 }
@@ -269,95 +341,79 @@ contig_sweep(SweepArgs a) {
 // ---- strided sweep ------------------------------------------------------------------
 
 template <typename T, int N, int MODE>
-__global__ void __launch_bounds__((Shape<T, N>::BLOCK_S))
+__global__ void __launch_bounds__((Shape<T, N>::BLOCK_S), TDSE_MIN_WAVES)
 strided_sweep(SweepArgs a) {
     using S = Shape<T, N>;
-    using Fwd = typename S::P::Fwd;
-    using Inv = Reverse<Fwd>;
-    constexpr int E = S::E, TX = S::TX;
-    constexpr int R1 = First<Fwd>::value, RL = Last<Fwd>::value;
+    constexpr int TX = S::TX, E = S::E;
+    using LT = Line<T, N, StridedX<T, TX>>;
+    constexpr bool KIN = MODE == S_KIN;
 
     __shared__ cx<T> lds[S::LDS_S];
+    __shared__ cx<double> lds_tw[S::EXCHANGE ? N : 1];
+    __shared__ cx<double> lds_kp[KIN ? N : 1];
     const int tid = threadIdx.x;
     const int xi = tid % TX, t = tid / TX;
-    const cx<T>* tw = stage_twiddles<T, N, S::TW_SLOTS>(lds + TX * S::LSTR, a.tw, tid, S::BLOCK_S);
+    if constexpr (S::EXCHANGE) stage_table<N>(lds_tw, a.tw, tid, S::BLOCK_S);
+    if constexpr (KIN) {
+        // line-axis kinetic table with the inverse normalisation folded in
+        const cx<double>* kp = static_cast<const cx<double>*>(a.kt[2]);
+        for (int i = tid; i < N; i += S::BLOCK_S) lds_kp[i] = cscale(kp[i], a.scale);
+    }
+    if constexpr (S::EXCHANGE || KIN) __syncthreads();
     StridedX<T, TX> x{lds + xi};
-
-    cx<T>* psi = static_cast<cx<T>*>(a.psi);
-    const T* phi = static_cast<const T*>(a.phi);
-    const T* mask = static_cast<const T*>(a.mask);
+    const Buf<cx<T>> P(a.psi, a.psi_bytes);
+    const Buf<T> F(a.phi, a.real_bytes);
+    const Buf<T> M(a.mask, a.mask ? a.real_bytes : 0u);
     const int xtiles = (a.nx + TX - 1) / TX;
     const long long ntiles = static_cast<long long>(xtiles) * a.nw;
 
     for (long long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
         const int w = static_cast<int>(tile / xtiles);
         const int xx = static_cast<int>(tile % xtiles) * TX + xi;
-        const bool valid = xx < a.nx;
-        const long long base = xx + a.wstride * w;
-        cx<T> v[E];
-        auto addr = [&](int i) { return base + a.estride * i; };
-        auto load = [&](int i) { return valid ? psi[addr(i)] : cx<T>{0, 0}; };
-        auto store = [&](int s, int i) {
-            if (valid) psi[addr(i)] = v[s];
-        };
-        auto kick = [&](int s, int i, T f, bool with_mask) {
-            if (!valid) return;
-            const long long g = addr(i);
-            const T m = (with_mask && mask) ? mask[g] : T(1);
-            v[s] = cmul(v[s], kick_factor<T>(phi[g], f, m));
-        };
-        auto twiddle_out = [&](int s, int i) {
-            v[s] = cmul(v[s], step_twiddle<T, -1>(xx, i, a.n_total));
-        };
-
+        LT L{a, P, F, M, x, lds_tw, t, static_cast<uint32_t>(xx) + a.wstride * w, a.estride,
+             xx < a.nx};
         if constexpr (MODE == S_FWD) {
-            pass_gather<T, N, E, R1>(v, t, load);
-            Transform<T, N, E, -1, 1, Fwd>::run(v, t, tw, x);
-            for_each_point<N, E, RL>(t, store);
+            L.template load<LT::R1>();
+            L.fft();
+            L.template store<LT::RL>();
         } else if constexpr (MODE == S_INV) {
-            pass_gather<T, N, E, RL>(v, t, load);
-            Transform<T, N, E, +1, 1, Inv>::run(v, t, tw, x);
-            for_each_point<N, E, R1>(t, store);
+            L.template load<LT::RL>();
+            L.ifft();
+            L.template store<LT::R1>();
         } else if constexpr (MODE == S_KIN) {
-            pass_gather<T, N, E, R1>(v, t, load);
-            Transform<T, N, E, -1, 1, Fwd>::run(v, t, tw, x);
-            const T scale = static_cast<T>(a.scale);
-            for_each_point<N, E, RL>(t, [&](int s, int i) {
-                v[s] = cmul(v[s], cis_scaled<T>(kin_angle(a, xx, w, i), scale));
+            cx<double> kxw = {1.0, 0.0};
+            if (a.kt[0] && L.valid) kxw = static_cast<const cx<double>*>(a.kt[0])[xx];
+            if (a.kt[1]) kxw = cmul(kxw, static_cast<const cx<double>*>(a.kt[1])[w]);
+            L.template load<LT::R1>();
+            L.fft();
+            for_each_point<N, E, LT::RL>(t, [&](int s, int i, int, int) {
+                L.v[s] = mul_d(L.v[s], cmul(kxw, lds_kp[i]));
             });
-            Transform<T, N, E, +1, 1, Inv>::run(v, t, tw, x);
-            for_each_point<N, E, R1>(t, store);
+            L.ifft();
+            L.template store<LT::R1>();
         } else if constexpr (MODE == S_KICK_FIRST_TW) {
-            pass_gather<T, N, E, R1>(v, t, load);
-            for_each_point<N, E, R1>(t, [&](int s, int i) { kick(s, i, T(1), false); });
-            Transform<T, N, E, -1, 1, Fwd>::run(v, t, tw, x);
-            for_each_point<N, E, RL>(t, [&](int s, int i) {
-                twiddle_out(s, i);
-                store(s, i);
-            });
+            L.template load<LT::R1>();
+            L.kick(1.0, false);
+            L.fft();
+            L.twiddle_out(xx);
+            L.template store<LT::RL>();
         } else if constexpr (MODE == S_KICK_MID_TW) {
-            pass_gather<T, N, E, RL>(v, t, load);
-            Transform<T, N, E, +1, 1, Inv>::run(v, t, tw, x);
-            for_each_point<N, E, R1>(t, [&](int s, int i) { kick(s, i, T(2), true); });
-            Transform<T, N, E, -1, 1, Fwd>::run(v, t, tw, x);
-            for_each_point<N, E, RL>(t, [&](int s, int i) {
-                twiddle_out(s, i);
-                store(s, i);
-            });
-        } else if constexpr (MODE == S_FWD_TW) {
-            pass_gather<T, N, E, R1>(v, t, load);
-            Transform<T, N, E, -1, 1, Fwd>::run(v, t, tw, x);
-            for_each_point<N, E, RL>(t, [&](int s, int i) {
-                twiddle_out(s, i);
-                store(s, i);
-            });
+            L.template load<LT::RL>();
+            L.ifft();
+            L.kick(2.0, true);
+            L.fft();
+            L.twiddle_out(xx);
+            L.template store<LT::RL>();
         } else if constexpr (MODE == S_KICK_LAST) {
-            pass_gather<T, N, E, RL>(v, t, load);
-            Transform<T, N, E, +1, 1, Inv>::run(v, t, tw, x);
-            for_each_point<N, E, R1>(t, [&](int s, int i) {
-                kick(s, i, T(1), true);
-                store(s, i);
-            });
+            L.template load<LT::RL>();
+            L.ifft();
+            L.kick(1.0, true);
+            L.template store<LT::R1>();
+        } else if constexpr (MODE == S_FWD_TW) {
+            L.template load<LT::R1>();
+            L.fft();
+            L.twiddle_out(xx);
+            L.template store<LT::RL>();
         }
     }
 }

@@ -94,6 +94,8 @@ struct tdse_plan {
     double* partials = nullptr;
     double* red_out = nullptr;
     std::map<int, void*> tw;  // line length -> W_N table
+    void* kt[3] = {nullptr, nullptr, nullptr};  // per squeezed axis exp(-i k^2 dt / 2m), FP64
+    long long kt_len[3] = {0, 0, 0};
     bool has_potential = false;
     uint64_t bytes = 0;
 
@@ -133,10 +135,12 @@ bool four_step_split(long long n, int* n1, int* n2) {
     return true;
 }
 
-uint64_t plan_bytes_for(long long total, const std::vector<int>& lines, bool fp64, bool mask) {
+uint64_t plan_bytes_for(long long total, const std::vector<int>& lines, bool fp64, bool mask,
+                        long long kt_points) {
     const uint64_t cs = fp64 ? 16 : 8, rs = fp64 ? 8 : 4;
     uint64_t b = total * cs + 2 * total * rs + (mask ? total * rs : 0);
-    for (int n : lines) b += static_cast<uint64_t>(n) * cs;
+    for (int n : lines) b += static_cast<uint64_t>(n) * 16;
+    b += static_cast<uint64_t>(kt_points) * 16;
     b += static_cast<uint64_t>(tdse::kMaxReduceBlocks) * tdse::kReduceQ * 8 + 64 * 8;
     return b;
 }
@@ -219,25 +223,42 @@ void release(tdse_plan* p, void*& ptr, uint64_t bytes) {
     p->bytes -= bytes;
 }
 
-// Twiddle table W_N^m = exp(-2 pi i m / N), m < N, FP64 then rounded.
+// Twiddle table W_N^m = exp(-2 pi i m / N), m < N, FP64 for both state
+// precisions (the kernels form and apply twiddles in FP64).
 int make_twiddles(tdse_plan* p, int N) {
     if (p->tw.count(N)) return TDSE_OK;
     void* d = nullptr;
-    int rc = alloc(p, &d, static_cast<uint64_t>(N) * p->csize());
+    int rc = alloc(p, &d, static_cast<uint64_t>(N) * 16);
     if (rc) return rc;
+    p->tw[N] = d;
     std::vector<double> w(2 * static_cast<size_t>(N));
     for (int m = 0; m < N; ++m) {
         const double th = -2.0 * 3.14159265358979323846 * static_cast<double>(m) / N;
         w[2 * m] = std::cos(th);
         w[2 * m + 1] = std::sin(th);
     }
-    if (p->fp64()) {
-        TDSE_HIP(hipMemcpy(d, w.data(), w.size() * 8, hipMemcpyHostToDevice));
-    } else {
-        std::vector<float> f(w.begin(), w.end());
-        TDSE_HIP(hipMemcpy(d, f.data(), f.size() * 4, hipMemcpyHostToDevice));
+    TDSE_HIP(hipMemcpy(d, w.data(), w.size() * 8, hipMemcpyHostToDevice));
+    return TDSE_OK;
+}
+
+// Kinetic phase table of one axis (or, for a four-step 1D grid, of the whole
+// line): exp(-i * 0.5 * k_j^2 / mass * dt) with k_j the fftfreq wavenumber
+// (core/src/spectral.ixx:14-23, core/src/propagator.ixx:34-37, 83-98), FP64.
+int make_kinetic(tdse_plan* p, int slot, const Axis& ax) {
+    void* d = nullptr;
+    int rc = alloc(p, &d, static_cast<uint64_t>(ax.n) * 16);
+    if (rc) return rc;
+    p->kt[slot] = d;
+    p->kt_len[slot] = ax.n;
+    const double dk = 2.0 * 3.14159265358979323846 / (ax.hi - ax.lo);
+    std::vector<double> k(2 * static_cast<size_t>(ax.n));
+    for (int j = 0; j < ax.n; ++j) {
+        const double kj = dk * ((2 * j < ax.n) ? j : j - ax.n);
+        const double th = -0.5 * kj * kj / p->mass * p->dt;
+        k[2 * j] = std::cos(th);
+        k[2 * j + 1] = std::sin(th);
     }
-    p->tw[N] = d;
+    TDSE_HIP(hipMemcpy(d, k.data(), k.size() * 8, hipMemcpyHostToDevice));
     return TDSE_OK;
 }
 
@@ -246,13 +267,10 @@ SweepArgs base_args(const tdse_plan* p) {
     a.psi = p->psi;
     a.phi = p->phi;
     a.mask = p->mask;
-    a.dt = p->dt;
-    a.mass = p->mass;
     a.scale = 1.0 / static_cast<double>(p->total);
-    for (int i = 0; i < 3; ++i) {
-        a.dk[i] = 0.0;
-        a.kn[i] = 1;
-    }
+    a.psi_bytes = static_cast<uint32_t>(p->total * p->csize());
+    a.real_bytes = static_cast<uint32_t>(p->total * p->rsize());
+    for (int i = 0; i < 3; ++i) a.kt[i] = nullptr;
     return a;
 }
 
@@ -277,7 +295,7 @@ Geometry geometry(const tdse_plan* p, void* target) {
         g.yf = a;
         g.yf.nx = n0;
         g.yf.nw = n2;
-        g.yf.wstride = static_cast<long long>(n0) * n1;
+        g.yf.wstride = static_cast<uint32_t>(n0) * n1;
         g.yf.estride = n0;
         g.yf.tw = p->tw.at(n1);
         g.yi = g.yf;
@@ -285,14 +303,11 @@ Geometry geometry(const tdse_plan* p, void* target) {
         g.z.nx = n0;
         g.z.nw = n1;
         g.z.wstride = n0;
-        g.z.estride = static_cast<long long>(n0) * n1;
+        g.z.estride = static_cast<uint32_t>(n0) * n1;
         g.z.tw = p->tw.at(n2);
-        g.z.dk[0] = dk_of(p->sq[0]);
-        g.z.kn[0] = n0;
-        g.z.dk[1] = dk_of(p->sq[1]);
-        g.z.kn[1] = n1;
-        g.z.dk[2] = dk_of(p->sq[2]);
-        g.z.kn[2] = n2;
+        g.z.kt[0] = p->kt[0];
+        g.z.kt[1] = p->kt[1];
+        g.z.kt[2] = p->kt[2];
     } else if (p->kind == kKind2D) {
         const int n0 = p->sq[0].n, n1 = p->sq[1].n;
         g.x = a;
@@ -304,17 +319,14 @@ Geometry geometry(const tdse_plan* p, void* target) {
         g.yf.wstride = 0;
         g.yf.estride = n0;
         g.yf.tw = p->tw.at(n1);
-        g.yf.dk[0] = dk_of(p->sq[0]);
-        g.yf.kn[0] = n0;
-        g.yf.dk[2] = dk_of(p->sq[1]);
-        g.yf.kn[2] = n1;
+        g.yf.kt[0] = p->kt[0];
+        g.yf.kt[2] = p->kt[1];
         g.yi = g.yf;
     } else if (p->kind == kKindSingle) {
         g.single = a;
         g.single.nlines = 1;
         g.single.tw = p->tw.at(p->sq[0].n);
-        g.single.dk[2] = dk_of(p->sq[0]);
-        g.single.kn[2] = p->sq[0].n;
+        g.single.kt[2] = p->kt[0];
     } else {
         const int N = p->sq[0].n;
         g.col = a;
@@ -327,8 +339,7 @@ Geometry geometry(const tdse_plan* p, void* target) {
         g.row = a;
         g.row.nlines = p->n2;
         g.row.tw = p->tw.at(p->n1);
-        g.row.dk[2] = dk_of(p->sq[0]);
-        g.row.kn[2] = N;
+        g.row.kt[2] = p->kt[0];
         g.row.n_total = N;
         g.row.n_rows = p->n2;
     }
@@ -464,7 +475,9 @@ int tdse_plan_bytes(const int n[3], int precision, int with_mask, uint64_t* out)
     int rc = classify(n, sq, nullptr, nullptr, &kind, &n1, &n2, &lines);
     if (rc) return rc;
     const long long total = static_cast<long long>(n[0]) * n[1] * n[2];
-    *out = plan_bytes_for(total, lines, precision == TDSE_FP64, with_mask != 0);
+    long long ktp = 0;
+    for (const Axis& ax : sq) ktp += ax.n;
+    *out = plan_bytes_for(total, lines, precision == TDSE_FP64, with_mask != 0, ktp);
     return TDSE_OK;
 }
 
@@ -482,7 +495,8 @@ int tdse_plan_create(tdse_ctx* ctx, const int n[3], const double lo[3], const do
         if (!(hi[a] > lo[a]) && n[a] > 1) return fail(TDSE_E_INVALID, "hi must exceed lo");
     }
     const long long total = static_cast<long long>(n[0]) * n[1] * n[2];
-    if (total > (1LL << 31) - 1) return fail(TDSE_E_UNSUPPORTED, "more than 2^31-1 points");
+    if (total * (precision == TDSE_FP64 ? 16 : 8) >= (1LL << 32))
+        return fail(TDSE_E_UNSUPPORTED, "state larger than 4 GiB (32-bit buffer offsets)");
     int rc = set_device(ctx);
     if (rc) return rc;
 
@@ -523,6 +537,8 @@ int tdse_plan_create(tdse_ctx* ctx, const int n[3], const double lo[3], const do
     p->red_out = p->partials + tdse::kMaxReduceBlocks * tdse::kReduceQ;
     for (int N : lines)
         if ((rc = make_twiddles(p, N))) return bail(rc);
+    for (size_t a = 0; a < p->sq.size(); ++a)
+        if ((rc = make_kinetic(p, static_cast<int>(a), p->sq[a]))) return bail(rc);
     if (hipMemset(p->psi, 0, total * p->csize()) != hipSuccess ||
         hipMemset(p->phi, 0, total * p->rsize()) != hipSuccess ||
         hipMemset(p->v, 0, total * p->rsize()) != hipSuccess) {
@@ -545,7 +561,8 @@ int tdse_plan_destroy(tdse_plan* p) {
     release(p, p->scratch, p->total * cs);
     void* part = p->partials;
     release(p, part, static_cast<uint64_t>(tdse::kMaxReduceBlocks) * tdse::kReduceQ * 8 + 64 * 8);
-    for (auto& kv : p->tw) release(p, kv.second, static_cast<uint64_t>(kv.first) * cs);
+    for (auto& kv : p->tw) release(p, kv.second, static_cast<uint64_t>(kv.first) * 16);
+    for (int a = 0; a < 3; ++a) release(p, p->kt[a], static_cast<uint64_t>(p->kt_len[a]) * 16);
     if (p->stream) (void)hipStreamDestroy(p->stream);
     delete p;
     return TDSE_OK;
@@ -562,7 +579,9 @@ int tdse_plan_set_potential(tdse_plan* p, const double* v) {
         if (!std::isfinite(v[i])) return fail(TDSE_E_INVALID, "potential not finite");
         th[i] = -0.5 * v[i] * p->dt;  // core/src/propagator.ixx:29, 79
     }
-    if ((rc = upload_real(p, p->phi, th.data(), !p->fp64()))) return rc;
+    // reduced modulo 2 pi in FP64 for both precisions (the kernels' sin/cos
+    // take |angle| <= 8)
+    if ((rc = upload_real(p, p->phi, th.data(), true))) return rc;
     if ((rc = upload_real(p, p->v, v, false))) return rc;
     p->has_potential = true;
     return TDSE_OK;

@@ -177,29 +177,58 @@ def test_zero_steps_identity(orc):
     assert np.array_equal(orc.so1d_evolve(np.zeros(n), lo, hi, 0.05, psi, 0), psi)
 
 
-def test_harmonic_ehrenfest_and_strang_order(orc):
-    """Quadratic V: <x> follows the exact Stormer-Verlet map of the split step;
-    the Strang error is O(dt^2) (harmonic_dynamics_test.cpp:75-92)."""
+def test_harmonic_ehrenfest(orc):
+    """Quadratic V: <x> follows the exact Stormer-Verlet map of the split step."""
     lo, hi, n = GRID
     w = 0.5
     v = orc.harmonic_1d(lo, hi, n, w)
     psi0 = orc.gaussian_1d(lo, hi, n, 3.0, 1 / math.sqrt(2 * w), 0.0)
     dt, steps = 0.05, 100
     psi = orc.so1d_evolve(v, lo, hi, dt, psi0, steps)
-    # half kick, drift, half kick on (x, p) is linear; iterate it exactly
-    x, p = 3.0, 0.0
+    x, p = 3.0, 0.0  # half kick, drift, half kick on (x, p) is linear
     for _ in range(steps):
         p -= 0.5 * dt * w * w * x
         x += dt * p
         p -= 0.5 * dt * w * w * x
     assert abs(orc.mean_position_1d(psi, lo, hi) - x) < 1e-8
-    T = 2 * math.pi / w
-    errs = []
-    for m in (200, 400):
-        d = T / m
-        ps = orc.so1d_evolve(v, lo, hi, d, psi0, m)
-        errs.append(abs(orc.mean_position_1d(ps, lo, hi) - 3.0))
-    assert 3.4 < errs[0] / errs[1] < 4.6
+
+
+HGRID = (-16.0, 16.0, 512)  # tests/harmonic_dynamics_test.cpp:25-31
+
+
+def coherent(orc):
+    lo, hi, n = HGRID
+    return orc.gaussian_1d(lo, hi, n, 2.0, 1.0 / math.sqrt(2.0), 0.0)
+
+
+def test_harmonic_dynamics_reference_cases(orc):
+    """tests/harmonic_dynamics_test.cpp: energy 2.5, mirror point at T/2,
+    no spreading, momentum at T/4, energy over T, Strang O(dt^2)."""
+    lo, hi, n = HGRID
+    h = (hi - lo) / n
+    x = lo + np.arange(n) * h
+    v = orc.harmonic_1d(lo, hi, n, 1.0)
+    dt = 2.0 * math.pi / 1000.0
+    psi0 = coherent(orc)
+    assert abs(orc.mean_energy_1d(psi0, lo, hi, v) - 2.5) < 1e-8
+    half = orc.so1d_evolve(v, lo, hi, dt, psi0, 500)
+    assert abs(orc.mean_position_1d(half, lo, hi) + 2.0) < 1e-5
+    assert abs(orc.norm_sq(half, h) - 1.0) < 1e-12
+    w = abs(half) ** 2
+    m = np.sum(x * w) / np.sum(w)
+    assert abs(math.sqrt(np.sum(x * x * w) / np.sum(w) - m * m) - 1 / math.sqrt(2)) < 1e-5
+    quarter = orc.so1d_evolve(v, lo, hi, dt, psi0, 250)
+    phi = orc.fft1d(quarter)
+    k = orc.wavenumbers(n, lo, hi)
+    assert abs(np.sum(k * abs(phi) ** 2) / np.sum(abs(phi) ** 2) + 2.0) < 1e-4
+    assert abs(orc.mean_position_1d(quarter, lo, hi)) < 1e-4
+    full = orc.so1d_evolve(v, lo, hi, dt, psi0, 1000)
+    assert abs(orc.mean_energy_1d(full, lo, hi, v) - 2.5) < 1e-4
+    exact = 2.0 * math.cos(1.0)
+    e_coarse = abs(orc.mean_position_1d(orc.so1d_evolve(v, lo, hi, 0.1, psi0, 10), lo, hi) - exact)
+    e_fine = abs(orc.mean_position_1d(orc.so1d_evolve(v, lo, hi, 0.05, psi0, 20), lo, hi) - exact)
+    assert e_coarse > 1e-9
+    assert 3.4 < e_coarse / e_fine < 4.6
 
 
 def test_3d_free_dispersion_exact_at_dt1(orc):
@@ -247,11 +276,18 @@ def test_barrier_half_open_and_coulomb(orc):
 
 
 def test_slab_probability_gaussian_tail(orc):
-    """P(x >= a) of a normalised Gaussian is erfc((a - x0) / (sqrt(2) sigma)) / 2."""
+    """P(x >= a) of a normalised Gaussian: the half-open sum over grid points is
+    the midpoint rule from the first included point minus h/2, i.e.
+    erfc((x_first - h/2 - x0) / (sqrt(2) sigma)) / 2."""
     dims, lo, hi = (256, 8, 8), (-20.0, -8.0, -8.0), (20.0, 8.0, 8.0)
     psi = orc.gaussian_3d(dims, lo, hi, (0.3, 0, 0), (1.2, 1.0, 1.0), (0, 0, 0))
-    p = orc.probability_in_slab_3d(psi, dims, lo, hi, 0, 1.0, 1e300)
-    assert abs(p - 0.5 * math.erfc((1.0 - 0.3) / (math.sqrt(2) * 1.2))) < 2e-3
+    h = 40.0 / 256
+    for a in (1.0, 0.3, -2.0):
+        first = lo[0] + math.ceil((a - lo[0]) / h - 1e-12) * h
+        want = 0.5 * math.erfc((first - 0.5 * h - 0.3) / (math.sqrt(2) * 1.2))
+        got = orc.probability_in_slab_3d(psi, dims, lo, hi, 0, a, 1e300)
+        assert abs(got - want) < 2e-4
+    assert abs(orc.probability_in_slab_3d(psi, dims, lo, hi, 0, -1e300, 1e300) - 1) < 1e-12
 
 
 # ---- bitwise against the reference build -------------------------------------
