@@ -153,6 +153,18 @@ def main():
             f"wall {args.steps / wall_s:.1f} steps/s)")
         props[prec] = p
 
+    # the FP64 "baseline-equivalent": the reference CPU data flow mapped one
+    # pass per operation onto the GPU (unfused, full complex128 tables)
+    if not args.no_fp64:
+        pb = t.Propagator(grid, v, solve.dt, precision="fp64", mask=mask, unfused_baseline=True)
+        pb.upload(psi0)
+        bsteps = max(10, args.steps // 5)
+        gpu_s, wall_s = time_steps(hip, pb, bsteps, max(2, args.warmup // 5))
+        result["fp64_unfused"] = {"steps_per_s": bsteps / gpu_s, "ms_per_step": 1e3 * gpu_s / bsteps,
+                                  "step_bytes": pb.step_bytes()}
+        log(f"fp64 unfused baseline: {bsteps / gpu_s:.1f} steps/s ({1e3 * gpu_s / bsteps:.4f} ms/step)")
+        pb.close()
+
     # roofline of the dominant sweep (the x sweep: IFFT, kick*mask, FFT)
     p32 = props["fp32"]
     prof = p32.profile(max(8, min(args.steps, 200)))
@@ -223,6 +235,11 @@ def main():
                    "V0": solve.params["V0"], "kx": solve.k0[0], "mask_width": solve.mask_width,
                    "seed": args.seed},
         "fp64_steps_per_s": round(result["fp64"]["steps_per_s"], 2) if "fp64" in result else None,
+        "fp64_unfused_baseline_steps_per_s": (round(result["fp64_unfused"]["steps_per_s"], 2)
+                                              if "fp64_unfused" in result else None),
+        "fp32_over_fp64_unfused": (round(result["fp32"]["steps_per_s"]
+                                         / result["fp64_unfused"]["steps_per_s"], 2)
+                                   if "fp64_unfused" in result else None),
         "wall_steps_per_s": round(result["fp32"]["wall_steps_per_s"], 2),
         "accuracy": accuracy,
         "roofline": roofline,

@@ -54,6 +54,15 @@ extern "C" {
 #define TDSE_FP32 0 /* complex64 state, FP64 phase arguments and reductions */
 #define TDSE_FP64 1 /* complex128 state */
 
+/* Plan flags.
+ * TDSE_FLAG_UNFUSED_BASELINE: FP64, 2D/3D only.  Runs the reference CPU data
+ * flow one GPU pass per operation (full complex128 half-kick and kinetic
+ * tables, per-axis forward FFT sweeps, conjugation-identity inverse with
+ * separate conj passes, separate mask pass; core/src/propagator.ixx:110-119,
+ * core/src/fft.ixx:102-188).  It is the unfused comparison point of the
+ * fused path, not a production mode. */
+#define TDSE_FLAG_UNFUSED_BASELINE 1u
+
 typedef struct tdse_ctx tdse_ctx;
 typedef struct tdse_plan tdse_plan;
 
@@ -76,7 +85,7 @@ int tdse_plan_bytes(const int n[3], int precision, int with_mask, uint64_t* out)
 /* A propagator for one grid: replaces the SplitOperator1D/3D constructor
  * (core/src/propagator.ixx:23, 72).  lo/hi are the periodic box bounds per
  * axis (Grid1D::xmin/xmax), mass the effective mass (default 1.0 in the
- * reference), flags reserved (pass 0). */
+ * reference), flags 0 or TDSE_FLAG_UNFUSED_BASELINE. */
 int tdse_plan_create(tdse_ctx* ctx, const int n[3], const double lo[3], const double hi[3],
                      double dt, double mass, int precision, unsigned flags, tdse_plan** out);
 int tdse_plan_destroy(tdse_plan* plan);

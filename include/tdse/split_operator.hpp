@@ -1,0 +1,259 @@
+// split_operator.hpp -- header-only C++ facade over the C ABI (include/tdse.h)
+// with the reference core's names and signatures, so reference-style call
+// sites compile unchanged in shape:
+//
+//   ses::Grid1D / Grid3D                 core/src/grid.ixx:10-30
+//   ses::Field1D / Field3D               core/src/field.ixx:11-66
+//   ses::SplitOperator1D(g, V, dt)       core/src/propagator.ixx:21-64
+//   ses::SplitOperator3D(g, V, dt, m)    core/src/propagator.ixx:66-134
+//   ses::WavepacketSimulation            core/src/simulation.ixx:22-88
+//
+// step(Field&, n) uploads the field, runs n Strang steps on the GPU and reads
+// it back (the reference's in-place contract).  Device-resident stepping is
+// tdse::Propagator.  Unsupported sizes throw std::invalid_argument like the
+// reference (core/src/fft.ixx:42-44); device errors throw std::runtime_error.
+#pragma once
+
+#include <complex>
+#include <cstdint>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../tdse.h"
+
+namespace tdse {
+
+inline void check(int status) {
+    if (status == TDSE_OK) return;
+    std::string msg = tdse_last_error();
+    if (msg.empty()) msg = tdse_status_string(status);
+    if (status == TDSE_E_INVALID || status == TDSE_E_UNSUPPORTED) throw std::invalid_argument(msg);
+    throw std::runtime_error(msg);
+}
+
+struct Grid1D {
+    double xmin{};
+    double xmax{};
+    int n{};
+    constexpr int size() const noexcept { return n; }
+    constexpr double spacing() const noexcept { return (xmax - xmin) / n; }
+    constexpr double coord(int i) const noexcept { return xmin + i * spacing(); }
+};
+
+struct Grid3D {
+    Grid1D x{}, y{}, z{};
+    constexpr int size() const noexcept { return x.n * y.n * z.n; }
+    constexpr int flat(int i, int j, int k) const noexcept { return i + x.n * (j + y.n * k); }
+    constexpr double cell_volume() const noexcept { return x.spacing() * y.spacing() * z.spacing(); }
+};
+
+struct Vec3d {
+    double x{}, y{}, z{};
+};
+
+class Field1D {
+public:
+    explicit Field1D(Grid1D g) : grid_(g), data_(static_cast<std::size_t>(g.n)) {}
+    int size() const noexcept { return grid_.n; }
+    const Grid1D& grid() const noexcept { return grid_; }
+    std::complex<double>& operator[](int i) noexcept { return data_[static_cast<std::size_t>(i)]; }
+    const std::complex<double>& operator[](int i) const noexcept {
+        return data_[static_cast<std::size_t>(i)];
+    }
+    std::vector<std::complex<double>>& data() noexcept { return data_; }
+    const std::vector<std::complex<double>>& data() const noexcept { return data_; }
+
+private:
+    Grid1D grid_;
+    std::vector<std::complex<double>> data_;
+};
+
+class Field3D {
+public:
+    explicit Field3D(Grid3D g) : grid_(g), data_(static_cast<std::size_t>(g.size())) {}
+    int size() const noexcept { return grid_.size(); }
+    const Grid3D& grid() const noexcept { return grid_; }
+    std::complex<double>& operator()(int i, int j, int k) noexcept {
+        return data_[static_cast<std::size_t>(grid_.flat(i, j, k))];
+    }
+    const std::complex<double>& operator()(int i, int j, int k) const noexcept {
+        return data_[static_cast<std::size_t>(grid_.flat(i, j, k))];
+    }
+    std::vector<std::complex<double>>& data() noexcept { return data_; }
+    const std::vector<std::complex<double>>& data() const noexcept { return data_; }
+
+private:
+    Grid3D grid_;
+    std::vector<std::complex<double>> data_;
+};
+
+// A device and its HBM budget (0 = none).  One default context per process.
+class Context {
+public:
+    explicit Context(int device = 0, std::uint64_t budget = 0) {
+        tdse_ctx* c = nullptr;
+        check(tdse_ctx_create(device, budget, &c));
+        ctx_.reset(c);
+    }
+    tdse_ctx* get() const noexcept { return ctx_.get(); }
+    static Context& instance() {
+        static Context c;
+        return c;
+    }
+
+private:
+    struct Del {
+        void operator()(tdse_ctx* c) const noexcept { tdse_ctx_destroy(c); }
+    };
+    std::unique_ptr<tdse_ctx, Del> ctx_;
+};
+
+enum class Precision : int { FP32 = TDSE_FP32, FP64 = TDSE_FP64 };
+
+// Device-resident propagator (one plan): the state lives in HBM between calls.
+class Propagator {
+public:
+    Propagator(const Grid3D& g, const std::vector<double>& potential, double dt, double mass = 1.0,
+               Precision prec = Precision::FP64, Context& ctx = Context::instance()) {
+        const int n[3] = {g.x.n, g.y.n, g.z.n};
+        const double lo[3] = {g.x.xmin, g.y.xmin, g.z.xmin};
+        const double hi[3] = {g.x.xmax, g.y.xmax, g.z.xmax};
+        if (static_cast<int>(potential.size()) != g.size())
+            throw std::invalid_argument("potential size does not match the grid");
+        tdse_plan* p = nullptr;
+        check(tdse_plan_create(ctx.get(), n, lo, hi, dt, mass, static_cast<int>(prec), 0u, &p));
+        plan_.reset(p);
+        size_ = g.size();
+        check(tdse_plan_set_potential(plan_.get(), potential.data()));
+    }
+    void set_mask(const std::vector<double>& m) { check(tdse_plan_set_mask(plan_.get(), m.data())); }
+    void clear_mask() { check(tdse_plan_set_mask(plan_.get(), nullptr)); }
+    void upload(const std::vector<std::complex<double>>& psi) {
+        check(tdse_state_upload(plan_.get(), reinterpret_cast<const double*>(psi.data())));
+    }
+    void download(std::vector<std::complex<double>>& psi) const {
+        psi.resize(static_cast<std::size_t>(size_));
+        check(tdse_state_download(plan_.get(), reinterpret_cast<double*>(psi.data())));
+    }
+    void step(int nsteps) { check(tdse_step(plan_.get(), nsteps)); }
+    void sync(int timeout_ms = -1) { check(tdse_plan_sync(plan_.get(), timeout_ms)); }
+    tdse_observables observe(unsigned what = TDSE_OBS_NORM | TDSE_OBS_POSITION | TDSE_OBS_ENERGY) {
+        tdse_observables o{};
+        check(tdse_observe(plan_.get(), what, &o));
+        return o;
+    }
+    tdse_plan* get() const noexcept { return plan_.get(); }
+
+private:
+    struct Del {
+        void operator()(tdse_plan* p) const noexcept { tdse_plan_destroy(p); }
+    };
+    std::unique_ptr<tdse_plan, Del> plan_;
+    int size_ = 0;
+};
+
+inline Grid3D as_grid3(const Grid1D& g) { return Grid3D{g, Grid1D{0.0, 1.0, 1}, Grid1D{0.0, 1.0, 1}}; }
+
+// ses::SplitOperator1D (core/src/propagator.ixx:21-64)
+class SplitOperator1D {
+public:
+    SplitOperator1D(const Grid1D& g, const std::vector<double>& potential, double dt,
+                    Precision prec = Precision::FP64)
+        : dt_(dt), prop_(std::make_unique<Propagator>(as_grid3(g), potential, dt, 1.0, prec)) {}
+    double dt() const noexcept { return dt_; }
+    void step(Field1D& psi, int nsteps = 1) const {
+        if (nsteps <= 0) return;
+        prop_->upload(psi.data());
+        prop_->step(nsteps);
+        prop_->download(psi.data());
+    }
+
+private:
+    double dt_;
+    std::unique_ptr<Propagator> prop_;
+};
+
+// ses::SplitOperator3D (core/src/propagator.ixx:66-134); 2D is nx x ny x 1.
+class SplitOperator3D {
+public:
+    SplitOperator3D(const Grid3D& g, const std::vector<double>& potential, double dt,
+                    double mass = 1.0, Precision prec = Precision::FP64)
+        : dt_(dt), prop_(std::make_unique<Propagator>(g, potential, dt, mass, prec)) {}
+    double dt() const noexcept { return dt_; }
+    void step(Field3D& psi, int nsteps = 1) const {
+        if (nsteps <= 0) return;
+        prop_->upload(psi.data());
+        prop_->step(nsteps);
+        prop_->download(psi.data());
+    }
+
+private:
+    double dt_;
+    std::unique_ptr<Propagator> prop_;
+};
+
+// core/src/wavepacket.ixx:27-46 (bit-identical builder in the library)
+inline Field3D gaussian_wavepacket(const Grid3D& g, Vec3d r0, Vec3d sigma, Vec3d k0) {
+    Field3D f{g};
+    const int n[3] = {g.x.n, g.y.n, g.z.n};
+    const double lo[3] = {g.x.xmin, g.y.xmin, g.z.xmin}, hi[3] = {g.x.xmax, g.y.xmax, g.z.xmax};
+    const double r[3] = {r0.x, r0.y, r0.z}, s[3] = {sigma.x, sigma.y, sigma.z},
+                 k[3] = {k0.x, k0.y, k0.z};
+    check(tdse_build_gaussian(n, lo, hi, r, s, k, reinterpret_cast<double*>(f.data().data())));
+    return f;
+}
+
+// ses::WavepacketSimulation (core/src/simulation.ixx:22-88): advance() keeps
+// the state on the GPU; psi() reads it back.
+class WavepacketSimulation {
+public:
+    struct Config {
+        Grid3D grid;
+        std::vector<double> potential;
+        Vec3d r0;
+        Vec3d sigma;
+        Vec3d k0;
+        double dt;
+    };
+    explicit WavepacketSimulation(Config cfg, Precision prec = Precision::FP64)
+        : grid_(cfg.grid), dt_(cfg.dt), potential_(std::move(cfg.potential)),
+          prop_(grid_, potential_, cfg.dt, 1.0, prec), psi_(grid_) {
+        psi_ = gaussian_wavepacket(grid_, cfg.r0, cfg.sigma, cfg.k0);
+        prop_.upload(psi_.data());
+    }
+    void advance(int nsteps) {
+        prop_.step(nsteps);
+        steps_ += nsteps;
+        stale_ = true;
+    }
+    double time() const noexcept { return steps_ * dt_; }
+    double dt() const noexcept { return dt_; }
+    const Grid3D& grid() const noexcept { return grid_; }
+    const std::vector<double>& potential() const noexcept { return potential_; }
+    const Field3D& psi() {
+        if (stale_) {
+            prop_.download(psi_.data());
+            stale_ = false;
+        }
+        return psi_;
+    }
+    void set_psi(const Field3D& psi) {
+        psi_ = psi;
+        prop_.upload(psi_.data());
+        stale_ = false;
+    }
+
+private:
+    Grid3D grid_;
+    double dt_;
+    std::vector<double> potential_;
+    Propagator prop_;
+    Field3D psi_;
+    long long steps_ = 0;
+    bool stale_ = false;
+};
+
+}  // namespace tdse

@@ -160,7 +160,7 @@ class Propagator:
     """Device-resident split-operator propagator for one grid (one C-ABI plan)."""
 
     def __init__(self, grid, potential, dt, mass=1.0, precision="fp64", mask=None,
-                 context: Context | None = None):
+                 context: Context | None = None, unfused_baseline: bool = False):
         self.grid = grid
         self.ctx = context or default_context()
         self.precision = PRECISIONS[precision]
@@ -174,7 +174,9 @@ class Propagator:
         try:
             capi.check(L.tdse_plan_create(self.ctx.handle, capi.ivec(n), capi.dvec(lo),
                                           capi.dvec(hi), C.c_double(dt), C.c_double(mass),
-                                          self.precision, 0, C.byref(h)))
+                                          self.precision,
+                                          capi.TDSE_FLAG_UNFUSED_BASELINE if unfused_baseline else 0,
+                                          C.byref(h)))
         except TdseError as e:
             raise _map_error(e) from None
         self._h = h

@@ -23,6 +23,8 @@ TDSE_E_STATE = -7
 TDSE_FP32 = 0
 TDSE_FP64 = 1
 
+TDSE_FLAG_UNFUSED_BASELINE = 1
+
 TDSE_OBS_NORM = 1
 TDSE_OBS_POSITION = 2
 TDSE_OBS_ENERGY = 4

@@ -132,7 +132,65 @@ scale_state(void* psi_v, long long total, double s) {
     }
 }
 
+// Unfused FP64 elementwise passes of the reference data flow (the baseline
+// the fused path is compared with): psi *= table, psi = conj(psi) * s,
+// psi *= mask.
+__global__ void __launch_bounds__(kBlock)
+table_mul(void* psi_v, const void* tab_v, long long total) {
+    cx<double>* psi = static_cast<cx<double>*>(psi_v);
+    const cx<double>* tab = static_cast<const cx<double>*>(tab_v);
+    const long long stride = static_cast<long long>(gridDim.x) * kBlock;
+    for (long long f = blockIdx.x * static_cast<long long>(kBlock) + threadIdx.x; f < total;
+         f += stride)
+        psi[f] = cmul(psi[f], tab[f]);
+}
+
+__global__ void __launch_bounds__(kBlock)
+conj_scale(void* psi_v, long long total, double s) {
+    cx<double>* psi = static_cast<cx<double>*>(psi_v);
+    const long long stride = static_cast<long long>(gridDim.x) * kBlock;
+    for (long long f = blockIdx.x * static_cast<long long>(kBlock) + threadIdx.x; f < total;
+         f += stride) {
+        const cx<double> z = psi[f];
+        psi[f] = {s * z.x, s * -z.y};
+    }
+}
+
+__global__ void __launch_bounds__(kBlock)
+mask_mul(void* psi_v, const double* mask, long long total) {
+    cx<double>* psi = static_cast<cx<double>*>(psi_v);
+    const long long stride = static_cast<long long>(gridDim.x) * kBlock;
+    for (long long f = blockIdx.x * static_cast<long long>(kBlock) + threadIdx.x; f < total;
+         f += stride)
+        psi[f] = cscale(psi[f], mask[f]);
+}
+
+int elementwise_blocks(long long total) {
+    long long b = (total + kBlock * 4 - 1) / (kBlock * 4);
+    if (b < 1) b = 1;
+    if (b > 16384) b = 16384;
+    return static_cast<int>(b);
+}
+
 }  // namespace
+
+hipError_t launch_table_mul(void* psi, const void* table, long long total, hipStream_t stream) {
+    hipLaunchKernelGGL(table_mul, dim3(elementwise_blocks(total)), dim3(kBlock), 0, stream, psi,
+                       table, total);
+    return hipGetLastError();
+}
+
+hipError_t launch_conj_scale(void* psi, long long total, double s, hipStream_t stream) {
+    hipLaunchKernelGGL(conj_scale, dim3(elementwise_blocks(total)), dim3(kBlock), 0, stream, psi,
+                       total, s);
+    return hipGetLastError();
+}
+
+hipError_t launch_mask_mul(void* psi, const double* mask, long long total, hipStream_t stream) {
+    hipLaunchKernelGGL(mask_mul, dim3(elementwise_blocks(total)), dim3(kBlock), 0, stream, psi,
+                       mask, total);
+    return hipGetLastError();
+}
 
 int reduce_blocks(long long total) {
     long long b = (total + kBlock * 8 - 1) / (kBlock * 8);

@@ -29,5 +29,9 @@ int reduce_blocks(long long total);
 hipError_t launch_reduce(bool fp64, bool kspace, const ReduceArgs& a, double* partials,
                          double* out, hipStream_t stream);
 hipError_t launch_scale(bool fp64, void* psi, long long total, double s, hipStream_t stream);
+// FP64 elementwise passes of the unfused baseline
+hipError_t launch_table_mul(void* psi, const void* table, long long total, hipStream_t stream);
+hipError_t launch_conj_scale(void* psi, long long total, double s, hipStream_t stream);
+hipError_t launch_mask_mul(void* psi, const double* mask, long long total, hipStream_t stream);
 
 }  // namespace tdse

@@ -98,6 +98,10 @@ struct tdse_plan {
     long long kt_len[3] = {0, 0, 0};
     bool has_potential = false;
     uint64_t bytes = 0;
+    unsigned flags = 0;
+    // unfused FP64 baseline: full complex128 tables of the reference propagator
+    void* base_half = nullptr;  // exp(-i V dt / 2) per cell
+    void* base_kin = nullptr;   // exp(-i |k|^2 dt / 2m) per cell
 
     size_t csize() const { return precision == TDSE_FP64 ? 16 : 8; }
     size_t rsize() const { return precision == TDSE_FP64 ? 8 : 4; }
@@ -262,6 +266,29 @@ int make_kinetic(tdse_plan* p, int slot, const Axis& ax) {
     return TDSE_OK;
 }
 
+// The reference SplitOperator3D tables (core/src/propagator.ixx:72-101):
+// full N-entry complex128 kinetic table over the three original axes.
+int make_baseline_kinetic(tdse_plan* p) {
+    std::vector<double> k[3];
+    for (int a = 0; a < 3; ++a) {
+        k[a].resize(p->n[a]);
+        const double dk = 2.0 * 3.14159265358979323846 / (p->hi[a] - p->lo[a]);
+        for (int j = 0; j < p->n[a]; ++j) k[a][j] = dk * ((2 * j < p->n[a]) ? j : j - p->n[a]);
+    }
+    std::vector<double> t(2 * static_cast<size_t>(p->total));
+    for (int kk = 0; kk < p->n[2]; ++kk)
+        for (int j = 0; j < p->n[1]; ++j)
+            for (int i = 0; i < p->n[0]; ++i) {
+                const double kx = k[0][i], ky = k[1][j], kz = k[2][kk];
+                const double th = -0.5 * (kx * kx + ky * ky + kz * kz) / p->mass * p->dt;
+                const long long f = i + static_cast<long long>(p->n[0]) * (j + static_cast<long long>(p->n[1]) * kk);
+                t[2 * f] = std::cos(th);
+                t[2 * f + 1] = std::sin(th);
+            }
+    TDSE_HIP(hipMemcpy(p->base_kin, t.data(), t.size() * 8, hipMemcpyHostToDevice));
+    return TDSE_OK;
+}
+
 SweepArgs base_args(const tdse_plan* p) {
     SweepArgs a{};
     a.psi = p->psi;
@@ -408,6 +435,36 @@ int run_launches(tdse_plan* p, const std::vector<Launch>& L) {
     return TDSE_OK;
 }
 
+// The reference data flow one pass per operation, FP64
+// (core/src/propagator.ixx:110-119, core/src/fft.ixx:102-188): table multiply,
+// per-axis forward line FFTs, table multiply, conj, forward FFTs, conj * 1/N,
+// table multiply, then the caller's mask.
+int run_baseline(tdse_plan* p, int nsteps) {
+    Geometry g = geometry(p, p->psi);
+    std::vector<Launch> fft;
+    auto add = [&](int mode, int n, const SweepArgs& a) {
+        fft.push_back({mode, n, a, launcher(p, n, mode), 0});
+    };
+    add(tdse::C_FWD, p->sq[0].n, g.x);
+    add(tdse::S_FWD, p->sq[1].n, g.yf);
+    if (p->kind == kKind3D) add(tdse::S_FWD, p->sq[2].n, g.z);
+    const double inv = 1.0 / static_cast<double>(p->total);
+    for (int s = 0; s < nsteps; ++s) {
+        TDSE_HIP(tdse::launch_table_mul(p->psi, p->base_half, p->total, p->stream));
+        int rc = run_launches(p, fft);
+        if (rc) return rc;
+        TDSE_HIP(tdse::launch_table_mul(p->psi, p->base_kin, p->total, p->stream));
+        TDSE_HIP(tdse::launch_conj_scale(p->psi, p->total, 1.0, p->stream));
+        if ((rc = run_launches(p, fft))) return rc;
+        TDSE_HIP(tdse::launch_conj_scale(p->psi, p->total, inv, p->stream));
+        TDSE_HIP(tdse::launch_table_mul(p->psi, p->base_half, p->total, p->stream));
+        if (p->mask)
+            TDSE_HIP(tdse::launch_mask_mul(p->psi, static_cast<const double*>(p->mask), p->total,
+                                           p->stream));
+    }
+    return TDSE_OK;
+}
+
 int check_plan(const tdse_plan* p) {
     if (!p) return fail(TDSE_E_INVALID, "null plan");
     return TDSE_OK;
@@ -483,8 +540,10 @@ int tdse_plan_bytes(const int n[3], int precision, int with_mask, uint64_t* out)
 
 int tdse_plan_create(tdse_ctx* ctx, const int n[3], const double lo[3], const double hi[3],
                      double dt, double mass, int precision, unsigned flags, tdse_plan** out) {
-    (void)flags;
     if (!ctx || !n || !lo || !hi || !out) return fail(TDSE_E_INVALID, "null argument");
+    if (flags & ~TDSE_FLAG_UNFUSED_BASELINE) return fail(TDSE_E_INVALID, "unknown flags");
+    if ((flags & TDSE_FLAG_UNFUSED_BASELINE) && precision != TDSE_FP64)
+        return fail(TDSE_E_INVALID, "the unfused baseline is FP64 only");
     *out = nullptr;
     if (precision != TDSE_FP32 && precision != TDSE_FP64)
         return fail(TDSE_E_INVALID, "precision must be TDSE_FP32 or TDSE_FP64");
@@ -511,11 +570,16 @@ int tdse_plan_create(tdse_ctx* ctx, const int n[3], const double lo[3], const do
     p->dt = dt;
     p->mass = mass;
     p->total = total;
+    p->flags = flags;
     std::vector<int> lines;
     rc = classify(n, p->sq, lo, hi, &p->kind, &p->n1, &p->n2, &lines);
     if (rc) {
         delete p;
         return rc;
+    }
+    if ((flags & TDSE_FLAG_UNFUSED_BASELINE) && p->kind != kKind3D && p->kind != kKind2D) {
+        delete p;
+        return fail(TDSE_E_UNSUPPORTED, "the unfused baseline covers 2D and 3D grids");
     }
     auto bail = [&](int code) {
         tdse_plan_destroy(p);
@@ -539,6 +603,11 @@ int tdse_plan_create(tdse_ctx* ctx, const int n[3], const double lo[3], const do
         if ((rc = make_twiddles(p, N))) return bail(rc);
     for (size_t a = 0; a < p->sq.size(); ++a)
         if ((rc = make_kinetic(p, static_cast<int>(a), p->sq[a]))) return bail(rc);
+    if (flags & TDSE_FLAG_UNFUSED_BASELINE) {
+        if ((rc = alloc(p, &p->base_half, total * 16))) return bail(rc);
+        if ((rc = alloc(p, &p->base_kin, total * 16))) return bail(rc);
+        if ((rc = make_baseline_kinetic(p))) return bail(rc);
+    }
     if (hipMemset(p->psi, 0, total * p->csize()) != hipSuccess ||
         hipMemset(p->phi, 0, total * p->rsize()) != hipSuccess ||
         hipMemset(p->v, 0, total * p->rsize()) != hipSuccess) {
@@ -559,6 +628,8 @@ int tdse_plan_destroy(tdse_plan* p) {
     release(p, p->v, p->total * rs);
     release(p, p->mask, p->total * rs);
     release(p, p->scratch, p->total * cs);
+    release(p, p->base_half, p->total * 16);
+    release(p, p->base_kin, p->total * 16);
     void* part = p->partials;
     release(p, part, static_cast<uint64_t>(tdse::kMaxReduceBlocks) * tdse::kReduceQ * 8 + 64 * 8);
     for (auto& kv : p->tw) release(p, kv.second, static_cast<uint64_t>(kv.first) * 16);
@@ -583,6 +654,14 @@ int tdse_plan_set_potential(tdse_plan* p, const double* v) {
     // take |angle| <= 8)
     if ((rc = upload_real(p, p->phi, th.data(), true))) return rc;
     if ((rc = upload_real(p, p->v, v, false))) return rc;
+    if (p->base_half) {
+        std::vector<double> h(2 * static_cast<size_t>(p->total));
+        for (long long i = 0; i < p->total; ++i) {
+            h[2 * i] = std::cos(th[i]);
+            h[2 * i + 1] = std::sin(th[i]);
+        }
+        TDSE_HIP(hipMemcpy(p->base_half, h.data(), h.size() * 8, hipMemcpyHostToDevice));
+    }
     p->has_potential = true;
     return TDSE_OK;
 }
@@ -659,6 +738,7 @@ int tdse_step(tdse_plan* p, int nsteps) {
     if (nsteps == 0) return TDSE_OK;
     if (!p->has_potential) return fail(TDSE_E_STATE, "tdse_plan_set_potential not called");
     if ((rc = set_device(p->ctx))) return rc;
+    if (p->flags & TDSE_FLAG_UNFUSED_BASELINE) return run_baseline(p, nsteps);
     return run_launches(p, schedule(p, nsteps));
 }
 
@@ -805,6 +885,8 @@ int tdse_profile_step(tdse_plan* p, int nsteps, int max_slots, double* ms_per_sw
     if (rc) return rc;
     if (nsteps < 1 || !ms_per_sweep || !launches || !n_slots)
         return fail(TDSE_E_INVALID, "bad profiling arguments");
+    if (p->flags & TDSE_FLAG_UNFUSED_BASELINE)
+        return fail(TDSE_E_UNSUPPORTED, "per-sweep profiling covers the fused schedule");
     if (!p->has_potential) return fail(TDSE_E_STATE, "tdse_plan_set_potential not called");
     if ((rc = set_device(p->ctx))) return rc;
     const std::vector<Launch> L = schedule(p, nsteps);
@@ -859,6 +941,12 @@ int tdse_plan_step_bytes(const tdse_plan* p, uint64_t* out) {
     if (p->kind == kKind3D) sweeps = 4;
     else if (p->kind == kKind2D || p->kind == kKindFourStep) sweeps = 2;
     const uint64_t N = static_cast<uint64_t>(p->total);
+    if (p->flags & TDSE_FLAG_UNFUSED_BASELINE) {
+        // 3 table passes (48 B), 2 x (axes) FFT sweeps (32 B), 2 conj passes (32 B), mask (40 B)
+        const int axes = p->kind == kKind3D ? 3 : 2;
+        *out = N * (3 * 48 + 2 * axes * 32 + 2 * 32 + (p->mask ? 40 : 0));
+        return TDSE_OK;
+    }
     *out = N * (sweeps * 2 * p->csize() + p->rsize() + (p->mask ? p->rsize() : 0));
     return TDSE_OK;
 }

@@ -116,3 +116,24 @@ def test_wavenumbers_match_oracle(tdse, orc):
     for n, lo, hi in [(8, 0.0, 2 * np.pi), (65536, -100.0, 100.0), (96, -3.0, 5.0), (1, 0.0, 1.0)]:
         np.testing.assert_allclose(tdse.wavenumbers(tdse.Grid1D(lo, hi, n)),
                                    orc.wavenumbers(n, lo, hi), rtol=1e-15, atol=0)
+
+
+def build_facade(tmp_path):
+    """Compile the reference-style C++ call site against the facade."""
+    import subprocess
+    lib_dir = os.path.join(ROOT, "shoeridingerequationsolver_amd", "_lib")
+    exe = str(tmp_path / "facade_main")
+    cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-I", os.path.join(ROOT, "include"),
+           os.path.join(ROOT, "tests", "cpp", "facade_main.cpp"), "-L", lib_dir, "-ltdse",
+           f"-Wl,-rpath,{lib_dir}", "-o", exe]
+    subprocess.run(cmd, check=True)
+    return exe
+
+
+def test_cpp_facade_compiles_and_reports_no_device(tdse, tmp_path):
+    import subprocess
+    exe = build_facade(tmp_path)
+    if os.path.exists("/dev/kfd") and os.access("/dev/kfd", os.R_OK):
+        pytest.skip("a GPU is visible here; the GPU tier runs the facade")
+    r = subprocess.run([exe], capture_output=True, text=True)
+    assert r.returncode == 77, r.stdout + r.stderr

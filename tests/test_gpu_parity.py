@@ -181,3 +181,19 @@ def test_unsupported_lengths_raise(tdse):
             tdse.SplitOperator3D(g, np.zeros(int(np.prod(dims))), 0.1)
     with pytest.raises(ValueError):
         tdse.SplitOperator1D(tdse.Grid1D(-1, 1, 1031), np.zeros(1031), 0.1)
+
+
+def test_unfused_baseline_matches_oracle(orc, tdse):
+    """The FP64 unfused baseline (reference data flow, one pass per operation)."""
+    dims, lo, hi = (32, 16, 16), (-12, -6, -6), (12, 6, 6)
+    v = orc.barrier_3d(dims, lo, hi, 1.5, 0.0, 2.0)
+    m = orc.absorbing_mask_3d(dims, lo, hi, 3.0)
+    psi0 = packet3(orc, dims, lo, hi, (-6.0, 0.0, 0.0), (1.5, 1.5, 1.5), (2.5, 0.0, 0.0))
+    want = orc.so3d_evolve(v, dims, lo, hi, 0.02, psi0, 30, mask=m)
+    p = tdse.Propagator(grid3(tdse, dims, lo, hi), v, 0.02, precision="fp64", mask=m,
+                        unfused_baseline=True)
+    p.upload(psi0)
+    p.step(30)
+    assert rel(p.download(), want) < FP64_TOL
+    with pytest.raises(ValueError):
+        tdse.Propagator(grid3(tdse, dims, lo, hi), v, 0.02, precision="fp32", unfused_baseline=True)
