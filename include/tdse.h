@@ -148,6 +148,23 @@ const char* tdse_plan_sweep_name(const tdse_plan* plan, int slot);
  * sweep, kick phase and mask once). */
 int tdse_plan_step_bytes(const tdse_plan* plan, uint64_t* out);
 
+/* Batches: many plans (any shapes, both precisions) advanced together.
+ * Plans that share a line length and sweep mode are swept by one batched
+ * launch; the launches of different plans overlap (each launch waits only for the sweeps of its
+ * own plans).  use_graph must be 0: replay of stream-captured hipGraphs is
+ * switched off (TDSE_E_UNSUPPORTED).  The batch snapshots each plan's buffers at
+ * creation: re-create it after tdse_plan_set_mask changes a plan's mask from
+ * null to non-null.  tdse_batch_step(b, n) == tdse_step(p, n) for every plan
+ * p of the batch (north-star batching; no reference counterpart). */
+typedef struct tdse_batch tdse_batch;
+int tdse_batch_create(tdse_ctx* ctx, tdse_plan* const* plans, int count, tdse_batch** out);
+int tdse_batch_destroy(tdse_batch* batch);
+int tdse_batch_step(tdse_batch* batch, int nsteps, int use_graph);
+int tdse_batch_sync(tdse_batch* batch, int timeout_ms);
+int tdse_batch_stream(tdse_batch* batch, void** out);
+/* Number of batched launches in one interior round (all sub-phases). */
+int tdse_batch_launches_per_round(const tdse_batch* batch, int* out);
+
 /* Host input builders, FP64, x-fastest, bit-identical to the reference
  * builders for the same arguments (n x 1 x 1 grids use the 1D forms).
  *   harmonic            0.5 w^2 |r - c|^2           core/src/potential.ixx:16-48

@@ -35,7 +35,7 @@ __all__ = [
     "Grid1D", "Grid3D", "wavenumbers", "Context", "Propagator", "SplitOperator1D",
     "SplitOperator3D", "WavepacketSimulation", "harmonic_potential", "barrier_potential",
     "regularized_coulomb_potential", "coulomb_floor_potential", "absorbing_mask",
-    "gaussian_wavepacket", "TDSE_FP32", "TDSE_FP64", "TdseError",
+    "gaussian_wavepacket", "TDSE_FP32", "TDSE_FP64", "TdseError", "Batch",
 ]
 
 PRECISIONS = {"fp32": TDSE_FP32, "fp64": TDSE_FP64, TDSE_FP32: TDSE_FP32, TDSE_FP64: TDSE_FP64}
@@ -267,6 +267,53 @@ class Propagator:
     def close(self):
         if getattr(self, "_h", None):
             capi.lib().tdse_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Many device-resident propagators advanced together (one C-ABI batch).
+
+    ``step(n)`` advances every member by n Strang steps, exactly as
+    ``Propagator.step(n)`` would one by one; members sharing a sweep shape are
+    swept by one batched launch.  Captured-graph replay is switched off."""
+
+    def __init__(self, props, context: Context | None = None):
+        self.props = list(props)
+        self.ctx = context or self.props[0].ctx
+        arr = (C.c_void_p * len(self.props))(*[p.handle for p in self.props])
+        h = C.c_void_p()
+        try:
+            capi.check(capi.lib().tdse_batch_create(self.ctx.handle, arr, len(self.props),
+                                                    C.byref(h)))
+        except TdseError as e:
+            raise _map_error(e) from None
+        self._h = h
+
+    def step(self, nsteps, use_graph=False):
+        capi.check(capi.lib().tdse_batch_step(self._h, int(nsteps), int(bool(use_graph))))
+
+    def sync(self, timeout_ms=-1):
+        capi.check(capi.lib().tdse_batch_sync(self._h, int(timeout_ms)))
+
+    def stream(self):
+        s = C.c_void_p()
+        capi.check(capi.lib().tdse_batch_stream(self._h, C.byref(s)))
+        return s.value
+
+    def launches_per_round(self):
+        v = C.c_int()
+        capi.check(capi.lib().tdse_batch_launches_per_round(self._h, C.byref(v)))
+        return v.value
+
+    def close(self):
+        if getattr(self, "_h", None):
+            capi.lib().tdse_batch_destroy(self._h)
             self._h = None
 
     def __del__(self):

@@ -27,6 +27,8 @@ DEVICE = ["--offload-arch=gfx950", "-Xclang", "-target-feature", "-Xclang", "-pa
 UNITS = [
     ("kernels_f32.hip", DEVICE),
     ("kernels_f64.hip", DEVICE),
+    ("kernels_f32_batch.hip", DEVICE),
+    ("kernels_f64_batch.hip", DEVICE),
     ("observe.hip", DEVICE),
     ("tdse_api.cpp", ["-x", "hip"] + DEVICE),
     ("builders.cpp", ["-ffp-contract=off"]),

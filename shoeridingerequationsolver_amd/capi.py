@@ -40,6 +40,8 @@ EXPORTS = (
     "tdse_profile_step", "tdse_plan_sweep_name", "tdse_plan_step_bytes",
     "tdse_build_harmonic", "tdse_build_barrier", "tdse_build_regularized_coulomb",
     "tdse_build_coulomb_floor", "tdse_build_absorbing_mask", "tdse_build_gaussian",
+    "tdse_batch_create", "tdse_batch_destroy", "tdse_batch_step", "tdse_batch_sync",
+    "tdse_batch_stream", "tdse_batch_launches_per_round",
 )
 
 
@@ -107,6 +109,13 @@ def lib() -> C.CDLL:
         L.tdse_build_barrier.argtypes = [_ip, _dp, _dp, C.c_double, C.c_double, C.c_double, _dp]
         L.tdse_build_absorbing_mask.argtypes = [_ip, _dp, _dp, C.c_double, _dp]
         L.tdse_build_gaussian.argtypes = [_ip, _dp, _dp, _dp, _dp, _dp, _dp]
+        L.tdse_batch_create.argtypes = [C.c_void_p, C.POINTER(C.c_void_p), C.c_int,
+                                        C.POINTER(C.c_void_p)]
+        L.tdse_batch_destroy.argtypes = [C.c_void_p]
+        L.tdse_batch_step.argtypes = [C.c_void_p, C.c_int, C.c_int]
+        L.tdse_batch_sync.argtypes = [C.c_void_p, C.c_int]
+        L.tdse_batch_stream.argtypes = [C.c_void_p, C.POINTER(C.c_void_p)]
+        L.tdse_batch_launches_per_round.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
         _LIB = L
     return _LIB
 

@@ -8,6 +8,7 @@
 // coord(i) = lo + i * spacing.
 #include <cmath>
 #include <cstddef>
+#include <vector>
 
 #include "../../include/tdse.h"
 
@@ -167,14 +168,19 @@ int tdse_build_gaussian(const int n[3], const double lo[3], const double hi[3], 
         for (int i = 0; i < 2 * n[0]; ++i) psi[i] = inv * psi[i];
         return TDSE_OK;
     }
+    // envelope() is a pure function of one coordinate: evaluate it once per
+    // grid line index (same doubles as evaluating it per cell)
+    std::vector<double> ex(n[0]), ey(n[1]), ez(n[2]);
+    for (int i = 0; i < n[0]; ++i) ex[i] = envelope(coord(lo[0], hi[0], n[0], i), r0[0], sigma[0]);
+    for (int j = 0; j < n[1]; ++j) ey[j] = envelope(coord(lo[1], hi[1], n[1], j), r0[1], sigma[1]);
+    for (int k = 0; k < n[2]; ++k) ez[k] = envelope(coord(lo[2], hi[2], n[2], k), r0[2], sigma[2]);
     for (int k = 0; k < n[2]; ++k)
         for (int j = 0; j < n[1]; ++j)
             for (int i = 0; i < n[0]; ++i) {
                 const double x = coord(lo[0], hi[0], n[0], i);
                 const double y = coord(lo[1], hi[1], n[1], j);
                 const double z = coord(lo[2], hi[2], n[2], k);
-                const double env =
-                    envelope(x, r0[0], sigma[0]) * envelope(y, r0[1], sigma[1]) * envelope(z, r0[2], sigma[2]);
+                const double env = ex[i] * ey[j] * ez[k];
                 const double phase = k0[0] * x + k0[1] * y + k0[2] * z;
                 const long long f = flat(n, i, j, k);
                 psi[2 * f] = env * std::cos(phase);

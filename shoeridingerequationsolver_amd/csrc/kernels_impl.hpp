@@ -13,7 +13,7 @@ hipError_t launch_sweep(const SweepArgs& a, int max_blocks, hipStream_t stream) 
     using S = Shape<T, N>;
     long long tiles;
     int block;
-    if constexpr (MODE <= C_SINGLE || MODE == C_FWD) {
+    if constexpr (mode_contiguous(MODE)) {
         tiles = (a.nlines + S::LPB - 1) / S::LPB;
         block = S::BLOCK_C;
     } else {
@@ -22,12 +22,45 @@ hipError_t launch_sweep(const SweepArgs& a, int max_blocks, hipStream_t stream) 
     }
     if (tiles <= 0) return hipSuccess;
     const int grid = static_cast<int>(tiles < max_blocks ? tiles : max_blocks);
-    if constexpr (MODE <= C_SINGLE || MODE == C_FWD) {
+    if constexpr (mode_contiguous(MODE)) {
         hipLaunchKernelGGL((contig_sweep<T, N, MODE>), dim3(grid), dim3(block), 0, stream, a);
     } else {
         hipLaunchKernelGGL((strided_sweep<T, N, MODE>), dim3(grid), dim3(block), 0, stream, a);
     }
     return hipGetLastError();
+}
+
+template <typename T, int N, int MODE>
+hipError_t launch_batch(const BatchTable& b, long long tiles, int max_blocks, hipStream_t stream) {
+    using S = Shape<T, N>;
+    if (tiles <= 0) return hipSuccess;
+    const int grid = static_cast<int>(tiles < max_blocks ? tiles : max_blocks);
+    if constexpr (mode_contiguous(MODE)) {
+        hipLaunchKernelGGL((contig_sweep_batch<T, N, MODE>), dim3(grid), dim3(S::BLOCK_C), 0,
+                           stream, b);
+    } else {
+        hipLaunchKernelGGL((strided_sweep_batch<T, N, MODE>), dim3(grid), dim3(S::BLOCK_S), 0,
+                           stream, b);
+    }
+    return hipGetLastError();
+}
+
+template <typename T, int N>
+BatchLaunchFn pick_batch_mode(int mode) {
+    switch (mode) {
+        case C_KICK_FIRST: return &launch_batch<T, N, C_KICK_FIRST>;
+        case C_KICK_MID: return &launch_batch<T, N, C_KICK_MID>;
+        case C_KICK_LAST: return &launch_batch<T, N, C_KICK_LAST>;
+        case C_KIN_TW: return &launch_batch<T, N, C_KIN_TW>;
+        case C_SINGLE: return &launch_batch<T, N, C_SINGLE>;
+        case S_FWD: return &launch_batch<T, N, S_FWD>;
+        case S_INV: return &launch_batch<T, N, S_INV>;
+        case S_KIN: return &launch_batch<T, N, S_KIN>;
+        case S_KICK_FIRST_TW: return &launch_batch<T, N, S_KICK_FIRST_TW>;
+        case S_KICK_MID_TW: return &launch_batch<T, N, S_KICK_MID_TW>;
+        case S_KICK_LAST: return &launch_batch<T, N, S_KICK_LAST>;
+        default: return nullptr;
+    }
 }
 
 template <typename T, int N>
@@ -62,6 +95,7 @@ SweepShape shape_of(int mode) {
 #define TDSE_CAT2(a, b) a##b
 #define TDSE_CAT(a, b) TDSE_CAT2(a, b)
 
+#ifndef TDSE_BATCH_UNIT
 LaunchFn TDSE_CAT(find_launcher_, TDSE_SUFFIX)(int n, int mode) {
     switch (n) {
 #define TDSE_CASE(N_) \
@@ -74,6 +108,22 @@ LaunchFn TDSE_CAT(find_launcher_, TDSE_SUFFIX)(int n, int mode) {
     }
 }
 
+#else
+BatchLaunchFn TDSE_CAT(find_batch_launcher_, TDSE_SUFFIX)(int n, int mode) {
+    switch (n) {
+#define TDSE_CASE(N_) \
+    case N_:          \
+        return pick_batch_mode<TDSE_REAL, N_>(mode);
+        TDSE_FOR_EACH_N(TDSE_CASE)
+#undef TDSE_CASE
+        default:
+            return nullptr;
+    }
+}
+
+#endif
+
+#ifndef TDSE_BATCH_UNIT
 SweepShape TDSE_CAT(sweep_shape_, TDSE_SUFFIX)(int n, int mode) {
     switch (n) {
 #define TDSE_CASE(N_) \
@@ -85,5 +135,7 @@ SweepShape TDSE_CAT(sweep_shape_, TDSE_SUFFIX)(int n, int mode) {
             return {0, 0};
     }
 }
+
+#endif
 
 }  // namespace tdse

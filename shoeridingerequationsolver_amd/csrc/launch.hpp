@@ -15,12 +15,20 @@ struct SweepShape {
     int per_block;  // lines (contiguous) or x positions (strided) per block
 };
 
+// Launch one batched sweep: the tiles of every plan in `b` (same line length
+// and mode), `tiles` in total.
+using BatchLaunchFn = hipError_t (*)(const BatchTable& b, long long tiles, int max_blocks,
+                                     hipStream_t stream);
+
+BatchLaunchFn find_batch_launcher_f32(int n, int mode);
+BatchLaunchFn find_batch_launcher_f64(int n, int mode);
+
 // Lookup of the compiled kernel for (line length, mode); null if not compiled.
 LaunchFn find_launcher_f32(int n, int mode);
 LaunchFn find_launcher_f64(int n, int mode);
 SweepShape sweep_shape_f32(int n, int mode);
 SweepShape sweep_shape_f64(int n, int mode);
 
-inline bool mode_is_contiguous(int mode) { return mode <= C_SINGLE || mode == C_FWD; }
+inline bool mode_is_contiguous(int mode) { return mode_contiguous(mode); }
 
 }  // namespace tdse

@@ -261,160 +261,250 @@ struct Line {
     }
 };
 
-// ---- contiguous sweep -------------------------------------------------------------
+// ---- per-tile bodies ---------------------------------------------------------------
 
+// One tile of a contiguous sweep: LPB whole lines.  `lds` holds LPB padded
+// lines, `tw` the staged FP64 twiddles of this line length.
 template <typename T, int N, int MODE>
-__global__ void __launch_bounds__((Shape<T, N>::BLOCK_C), TDSE_MIN_WAVES)
-contig_sweep(SweepArgs a) {
+__device__ __forceinline__ void contig_tile(const SweepArgs& a, long long tile, cx<T>* lds,
+                                            const cx<double>* tw, int tid) {
     using S = Shape<T, N>;
     using LT = Line<T, N, ContigX<T>>;
     constexpr int TT = S::TT, LPB = S::LPB, E = S::E;
-
-    __shared__ cx<T> lds[S::LDS_C];
-    __shared__ cx<double> lds_tw[S::EXCHANGE ? N : 1];
-    const int tid = threadIdx.x;
     const int l = tid / TT, t = tid % TT;
-    if constexpr (S::EXCHANGE) {
-        stage_table<N>(lds_tw, a.tw, tid, S::BLOCK_C);
-        __syncthreads();
-    }
     ContigX<T> x{lds + l * S::LSTR};
     const Buf<cx<T>> P(a.psi, a.psi_bytes);
     const Buf<T> F(a.phi, a.real_bytes);
     const Buf<T> M(a.mask, a.mask ? a.real_bytes : 0u);
-    const long long ntiles = (a.nlines + LPB - 1) / LPB;
-
-    for (long long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-        const long long line = tile * LPB + l;
-        LT L{a, P, F, M, x, lds_tw, t, static_cast<uint32_t>(line * N), 1u, line < a.nlines};
-        if constexpr (MODE == C_KICK_FIRST) {
-            L.template load<LT::R1>();
-            L.kick(1.0, false);
-            L.fft();
-            L.template store<LT::RL>();
-        } else if constexpr (MODE == C_KICK_MID) {
-            L.template load<LT::RL>();
-            L.ifft();
-            L.kick(2.0, true);
-            L.fft();
-            L.template store<LT::RL>();
-        } else if constexpr (MODE == C_KICK_LAST) {
-            L.template load<LT::RL>();
-            L.ifft();
-            L.kick(1.0, true);
-            L.template store<LT::R1>();
-        } else if constexpr (MODE == C_FWD) {
-            L.template load<LT::R1>();
-            L.fft();
-            L.template store<LT::RL>();
-        } else if constexpr (MODE == C_KIN_TW) {
-            // row k2 = line; after the row FFT position k1 holds bin k2 + N2*k1
-            const cx<double>* kt = static_cast<const cx<double>*>(a.kt[2]);
-            L.template load<LT::R1>();
+    const long long line = tile * LPB + l;
+    LT L{a, P, F, M, x, tw, t, static_cast<uint32_t>(line * N), 1u, line < a.nlines};
+    if constexpr (MODE == C_KICK_FIRST) {
+        L.template load<LT::R1>();
+        L.kick(1.0, false);
+        L.fft();
+        L.template store<LT::RL>();
+    } else if constexpr (MODE == C_KICK_MID) {
+        L.template load<LT::RL>();
+        L.ifft();
+        L.kick(2.0, true);
+        L.fft();
+        L.template store<LT::RL>();
+    } else if constexpr (MODE == C_KICK_LAST) {
+        L.template load<LT::RL>();
+        L.ifft();
+        L.kick(1.0, true);
+        L.template store<LT::R1>();
+    } else if constexpr (MODE == C_FWD) {
+        L.template load<LT::R1>();
+        L.fft();
+        L.template store<LT::RL>();
+    } else if constexpr (MODE == C_KIN_TW) {
+        // row k2 = line; after the row FFT position k1 holds bin k2 + N2*k1
+        const cx<double>* kt = static_cast<const cx<double>*>(a.kt[2]);
+        L.template load<LT::R1>();
+        L.fft();
+        for_each_point<N, E, LT::RL>(t, [&](int s, int i, int, int) {
+            const int j = static_cast<int>(line) + a.n_rows * i;
+            L.v[s] = mul_d(L.v[s], cscale(kt[j], a.scale));
+        });
+        L.ifft();
+        for_each_point<N, E, LT::R1>(t, [&](int s, int i, int, int) {
+            L.v[s] = mul_d(L.v[s], step_twiddle<+1>(i, line, a.n_total));
+        });
+        L.template store<LT::R1>();
+    } else if constexpr (MODE == C_SINGLE) {
+        const cx<double>* kt = static_cast<const cx<double>*>(a.kt[2]);
+        L.template load<LT::R1>();
+        L.kick(1.0, false);
+        for (int step = 0; step < a.nsteps; ++step) {
             L.fft();
             for_each_point<N, E, LT::RL>(t, [&](int s, int i, int, int) {
-                const int j = static_cast<int>(line) + a.n_rows * i;
-                L.v[s] = mul_d(L.v[s], cscale(kt[j], a.scale));
+                L.v[s] = mul_d(L.v[s], cscale(kt[i], a.scale));
             });
             L.ifft();
-            for_each_point<N, E, LT::R1>(t, [&](int s, int i, int, int) {
-                L.v[s] = mul_d(L.v[s], step_twiddle<+1>(i, line, a.n_total));
-            });
-            L.template store<LT::R1>();
-        } else if constexpr (MODE == C_SINGLE) {
-            const cx<double>* kt = static_cast<const cx<double>*>(a.kt[2]);
-            L.template load<LT::R1>();
-            L.kick(1.0, false);
-            for (int step = 0; step < a.nsteps; ++step) {
-                L.fft();
-                for_each_point<N, E, LT::RL>(t, [&](int s, int i, int, int) {
-                    L.v[s] = mul_d(L.v[s], cscale(kt[i], a.scale));
-                });
-                L.ifft();
-                L.kick(step + 1 == a.nsteps ? 1.0 : 2.0, true);
-            }
-            L.template store<LT::R1>();
+            L.kick(step + 1 == a.nsteps ? 1.0 : 2.0, true);
         }
+        L.template store<LT::R1>();
     }
 }
 
-// ---- strided sweep ------------------------------------------------------------------
-
+// One tile of a strided sweep: TX consecutive lines.  `kp` holds the staged
+// line-axis kinetic table (times 1/N) for S_KIN.
 template <typename T, int N, int MODE>
-__global__ void __launch_bounds__((Shape<T, N>::BLOCK_S), TDSE_MIN_WAVES)
-strided_sweep(SweepArgs a) {
+__device__ __forceinline__ void strided_tile(const SweepArgs& a, long long tile, cx<T>* lds,
+                                             const cx<double>* tw, const cx<double>* kp, int tid) {
     using S = Shape<T, N>;
     constexpr int TX = S::TX, E = S::E;
     using LT = Line<T, N, StridedX<T, TX>>;
-    constexpr bool KIN = MODE == S_KIN;
-
-    __shared__ cx<T> lds[S::LDS_S];
-    __shared__ cx<double> lds_tw[S::EXCHANGE ? N : 1];
-    __shared__ cx<double> lds_kp[KIN ? N : 1];
-    const int tid = threadIdx.x;
     const int xi = tid % TX, t = tid / TX;
-    if constexpr (S::EXCHANGE) stage_table<N>(lds_tw, a.tw, tid, S::BLOCK_S);
-    if constexpr (KIN) {
-        // line-axis kinetic table with the inverse normalisation folded in
-        const cx<double>* kp = static_cast<const cx<double>*>(a.kt[2]);
-        for (int i = tid; i < N; i += S::BLOCK_S) lds_kp[i] = cscale(kp[i], a.scale);
-    }
-    if constexpr (S::EXCHANGE || KIN) __syncthreads();
     StridedX<T, TX> x{lds + xi};
     const Buf<cx<T>> P(a.psi, a.psi_bytes);
     const Buf<T> F(a.phi, a.real_bytes);
     const Buf<T> M(a.mask, a.mask ? a.real_bytes : 0u);
     const int xtiles = (a.nx + TX - 1) / TX;
-    const long long ntiles = static_cast<long long>(xtiles) * a.nw;
+    const int w = static_cast<int>(tile / xtiles);
+    const int xx = static_cast<int>(tile % xtiles) * TX + xi;
+    LT L{a, P, F, M, x, tw, t, static_cast<uint32_t>(xx) + a.wstride * w, a.estride,
+         xx < a.nx};
+    if constexpr (MODE == S_FWD) {
+        L.template load<LT::R1>();
+        L.fft();
+        L.template store<LT::RL>();
+    } else if constexpr (MODE == S_INV) {
+        L.template load<LT::RL>();
+        L.ifft();
+        L.template store<LT::R1>();
+    } else if constexpr (MODE == S_KIN) {
+        cx<double> kxw = {1.0, 0.0};
+        if (a.kt[0] && L.valid) kxw = static_cast<const cx<double>*>(a.kt[0])[xx];
+        if (a.kt[1]) kxw = cmul(kxw, static_cast<const cx<double>*>(a.kt[1])[w]);
+        L.template load<LT::R1>();
+        L.fft();
+        for_each_point<N, E, LT::RL>(t, [&](int s, int i, int, int) {
+            L.v[s] = mul_d(L.v[s], cmul(kxw, kp[i]));
+        });
+        L.ifft();
+        L.template store<LT::R1>();
+    } else if constexpr (MODE == S_KICK_FIRST_TW) {
+        L.template load<LT::R1>();
+        L.kick(1.0, false);
+        L.fft();
+        L.twiddle_out(xx);
+        L.template store<LT::RL>();
+    } else if constexpr (MODE == S_KICK_MID_TW) {
+        L.template load<LT::RL>();
+        L.ifft();
+        L.kick(2.0, true);
+        L.fft();
+        L.twiddle_out(xx);
+        L.template store<LT::RL>();
+    } else if constexpr (MODE == S_KICK_LAST) {
+        L.template load<LT::RL>();
+        L.ifft();
+        L.kick(1.0, true);
+        L.template store<LT::R1>();
+    } else if constexpr (MODE == S_FWD_TW) {
+        L.template load<LT::R1>();
+        L.fft();
+        L.twiddle_out(xx);
+        L.template store<LT::RL>();
+    }
+}
 
-    for (long long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-        const int w = static_cast<int>(tile / xtiles);
-        const int xx = static_cast<int>(tile % xtiles) * TX + xi;
-        LT L{a, P, F, M, x, lds_tw, t, static_cast<uint32_t>(xx) + a.wstride * w, a.estride,
-             xx < a.nx};
-        if constexpr (MODE == S_FWD) {
-            L.template load<LT::R1>();
-            L.fft();
-            L.template store<LT::RL>();
-        } else if constexpr (MODE == S_INV) {
-            L.template load<LT::RL>();
-            L.ifft();
-            L.template store<LT::R1>();
-        } else if constexpr (MODE == S_KIN) {
-            cx<double> kxw = {1.0, 0.0};
-            if (a.kt[0] && L.valid) kxw = static_cast<const cx<double>*>(a.kt[0])[xx];
-            if (a.kt[1]) kxw = cmul(kxw, static_cast<const cx<double>*>(a.kt[1])[w]);
-            L.template load<LT::R1>();
-            L.fft();
-            for_each_point<N, E, LT::RL>(t, [&](int s, int i, int, int) {
-                L.v[s] = mul_d(L.v[s], cmul(kxw, lds_kp[i]));
-            });
-            L.ifft();
-            L.template store<LT::R1>();
-        } else if constexpr (MODE == S_KICK_FIRST_TW) {
-            L.template load<LT::R1>();
-            L.kick(1.0, false);
-            L.fft();
-            L.twiddle_out(xx);
-            L.template store<LT::RL>();
-        } else if constexpr (MODE == S_KICK_MID_TW) {
-            L.template load<LT::RL>();
-            L.ifft();
-            L.kick(2.0, true);
-            L.fft();
-            L.twiddle_out(xx);
-            L.template store<LT::RL>();
-        } else if constexpr (MODE == S_KICK_LAST) {
-            L.template load<LT::RL>();
-            L.ifft();
-            L.kick(1.0, true);
-            L.template store<LT::R1>();
-        } else if constexpr (MODE == S_FWD_TW) {
-            L.template load<LT::R1>();
-            L.fft();
-            L.twiddle_out(xx);
-            L.template store<LT::RL>();
+template <typename T, int N>
+__device__ __forceinline__ void stage_kinetic(cx<double>* kp, const SweepArgs& a, int tid,
+                                              int nthreads) {
+    const cx<double>* src = static_cast<const cx<double>*>(a.kt[2]);
+    for (int i = tid; i < N; i += nthreads) kp[i] = cscale(src[i], a.scale);
+}
+
+// ---- single-plan kernels (grid-stride over tiles) -------------------------------------
+
+template <typename T, int N, int MODE>
+__global__ void __launch_bounds__((Shape<T, N>::BLOCK_C), TDSE_MIN_WAVES)
+contig_sweep(SweepArgs a) {
+    using S = Shape<T, N>;
+    __shared__ cx<T> lds[S::LDS_C];
+    __shared__ cx<double> lds_tw[S::EXCHANGE ? N : 1];
+    const int tid = threadIdx.x;
+    if constexpr (S::EXCHANGE) {
+        stage_table<N>(lds_tw, a.tw, tid, S::BLOCK_C);
+        __syncthreads();
+    }
+    const long long ntiles = (a.nlines + S::LPB - 1) / S::LPB;
+    for (long long tile = blockIdx.x; tile < ntiles; tile += gridDim.x)
+        contig_tile<T, N, MODE>(a, tile, lds, lds_tw, tid);
+}
+
+template <typename T, int N, int MODE>
+__global__ void __launch_bounds__((Shape<T, N>::BLOCK_S), TDSE_MIN_WAVES)
+strided_sweep(SweepArgs a) {
+    using S = Shape<T, N>;
+    constexpr bool KIN = MODE == S_KIN;
+    __shared__ cx<T> lds[S::LDS_S];
+    __shared__ cx<double> lds_tw[S::EXCHANGE ? N : 1];
+    __shared__ cx<double> lds_kp[KIN ? N : 1];
+    const int tid = threadIdx.x;
+    if constexpr (S::EXCHANGE) stage_table<N>(lds_tw, a.tw, tid, S::BLOCK_S);
+    if constexpr (KIN) stage_kinetic<T, N>(lds_kp, a, tid, S::BLOCK_S);
+    if constexpr (S::EXCHANGE || KIN) __syncthreads();
+    const long long ntiles = static_cast<long long>((a.nx + S::TX - 1) / S::TX) * a.nw;
+    for (long long tile = blockIdx.x; tile < ntiles; tile += gridDim.x)
+        strided_tile<T, N, MODE>(a, tile, lds, lds_tw, lds_kp, tid);
+}
+
+// ---- batched kernels: one launch over the tiles of many plans ------------------------
+
+// Plans that share a line length and mode are swept by one launch.  Item i
+// owns global tiles [first_tile[i], first_tile[i+1]); the last entry of
+// first_tile is the total.  All items' twiddle tables are equal (same N).
+struct BatchTable {
+    const SweepArgs* items;
+    const long long* first_tile;  // nitems + 1 entries
+    int nitems;
+};
+
+__device__ __forceinline__ int find_item(const BatchTable& b, long long tile) {
+    int lo = 0, hi = b.nitems - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (b.first_tile[mid] <= tile) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+template <typename T, int N, int MODE>
+__global__ void __launch_bounds__((Shape<T, N>::BLOCK_C), TDSE_MIN_WAVES)
+contig_sweep_batch(BatchTable b) {
+    using S = Shape<T, N>;
+    __shared__ cx<T> lds[S::LDS_C];
+    __shared__ cx<double> lds_tw[S::EXCHANGE ? N : 1];
+    const int tid = threadIdx.x;
+    if constexpr (S::EXCHANGE) {
+        stage_table<N>(lds_tw, b.items[0].tw, tid, S::BLOCK_C);
+        __syncthreads();
+    }
+    // contiguous chunk of tiles per block: consecutive tiles mostly share a plan
+    const long long total = b.first_tile[b.nitems];
+    const long long chunk = (total + gridDim.x - 1) / gridDim.x;
+    const long long g0 = blockIdx.x * chunk;
+    const long long g1 = g0 + chunk < total ? g0 + chunk : total;
+    for (long long g = g0; g < g1; ++g) {
+        const int it = find_item(b, g);
+        contig_tile<T, N, MODE>(b.items[it], g - b.first_tile[it], lds, lds_tw, tid);
+    }
+}
+
+template <typename T, int N, int MODE>
+__global__ void __launch_bounds__((Shape<T, N>::BLOCK_S), TDSE_MIN_WAVES)
+strided_sweep_batch(BatchTable b) {
+    using S = Shape<T, N>;
+    constexpr bool KIN = MODE == S_KIN;
+    __shared__ cx<T> lds[S::LDS_S];
+    __shared__ cx<double> lds_tw[S::EXCHANGE ? N : 1];
+    __shared__ cx<double> lds_kp[KIN ? N : 1];
+    const int tid = threadIdx.x;
+    if constexpr (S::EXCHANGE) {
+        stage_table<N>(lds_tw, b.items[0].tw, tid, S::BLOCK_S);
+        __syncthreads();
+    }
+    const long long total = b.first_tile[b.nitems];
+    const long long chunk = (total + gridDim.x - 1) / gridDim.x;
+    const long long g0 = blockIdx.x * chunk;
+    const long long g1 = g0 + chunk < total ? g0 + chunk : total;
+    int staged = -1;
+    for (long long g = g0; g < g1; ++g) {
+        const int it = find_item(b, g);
+        if constexpr (KIN) {
+            if (it != staged) {  // block-uniform: every thread sees the same tile
+                __syncthreads();
+                stage_kinetic<T, N>(lds_kp, b.items[it], tid, S::BLOCK_S);
+                __syncthreads();
+                staged = it;
+            }
         }
+        strided_tile<T, N, MODE>(b.items[it], g - b.first_tile[it], lds, lds_tw, lds_kp, tid);
     }
 }
 

@@ -152,3 +152,36 @@ def barrier(seed: int = 3, n: int = 256, steps: int = 2000, precision: str = "fp
     return Solve("barrier", (n, n, n), (-half,) * 3, (half,) * 3, "barrier",
                  {"V0": v0, "x_lo": 0.0, "x_hi": 2.0}, (-15.0, 0.0, 0.0), (3.0, 3.0, 3.0),
                  (kx, 0.0, 0.0), 0.01, steps, precision, mask_width=8.0, seed=seed)
+
+
+BATCH_SHAPES = [(64, 64, 64), (96, 96, 96), (128, 128, 128), (160, 160, 160),
+                (512, 512, 1), (1024, 1024, 1), (65536, 1, 1)]
+
+
+def batch_instances(seed: int = 4, count: int = 48, steps: int = 500,
+                    precision: str = "fp32", shapes=None) -> list:
+    """Uneven independent solves: shape from the list above (3D boxes of 40 a.u.,
+    2D of 80 a.u., 1D of 200 a.u.), harmonic (w ~ U(0.05, 0.5)) or
+    regularized-Coulomb (Z ~ U(0.5, 2), 3D/2D only) potential, Gaussian packet
+    with random centre, width and momentum."""
+    rng = SplitMix64(seed)
+    shapes = shapes or BATCH_SHAPES
+    out = []
+    for i in range(count):
+        n = rng.choice(shapes)
+        dims = sum(1 for v in n if v > 1)
+        half = {3: 20.0, 2: 40.0, 1: 100.0}[dims]
+        lo = tuple(-half if v > 1 else 0.0 for v in n)
+        hi = tuple(half if v > 1 else 1.0 for v in n)
+        coulomb = dims > 1 and rng.uniform() < 0.5
+        if coulomb:
+            pot, params, dt = "regularized_coulomb", {"Z": rng.uniform(0.5, 2.0)}, 0.005
+        else:
+            pot, params, dt = "harmonic", {"omega": rng.uniform(0.05, 0.5)}, 0.01
+        r0 = tuple(rng.uniform(-0.2 * half, 0.2 * half) if v > 1 else 0.0 for v in n)
+        sig = rng.uniform(1.0, 2.5)
+        sigma = tuple(sig if v > 1 else 1.0 for v in n)
+        k0 = tuple(rng.uniform(-0.8, 0.8) if v > 1 else 0.0 for v in n)
+        out.append(Solve(f"batch{i}", n, lo, hi, pot, params, r0, sigma, k0, dt, steps,
+                         precision, seed=seed))
+    return out

@@ -137,3 +137,16 @@ def test_cpp_facade_compiles_and_reports_no_device(tdse, tmp_path):
         pytest.skip("a GPU is visible here; the GPU tier runs the facade")
     r = subprocess.run([exe], capture_output=True, text=True)
     assert r.returncode == 77, r.stdout + r.stderr
+
+
+def test_request_stream_is_seeded_and_poisson(tdse):
+    from shoeridingerequationsolver_amd import serving
+    a = serving.request_stream(seed=5, count=400, rate=50.0)
+    b = serving.request_stream(seed=5, count=400, rate=50.0)
+    assert [(t, s.n, s.steps, s.precision) for t, s in a] == \
+           [(t, s.n, s.steps, s.precision) for t, s in b]
+    gaps = np.diff([0.0] + [t for t, _ in a])
+    assert abs(gaps.mean() - 1 / 50.0) < 0.2 / 50.0          # exponential gaps, mean 1/rate
+    assert all(100 <= s.steps <= 2000 for _, s in a)
+    assert {s.precision for _, s in a} == {"fp32", "fp64"}
+    assert {s.n for _, s in a} == set(serving.STREAM_SHAPES)

@@ -1,0 +1,46 @@
+"""Continuous batching: requests served in a mix (admitted under an HBM
+budget, stepped together in quanta, joining while others run) give the same
+state as each request solved alone, within the precision's tolerance."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [(65536, 1, 1), (256, 256, 1), (32, 32, 32), (64, 64, 64), (96, 96, 96), (1024, 1, 1)]
+
+
+def test_mix_equals_alone(tdse):
+    from shoeridingerequationsolver_amd import serving
+    trace = serving.request_stream(seed=21, count=14, rate=400.0, max_steps=160, shapes=SHAPES)
+    # a budget that holds only a few requests at once, so admission waits
+    need = sorted(serving.plan_bytes(s.grid(), s.precision, s.mask_width > 0) for _, s in trace)
+    budget = need[-1] + need[-2] + need[-3]
+    srv = serving.Server(budget, quantum=40)
+    rep = srv.run(trace)
+    srv.close()
+    summ = rep.summary()
+    print(summ, flush=True)
+    assert summ["solves"] == len(trace)
+    assert rep.quanta > 1 and rep.max_active > 1
+    assert rep.peak_bytes <= budget
+    by_name = {r.name: r for r in rep.results}
+    for _, s in trace:
+        p = tdse.Propagator(s.grid(), s.build_potential(), s.dt, precision=s.precision,
+                            mask=s.build_mask())
+        p.upload(s.build_psi0())
+        p.step(s.steps)
+        alone = p.download()
+        mixed = by_name[s.name].psi
+        err = np.abs(mixed - alone).max() / np.abs(alone).max()
+        tol = 1e-12 if s.precision == "fp64" else 2e-6
+        assert err < tol, f"{s.name} {s.n} {s.precision} steps={s.steps}: {err:.3e}"
+
+
+def test_budget_too_small_is_rejected(tdse):
+    from shoeridingerequationsolver_amd import serving
+    trace = serving.request_stream(seed=22, count=2, rate=100.0, max_steps=10,
+                                   shapes=[(64, 64, 64)])
+    srv = serving.Server(1 << 20, quantum=10)
+    with pytest.raises(ValueError):
+        srv.run(trace)
+    srv.close()
