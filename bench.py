@@ -109,6 +109,68 @@ def cpu_baseline(solve, v, mask, psi0, sample_steps, threads):
                       f"barrier workload after 1 warm-up step; table build subtracted"}
 
 
+def batch_leg(t, workloads, steps):
+    """48 uneven instances (seeded shapes 64^3..160^3, 512^2, 1024^2, 65536),
+    complex64, inputs built in HBM; one batch vs the same plans stepped
+    independently (one stream each)."""
+    solves = workloads.batch_instances(seed=4, count=48, steps=steps)
+    props = []
+    for s in solves:
+        p = t.Propagator(s.grid(), None, s.dt, precision=s.precision)
+        p.build(s.recipe())
+        props.append(p)
+    cells = sum(s.points for s in solves)
+    for p in props:
+        p.step(2)
+    for p in props:
+        p.sync()
+    t0 = time.perf_counter()
+    for p in props:
+        p.step(steps)
+    for p in props:
+        p.sync()
+    indep = time.perf_counter() - t0
+    bt = t.Batch(props)
+    bt.step(2)
+    bt.sync()
+    t0 = time.perf_counter()
+    bt.step(steps)
+    bt.sync()
+    batched = time.perf_counter() - t0
+    out = {"instances": len(solves), "steps": steps, "cells": cells,
+           "launches_per_round": bt.launches_per_round(),
+           "batched_s": round(batched, 4), "independent_streams_s": round(indep, 4),
+           "instance_steps_per_s": round(len(solves) * steps / batched, 1),
+           "gcell_steps_per_s": round(cells * steps / batched / 1e9, 2)}
+    bt.close()
+    for p in props:
+        p.close()
+    return out
+
+
+def serving_leg(rates, count, budget):
+    """Poisson request streams (mixed 1D/2D/3D up to 256^3, harmonic / Coulomb /
+    barrier+mask, 100-2000 steps, FP32 or FP64), continuous batching under an
+    HBM budget vs the same trace solved one at a time in plain FP64."""
+    from shoeridingerequationsolver_amd import serving
+    out = {"requests": count, "hbm_budget_bytes": budget, "rates": {}}
+    for rate in rates:
+        trace = serving.request_stream(seed=5, count=count, rate=rate)
+        srv = serving.Server(budget, quantum=200, keep_results=False)
+        rep = srv.run(trace)
+        srv.close()
+        s = rep.summary()
+        out["rates"][str(rate)] = {k: (round(v, 4) if isinstance(v, float) else v)
+                                   for k, v in s.items()}
+        log(f"serving rate {rate}/s: {s['solves_per_s']:.2f} solves/s, p50 {s['p50_latency_s']:.3f} s,"
+            f" p99 {s['p99_latency_s']:.3f} s")
+    trace = serving.request_stream(seed=5, count=count, rate=max(rates))
+    base = serving.run_sequential(trace, "fp64").summary()
+    out["sequential_fp64"] = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in base.items()}
+    log(f"sequential FP64: {base['solves_per_s']:.2f} solves/s, p99 {base['p99_latency_s']:.3f} s")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -121,6 +183,11 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=30)
     ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--batch-steps", type=int, default=500)
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=30)
+    ap.add_argument("--serve-budget-gib", type=float, default=6.0)
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -209,6 +276,15 @@ def main():
                     "transmitted_err_fp32": outs["fp32"][2] - outs["fp64"][2]}
         log(f"accuracy after {full} steps: max|dpsi| {d:.3e}, transmitted {outs['fp64'][2]:.6f}")
 
+    batch = None
+    if not args.no_batch:
+        batch = batch_leg(t, workloads, args.batch_steps)
+        log(f"batch: {batch}")
+    serve = None
+    if not args.no_serve:
+        serve = serving_leg([5.0, 20.0, 200.0], args.serve_requests,
+                            int(args.serve_budget_gib * (1 << 30)))
+
     cpu = None
     if not args.no_cpu:
         try:
@@ -244,6 +320,8 @@ def main():
         "accuracy": accuracy,
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "batch": batch,
+        "serving": serve,
     }
     print(json.dumps(line), flush=True)
 

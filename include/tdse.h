@@ -165,6 +165,26 @@ int tdse_batch_stream(tdse_batch* batch, void** out);
 /* Number of batched launches in one interior round (all sub-phases). */
 int tdse_batch_launches_per_round(const tdse_batch* batch, int* out);
 
+/* Build a plan's inputs on the GPU from a recipe: potential (and its kick
+ * phase), optional absorbing mask, and the normalised Gaussian packet as the
+ * state.  Same formulas as the host builders below, evaluated in FP64 on the
+ * device (equal to a few ulps, not bit for bit).  Replaces
+ * set_potential + set_mask + state_upload without host arrays. */
+#define TDSE_POT_NONE 0
+#define TDSE_POT_HARMONIC 1            /* 0.5 strength^2 |r - center|^2 */
+#define TDSE_POT_REGULARIZED_COULOMB 2 /* -strength / |r - center|, nucleus cell average */
+#define TDSE_POT_COULOMB_FLOOR 3       /* -strength / max(|r - center|, h/2) */
+#define TDSE_POT_BARRIER 4             /* strength on x_lo <= x < x_hi */
+typedef struct tdse_recipe {
+    int potential;
+    double strength;
+    double center[3];
+    double x_lo, x_hi;
+    double mask_width;                 /* <= 0: no mask */
+    double r0[3], sigma[3], k0[3];     /* Gaussian packet (sigma: density std-dev) */
+} tdse_recipe;
+int tdse_plan_build(tdse_plan* plan, const tdse_recipe* recipe);
+
 /* Host input builders, FP64, x-fastest, bit-identical to the reference
  * builders for the same arguments (n x 1 x 1 grids use the 1D forms).
  *   harmonic            0.5 w^2 |r - c|^2           core/src/potential.ixx:16-48

@@ -180,7 +180,8 @@ class Propagator:
         except TdseError as e:
             raise _map_error(e) from None
         self._h = h
-        self.set_potential(potential)
+        if potential is not None:
+            self.set_potential(potential)
         if mask is not None:
             self.set_mask(mask)
 
@@ -197,6 +198,11 @@ class Propagator:
     def set_potential(self, v):
         v = self._vec(v, "potential")
         capi.check(capi.lib().tdse_plan_set_potential(self._h, capi.dptr(v)))
+
+    def build(self, recipe):
+        """Build potential, mask and the normalised Gaussian state on the GPU
+        from a ``capi.Recipe`` (see ``workloads.Solve.recipe``)."""
+        capi.check(capi.lib().tdse_plan_build(self._h, C.byref(recipe)))
 
     def set_mask(self, m):
         if m is None:

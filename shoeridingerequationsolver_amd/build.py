@@ -30,6 +30,7 @@ UNITS = [
     ("kernels_f32_batch.hip", DEVICE),
     ("kernels_f64_batch.hip", DEVICE),
     ("observe.hip", DEVICE),
+    ("device_build.hip", DEVICE),
     ("tdse_api.cpp", ["-x", "hip"] + DEVICE),
     ("builders.cpp", ["-ffp-contract=off"]),
 ]

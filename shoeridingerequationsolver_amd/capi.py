@@ -41,8 +41,24 @@ EXPORTS = (
     "tdse_build_harmonic", "tdse_build_barrier", "tdse_build_regularized_coulomb",
     "tdse_build_coulomb_floor", "tdse_build_absorbing_mask", "tdse_build_gaussian",
     "tdse_batch_create", "tdse_batch_destroy", "tdse_batch_step", "tdse_batch_sync",
-    "tdse_batch_stream", "tdse_batch_launches_per_round",
+    "tdse_batch_stream", "tdse_batch_launches_per_round", "tdse_plan_build",
 )
+
+TDSE_POT = {"none": 0, "harmonic": 1, "regularized_coulomb": 2, "coulomb_floor": 3, "barrier": 4}
+
+
+class Recipe(C.Structure):
+    _fields_ = [
+        ("potential", C.c_int),
+        ("strength", C.c_double),
+        ("center", C.c_double * 3),
+        ("x_lo", C.c_double),
+        ("x_hi", C.c_double),
+        ("mask_width", C.c_double),
+        ("r0", C.c_double * 3),
+        ("sigma", C.c_double * 3),
+        ("k0", C.c_double * 3),
+    ]
 
 
 class TdseError(RuntimeError):
@@ -116,6 +132,7 @@ def lib() -> C.CDLL:
         L.tdse_batch_sync.argtypes = [C.c_void_p, C.c_int]
         L.tdse_batch_stream.argtypes = [C.c_void_p, C.POINTER(C.c_void_p)]
         L.tdse_batch_launches_per_round.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
+        L.tdse_plan_build.argtypes = [C.c_void_p, C.POINTER(Recipe)]
         _LIB = L
     return _LIB
 

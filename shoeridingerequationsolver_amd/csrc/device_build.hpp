@@ -1,0 +1,24 @@
+// device_build.hpp -- host view of the device input builders.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "../../include/tdse.h"
+
+namespace tdse {
+
+struct BuildArgs {
+    void* phi;
+    void* v;
+    void* mask;   // null: no mask
+    void* psi;
+    long long total;
+    int n[3];
+    double lo[3], hi[3], h[3];
+    double dt;
+    tdse_recipe r;
+};
+
+hipError_t launch_build(bool fp64, const BuildArgs& a, hipStream_t stream);
+
+}  // namespace tdse

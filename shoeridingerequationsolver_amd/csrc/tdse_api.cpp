@@ -21,6 +21,7 @@
 #include "../../include/tdse.h"
 #include "launch.hpp"
 #include "observe.hpp"
+#include "device_build.hpp"
 
 using tdse::SweepArgs;
 
@@ -935,6 +936,44 @@ int tdse_plan_set_mask(tdse_plan* p, const double* mask) {
     }
     if (!p->mask && (rc = alloc(p, &p->mask, p->total * p->rsize()))) return rc;
     return upload_real(p, p->mask, mask, false);
+}
+
+int tdse_plan_build(tdse_plan* p, const tdse_recipe* r) {
+    int rc = check_plan(p);
+    if (rc) return rc;
+    if (!r) return fail(TDSE_E_INVALID, "null recipe");
+    if (p->flags & TDSE_FLAG_UNFUSED_BASELINE)
+        return fail(TDSE_E_INVALID, "the unfused baseline takes host inputs");
+    for (int a = 0; a < 3; ++a)
+        if (!(r->sigma[a] > 0.0)) return fail(TDSE_E_INVALID, "sigma must be positive");
+    if ((rc = set_device(p->ctx))) return rc;
+    TDSE_HIP(hipStreamSynchronize(p->stream));
+    if (r->mask_width > 0.0) {
+        if (!p->mask && (rc = alloc(p, &p->mask, p->total * p->rsize()))) return rc;
+    } else {
+        release(p, p->mask, p->total * p->rsize());
+    }
+    tdse::BuildArgs b{};
+    b.phi = p->phi;
+    b.v = p->v;
+    b.mask = p->mask;
+    b.psi = p->psi;
+    b.total = p->total;
+    for (int a = 0; a < 3; ++a) {
+        b.n[a] = p->n[a];
+        b.lo[a] = p->lo[a];
+        b.hi[a] = p->hi[a];
+        b.h[a] = (p->hi[a] - p->lo[a]) / p->n[a];
+    }
+    b.dt = p->dt;
+    b.r = *r;
+    TDSE_HIP(tdse::launch_build(p->fp64(), b, p->stream));
+    p->has_potential = true;
+    // normalise the packet: sum |psi|^2 dV -> scale (core/src/field.ixx:78-87)
+    tdse_observables o;
+    if ((rc = tdse_observe(p, TDSE_OBS_NORM, &o))) return rc;
+    if (o.norm > 0.0) TDSE_HIP(tdse::launch_scale(p->fp64(), p->psi, p->total, 1.0 / std::sqrt(o.norm), p->stream));
+    return TDSE_OK;
 }
 
 int tdse_state_upload(tdse_plan* p, const double* psi) {

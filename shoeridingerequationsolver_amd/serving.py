@@ -4,8 +4,9 @@ Requests (seeded solves of mixed shapes, potentials, step counts and
 precisions) arrive over time.  The server
 
 * admits a waiting request as soon as its device footprint fits the HBM
-  budget (``tdse_plan_bytes``; the context enforces the same cap), building
-  its inputs on host threads while the GPU works,
+  budget (``tdse_plan_bytes``; the context enforces the same cap) and builds
+  its potential, mask and packet directly in HBM (``tdse_plan_build``; or,
+  with ``device_build=False``, on host threads while the GPU works),
 * keeps every admitted state resident in HBM (one plan each),
 * advances all active requests together in quanta: one ``tdse_batch_step``
   over the active set, of ``min(quantum, fewest remaining steps)`` steps, so
@@ -130,11 +131,13 @@ class Server:
     """Continuous-batching server over one device context with an HBM budget."""
 
     def __init__(self, hbm_budget_bytes: int, quantum: int = 200, device: int = 0,
-                 keep_results: bool = True, builder_threads: int = 8):
+                 keep_results: bool = True, builder_threads: int = 8,
+                 device_build: bool = True):
         self.ctx = Context(device, hbm_budget_bytes)
         self.budget = hbm_budget_bytes
         self.quantum = quantum
         self.keep = keep_results
+        self.device_build = device_build
         self.pool = cf.ThreadPoolExecutor(max_workers=builder_threads)
 
     def close(self):
@@ -165,8 +168,9 @@ class Server:
 
         while pending or active:
             clock = now()
-            # prebuild inputs of requests that have arrived (bounded look-ahead)
-            while nxt < len(trace) and trace[nxt][0] <= clock and len(building) < 16:
+            # prebuild host inputs of requests that have arrived (bounded look-ahead)
+            while (not self.device_build and nxt < len(trace) and trace[nxt][0] <= clock
+                   and len(building) < 16):
                 building[nxt] = self.pool.submit(_build, trace[nxt][1])
                 nxt += 1
             # admit in arrival order while memory allows
@@ -176,11 +180,16 @@ class Server:
                 need = self.footprint(req)
                 if reserved + need > self.budget:
                     break
-                fut = building.pop(idx, None) or self.pool.submit(_build, req)
-                v, m, psi0 = fut.result()
-                prop = Propagator(req.grid(), v, req.dt, precision=req.precision, mask=m,
-                                  context=self.ctx)
-                prop.upload(psi0)
+                if self.device_build:
+                    prop = Propagator(req.grid(), None, req.dt, precision=req.precision,
+                                      context=self.ctx)
+                    prop.build(req.recipe())
+                else:
+                    fut = building.pop(idx, None) or self.pool.submit(_build, req)
+                    v, m, psi0 = fut.result()
+                    prop = Propagator(req.grid(), v, req.dt, precision=req.precision, mask=m,
+                                      context=self.ctx)
+                    prop.upload(psi0)
                 reserved += need
                 active.append(_Active(req, arr, now(), prop, req.steps))
                 pending.pop(0)
@@ -215,7 +224,7 @@ class Server:
 
 
 def run_sequential(trace: list, precision: str = "fp64", device: int = 0,
-                   keep_results: bool = False) -> ServeReport:
+                   keep_results: bool = False, device_build: bool = True) -> ServeReport:
     """Baseline: the same trace solved one request at a time, plain FP64 (or
     `precision`), each plan created, stepped to the end and read back before
     the next request starts."""
@@ -227,9 +236,13 @@ def run_sequential(trace: list, precision: str = "fp64", device: int = 0,
         if wait > 0:
             time.sleep(wait)
         start = time.perf_counter() - t0
-        v, m, psi0 = _build(req)
-        p = Propagator(req.grid(), v, req.dt, precision=precision, mask=m, context=ctx)
-        p.upload(psi0)
+        if device_build:
+            p = Propagator(req.grid(), None, req.dt, precision=precision, context=ctx)
+            p.build(req.recipe())
+        else:
+            v, m, psi0 = _build(req)
+            p = Propagator(req.grid(), v, req.dt, precision=precision, mask=m, context=ctx)
+            p.upload(psi0)
         p.step(req.steps)
         psi = p.download()
         p.close()

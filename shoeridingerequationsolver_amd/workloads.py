@@ -114,6 +114,29 @@ class Solve:
         from . import absorbing_mask
         return absorbing_mask(self.grid(), self.mask_width)
 
+    def recipe(self):
+        """The same inputs as a device-build recipe (``Propagator.build``)."""
+        from .capi import TDSE_POT, Recipe
+        r = Recipe()
+        p = self.params
+        if self.potential == "harmonic":
+            r.potential, r.strength = TDSE_POT["harmonic"], p["omega"]
+        elif self.potential in ("regularized_coulomb", "coulomb_floor"):
+            r.potential, r.strength = TDSE_POT[self.potential], p["Z"]
+        elif self.potential == "barrier":
+            r.potential, r.strength = TDSE_POT["barrier"], p["V0"]
+            r.x_lo, r.x_hi = p["x_lo"], p["x_hi"]
+        elif self.potential == "free":
+            r.potential = TDSE_POT["none"]
+        else:
+            raise ValueError(f"unknown potential {self.potential}")
+        c = p.get("center", (0.0, 0.0, 0.0))
+        for a in range(3):
+            r.center[a] = c[a]
+            r.r0[a], r.sigma[a], r.k0[a] = self.r0[a], self.sigma[a], self.k0[a]
+        r.mask_width = self.mask_width
+        return r
+
     def build_psi0(self) -> np.ndarray:
         from . import gaussian_wavepacket
         g = self.grid()

@@ -25,9 +25,8 @@ def test_mix_equals_alone(tdse):
     assert rep.peak_bytes <= budget
     by_name = {r.name: r for r in rep.results}
     for _, s in trace:
-        p = tdse.Propagator(s.grid(), s.build_potential(), s.dt, precision=s.precision,
-                            mask=s.build_mask())
-        p.upload(s.build_psi0())
+        p = tdse.Propagator(s.grid(), None, s.dt, precision=s.precision)
+        p.build(s.recipe())
         p.step(s.steps)
         alone = p.download()
         mixed = by_name[s.name].psi
@@ -44,3 +43,33 @@ def test_budget_too_small_is_rejected(tdse):
     with pytest.raises(ValueError):
         srv.run(trace)
     srv.close()
+
+
+@pytest.mark.parametrize("shape", [(32, 32, 32), (64, 32, 1), (4096, 1, 1)])
+@pytest.mark.parametrize("kind", ["harmonic", "regularized_coulomb", "barrier"])
+def test_device_build_matches_host_builders(tdse, shape, kind):
+    """Inputs built on the GPU from a recipe equal the host builders (which
+    equal the reference bit for bit) to a few ulps."""
+    from shoeridingerequationsolver_amd import workloads
+    dims = sum(1 for v in shape if v > 1)
+    if kind == "regularized_coulomb" and dims == 1:
+        pytest.skip("no 1D Coulomb builder")
+    half = 8.0
+    lo = tuple(-half if v > 1 else 0.0 for v in shape)
+    hi = tuple(half if v > 1 else 1.0 for v in shape)
+    params = {"harmonic": {"omega": 0.4}, "regularized_coulomb": {"Z": 1.3},
+              "barrier": {"V0": 1.1, "x_lo": 0.0, "x_hi": 2.0}}[kind]
+    s = workloads.Solve("b", shape, lo, hi, kind, params,
+                        tuple(0.7 if v > 1 else 0.0 for v in shape),
+                        tuple(1.3 if v > 1 else 1.0 for v in shape),
+                        tuple(0.4 if v > 1 else 0.0 for v in shape), 0.01, 10, "fp64",
+                        mask_width=2.0 if kind == "barrier" else 0.0)
+    a = tdse.Propagator(s.grid(), s.build_potential(), s.dt, precision="fp64", mask=s.build_mask())
+    a.upload(s.build_psi0())
+    b = tdse.Propagator(s.grid(), None, s.dt, precision="fp64")
+    b.build(s.recipe())
+    pa, pb = a.download(), b.download()
+    assert np.abs(pa - pb).max() < 1e-13 * np.abs(pa).max() * 100
+    a.step(10)
+    b.step(10)
+    assert np.abs(a.download() - b.download()).max() < 1e-12 * np.abs(pa).max()
