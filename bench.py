@@ -6,7 +6,8 @@ Workload (the headline configuration): 3D barrier tunnelling, 256^3 grid on
 reductions), rectangular barrier V0 ~ U(0.5, 2) on 0 <= x < 2, sin^2 absorbing
 mask over the outer 8 a.u., Gaussian packet sigma = 3 at (-15, 0, 0) with
 kx ~ U(0.8, 1.5), dt = 0.01, seed 3.  Inputs are built on the host, uploaded
-once, and stay resident in HBM; the timed region is `steps` Strang steps
+once, and stay resident in HBM; each plan's kernel variants are autotuned
+(tdse_plan_autotune) before the timed region, which is `steps` Strang steps
 (one tdse_step call) bracketed by HIP events on the plan's stream.
 
 Printed JSON line (stdout, one line):
@@ -188,11 +189,13 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=30)
     ap.add_argument("--serve-budget-gib", type=float, default=6.0)
+    ap.add_argument("--no-tune", action="store_true",
+                    help="run every sweep in kernel variant 0 (no autotuning)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(independent solves shard across GPUs with no exchange; see DESIGN.md)",
+              "(independent solves would shard across GPUs with no exchange between them)",
               file=sys.stderr)
         sys.exit(2)
 
@@ -210,9 +213,14 @@ def main():
 
     result = {}
     props = {}
+    variants = {}
     for prec in (["fp32"] if args.no_fp64 else ["fp32", "fp64"]):
         p = t.Propagator(grid, v, solve.dt, precision=prec, mask=mask)
         p.upload(psi0)
+        if not args.no_tune:
+            # kernel variants per sweep, chosen before (and outside) the timed region
+            variants[prec] = p.autotune(reps=6)
+            log(f"{prec} variants: {variants[prec]}")
         gpu_s, wall_s = time_steps(hip, p, args.steps, args.warmup)
         result[prec] = {"steps_per_s": args.steps / gpu_s, "ms_per_step": 1e3 * gpu_s / args.steps,
                         "wall_steps_per_s": args.steps / wall_s}
@@ -248,7 +256,8 @@ def main():
     if os.path.exists(TRAFFIC_FILE):
         with open(TRAFFIC_FILE) as f:
             traffic = json.load(f).get(dom)
-    roofline = {"bound": "hbm", "kernel": "contig_sweep<float,256,C_KICK_MID> (x sweep)",
+    xv = p32.variants.get(dom, 0)
+    roofline = {"bound": "hbm", "kernel": f"contig_sweep<float,256,C_KICK_MID,{xv}> (x sweep)",
                 "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "bytes_per_launch": sweep_bytes[dom], "avg_launch_ms": round(avg_s * 1e3, 5),
@@ -309,7 +318,7 @@ def main():
         "config": {"workload": f"barrier-tunnelling {args.n}^3 complex64 + absorbing mask",
                    "grid": list(solve.n), "box": [solve.lo[0], solve.hi[0]], "dt": solve.dt,
                    "V0": solve.params["V0"], "kx": solve.k0[0], "mask_width": solve.mask_width,
-                   "seed": args.seed},
+                   "seed": args.seed, "kernel_variants": variants or None},
         "fp64_steps_per_s": round(result["fp64"]["steps_per_s"], 2) if "fp64" in result else None,
         "fp64_unfused_baseline_steps_per_s": (round(result["fp64_unfused"]["steps_per_s"], 2)
                                               if "fp64_unfused" in result else None),

@@ -148,6 +148,26 @@ const char* tdse_plan_sweep_name(const tdse_plan* plan, int slot);
  * sweep, kick phase and mask once). */
 int tdse_plan_step_bytes(const tdse_plan* plan, uint64_t* out);
 
+/* Kernel variants.  Each sweep of the step schedule (slot, as above) can run
+ * one of up to 4 compiled variants of its kernel: the base or an alternative
+ * radix list (a different number of points per thread) times one or two
+ * times the lines per block.  Variant 0 is the default of every slot.
+ * tdse_plan_autotune times the compiled variants on a scratch copy of the
+ * state (allocated on first use, counted in the context budget; the state
+ * itself is left unchanged) and keeps the fastest: the sweeps of every step
+ * by the time of whole steps (at least `reps` steps and about 2 ms per
+ * measurement, one pass of coordinate descent over the slots), the sweeps run
+ * once per call by the time of `reps` launches of their own.  A change from
+ * the current choice must be at least 2% faster.  variants[] receives the
+ * choice per slot.  Lengths 64-1024 and 96 have variants; the others, the
+ * one-block 1D kernel and the
+ * batched kernels run variant 0.  No reference counterpart (the reference
+ * has one FFT plan per length, core/src/fft.ixx:64-100). */
+int tdse_plan_autotune(tdse_plan* plan, int reps, int max_slots, int* variants, int* n_slots);
+int tdse_plan_get_variants(const tdse_plan* plan, int max_slots, int* variants, int* n_slots);
+/* TDSE_E_UNSUPPORTED when a slot's variant is not compiled for its sweep. */
+int tdse_plan_set_variants(tdse_plan* plan, const int* variants, int n_slots);
+
 /* Batches: many plans (any shapes, both precisions) advanced together.
  * Plans that share a line length and sweep mode are swept by one batched
  * launch; the launches of different plans overlap (each launch waits only for the sweeps of its
@@ -155,7 +175,7 @@ int tdse_plan_step_bytes(const tdse_plan* plan, uint64_t* out);
  * switched off (TDSE_E_UNSUPPORTED).  The batch snapshots each plan's buffers at
  * creation: re-create it after tdse_plan_set_mask changes a plan's mask from
  * null to non-null.  tdse_batch_step(b, n) == tdse_step(p, n) for every plan
- * p of the batch (north-star batching; no reference counterpart). */
+ * p of the batch (no reference counterpart: the reference steps one scene at a time). */
 typedef struct tdse_batch tdse_batch;
 int tdse_batch_create(tdse_ctx* ctx, tdse_plan* const* plans, int count, tdse_batch** out);
 int tdse_batch_destroy(tdse_batch* batch);

@@ -180,6 +180,7 @@ class Propagator:
         except TdseError as e:
             raise _map_error(e) from None
         self._h = h
+        self._has_mask = False
         if potential is not None:
             self.set_potential(potential)
         if mask is not None:
@@ -203,13 +204,16 @@ class Propagator:
         """Build potential, mask and the normalised Gaussian state on the GPU
         from a ``capi.Recipe`` (see ``workloads.Solve.recipe``)."""
         capi.check(capi.lib().tdse_plan_build(self._h, C.byref(recipe)))
+        self._has_mask = self._has_mask or recipe.mask_width > 0.0
 
     def set_mask(self, m):
         if m is None:
             capi.check(capi.lib().tdse_plan_set_mask(self._h, None))
+            self._has_mask = False
             return
         m = self._vec(m, "mask")
         capi.check(capi.lib().tdse_plan_set_mask(self._h, capi.dptr(m)))
+        self._has_mask = True
 
     def upload(self, psi):
         a = np.ascontiguousarray(psi, dtype=np.complex128).ravel()
@@ -269,6 +273,61 @@ class Propagator:
         L = capi.lib()
         return {L.tdse_plan_sweep_name(self._h, i).decode(): (ms[i], cnt[i])
                 for i in range(ns.value)}
+
+    @property
+    def variants(self) -> dict:
+        """Kernel variant per schedule slot: {slot name: variant}."""
+        v = (C.c_int * 16)()
+        ns = C.c_int()
+        L = capi.lib()
+        capi.check(L.tdse_plan_get_variants(self._h, 16, v, C.byref(ns)))
+        return {L.tdse_plan_sweep_name(self._h, i).decode(): v[i] for i in range(ns.value)}
+
+    def set_variants(self, variants: dict):
+        """Select kernel variants by slot name (every slot of the schedule)."""
+        L = capi.lib()
+        names = list(self.variants)
+        if set(variants) != set(names):
+            raise ValueError(f"expected variants for slots {names}")
+        arr = (C.c_int * len(names))(*[int(variants[k]) for k in names])
+        try:
+            capi.check(L.tdse_plan_set_variants(self._h, arr, len(names)))
+        except TdseError as e:
+            raise _map_error(e) from None
+
+    def tuning_key(self) -> str:
+        """Key of this plan's sweep shapes in a tuning cache."""
+        prec = "fp64" if self.precision == capi.TDSE_FP64 else "fp32"
+        n, _, _ = _axes(self.grid)
+        return f"{prec}:{n[0]}x{n[1]}x{n[2]}:{'mask' if self._has_mask else 'nomask'}"
+
+    def autotune(self, reps: int = 6, cache: str | None = None) -> dict:
+        """Time every compiled kernel variant of each sweep on a scratch copy of
+        the state and keep the fastest.  With `cache` (a JSON file), a stored
+        choice for the same precision, shape and mask presence is applied
+        without timing, and a new choice is stored."""
+        import json
+        import os
+        key = self.tuning_key()
+        table = {}
+        if cache and os.path.exists(cache):
+            with open(cache) as f:
+                table = json.load(f)
+            if key in table:
+                self.set_variants(table[key])
+                return dict(table[key])
+        v = (C.c_int * 16)()
+        ns = C.c_int()
+        L = capi.lib()
+        capi.check(L.tdse_plan_autotune(self._h, int(reps), 16, v, C.byref(ns)))
+        got = {L.tdse_plan_sweep_name(self._h, i).decode(): v[i] for i in range(ns.value)}
+        if cache:
+            table[key] = got
+            tmp = cache + ".tmp"
+            with open(tmp, "w") as f:
+                json.dump(table, f, indent=1, sort_keys=True)
+            os.replace(tmp, cache)
+        return got
 
     def close(self):
         if getattr(self, "_h", None):

@@ -29,6 +29,12 @@ UNITS = [
     ("kernels_f64.hip", DEVICE),
     ("kernels_f32_batch.hip", DEVICE),
     ("kernels_f64_batch.hip", DEVICE),
+    ("kernels_f32_v1.hip", DEVICE),
+    ("kernels_f32_v2.hip", DEVICE),
+    ("kernels_f32_v3.hip", DEVICE),
+    ("kernels_f64_v1.hip", DEVICE),
+    ("kernels_f64_v2.hip", DEVICE),
+    ("kernels_f64_v3.hip", DEVICE),
     ("observe.hip", DEVICE),
     ("device_build.hip", DEVICE),
     ("tdse_api.cpp", ["-x", "hip"] + DEVICE),

@@ -38,6 +38,7 @@ EXPORTS = (
     "tdse_step", "tdse_plan_sync", "tdse_plan_stream",
     "tdse_observe", "tdse_probability_in_slab", "tdse_state_scale",
     "tdse_profile_step", "tdse_plan_sweep_name", "tdse_plan_step_bytes",
+    "tdse_plan_autotune", "tdse_plan_get_variants", "tdse_plan_set_variants",
     "tdse_build_harmonic", "tdse_build_barrier", "tdse_build_regularized_coulomb",
     "tdse_build_coulomb_floor", "tdse_build_absorbing_mask", "tdse_build_gaussian",
     "tdse_batch_create", "tdse_batch_destroy", "tdse_batch_step", "tdse_batch_sync",
@@ -119,6 +120,11 @@ def lib() -> C.CDLL:
         L.tdse_profile_step.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, _ip, _ip]
         L.tdse_plan_sweep_name.argtypes = [C.c_void_p, C.c_int]
         L.tdse_plan_step_bytes.argtypes = [C.c_void_p, C.POINTER(C.c_uint64)]
+        L.tdse_plan_autotune.argtypes = [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_int),
+                                         C.POINTER(C.c_int)]
+        L.tdse_plan_get_variants.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int),
+                                             C.POINTER(C.c_int)]
+        L.tdse_plan_set_variants.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.c_int]
         for name in ("tdse_build_harmonic", "tdse_build_regularized_coulomb",
                      "tdse_build_coulomb_floor"):
             getattr(L, name).argtypes = [_ip, _dp, _dp, C.c_double, _dp, _dp]

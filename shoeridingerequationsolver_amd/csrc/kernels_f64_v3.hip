@@ -1,0 +1,5 @@
+// F64 sweep kernels, variant 3 (autotuner candidate).
+#define TDSE_REAL double
+#define TDSE_SUFFIX f64
+#define TDSE_VARIANT_UNIT 3
+#include "kernels_impl.hpp"

@@ -8,9 +8,9 @@
 namespace tdse {
 namespace {
 
-template <typename T, int N, int MODE>
+template <typename T, int N, int MODE, int V = 0>
 hipError_t launch_sweep(const SweepArgs& a, int max_blocks, hipStream_t stream) {
-    using S = Shape<T, N>;
+    using S = Shape<T, N, V>;
     long long tiles;
     int block;
     if constexpr (mode_contiguous(MODE)) {
@@ -23,9 +23,9 @@ hipError_t launch_sweep(const SweepArgs& a, int max_blocks, hipStream_t stream) 
     if (tiles <= 0) return hipSuccess;
     const int grid = static_cast<int>(tiles < max_blocks ? tiles : max_blocks);
     if constexpr (mode_contiguous(MODE)) {
-        hipLaunchKernelGGL((contig_sweep<T, N, MODE>), dim3(grid), dim3(block), 0, stream, a);
+        hipLaunchKernelGGL((contig_sweep<T, N, MODE, V>), dim3(grid), dim3(block), 0, stream, a);
     } else {
-        hipLaunchKernelGGL((strided_sweep<T, N, MODE>), dim3(grid), dim3(block), 0, stream, a);
+        hipLaunchKernelGGL((strided_sweep<T, N, MODE, V>), dim3(grid), dim3(block), 0, stream, a);
     }
     return hipGetLastError();
 }
@@ -90,12 +90,45 @@ SweepShape shape_of(int mode) {
     return {S::BLOCK_S, S::TX};
 }
 
+// Variants 1..3 of the step-path modes (tuned lengths only).
+template <typename T, int N, int V>
+LaunchFn pick_variant_mode(int mode) {
+    switch (mode) {
+        case C_KICK_FIRST: return &launch_sweep<T, N, C_KICK_FIRST, V>;
+        case C_KICK_MID: return &launch_sweep<T, N, C_KICK_MID, V>;
+        case C_KICK_LAST: return &launch_sweep<T, N, C_KICK_LAST, V>;
+        case C_KIN_TW: return &launch_sweep<T, N, C_KIN_TW, V>;
+        case S_FWD: return &launch_sweep<T, N, S_FWD, V>;
+        case S_INV: return &launch_sweep<T, N, S_INV, V>;
+        case S_KIN: return &launch_sweep<T, N, S_KIN, V>;
+        case S_KICK_FIRST_TW: return &launch_sweep<T, N, S_KICK_FIRST_TW, V>;
+        case S_KICK_MID_TW: return &launch_sweep<T, N, S_KICK_MID_TW, V>;
+        case S_KICK_LAST: return &launch_sweep<T, N, S_KICK_LAST, V>;
+        default: return nullptr;
+    }
+}
+
+
+
 }  // namespace
 
 #define TDSE_CAT2(a, b) a##b
 #define TDSE_CAT(a, b) TDSE_CAT2(a, b)
 
-#ifndef TDSE_BATCH_UNIT
+#if defined(TDSE_VARIANT_UNIT)
+LaunchFn TDSE_CAT(TDSE_CAT(find_variant_launcher_, TDSE_SUFFIX), TDSE_CAT(_v, TDSE_VARIANT_UNIT))(
+    int n, int mode) {
+    switch (n) {
+#define TDSE_CASE(N_) \
+    case N_:          \
+        return pick_variant_mode<TDSE_REAL, N_, TDSE_VARIANT_UNIT>(mode);
+        TDSE_FOR_EACH_TUNED_N(TDSE_CASE)
+#undef TDSE_CASE
+        default:
+            return nullptr;
+    }
+}
+#elif !defined(TDSE_BATCH_UNIT)
 LaunchFn TDSE_CAT(find_launcher_, TDSE_SUFFIX)(int n, int mode) {
     switch (n) {
 #define TDSE_CASE(N_) \
@@ -123,7 +156,7 @@ BatchLaunchFn TDSE_CAT(find_batch_launcher_, TDSE_SUFFIX)(int n, int mode) {
 
 #endif
 
-#ifndef TDSE_BATCH_UNIT
+#if !defined(TDSE_BATCH_UNIT) && !defined(TDSE_VARIANT_UNIT)
 SweepShape TDSE_CAT(sweep_shape_, TDSE_SUFFIX)(int n, int mode) {
     switch (n) {
 #define TDSE_CASE(N_) \

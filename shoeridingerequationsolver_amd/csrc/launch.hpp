@@ -23,6 +23,32 @@ using BatchLaunchFn = hipError_t (*)(const BatchTable& b, long long tiles, int m
 BatchLaunchFn find_batch_launcher_f32(int n, int mode);
 BatchLaunchFn find_batch_launcher_f64(int n, int mode);
 
+// Variants 1..3 of a (line length, mode) sweep, one translation unit each;
+// null if not compiled.
+LaunchFn find_variant_launcher_f32_v1(int n, int mode);
+LaunchFn find_variant_launcher_f32_v2(int n, int mode);
+LaunchFn find_variant_launcher_f32_v3(int n, int mode);
+LaunchFn find_variant_launcher_f64_v1(int n, int mode);
+LaunchFn find_variant_launcher_f64_v2(int n, int mode);
+LaunchFn find_variant_launcher_f64_v3(int n, int mode);
+
+inline LaunchFn find_variant_launcher_f32(int n, int mode, int variant) {
+    switch (variant) {
+        case 1: return find_variant_launcher_f32_v1(n, mode);
+        case 2: return find_variant_launcher_f32_v2(n, mode);
+        case 3: return find_variant_launcher_f32_v3(n, mode);
+        default: return nullptr;
+    }
+}
+inline LaunchFn find_variant_launcher_f64(int n, int mode, int variant) {
+    switch (variant) {
+        case 1: return find_variant_launcher_f64_v1(n, mode);
+        case 2: return find_variant_launcher_f64_v2(n, mode);
+        case 3: return find_variant_launcher_f64_v3(n, mode);
+        default: return nullptr;
+    }
+}
+
 // Lookup of the compiled kernel for (line length, mode); null if not compiled.
 LaunchFn find_launcher_f32(int n, int mode);
 LaunchFn find_launcher_f64(int n, int mode);

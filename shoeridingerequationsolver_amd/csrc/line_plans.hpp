@@ -2,7 +2,13 @@
 //
 // For each supported line length N: E points per thread (T = N / E threads
 // per line) and the forward radix list (product N, every radix divides E).
-// The inverse transform runs the same radices in reverse order.
+// The inverse transform runs the same radices in reverse order.  ALT = 1
+// selects an alternative radix list with a different E (fewer or more points
+// per thread); lengths without one reuse the base plan.
+//
+// A kernel variant V (0..3) combines the radix list (V & 1) with a tile-width
+// multiplier (V & 2: twice the lines per block).  The autotuner times the
+// variants of each sweep and keeps the fastest.
 //
 // Launch shapes:
 //   contiguous sweeps (lines along the unit-stride axis): LPB lines per block;
@@ -15,35 +21,48 @@
 
 namespace tdse {
 
-template <int N>
+template <int N, int ALT = 0>
 struct LinePlan;
 
-#define TDSE_LINE_PLAN(N_, E_, ...)        \
+#define TDSE_LINE_PLAN(N_, ALT_, E_, ...)  \
     template <>                            \
-    struct LinePlan<N_> {                  \
+    struct LinePlan<N_, ALT_> {            \
         static constexpr int N = N_;       \
         static constexpr int E = E_;       \
         using Fwd = Radices<__VA_ARGS__>;  \
     };
 
-TDSE_LINE_PLAN(2, 2, 2)
-TDSE_LINE_PLAN(4, 4, 4)
-TDSE_LINE_PLAN(8, 8, 8)
-TDSE_LINE_PLAN(16, 16, 16)
-TDSE_LINE_PLAN(32, 8, 8, 4)
-TDSE_LINE_PLAN(64, 8, 8, 8)
-TDSE_LINE_PLAN(128, 16, 16, 8)
-TDSE_LINE_PLAN(256, 16, 16, 16)
-TDSE_LINE_PLAN(512, 16, 8, 8, 8)
-TDSE_LINE_PLAN(1024, 16, 16, 16, 4)
-TDSE_LINE_PLAN(96, 12, 6, 4, 4)
-TDSE_LINE_PLAN(160, 20, 10, 4, 4)
-TDSE_LINE_PLAN(192, 24, 8, 6, 4)
+TDSE_LINE_PLAN(2, 0, 2, 2)
+TDSE_LINE_PLAN(4, 0, 4, 4)
+TDSE_LINE_PLAN(8, 0, 8, 8)
+TDSE_LINE_PLAN(16, 0, 16, 16)
+TDSE_LINE_PLAN(32, 0, 8, 8, 4)
+TDSE_LINE_PLAN(64, 0, 8, 8, 8)
+TDSE_LINE_PLAN(128, 0, 16, 16, 8)
+TDSE_LINE_PLAN(256, 0, 16, 16, 16)
+TDSE_LINE_PLAN(512, 0, 16, 8, 8, 8)
+TDSE_LINE_PLAN(1024, 0, 16, 16, 16, 4)
+TDSE_LINE_PLAN(96, 0, 12, 6, 4, 4)
+TDSE_LINE_PLAN(160, 0, 20, 10, 4, 4)
+TDSE_LINE_PLAN(192, 0, 24, 8, 6, 4)
+// alternatives
+TDSE_LINE_PLAN(64, 1, 4, 4, 4, 4)
+TDSE_LINE_PLAN(128, 1, 8, 8, 4, 4)
+TDSE_LINE_PLAN(256, 1, 8, 8, 8, 4)
+TDSE_LINE_PLAN(512, 1, 8, 8, 8, 8)
+TDSE_LINE_PLAN(1024, 1, 32, 32, 32)
+TDSE_LINE_PLAN(96, 1, 24, 12, 8)
 #undef TDSE_LINE_PLAN
+
+template <int N>
+struct LinePlan<N, 1> : LinePlan<N, 0> {};
 
 // X-macro over every compiled line length.
 #define TDSE_FOR_EACH_N(X) \
     X(2) X(4) X(8) X(16) X(32) X(64) X(128) X(256) X(512) X(1024) X(96) X(160) X(192)
+// Lengths whose variants 1..3 are compiled (the others run variant 0 only).
+#define TDSE_FOR_EACH_TUNED_N(X) X(64) X(128) X(256) X(512) X(1024) X(96)
+constexpr int kVariants = 4;
 
 constexpr bool line_supported(int n) {
     switch (n) {
@@ -71,9 +90,10 @@ constexpr int pow2_floor(int a) {
 
 constexpr int kLdsCap = 80 * 1024;
 
-template <typename T, int N>
+template <typename T, int N, int V = 0>
 struct Shape {
-    using P = LinePlan<N>;
+    using P = LinePlan<N, V & 1>;
+    static constexpr int WIDE = (V & 2) ? 2 : 1;  // tile-width multiplier
     static constexpr int E = P::E;
     static constexpr int TT = N / E;  // threads per line
     static constexpr int PASSES = P::Fwd::count;
@@ -84,14 +104,14 @@ struct Shape {
     static constexpr int DATA_CAP = (kLdsCap - 2 * N * 16) / CSZ;
 
     // contiguous sweeps
-    static constexpr int LPB_WANT = cmax(1, 256 / TT);
+    static constexpr int LPB_WANT = cmax(1, WIDE * 256 / TT);
     static constexpr int LPB_LDS = EXCHANGE ? cmax(1, DATA_CAP / LSTR) : LPB_WANT;
     static constexpr int LPB = pow2_floor(cmin(LPB_WANT, LPB_LDS));
     static constexpr int BLOCK_C = LPB * TT;
     static constexpr int LDS_C = EXCHANGE ? LPB * LSTR : 1;
 
     // strided sweeps
-    static constexpr int TX_WANT = cmax(1, cmin(64, 256 / TT));
+    static constexpr int TX_WANT = cmax(1, cmin(64, WIDE * 256 / TT));
     static constexpr int TX_LDS = EXCHANGE ? cmax(1, DATA_CAP / LSTR) : TX_WANT;
     static constexpr int TX = pow2_floor(cmin(TX_WANT, TX_LDS));
     static constexpr int BLOCK_S = TX * TT;

@@ -205,9 +205,9 @@ __device__ __forceinline__ void stage_table(cx<double>* lds, const void* g, int 
 
 // One line (or one lane's share of it) of a sweep.  Point (b, r) of the
 // thread sits at element  base + (j_b + r*N/R) * es  of psi/phi/mask.
-template <typename T, int N, class X>
+template <typename T, int N, class X, int V = 0>
 struct Line {
-    using S = Shape<T, N>;
+    using S = Shape<T, N, V>;
     using Fwd = typename S::P::Fwd;
     using Inv = Reverse<Fwd>;
     static constexpr int E = S::E;
@@ -227,29 +227,53 @@ struct Line {
 
     template <int R>
     __device__ __forceinline__ void load() {
-        for_each_point<N, E, R>(t, [&](int s, int, int j, int ro) {
-            v[s] = valid ? P.ld((base + j * es) * CS, ro * es * CS) : cx<T>{0, 0};
-        });
+        if (valid) {
+            for_each_point<N, E, R>(t, [&](int s, int, int j, int ro) {
+                v[s] = P.ld((base + j * es) * CS, ro * es * CS);
+            });
+        } else {
+            for (int s = 0; s < E; ++s) v[s] = cx<T>{0, 0};
+        }
     }
     template <int R>
     __device__ __forceinline__ void store() {
+        if (!valid) return;
         for_each_point<N, E, R>(t, [&](int s, int, int j, int ro) {
-            if (valid) P.st(v[s], (base + j * es) * CS, ro * es * CS);
+            P.st(v[s], (base + j * es) * CS, ro * es * CS);
         });
     }
-    // kick in pattern R1 (position space after IFFT / before FFT)
-    __device__ __forceinline__ void kick(double f, bool with_mask) {
+
+    // Kick inputs in pattern R1 (position space after IFFT / before FFT):
+    // all loads are issued together, so their latency is paid once per line.
+    // (Issuing them before the inverse transform measured no faster.)
+    T kp[E];  // phase
+    T km[E];  // mask (1 without one)
+    __device__ __forceinline__ void load_kick(bool with_mask) {
+        if (!valid) return;
+        const bool m = with_mask && a.mask;
         for_each_point<N, E, R1>(t, [&](int s, int, int j, int ro) {
-            if (!valid) return;
             const uint32_t vo = (base + j * es) * RS, so = ro * es * RS;
-            const T p = F.ld(vo, so);
-            const T m = (with_mask && a.mask) ? M.ld(vo, so) : T(1);
-            if (__any(p != T(0))) {
-                v[s] = mul_d(v[s], kick_factor<T>(p, f, static_cast<double>(m)));
-            } else if (with_mask && a.mask) {
-                v[s] = cscale(v[s], m);  // real factor: exact per component
-            }
+            kp[s] = F.ld(vo, so);
+            km[s] = m ? M.ld(vo, so) : T(1);
         });
+    }
+    // v *= m exp(i f phi), the exponential in FP64.  Waves whose phases are
+    // all zero take the real product instead, which gives the same bits
+    // (the factor (m, 0) times v rounds each component once either way).
+    __device__ __forceinline__ void apply_kick(double f, bool use_mask) {
+        if (!valid) return;
+        bool nz = false;
+        for (int s = 0; s < E; ++s) nz |= kp[s] != T(0);
+        if (__any(nz)) {
+            for (int s = 0; s < E; ++s)
+                v[s] = mul_d(v[s], kick_factor<T>(kp[s], f, use_mask ? static_cast<double>(km[s]) : 1.0));
+        } else if (use_mask && a.mask) {
+            for (int s = 0; s < E; ++s) v[s] = cscale(v[s], km[s]);
+        }
+    }
+    __device__ __forceinline__ void kick(double f, bool with_mask) {
+        load_kick(with_mask);
+        apply_kick(f, with_mask);
     }
     __device__ __forceinline__ void fft() { Transform<T, N, E, -1, 1, Fwd>::run(v, t, tw, x); }
     __device__ __forceinline__ void ifft() { Transform<T, N, E, +1, 1, Inv>::run(v, t, tw, x); }
@@ -265,11 +289,11 @@ struct Line {
 
 // One tile of a contiguous sweep: LPB whole lines.  `lds` holds LPB padded
 // lines, `tw` the staged FP64 twiddles of this line length.
-template <typename T, int N, int MODE>
+template <typename T, int N, int MODE, int V = 0>
 __device__ __forceinline__ void contig_tile(const SweepArgs& a, long long tile, cx<T>* lds,
                                             const cx<double>* tw, int tid) {
-    using S = Shape<T, N>;
-    using LT = Line<T, N, ContigX<T>>;
+    using S = Shape<T, N, V>;
+    using LT = Line<T, N, ContigX<T>, V>;
     constexpr int TT = S::TT, LPB = S::LPB, E = S::E;
     const int l = tid / TT, t = tid % TT;
     ContigX<T> x{lds + l * S::LSTR};
@@ -286,13 +310,15 @@ __device__ __forceinline__ void contig_tile(const SweepArgs& a, long long tile, 
     } else if constexpr (MODE == C_KICK_MID) {
         L.template load<LT::RL>();
         L.ifft();
-        L.kick(2.0, true);
+        L.load_kick(true);
+        L.apply_kick(2.0, true);
         L.fft();
         L.template store<LT::RL>();
     } else if constexpr (MODE == C_KICK_LAST) {
         L.template load<LT::RL>();
         L.ifft();
-        L.kick(1.0, true);
+        L.load_kick(true);
+        L.apply_kick(1.0, true);
         L.template store<LT::R1>();
     } else if constexpr (MODE == C_FWD) {
         L.template load<LT::R1>();
@@ -315,14 +341,15 @@ __device__ __forceinline__ void contig_tile(const SweepArgs& a, long long tile, 
     } else if constexpr (MODE == C_SINGLE) {
         const cx<double>* kt = static_cast<const cx<double>*>(a.kt[2]);
         L.template load<LT::R1>();
-        L.kick(1.0, false);
+        L.load_kick(true);  // phases and mask stay in registers for every step
+        L.apply_kick(1.0, false);
         for (int step = 0; step < a.nsteps; ++step) {
             L.fft();
             for_each_point<N, E, LT::RL>(t, [&](int s, int i, int, int) {
                 L.v[s] = mul_d(L.v[s], cscale(kt[i], a.scale));
             });
             L.ifft();
-            L.kick(step + 1 == a.nsteps ? 1.0 : 2.0, true);
+            L.apply_kick(step + 1 == a.nsteps ? 1.0 : 2.0, true);
         }
         L.template store<LT::R1>();
     }
@@ -330,12 +357,12 @@ __device__ __forceinline__ void contig_tile(const SweepArgs& a, long long tile, 
 
 // One tile of a strided sweep: TX consecutive lines.  `kp` holds the staged
 // line-axis kinetic table (times 1/N) for S_KIN.
-template <typename T, int N, int MODE>
+template <typename T, int N, int MODE, int V = 0>
 __device__ __forceinline__ void strided_tile(const SweepArgs& a, long long tile, cx<T>* lds,
                                              const cx<double>* tw, const cx<double>* kp, int tid) {
-    using S = Shape<T, N>;
+    using S = Shape<T, N, V>;
     constexpr int TX = S::TX, E = S::E;
-    using LT = Line<T, N, StridedX<T, TX>>;
+    using LT = Line<T, N, StridedX<T, TX>, V>;
     const int xi = tid % TX, t = tid / TX;
     StridedX<T, TX> x{lds + xi};
     const Buf<cx<T>> P(a.psi, a.psi_bytes);
@@ -374,14 +401,16 @@ __device__ __forceinline__ void strided_tile(const SweepArgs& a, long long tile,
     } else if constexpr (MODE == S_KICK_MID_TW) {
         L.template load<LT::RL>();
         L.ifft();
-        L.kick(2.0, true);
+        L.load_kick(true);
+        L.apply_kick(2.0, true);
         L.fft();
         L.twiddle_out(xx);
         L.template store<LT::RL>();
     } else if constexpr (MODE == S_KICK_LAST) {
         L.template load<LT::RL>();
         L.ifft();
-        L.kick(1.0, true);
+        L.load_kick(true);
+        L.apply_kick(1.0, true);
         L.template store<LT::R1>();
     } else if constexpr (MODE == S_FWD_TW) {
         L.template load<LT::R1>();
@@ -400,10 +429,10 @@ __device__ __forceinline__ void stage_kinetic(cx<double>* kp, const SweepArgs& a
 
 // ---- single-plan kernels (grid-stride over tiles) -------------------------------------
 
-template <typename T, int N, int MODE>
-__global__ void __launch_bounds__((Shape<T, N>::BLOCK_C), TDSE_MIN_WAVES)
+template <typename T, int N, int MODE, int V = 0>
+__global__ void __launch_bounds__((Shape<T, N, V>::BLOCK_C), TDSE_MIN_WAVES)
 contig_sweep(SweepArgs a) {
-    using S = Shape<T, N>;
+    using S = Shape<T, N, V>;
     __shared__ cx<T> lds[S::LDS_C];
     __shared__ cx<double> lds_tw[S::EXCHANGE ? N : 1];
     const int tid = threadIdx.x;
@@ -413,13 +442,13 @@ contig_sweep(SweepArgs a) {
     }
     const long long ntiles = (a.nlines + S::LPB - 1) / S::LPB;
     for (long long tile = blockIdx.x; tile < ntiles; tile += gridDim.x)
-        contig_tile<T, N, MODE>(a, tile, lds, lds_tw, tid);
+        contig_tile<T, N, MODE, V>(a, tile, lds, lds_tw, tid);
 }
 
-template <typename T, int N, int MODE>
-__global__ void __launch_bounds__((Shape<T, N>::BLOCK_S), TDSE_MIN_WAVES)
+template <typename T, int N, int MODE, int V = 0>
+__global__ void __launch_bounds__((Shape<T, N, V>::BLOCK_S), TDSE_MIN_WAVES)
 strided_sweep(SweepArgs a) {
-    using S = Shape<T, N>;
+    using S = Shape<T, N, V>;
     constexpr bool KIN = MODE == S_KIN;
     __shared__ cx<T> lds[S::LDS_S];
     __shared__ cx<double> lds_tw[S::EXCHANGE ? N : 1];
@@ -430,7 +459,7 @@ strided_sweep(SweepArgs a) {
     if constexpr (S::EXCHANGE || KIN) __syncthreads();
     const long long ntiles = static_cast<long long>((a.nx + S::TX - 1) / S::TX) * a.nw;
     for (long long tile = blockIdx.x; tile < ntiles; tile += gridDim.x)
-        strided_tile<T, N, MODE>(a, tile, lds, lds_tw, lds_kp, tid);
+        strided_tile<T, N, MODE, V>(a, tile, lds, lds_tw, lds_kp, tid);
 }
 
 // ---- batched kernels: one launch over the tiles of many plans ------------------------

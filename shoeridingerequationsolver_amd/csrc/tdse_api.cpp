@@ -58,6 +58,8 @@ struct Axis {
 
 enum Kind { kKind3D, kKind2D, kKindSingle, kKindFourStep };
 
+constexpr int kMaxSlots = 6;
+
 struct Launch {
     int mode;
     int n;          // line length
@@ -100,6 +102,7 @@ struct tdse_plan {
     bool has_potential = false;
     uint64_t bytes = 0;
     unsigned flags = 0;
+    int variant[kMaxSlots] = {0, 0, 0, 0, 0, 0};  // kernel variant per schedule slot
     // unfused FP64 baseline: full complex128 tables of the reference propagator
     void* base_half = nullptr;  // exp(-i V dt / 2) per cell
     void* base_kin = nullptr;   // exp(-i |k|^2 dt / 2m) per cell
@@ -111,7 +114,10 @@ struct tdse_plan {
 
 namespace {
 
-tdse::LaunchFn launcher(const tdse_plan* p, int n, int mode) {
+tdse::LaunchFn launcher(const tdse_plan* p, int n, int mode, int variant = 0) {
+    if (variant != 0)
+        return p->fp64() ? tdse::find_variant_launcher_f64(n, mode, variant)
+                         : tdse::find_variant_launcher_f32(n, mode, variant);
     return p->fp64() ? tdse::find_launcher_f64(n, mode) : tdse::find_launcher_f32(n, mode);
 }
 
@@ -374,41 +380,69 @@ Geometry geometry(const tdse_plan* p, void* target) {
     return g;
 }
 
-// The launch list of `nsteps` steps (nsteps >= 1).
-std::vector<Launch> schedule(const tdse_plan* p, int nsteps) {
-    std::vector<Launch> L;
-    Geometry g = geometry(p, p->psi);
-    auto add = [&](int mode, int n, const SweepArgs& a, int slot) {
-        L.push_back({mode, n, a, launcher(p, n, mode), slot});
-    };
+// Sweep of schedule slot `slot` (mode, line length, descriptor); the slots are
+// named by kSlots* below.
+struct SlotSweep {
+    int mode;
+    int n;
+    SweepArgs args;
+};
+
+SlotSweep slot_sweep(const tdse_plan* p, const Geometry& g, int slot) {
     if (p->kind == kKind3D) {
         const int n0 = p->sq[0].n, n1 = p->sq[1].n, n2 = p->sq[2].n;
-        add(tdse::C_KICK_FIRST, n0, g.x, 0);
-        for (int s = 0; s < nsteps; ++s) {
-            add(tdse::S_FWD, n1, g.yf, 1);
-            add(tdse::S_KIN, n2, g.z, 2);
-            add(tdse::S_INV, n1, g.yi, 3);
-            if (s + 1 < nsteps) add(tdse::C_KICK_MID, n0, g.x, 4);
-            else add(tdse::C_KICK_LAST, n0, g.x, 5);
+        switch (slot) {
+            case 0: return {tdse::C_KICK_FIRST, n0, g.x};
+            case 1: return {tdse::S_FWD, n1, g.yf};
+            case 2: return {tdse::S_KIN, n2, g.z};
+            case 3: return {tdse::S_INV, n1, g.yi};
+            case 4: return {tdse::C_KICK_MID, n0, g.x};
+            default: return {tdse::C_KICK_LAST, n0, g.x};
         }
-    } else if (p->kind == kKind2D) {
+    }
+    if (p->kind == kKind2D) {
         const int n0 = p->sq[0].n, n1 = p->sq[1].n;
-        add(tdse::C_KICK_FIRST, n0, g.x, 0);
+        switch (slot) {
+            case 0: return {tdse::C_KICK_FIRST, n0, g.x};
+            case 1: return {tdse::S_KIN, n1, g.yf};
+            case 2: return {tdse::C_KICK_MID, n0, g.x};
+            default: return {tdse::C_KICK_LAST, n0, g.x};
+        }
+    }
+    if (p->kind == kKindSingle) return {tdse::C_SINGLE, p->sq[0].n, g.single};
+    switch (slot) {
+        case 0: return {tdse::S_KICK_FIRST_TW, p->n2, g.col};
+        case 1: return {tdse::C_KIN_TW, p->n1, g.row};
+        case 2: return {tdse::S_KICK_MID_TW, p->n2, g.col};
+        default: return {tdse::S_KICK_LAST, p->n2, g.col};
+    }
+}
+
+// The launch list of `nsteps` steps (nsteps >= 1), each sweep in the kernel
+// variant chosen for its slot.
+std::vector<Launch> schedule(const tdse_plan* p, int nsteps, void* target = nullptr) {
+    std::vector<Launch> L;
+    const Geometry g = geometry(p, target ? target : p->psi);
+    auto add = [&](int slot) {
+        SlotSweep w = slot_sweep(p, g, slot);
+        if (w.mode == tdse::C_SINGLE) w.args.nsteps = nsteps;
+        L.push_back({w.mode, w.n, w.args, launcher(p, w.n, w.mode, p->variant[slot]), slot});
+    };
+    if (p->kind == kKind3D) {
+        add(0);
         for (int s = 0; s < nsteps; ++s) {
-            add(tdse::S_KIN, n1, g.yf, 1);
-            if (s + 1 < nsteps) add(tdse::C_KICK_MID, n0, g.x, 2);
-            else add(tdse::C_KICK_LAST, n0, g.x, 3);
+            add(1);
+            add(2);
+            add(3);
+            add(s + 1 < nsteps ? 4 : 5);
         }
     } else if (p->kind == kKindSingle) {
-        SweepArgs a = g.single;
-        a.nsteps = nsteps;
-        add(tdse::C_SINGLE, p->sq[0].n, a, 0);
-    } else {
-        add(tdse::S_KICK_FIRST_TW, p->n2, g.col, 0);
+        add(0);
+    } else {  // 2D and four-step 1D: first, then (kinetic, mid | last) per step
+        add(0);
         for (int s = 0; s < nsteps; ++s) {
-            add(tdse::C_KIN_TW, p->n1, g.row, 1);
-            if (s + 1 < nsteps) add(tdse::S_KICK_MID_TW, p->n2, g.col, 2);
-            else add(tdse::S_KICK_LAST, p->n2, g.col, 3);
+            add(1);
+            add(s + 1 < nsteps ? 2 : 3);
         }
     }
     return L;
@@ -1230,6 +1264,195 @@ const char* tdse_plan_sweep_name(const tdse_plan* p, int slot) {
         case kKindSingle: return kSlotsSingle[slot];
         default: return kSlotsFour[slot];
     }
+}
+
+int tdse_plan_get_variants(const tdse_plan* p, int max_slots, int* variants, int* n_slots) {
+    if (!p || !variants || !n_slots || max_slots < 0) return fail(TDSE_E_INVALID, "null argument");
+    const int slots = slot_count(p);
+    *n_slots = slots;
+    for (int s = 0; s < std::min(slots, max_slots); ++s) variants[s] = p->variant[s];
+    return TDSE_OK;
+}
+
+int tdse_plan_set_variants(tdse_plan* p, const int* variants, int n_slots) {
+    if (!p || !variants) return fail(TDSE_E_INVALID, "null argument");
+    const int slots = slot_count(p);
+    if (n_slots != slots) return fail(TDSE_E_INVALID, "expected one variant per schedule slot");
+    const Geometry g = geometry(p, p->psi);
+    for (int s = 0; s < slots; ++s) {
+        const int v = variants[s];
+        if (v < 0 || v >= tdse::kVariants) return fail(TDSE_E_INVALID, "variant out of range");
+        const SlotSweep w = slot_sweep(p, g, s);
+        if (!launcher(p, w.n, w.mode, v))
+            return fail(TDSE_E_UNSUPPORTED, "variant " + std::to_string(v) + " not compiled for slot " +
+                                                std::to_string(s));
+    }
+    for (int s = 0; s < slots; ++s) p->variant[s] = variants[s];
+    return TDSE_OK;
+}
+
+namespace {
+
+// Timing helpers of the autotuner.  Everything runs on the plan's scratch
+// array, refreshed from the state before each measurement; the state is
+// never written.
+struct Tuner {
+    tdse_plan* p;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    size_t bytes = 0;
+
+    int refresh() {
+        TDSE_HIP(hipMemcpyAsync(p->scratch, p->psi, bytes, hipMemcpyDeviceToDevice, p->stream));
+        return TDSE_OK;
+    }
+
+    // Milliseconds per steady-state step of the current variants: a schedule
+    // of `k` steps, timed from after its first sweep to before its last; the
+    // best of three runs.
+    int per_step(int k, double* out) {
+        const std::vector<Launch> L = schedule(p, k, p->scratch);
+        for (const Launch& l : L)
+            if (!l.fn) return fail(TDSE_E_UNSUPPORTED, "no kernel for a tuned sweep");
+        double best = 1e30;
+        for (int round = 0; round < 3; ++round) {
+            int rc = refresh();
+            if (rc) return rc;
+            TDSE_HIP(L.front().fn(L.front().args, p->ctx->max_blocks, p->stream));
+            TDSE_HIP(hipEventRecord(e0, p->stream));
+            for (size_t i = 1; i + 1 < L.size(); ++i)
+                TDSE_HIP(L[i].fn(L[i].args, p->ctx->max_blocks, p->stream));
+            TDSE_HIP(hipEventRecord(e1, p->stream));
+            TDSE_HIP(L.back().fn(L.back().args, p->ctx->max_blocks, p->stream));
+            TDSE_HIP(hipStreamSynchronize(p->stream));
+            float ms = 0.f;
+            TDSE_HIP(hipEventElapsedTime(&ms, e0, e1));
+            best = std::min(best, static_cast<double>(ms) / k);
+        }
+        *out = best;
+        return TDSE_OK;
+    }
+
+    // Milliseconds per launch of one sweep repeated `k` times (after two
+    // untimed launches); the best of three runs.
+    int per_launch(tdse::LaunchFn fn, const SweepArgs& a, int k, double* out) {
+        double best = 1e30;
+        for (int round = 0; round < 3; ++round) {
+            int rc = refresh();
+            if (rc) return rc;
+            for (int i = 0; i < 2; ++i) TDSE_HIP(fn(a, p->ctx->max_blocks, p->stream));
+            TDSE_HIP(hipEventRecord(e0, p->stream));
+            for (int i = 0; i < k; ++i) TDSE_HIP(fn(a, p->ctx->max_blocks, p->stream));
+            TDSE_HIP(hipEventRecord(e1, p->stream));
+            TDSE_HIP(hipStreamSynchronize(p->stream));
+            float ms = 0.f;
+            TDSE_HIP(hipEventElapsedTime(&ms, e0, e1));
+            best = std::min(best, static_cast<double>(ms) / k);
+        }
+        *out = best;
+        return TDSE_OK;
+    }
+};
+
+// Slots swept once per step (tuned on whole steps) and once per call (tuned
+// per launch).
+void slot_roles(const tdse_plan* p, std::vector<int>* per_step, std::vector<int>* per_call) {
+    switch (p->kind) {
+        case kKind3D:
+            *per_step = {1, 2, 3, 4};
+            *per_call = {0, 5};
+            break;
+        case kKindSingle:
+            *per_step = {};
+            *per_call = {};
+            break;
+        default:
+            *per_step = {1, 2};
+            *per_call = {0, 3};
+    }
+}
+
+int autotune_impl(tdse_plan* p, Tuner& t, int reps, int* chosen) {
+    std::vector<int> per_step, per_call;
+    slot_roles(p, &per_step, &per_call);
+    const Geometry g = geometry(p, p->scratch);
+    for (int s = 0; s < kMaxSlots; ++s) p->variant[s] = 0;
+    int rc;
+    if (!per_step.empty()) {
+        // enough steps for about 2 ms of work per measurement
+        int k = reps;
+        double ms = 0.0;
+        if ((rc = t.per_step(k, &ms))) return rc;
+        if (ms * k < 2.0) k = static_cast<int>(std::min(256.0, std::ceil(2.0 / std::max(ms, 1e-4))));
+        double best = 0.0;
+        if ((rc = t.per_step(k, &best))) return rc;
+        // one pass of coordinate descent over the per-step slots
+        for (int s : per_step) {
+            const SlotSweep w = slot_sweep(p, g, s);
+            int pick = p->variant[s];
+            for (int v = 1; v < tdse::kVariants; ++v) {
+                if (!launcher(p, w.n, w.mode, v)) continue;
+                p->variant[s] = v;
+                double t_v = 0.0;
+                if ((rc = t.per_step(k, &t_v))) return rc;
+                if (t_v < 0.98 * best) {
+                    best = t_v;
+                    pick = v;
+                }
+                p->variant[s] = pick;
+            }
+        }
+    }
+    for (int s : per_call) {
+        const SlotSweep w = slot_sweep(p, g, s);
+        double t0 = 0.0;
+        if ((rc = t.per_launch(launcher(p, w.n, w.mode, 0), w.args, reps, &t0))) return rc;
+        double best = t0;
+        for (int v = 1; v < tdse::kVariants; ++v) {
+            const tdse::LaunchFn fn = launcher(p, w.n, w.mode, v);
+            if (!fn) continue;
+            double t_v = 0.0;
+            if ((rc = t.per_launch(fn, w.args, reps, &t_v))) return rc;
+            if (t_v < 0.98 * best) {
+                best = t_v;
+                p->variant[s] = v;
+            }
+        }
+    }
+    for (int s = 0; s < kMaxSlots; ++s) chosen[s] = p->variant[s];
+    return TDSE_OK;
+}
+
+}  // namespace
+
+int tdse_plan_autotune(tdse_plan* p, int reps, int max_slots, int* variants, int* n_slots) {
+    int rc = check_plan(p);
+    if (rc) return rc;
+    if (reps < 1 || reps > 64 || !variants || !n_slots || max_slots < 0)
+        return fail(TDSE_E_INVALID, "bad autotune arguments");
+    if (!p->has_potential) return fail(TDSE_E_STATE, "tdse_plan_set_potential not called");
+    if (p->flags & TDSE_FLAG_UNFUSED_BASELINE)
+        return fail(TDSE_E_UNSUPPORTED, "the unfused baseline has no kernel variants");
+    if ((rc = set_device(p->ctx))) return rc;
+    const int slots = slot_count(p);
+    *n_slots = slots;
+    if (!p->scratch && (rc = alloc(p, &p->scratch, p->total * p->csize()))) return rc;
+    int saved[kMaxSlots], chosen[kMaxSlots];
+    for (int s = 0; s < kMaxSlots; ++s) saved[s] = p->variant[s];
+    Tuner t{p};
+    t.bytes = static_cast<size_t>(p->total) * p->csize();
+    if (hipEventCreate(&t.e0) != hipSuccess || hipEventCreate(&t.e1) != hipSuccess) {
+        if (t.e0) (void)hipEventDestroy(t.e0);
+        return fail(TDSE_E_DEVICE, "event creation failed");
+    }
+    rc = autotune_impl(p, t, reps, chosen);
+    (void)hipEventDestroy(t.e0);
+    (void)hipEventDestroy(t.e1);
+    if (rc) {
+        for (int s = 0; s < kMaxSlots; ++s) p->variant[s] = saved[s];
+        return rc;
+    }
+    for (int s = 0; s < std::min(slots, max_slots); ++s) variants[s] = chosen[s];
+    return TDSE_OK;
 }
 
 int tdse_plan_step_bytes(const tdse_plan* p, uint64_t* out) {
