@@ -85,7 +85,7 @@ def time_steps(hip, prop, steps, warmup):
 
 def cpu_baseline(solve, v, mask, psi0, sample_steps, threads):
     """Reference CPU FP64 propagator (oracle/_ref) on `sample_steps` steps."""
-    os.environ.setdefault("SES_PARALLEL_WORKERS", str(threads))
+    os.environ["SES_PARALLEL_WORKERS"] = str(threads)
     from oracle import oracle as orc
     kind = "reference" if orc.ref_available() else "port"
     dims, lo, hi = solve.n, solve.lo, solve.hi
@@ -106,8 +106,26 @@ def cpu_baseline(solve, v, mask, psi0, sample_steps, threads):
     t2 = time.perf_counter()
     per_step = ((t2 - t1) - (t1 - t0)) / sample_steps
     return {"value": 1.0 / per_step, "unit": "steps/s", "cores": workers, "kind": kind,
-            "sample": f"{sample_steps} FP64 Strang steps (+ per-step mask) of the same 256^3 "
-                      f"barrier workload after 1 warm-up step; table build subtracted"}
+            "sample": f"{sample_steps} FP64 Strang steps (+ per-step mask) of the same "
+                      f"{dims[0]}x{dims[1]}x{dims[2]} barrier workload after 1 warm-up step; "
+                      f"table build subtracted"}
+
+
+def cpu_baseline_subprocess(args, threads, sample_steps):
+    """cpu_baseline in a child process (the reference's worker pool is sized
+    once per process), e.g. on one core."""
+    import subprocess
+    env = dict(os.environ, SES_PARALLEL_WORKERS=str(threads), OMP_NUM_THREADS=str(threads))
+    code = ("import json, sys; sys.path.insert(0, %r); import bench; "
+            "from shoeridingerequationsolver_amd import workloads; "
+            "s = workloads.barrier(seed=%d, n=%d); "
+            "print(json.dumps(bench.cpu_baseline(s, s.build_potential(), s.build_mask(), "
+            "s.build_psi0(), %d, %d)))") % (ROOT, args.seed, args.n, sample_steps, threads)
+    res = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True,
+                         timeout=600)
+    if res.returncode != 0:
+        raise RuntimeError(res.stderr[-500:])
+    return json.loads(res.stdout.strip().splitlines()[-1])
 
 
 def batch_leg(t, workloads, steps):
@@ -165,11 +183,42 @@ def serving_leg(rates, count, budget):
                                    for k, v in s.items()}
         log(f"serving rate {rate}/s: {s['solves_per_s']:.2f} solves/s, p50 {s['p50_latency_s']:.3f} s,"
             f" p99 {s['p99_latency_s']:.3f} s")
+    # per-request accuracy: the served results at the highest rate (FP32 and
+    # FP64 requests, batched, stepped in quanta) against the sequential plain
+    # FP64 solves of the same requests (the same requests at every rate: only
+    # the arrival times scale with the rate)
     trace = serving.request_stream(seed=5, count=count, rate=max(rates))
-    base = serving.run_sequential(trace, "fp64").summary()
+    srv = serving.Server(budget, quantum=200, keep_results=True)
+    served = {r.name: r for r in srv.run(trace).results}
+    srv.close()
+    seq = serving.run_sequential(trace, "fp64", keep_results=True)
+    base = seq.summary()
+    err = {"fp32": [], "fp64": []}
+    for r in seq.results:
+        ref = r.psi
+        got = served[r.name]
+        err[got.precision].append(float(np.abs(got.psi - ref).max() / np.abs(ref).max()))
     out["sequential_fp64"] = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in base.items()}
-    log(f"sequential FP64: {base['solves_per_s']:.2f} solves/s, p99 {base['p99_latency_s']:.3f} s")
+    out["max_rel_dpsi_vs_sequential_fp64"] = {k: (max(v) if v else None) for k, v in err.items()}
+    out["requests_by_precision"] = {k: len(v) for k, v in err.items()}
+    log(f"sequential FP64: {base['solves_per_s']:.2f} solves/s, p99 {base['p99_latency_s']:.3f} s; "
+        f"served max rel |dpsi| {out['max_rel_dpsi_vs_sequential_fp64']}")
     return out
+
+
+def copy_probe():
+    """A plain device copy of 512 MiB in and 512 MiB out, 5 launches: under
+    rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE its counters give the factor from
+    counter units to bytes."""
+    import torch
+    n = 512 << 20
+    src = torch.ones(n // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    log(f"copy probe: 5 x {n} bytes read and written")
 
 
 def main():
@@ -184,11 +233,15 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=30)
     ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--cpu-steps-1core", type=int, default=4)
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-steps", type=int, default=500)
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=30)
     ap.add_argument("--serve-budget-gib", type=float, default=6.0)
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a plain 512 MiB device copy (torch) 5 times: the "
+                         "reference kernel for scaling HBM byte counters")
     ap.add_argument("--no-tune", action="store_true",
                     help="run every sweep in kernel variant 0 (no autotuning)")
     args = ap.parse_args()
@@ -199,6 +252,9 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
+    if args.copy_probe:
+        copy_probe()
+        return
     import shoeridingerequationsolver_amd as t
     from shoeridingerequationsolver_amd import workloads
 
@@ -291,7 +347,7 @@ def main():
         log(f"batch: {batch}")
     serve = None
     if not args.no_serve:
-        serve = serving_leg([5.0, 20.0, 200.0], args.serve_requests,
+        serve = serving_leg([5.0, 10.0, 20.0, 50.0, 100.0, 200.0], args.serve_requests,
                             int(args.serve_budget_gib * (1 << 30)))
 
     cpu = None
@@ -299,6 +355,9 @@ def main():
         try:
             cpu = cpu_baseline(solve, v, mask, psi0, args.cpu_steps, args.cpu_threads)
             log(f"cpu baseline: {cpu}")
+            one = cpu_baseline_subprocess(args, 1, args.cpu_steps_1core)
+            cpu["one_core"] = {k: one[k] for k in ("value", "unit", "cores", "kind", "sample")}
+            log(f"cpu baseline, one core: {one}")
         except Exception as e:  # the baseline is reported, never required
             log(f"cpu baseline unavailable: {e}")
 

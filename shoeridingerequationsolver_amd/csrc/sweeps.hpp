@@ -471,6 +471,7 @@ struct BatchTable {
     const SweepArgs* items;
     const long long* first_tile;  // nitems + 1 entries
     int nitems;
+    int nsteps;  // steps of the one-block all-steps mode (C_SINGLE) in this call
 };
 
 __device__ __forceinline__ int find_item(const BatchTable& b, long long tile) {
@@ -501,7 +502,13 @@ contig_sweep_batch(BatchTable b) {
     const long long g1 = g0 + chunk < total ? g0 + chunk : total;
     for (long long g = g0; g < g1; ++g) {
         const int it = find_item(b, g);
-        contig_tile<T, N, MODE>(b.items[it], g - b.first_tile[it], lds, lds_tw, tid);
+        if constexpr (MODE == C_SINGLE) {
+            SweepArgs a = b.items[it];
+            a.nsteps = b.nsteps;
+            contig_tile<T, N, MODE>(a, g - b.first_tile[it], lds, lds_tw, tid);
+        } else {
+            contig_tile<T, N, MODE>(b.items[it], g - b.first_tile[it], lds, lds_tw, tid);
+        }
     }
 }
 

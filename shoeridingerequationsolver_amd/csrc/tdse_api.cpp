@@ -188,6 +188,22 @@ int classify(const int n[3], std::vector<Axis>& sq, const double lo[3], const do
     return TDSE_OK;
 }
 
+// Host <-> device copies ordered on the plan's stream, complete on return
+// (the host buffer may then be reused).  Plan streams are non-blocking, so
+// nothing queued on them waits for null-stream copies or memsets: every
+// host-initiated write of plan memory goes through the plan's stream.
+int h2d(const tdse_plan* p, void* dst, const void* src, size_t bytes) {
+    TDSE_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, p->stream));
+    TDSE_HIP(hipStreamSynchronize(p->stream));
+    return TDSE_OK;
+}
+
+int d2h(const tdse_plan* p, void* dst, const void* src, size_t bytes) {
+    TDSE_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, p->stream));
+    TDSE_HIP(hipStreamSynchronize(p->stream));
+    return TDSE_OK;
+}
+
 template <typename T>
 std::vector<T> to_real(const double* src, long long n, bool reduce) {
     std::vector<T> out(static_cast<size_t>(n));
@@ -199,12 +215,10 @@ std::vector<T> to_real(const double* src, long long n, bool reduce) {
 int upload_real(tdse_plan* p, void* dst, const double* src, bool reduce) {
     if (p->fp64()) {
         std::vector<double> h = to_real<double>(src, p->total, reduce);
-        TDSE_HIP(hipMemcpy(dst, h.data(), h.size() * 8, hipMemcpyHostToDevice));
-    } else {
-        std::vector<float> h = to_real<float>(src, p->total, reduce);
-        TDSE_HIP(hipMemcpy(dst, h.data(), h.size() * 4, hipMemcpyHostToDevice));
+        return h2d(p, dst, h.data(), h.size() * 8);
     }
-    return TDSE_OK;
+    std::vector<float> h = to_real<float>(src, p->total, reduce);
+    return h2d(p, dst, h.data(), h.size() * 4);
 }
 
 int alloc(tdse_plan* p, void** out, uint64_t bytes) {
@@ -248,8 +262,7 @@ int make_twiddles(tdse_plan* p, int N) {
         w[2 * m] = std::cos(th);
         w[2 * m + 1] = std::sin(th);
     }
-    TDSE_HIP(hipMemcpy(d, w.data(), w.size() * 8, hipMemcpyHostToDevice));
-    return TDSE_OK;
+    return h2d(p, d, w.data(), w.size() * 8);
 }
 
 // Kinetic phase table of one axis (or, for a four-step 1D grid, of the whole
@@ -269,8 +282,7 @@ int make_kinetic(tdse_plan* p, int slot, const Axis& ax) {
         k[2 * j] = std::cos(th);
         k[2 * j + 1] = std::sin(th);
     }
-    TDSE_HIP(hipMemcpy(d, k.data(), k.size() * 8, hipMemcpyHostToDevice));
-    return TDSE_OK;
+    return h2d(p, d, k.data(), k.size() * 8);
 }
 
 // The reference SplitOperator3D tables (core/src/propagator.ixx:72-101):
@@ -292,8 +304,7 @@ int make_baseline_kinetic(tdse_plan* p) {
                 t[2 * f] = std::cos(th);
                 t[2 * f + 1] = std::sin(th);
             }
-    TDSE_HIP(hipMemcpy(p->base_kin, t.data(), t.size() * 8, hipMemcpyHostToDevice));
-    return TDSE_OK;
+    return h2d(p, p->base_kin, t.data(), t.size() * 8);
 }
 
 SweepArgs base_args(const tdse_plan* p) {
@@ -526,9 +537,9 @@ struct BatchGroup {
     void* d_first = nullptr;  // long long[nitems + 1]
     int nitems = 0;
     long long tiles = 0;
-    tdse::BatchTable table() const {
+    tdse::BatchTable table(int nsteps) const {
         return {static_cast<const SweepArgs*>(d_items), static_cast<const long long*>(d_first),
-                nitems};
+                nitems, nsteps};
     }
 };
 
@@ -625,10 +636,12 @@ int upload_group(tdse_batch* b, BatchGroup& g, const std::vector<SweepArgs>& ite
     int rc;
     if ((rc = batch_alloc(b, &g.d_items, items.size() * sizeof(SweepArgs)))) return rc;
     if ((rc = batch_alloc(b, &g.d_first, first.size() * sizeof(long long)))) return rc;
-    TDSE_HIP(hipMemcpy(g.d_items, items.data(), items.size() * sizeof(SweepArgs),
-                       hipMemcpyHostToDevice));
-    TDSE_HIP(hipMemcpy(g.d_first, first.data(), first.size() * sizeof(long long),
-                       hipMemcpyHostToDevice));
+    // ordered on the batch stream (non-blocking, like the lanes that wait on it)
+    TDSE_HIP(hipMemcpyAsync(g.d_items, items.data(), items.size() * sizeof(SweepArgs),
+                            hipMemcpyHostToDevice, b->stream));
+    TDSE_HIP(hipMemcpyAsync(g.d_first, first.data(), first.size() * sizeof(long long),
+                            hipMemcpyHostToDevice, b->stream));
+    TDSE_HIP(hipStreamSynchronize(b->stream));
     g.fn = g.precision == TDSE_FP64 ? tdse::find_batch_launcher_f64(g.n, g.mode)
                                     : tdse::find_batch_launcher_f32(g.n, g.mode);
     if (!g.fn) return fail(TDSE_E_UNSUPPORTED, "no batched kernel for line length " + std::to_string(g.n));
@@ -670,7 +683,7 @@ int build_groups(tdse_batch* b, std::vector<BatchGroup>& out,
 // after which its own plans are ready (their previous sweeps), so groups of
 // different plans overlap freely across lanes: a dependency DAG, not a
 // barrier per sub-phase.
-int launch_phase(tdse_batch* b, const std::vector<BatchGroup>& gs) {
+int launch_phase(tdse_batch* b, const std::vector<BatchGroup>& gs, int nsteps = 1) {
     const int max_blocks = kBatchMaxBlocks;
     for (const BatchGroup& g : gs) {
         hipStream_t st = g.lane == 0 ? b->stream : b->side[g.lane - 1];
@@ -684,7 +697,7 @@ int launch_phase(tdse_batch* b, const std::vector<BatchGroup>& gs) {
             TDSE_HIP(hipStreamWaitEvent(st, e, 0));
             if (nw < 8) waited[nw++] = e;
         }
-        TDSE_HIP(g.fn(g.table(), g.tiles, max_blocks, st));
+        TDSE_HIP(g.fn(g.table(nsteps), g.tiles, max_blocks, st));
         TDSE_HIP(hipEventRecord(g.done, st));
         for (int pi : g.plans) b->ready[pi] = g.done;
     }
@@ -901,9 +914,10 @@ int tdse_plan_create(tdse_ctx* ctx, const int n[3], const double lo[3], const do
         if ((rc = alloc(p, &p->base_kin, total * 16))) return bail(rc);
         if ((rc = make_baseline_kinetic(p))) return bail(rc);
     }
-    if (hipMemset(p->psi, 0, total * p->csize()) != hipSuccess ||
-        hipMemset(p->phi, 0, total * p->rsize()) != hipSuccess ||
-        hipMemset(p->v, 0, total * p->rsize()) != hipSuccess) {
+    if (hipMemsetAsync(p->psi, 0, total * p->csize(), p->stream) != hipSuccess ||
+        hipMemsetAsync(p->phi, 0, total * p->rsize(), p->stream) != hipSuccess ||
+        hipMemsetAsync(p->v, 0, total * p->rsize(), p->stream) != hipSuccess ||
+        hipStreamSynchronize(p->stream) != hipSuccess) {
         (void)hipGetLastError();
         return bail(fail(TDSE_E_DEVICE, "hipMemset failed"));
     }
@@ -953,7 +967,7 @@ int tdse_plan_set_potential(tdse_plan* p, const double* v) {
             h[2 * i] = std::cos(th[i]);
             h[2 * i + 1] = std::sin(th[i]);
         }
-        TDSE_HIP(hipMemcpy(p->base_half, h.data(), h.size() * 8, hipMemcpyHostToDevice));
+        if ((rc = h2d(p, p->base_half, h.data(), h.size() * 8))) return rc;
     }
     p->has_potential = true;
     return TDSE_OK;
@@ -1015,15 +1029,10 @@ int tdse_state_upload(tdse_plan* p, const double* psi) {
     if (rc) return rc;
     if (!psi) return fail(TDSE_E_INVALID, "null state");
     if ((rc = set_device(p->ctx))) return rc;
-    TDSE_HIP(hipStreamSynchronize(p->stream));
-    if (p->fp64()) {
-        TDSE_HIP(hipMemcpy(p->psi, psi, p->total * 16, hipMemcpyHostToDevice));
-    } else {
-        std::vector<float> f(2 * static_cast<size_t>(p->total));
-        for (long long i = 0; i < 2 * p->total; ++i) f[i] = static_cast<float>(psi[i]);
-        TDSE_HIP(hipMemcpy(p->psi, f.data(), f.size() * 4, hipMemcpyHostToDevice));
-    }
-    return TDSE_OK;
+    if (p->fp64()) return h2d(p, p->psi, psi, p->total * 16);
+    std::vector<float> f(2 * static_cast<size_t>(p->total));
+    for (long long i = 0; i < 2 * p->total; ++i) f[i] = static_cast<float>(psi[i]);
+    return h2d(p, p->psi, f.data(), f.size() * 4);
 }
 
 int tdse_state_download(tdse_plan* p, double* psi) {
@@ -1031,14 +1040,10 @@ int tdse_state_download(tdse_plan* p, double* psi) {
     if (rc) return rc;
     if (!psi) return fail(TDSE_E_INVALID, "null state");
     if ((rc = set_device(p->ctx))) return rc;
-    TDSE_HIP(hipStreamSynchronize(p->stream));
-    if (p->fp64()) {
-        TDSE_HIP(hipMemcpy(psi, p->psi, p->total * 16, hipMemcpyDeviceToHost));
-    } else {
-        std::vector<float> f(2 * static_cast<size_t>(p->total));
-        TDSE_HIP(hipMemcpy(f.data(), p->psi, f.size() * 4, hipMemcpyDeviceToHost));
-        for (long long i = 0; i < 2 * p->total; ++i) psi[i] = f[i];
-    }
+    if (p->fp64()) return d2h(p, psi, p->psi, p->total * 16);
+    std::vector<float> f(2 * static_cast<size_t>(p->total));
+    if ((rc = d2h(p, f.data(), p->psi, f.size() * 4))) return rc;
+    for (long long i = 0; i < 2 * p->total; ++i) psi[i] = f[i];
     return TDSE_OK;
 }
 
@@ -1581,18 +1586,9 @@ int tdse_batch_step(tdse_batch* b, int nsteps, int use_graph) {
     if (rc) return rc;
     // the plans' own queues must be drained before the batch touches their state
     for (tdse_plan* p : b->plans) TDSE_HIP(hipStreamSynchronize(p->stream));
-    // refresh nsteps of the single-line groups (tables are small)
-    for (BatchGroup& g : b->init) {
-        if (g.mode != tdse::C_SINGLE) continue;
-        std::vector<SweepArgs> items(g.nitems);
-        TDSE_HIP(hipMemcpy(items.data(), g.d_items, items.size() * sizeof(SweepArgs),
-                           hipMemcpyDeviceToHost));
-        for (auto& a : items) a.nsteps = nsteps;
-        TDSE_HIP(hipMemcpy(g.d_items, items.data(), items.size() * sizeof(SweepArgs),
-                           hipMemcpyHostToDevice));
-    }
     if ((rc = open_call(b))) return rc;
-    if ((rc = launch_phase(b, b->init))) return rc;
+    // (the one-block 1D groups of the first phase take nsteps as an argument)
+    if ((rc = launch_phase(b, b->init, nsteps))) return rc;
     if ((rc = close_call(b))) return rc;
     const int mid_rounds = nsteps - 1;
     if ((rc = open_call(b))) return rc;

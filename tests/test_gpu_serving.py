@@ -73,3 +73,27 @@ def test_device_build_matches_host_builders(tdse, shape, kind):
     a.step(10)
     b.step(10)
     assert np.abs(a.download() - b.download()).max() < 1e-12 * np.abs(pa).max()
+
+
+def test_large_device_builds_are_complete(tdse):
+    """Plan memory is cleared and built on the plan's own stream: a large
+    FP64 plan built on the GPU equals the host builders everywhere, every
+    time (a clear left on another stream could overwrite part of the build)."""
+    from shoeridingerequationsolver_amd import workloads
+    s = workloads.hydrogen(seed=7, n=256, steps=3, precision="fp64")
+    s.potential, s.params = "regularized_coulomb", {"Z": 1.0}
+    host_psi = s.build_psi0()
+    a = tdse.Propagator(s.grid(), s.build_potential(), s.dt, precision="fp64")
+    a.upload(host_psi)
+    a.step(3)
+    want = a.download()
+    a.close()
+    scale = np.abs(host_psi).max()
+    for rep in range(3):
+        print(f"case 256^3 fp64 device build rep {rep}", flush=True)
+        b = tdse.Propagator(s.grid(), None, s.dt, precision="fp64")
+        b.build(s.recipe())
+        assert np.abs(b.download() - host_psi).max() < 1e-11 * scale
+        b.step(3)
+        assert np.abs(b.download() - want).max() < 1e-11 * scale
+        b.close()
