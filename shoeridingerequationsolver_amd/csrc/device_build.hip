@@ -79,9 +79,11 @@ build_cells(BuildArgs a) {
         v[f] = static_cast<T>(pot);
         phi[f] = static_cast<T>(reduce_2pi_dev(-0.5 * pot * a.dt));
         if (mask) {
-            mask[f] = static_cast<T>(ramp(x, a.lo[0], a.hi[0], a.n[0], r.mask_width) *
-                                     ramp(y, a.lo[1], a.hi[1], a.n[1], r.mask_width) *
-                                     ramp(z, a.lo[2], a.hi[2], a.n[2], r.mask_width));
+            const double m = ramp(x, a.lo[0], a.hi[0], a.n[0], r.mask_width) *
+                             ramp(y, a.lo[1], a.hi[1], a.n[1], r.mask_width) *
+                             ramp(z, a.lo[2], a.hi[2], a.n[2], r.mask_width);
+            // FP32 plans store 1 - m (full precision just below m = 1)
+            mask[f] = static_cast<T>(sizeof(T) == 4 ? 1.0 - m : m);
         }
         const double env = envelope(x, r.r0[0], r.sigma[0]) * envelope(y, r.r0[1], r.sigma[1]) *
                            envelope(z, r.r0[2], r.sigma[2]);

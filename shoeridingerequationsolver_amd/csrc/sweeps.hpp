@@ -160,6 +160,18 @@ __device__ __forceinline__ void sincos_small(double x, double* s, double* c) {
     *c = ((qi + 1) & 2) ? -cc : cc;
 }
 
+// Absorbing-mask storage: FP64 plans store m, FP32 plans store 1 - m, so a
+// value just below 1 (the inner edge of the absorbing layer, where most of
+// the absorption happens) keeps its full precision as a small complement
+// instead of rounding to 1.
+template <typename T>
+__device__ __forceinline__ double mask_value(T stored) {
+    if constexpr (sizeof(T) == 8) return stored;
+    else return 1.0 - static_cast<double>(stored);
+}
+template <typename T>
+constexpr T kMaskOne = sizeof(T) == 8 ? T(1) : T(0);  // stored form of m = 1
+
 // kick factor m * exp(i * f * phi); f = 1 (half kick) or 2 (merged full kick)
 template <typename T>
 __device__ __forceinline__ cx<double> kick_factor(T phi, double f, double m) {
@@ -247,28 +259,34 @@ struct Line {
     // all loads are issued together, so their latency is paid once per line.
     // (Issuing them before the inverse transform measured no faster.)
     T kp[E];  // phase
-    T km[E];  // mask (1 without one)
+    T km[E];  // mask, stored form (see mask_value)
     __device__ __forceinline__ void load_kick(bool with_mask) {
         if (!valid) return;
         const bool m = with_mask && a.mask;
         for_each_point<N, E, R1>(t, [&](int s, int, int j, int ro) {
             const uint32_t vo = (base + j * es) * RS, so = ro * es * RS;
             kp[s] = F.ld(vo, so);
-            km[s] = m ? M.ld(vo, so) : T(1);
+            km[s] = m ? M.ld(vo, so) : kMaskOne<T>;
         });
     }
     // v *= m exp(i f phi), the exponential in FP64.  Waves whose phases are
-    // all zero take the real product instead, which gives the same bits
-    // (the factor (m, 0) times v rounds each component once either way).
+    // all zero skip the exponential and multiply by the real m alone,
+    // rounding each component once.
     __device__ __forceinline__ void apply_kick(double f, bool use_mask) {
         if (!valid) return;
         bool nz = false;
         for (int s = 0; s < E; ++s) nz |= kp[s] != T(0);
         if (__any(nz)) {
             for (int s = 0; s < E; ++s)
-                v[s] = mul_d(v[s], kick_factor<T>(kp[s], f, use_mask ? static_cast<double>(km[s]) : 1.0));
+                v[s] = mul_d(v[s], kick_factor<T>(kp[s], f, use_mask ? mask_value(km[s]) : 1.0));
         } else if (use_mask && a.mask) {
-            for (int s = 0; s < E; ++s) v[s] = cscale(v[s], km[s]);
+            if constexpr (sizeof(T) == 4) {
+                // v (1 - c) = v - v c with one rounding per component
+                for (int s = 0; s < E; ++s)
+                    v[s] = {__builtin_fmaf(-v[s].x, km[s], v[s].x), __builtin_fmaf(-v[s].y, km[s], v[s].y)};
+            } else {
+                for (int s = 0; s < E; ++s) v[s] = cscale(v[s], km[s]);
+            }
         }
     }
     __device__ __forceinline__ void kick(double f, bool with_mask) {

@@ -204,20 +204,29 @@ int d2h(const tdse_plan* p, void* dst, const void* src, size_t bytes) {
     return TDSE_OK;
 }
 
+enum RealForm { kPlain, kPhase, kMask };
+
+// A per-cell real array in the plan's precision: plain values, kick phases
+// reduced to [-pi, pi], or an absorbing mask (stored as 1 - m in FP32 plans,
+// m in FP64 plans; the kernels undo it in FP64).
 template <typename T>
-std::vector<T> to_real(const double* src, long long n, bool reduce) {
+std::vector<T> to_real(const double* src, long long n, RealForm form) {
     std::vector<T> out(static_cast<size_t>(n));
-    for (long long i = 0; i < n; ++i)
-        out[i] = static_cast<T>(reduce ? reduce_2pi_host(src[i]) : src[i]);
+    for (long long i = 0; i < n; ++i) {
+        double x = src[i];
+        if (form == kPhase) x = reduce_2pi_host(x);
+        else if (form == kMask && sizeof(T) == 4) x = 1.0 - x;
+        out[i] = static_cast<T>(x);
+    }
     return out;
 }
 
-int upload_real(tdse_plan* p, void* dst, const double* src, bool reduce) {
+int upload_real(tdse_plan* p, void* dst, const double* src, RealForm form) {
     if (p->fp64()) {
-        std::vector<double> h = to_real<double>(src, p->total, reduce);
+        std::vector<double> h = to_real<double>(src, p->total, form);
         return h2d(p, dst, h.data(), h.size() * 8);
     }
-    std::vector<float> h = to_real<float>(src, p->total, reduce);
+    std::vector<float> h = to_real<float>(src, p->total, form);
     return h2d(p, dst, h.data(), h.size() * 4);
 }
 
@@ -959,8 +968,8 @@ int tdse_plan_set_potential(tdse_plan* p, const double* v) {
     }
     // reduced modulo 2 pi in FP64 for both precisions (the kernels' sin/cos
     // take |angle| <= 8)
-    if ((rc = upload_real(p, p->phi, th.data(), true))) return rc;
-    if ((rc = upload_real(p, p->v, v, false))) return rc;
+    if ((rc = upload_real(p, p->phi, th.data(), kPhase))) return rc;
+    if ((rc = upload_real(p, p->v, v, kPlain))) return rc;
     if (p->base_half) {
         std::vector<double> h(2 * static_cast<size_t>(p->total));
         for (long long i = 0; i < p->total; ++i) {
@@ -983,7 +992,7 @@ int tdse_plan_set_mask(tdse_plan* p, const double* mask) {
         return TDSE_OK;
     }
     if (!p->mask && (rc = alloc(p, &p->mask, p->total * p->rsize()))) return rc;
-    return upload_real(p, p->mask, mask, false);
+    return upload_real(p, p->mask, mask, kMask);
 }
 
 int tdse_plan_build(tdse_plan* p, const tdse_recipe* r) {
