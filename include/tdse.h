@@ -60,8 +60,14 @@ extern "C" {
  * tables, per-axis forward FFT sweeps, conjugation-identity inverse with
  * separate conj passes, separate mask pass; core/src/propagator.ixx:110-119,
  * core/src/fft.ixx:102-188).  It is the unfused comparison point of the
- * fused path, not a production mode. */
+ * fused path, not a production mode.
+ * TDSE_FLAG_IMAGINARY_TIME: the plan relaxes instead of propagating
+ * (ImaginaryTimePropagator1D/3D, core/src/imaginary_time.ixx:21-144): dt is
+ * dtau, the weights are real (exp(-V dtau/2), exp(-k^2 dtau/2m)) and every
+ * tdse_step step ends with psi /= sqrt(sum |psi|^2 dV) (left unchanged for a
+ * zero field, core/src/field.ixx:78-87).  No mask, no batching. */
 #define TDSE_FLAG_UNFUSED_BASELINE 1u
+#define TDSE_FLAG_IMAGINARY_TIME 2u
 
 typedef struct tdse_ctx tdse_ctx;
 typedef struct tdse_plan tdse_plan;
@@ -85,7 +91,7 @@ int tdse_plan_bytes(const int n[3], int precision, int with_mask, uint64_t* out)
 /* A propagator for one grid: replaces the SplitOperator1D/3D constructor
  * (core/src/propagator.ixx:23, 72).  lo/hi are the periodic box bounds per
  * axis (Grid1D::xmin/xmax), mass the effective mass (default 1.0 in the
- * reference), flags 0 or TDSE_FLAG_UNFUSED_BASELINE. */
+ * reference), flags 0, TDSE_FLAG_UNFUSED_BASELINE or TDSE_FLAG_IMAGINARY_TIME. */
 int tdse_plan_create(tdse_ctx* ctx, const int n[3], const double lo[3], const double hi[3],
                      double dt, double mass, int precision, unsigned flags, tdse_plan** out);
 int tdse_plan_destroy(tdse_plan* plan);
@@ -148,6 +154,27 @@ const char* tdse_plan_sweep_name(const tdse_plan* plan, int slot);
  * sweep, kick phase and mask once). */
 int tdse_plan_step_bytes(const tdse_plan* plan, uint64_t* out);
 
+/* Imaginary time (plans created with TDSE_FLAG_IMAGINARY_TIME).
+ * tdse_step(plan, n) = ImaginaryTimePropagator*::relax(psi, n)
+ * (core/src/imaginary_time.ixx:43-53, 112-122).
+ * tdse_relax_deflated = relax_deflated(psi, lower, n)
+ * (core/src/imaginary_time.ixx:124-141): after each step, psi -= <phi|psi> phi
+ * for every lower state phi in order (<a|b> = sum conj(a) b dV,
+ * core/src/field.ixx:88-99), then normalise.  Lower states are plans of the
+ * same shape and precision (their current states).
+ * tdse_plan_relax_stats: the norm before renormalisation of the last step, the
+ * energy estimate -ln(norm_sq) / (2 dtau) (the norm of a normalised state
+ * decays as exp(-2 E dtau), solver/src/vk_engine.ixx:3017-3023) and the peak
+ * density max |psi|^2 / norm_sq of that step (after a deflated call: of the
+ * step before projection). */
+typedef struct tdse_relax_stats {
+    double norm_sq;
+    double energy;
+    double peak;
+} tdse_relax_stats;
+int tdse_relax_deflated(tdse_plan* plan, const tdse_plan* const* lower, int nlower, int nsteps);
+int tdse_plan_relax_stats(tdse_plan* plan, tdse_relax_stats* out);
+
 /* Kernel variants.  Each sweep of the step schedule (slot, as above) can run
  * one of up to 4 compiled variants of its kernel: the base or an alternative
  * radix list (a different number of points per thread) times one or two
@@ -195,6 +222,7 @@ int tdse_batch_launches_per_round(const tdse_batch* batch, int* out);
 #define TDSE_POT_REGULARIZED_COULOMB 2 /* -strength / |r - center|, nucleus cell average */
 #define TDSE_POT_COULOMB_FLOOR 3       /* -strength / max(|r - center|, h/2) */
 #define TDSE_POT_BARRIER 4             /* strength on x_lo <= x < x_hi */
+#define TDSE_POT_SOFT_COULOMB 5        /* -strength / sqrt(|r - center|^2 + softening^2) */
 typedef struct tdse_recipe {
     int potential;
     double strength;
@@ -202,6 +230,7 @@ typedef struct tdse_recipe {
     double x_lo, x_hi;
     double mask_width;                 /* <= 0: no mask */
     double r0[3], sigma[3], k0[3];     /* Gaussian packet (sigma: density std-dev) */
+    double softening;                  /* TDSE_POT_SOFT_COULOMB */
 } tdse_recipe;
 int tdse_plan_build(tdse_plan* plan, const tdse_recipe* recipe);
 
@@ -211,6 +240,7 @@ int tdse_plan_build(tdse_plan* plan, const tdse_recipe* recipe);
  *   barrier             v0 on x_lo <= x < x_hi      core/src/potential.ixx:102-132
  *   regularized_coulomb -Z/r, nucleus cell -2.3800774 Z/h   core/src/potential.ixx:134-154
  *   coulomb_floor       -Z / max(r, h/2)            (no reference builder)
+ *   soft_coulomb        -Z / sqrt(r^2 + a^2)        core/src/potential.ixx:25-33, 51-65
  *   absorbing_mask      separable sin^2 ramp        core/src/potential.ixx:194-233
  *   gaussian            normalised packet, interleaved complex128 out
  *                                                   core/src/wavepacket.ixx:16-48 */
@@ -222,6 +252,8 @@ int tdse_build_regularized_coulomb(const int n[3], const double lo[3], const dou
                                    double z, const double c[3], double* v);
 int tdse_build_coulomb_floor(const int n[3], const double lo[3], const double hi[3], double z,
                              const double c[3], double* v);
+int tdse_build_soft_coulomb(const int n[3], const double lo[3], const double hi[3], double z,
+                            double a, const double c[3], double* v);
 int tdse_build_absorbing_mask(const int n[3], const double lo[3], const double hi[3],
                               double width, double* m);
 int tdse_build_gaussian(const int n[3], const double lo[3], const double hi[3], const double r0[3],

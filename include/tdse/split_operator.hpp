@@ -7,6 +7,7 @@
 //   ses::SplitOperator1D(g, V, dt)       core/src/propagator.ixx:21-64
 //   ses::SplitOperator3D(g, V, dt, m)    core/src/propagator.ixx:66-134
 //   ses::WavepacketSimulation            core/src/simulation.ixx:22-88
+//   ses::ImaginaryTimePropagator1D/3D    core/src/imaginary_time.ixx:21-144
 //
 // step(Field&, n) uploads the field, runs n Strang steps on the GPU and reads
 // it back (the reference's in-place contract).  Device-resident stepping is
@@ -117,14 +118,15 @@ enum class Precision : int { FP32 = TDSE_FP32, FP64 = TDSE_FP64 };
 class Propagator {
 public:
     Propagator(const Grid3D& g, const std::vector<double>& potential, double dt, double mass = 1.0,
-               Precision prec = Precision::FP64, Context& ctx = Context::instance()) {
+               Precision prec = Precision::FP64, Context& ctx = Context::instance(),
+               unsigned flags = 0u) {
         const int n[3] = {g.x.n, g.y.n, g.z.n};
         const double lo[3] = {g.x.xmin, g.y.xmin, g.z.xmin};
         const double hi[3] = {g.x.xmax, g.y.xmax, g.z.xmax};
         if (static_cast<int>(potential.size()) != g.size())
             throw std::invalid_argument("potential size does not match the grid");
         tdse_plan* p = nullptr;
-        check(tdse_plan_create(ctx.get(), n, lo, hi, dt, mass, static_cast<int>(prec), 0u, &p));
+        check(tdse_plan_create(ctx.get(), n, lo, hi, dt, mass, static_cast<int>(prec), flags, &p));
         plan_.reset(p);
         size_ = g.size();
         check(tdse_plan_set_potential(plan_.get(), potential.data()));
@@ -192,6 +194,72 @@ public:
 
 private:
     double dt_;
+    std::unique_ptr<Propagator> prop_;
+};
+
+// ses::ImaginaryTimePropagator1D (core/src/imaginary_time.ixx:21-63): relax
+// steps e^{-H dtau} with a renormalisation after each.
+class ImaginaryTimePropagator1D {
+public:
+    ImaginaryTimePropagator1D(const Grid1D& g, const std::vector<double>& potential, double dtau,
+                              Precision prec = Precision::FP64)
+        : prop_(std::make_unique<Propagator>(as_grid3(g), potential, dtau, 1.0, prec,
+                                             Context::instance(), TDSE_FLAG_IMAGINARY_TIME)) {}
+    void relax(Field1D& psi, int nsteps) const {
+        if (nsteps <= 0) return;
+        prop_->upload(psi.data());
+        prop_->step(nsteps);
+        prop_->download(psi.data());
+    }
+    tdse_relax_stats stats() const {
+        tdse_relax_stats s{};
+        check(tdse_plan_relax_stats(prop_->get(), &s));
+        return s;
+    }
+
+private:
+    std::unique_ptr<Propagator> prop_;
+};
+
+// ses::ImaginaryTimePropagator3D (core/src/imaginary_time.ixx:65-144).
+class ImaginaryTimePropagator3D {
+public:
+    ImaginaryTimePropagator3D(const Grid3D& g, const std::vector<double>& potential, double dtau,
+                              double mass = 1.0, Precision prec = Precision::FP64)
+        : grid_(g), dtau_(dtau), prec_(prec),
+          prop_(std::make_unique<Propagator>(g, potential, dtau, mass, prec, Context::instance(),
+                                             TDSE_FLAG_IMAGINARY_TIME)) {}
+    void relax(Field3D& psi, int nsteps) const {
+        if (nsteps <= 0) return;
+        prop_->upload(psi.data());
+        prop_->step(nsteps);
+        prop_->download(psi.data());
+    }
+    // Gram-Schmidt deflation of `lower` each step -> the next state up.
+    void relax_deflated(Field3D& psi, const std::vector<const Field3D*>& lower, int nsteps) const {
+        std::vector<std::unique_ptr<Propagator>> held;
+        std::vector<const tdse_plan*> plans;
+        const std::vector<double> zero(static_cast<std::size_t>(grid_.size()), 0.0);
+        for (const Field3D* q : lower) {
+            held.push_back(std::make_unique<Propagator>(grid_, zero, dtau_, 1.0, prec_));
+            held.back()->upload(q->data());
+            plans.push_back(held.back()->get());
+        }
+        prop_->upload(psi.data());
+        check(tdse_relax_deflated(prop_->get(), plans.data(), static_cast<int>(plans.size()),
+                                  nsteps));
+        prop_->download(psi.data());
+    }
+    tdse_relax_stats stats() const {
+        tdse_relax_stats s{};
+        check(tdse_plan_relax_stats(prop_->get(), &s));
+        return s;
+    }
+
+private:
+    Grid3D grid_;
+    double dtau_;
+    Precision prec_;
     std::unique_ptr<Propagator> prop_;
 };
 

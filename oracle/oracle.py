@@ -197,6 +197,79 @@ def so3d_evolve(v, dims, lo, hi, dt, psi, nsteps, mass=1.0, mask=None, any_size=
     return psi.reshape(shape)
 
 
+def it1d_tables(v, xmin, xmax, dtau):
+    """ImaginaryTimePropagator1D weights (core/src/imaginary_time.ixx:24-40)."""
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    hw = np.empty(v.size)
+    kw = np.empty(v.size)
+    orc().orc_it1d_tables(_p(v), v.size, C.c_double(xmin), C.c_double(xmax), C.c_double(dtau),
+                          _p(hw), _p(kw))
+    return hw, kw
+
+
+def it1d_relax(v, xmin, xmax, dtau, psi, nsteps):
+    """ImaginaryTimePropagator1D::relax; returns (psi, norm of the last step
+    before its renormalisation)."""
+    hw, kw = it1d_tables(v, xmin, xmax, dtau)
+    psi = _c128(psi).copy()
+    last = C.c_double(0.0)
+    dx = (xmax - xmin) / psi.size
+    if orc().orc_it1d_relax(_p(psi), psi.size, C.c_double(dx), _p(hw), _p(kw), nsteps,
+                            C.byref(last)) != 0:
+        raise ValueError("size must be a power of two")
+    return psi, last.value
+
+
+def it3d_tables(v, dims, lo, hi, dtau, mass=1.0):
+    v = np.ascontiguousarray(v, dtype=np.float64).ravel()
+    hw = np.empty(v.size)
+    kw = np.empty(v.size)
+    orc().orc_it3d_tables(_p(v), _ivec(dims), _dvec(lo), _dvec(hi), C.c_double(dtau),
+                          C.c_double(mass), _p(hw), _p(kw))
+    return hw, kw
+
+
+def it3d_relax(v, dims, lo, hi, dtau, psi, nsteps, mass=1.0, lower=(), any_size=False):
+    """ImaginaryTimePropagator3D::relax, or relax_deflated against `lower`
+    (core/src/imaginary_time.ixx:100-134); returns (psi, last pre-renorm norm)."""
+    hw, kw = it3d_tables(v, dims, lo, hi, dtau, mass)
+    psi = _c128(psi).copy()
+    shape = psi.shape
+    psi = psi.ravel()
+    low = [_c128(q).ravel() for q in lower]
+    arr = (C.c_void_p * max(1, len(low)))(*[q.ctypes.data for q in low])
+    dv = 1.0
+    for a in range(3):
+        dv *= (hi[a] - lo[a]) / dims[a]
+    last = C.c_double(0.0)
+    if orc().orc_it3d_relax(_p(psi), _ivec(dims), C.c_double(dv), _p(hw), _p(kw), arr, len(low),
+                            nsteps, 1 if any_size else 0, C.byref(last)) != 0:
+        raise ValueError("size must be a power of two")
+    return psi.reshape(shape), last.value
+
+
+def inner_product(a, b, dv):
+    a = _c128(a).ravel()
+    b = _c128(b).ravel()
+    out = np.empty(2)
+    orc().orc_inner_product(_p(a), _p(b), C.c_size_t(a.size), C.c_double(dv), _p(out))
+    return complex(out[0], out[1])
+
+
+def soft_coulomb_1d(xmin, xmax, n, z, a, x0=0.0):
+    v = np.empty(n)
+    orc().orc_soft_coulomb_1d(C.c_double(xmin), C.c_double(xmax), n, C.c_double(z), C.c_double(a),
+                              C.c_double(x0), _p(v))
+    return v
+
+
+def soft_coulomb_3d(dims, lo, hi, z, a, c):
+    v = np.empty(int(np.prod(dims)))
+    orc().orc_soft_coulomb_3d(_ivec(dims), _dvec(lo), _dvec(hi), C.c_double(z), C.c_double(a),
+                              _dvec(c), _p(v))
+    return v
+
+
 def norm_sq(psi, cell_volume):
     psi = _c128(psi)
     return orc().orc_norm_sq(_p(psi), C.c_size_t(psi.size), C.c_double(cell_volume))
@@ -484,3 +557,56 @@ class Ref:
         v = np.ascontiguousarray(v, dtype=np.float64).ravel()
         return ref().ref_mean_energy_3d(_p(psi), _ivec(dims), _dvec(lo), _dvec(hi), _p(v),
                                         C.c_double(mass))
+
+    @staticmethod
+    def it1d_relax(v, xmin, xmax, dtau, psi, nsteps):
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        psi = _c128(psi).copy()
+        if ref().ref_it1d_relax(_p(v), v.size, C.c_double(xmin), C.c_double(xmax),
+                                C.c_double(dtau), _p(psi), nsteps) != 0:
+            raise ValueError("size must be a power of two")
+        return psi
+
+    @staticmethod
+    def it3d_tables(v, dims, lo, hi, dtau, mass=1.0):
+        v = np.ascontiguousarray(v, dtype=np.float64).ravel()
+        hw = np.empty(v.size)
+        kw = np.empty(v.size)
+        ref().ref_it3d_tables(_p(v), _ivec(dims), _dvec(lo), _dvec(hi), C.c_double(dtau),
+                              C.c_double(mass), _p(hw), _p(kw))
+        return hw, kw
+
+    @staticmethod
+    def it3d_relax(v, dims, lo, hi, dtau, psi, nsteps, mass=1.0, lower=()):
+        v = np.ascontiguousarray(v, dtype=np.float64).ravel()
+        psi = _c128(psi).copy()
+        shape = psi.shape
+        psi = psi.ravel()
+        low = [_c128(q).ravel() for q in lower]
+        arr = (C.c_void_p * max(1, len(low)))(*[q.ctypes.data for q in low])
+        if ref().ref_it3d_relax(_p(v), _ivec(dims), _dvec(lo), _dvec(hi), C.c_double(dtau),
+                                C.c_double(mass), _p(psi), arr, len(low), nsteps) != 0:
+            raise ValueError("size must be a power of two")
+        return psi.reshape(shape)
+
+    @staticmethod
+    def inner_product(a, b, dims, lo, hi):
+        a = _c128(a).ravel()
+        b = _c128(b).ravel()
+        out = np.empty(2)
+        ref().ref_inner_product(_p(a), _p(b), _ivec(dims), _dvec(lo), _dvec(hi), _p(out))
+        return complex(out[0], out[1])
+
+    @staticmethod
+    def soft_coulomb_1d(xmin, xmax, n, z, a, x0=0.0):
+        v = np.empty(n)
+        ref().ref_soft_coulomb_1d(C.c_double(xmin), C.c_double(xmax), n, C.c_double(z),
+                                  C.c_double(a), C.c_double(x0), _p(v))
+        return v
+
+    @staticmethod
+    def soft_coulomb_3d(dims, lo, hi, z, a, c):
+        v = np.empty(int(np.prod(dims)))
+        ref().ref_soft_coulomb_3d(_ivec(dims), _dvec(lo), _dvec(hi), C.c_double(z), C.c_double(a),
+                                  _dvec(c), _p(v))
+        return v

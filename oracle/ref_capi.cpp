@@ -25,6 +25,7 @@ import ses.potential;
 import ses.wavepacket;
 import ses.observables;
 import ses.parallel;
+import ses.imaginary_time;
 
 namespace {
 
@@ -160,6 +161,85 @@ int ref_so3d_evolve(const double* v, const int d[3], const double lo[3], const d
     }
     store(psi, f.data());
     return 0;
+}
+
+// ImaginaryTimePropagator1D::relax (core/src/imaginary_time.ixx:21-63).
+int ref_it1d_relax(const double* v, int n, double xmin, double xmax, double dtau, double* psi,
+                   int nsteps) {
+    const ses::Grid1D g{xmin, xmax, n};
+    std::vector<double> pot(v, v + n);
+    const ses::ImaginaryTimePropagator1D op{g, pot, dtau};
+    ses::Field1D f{g};
+    load(f.data(), psi);
+    try {
+        op.relax(f, nsteps);
+    } catch (const std::invalid_argument&) {
+        return -1;
+    }
+    store(psi, f.data());
+    return 0;
+}
+
+// ImaginaryTimePropagator3D weights through its accessors (imaginary_time.ixx:96-98).
+void ref_it3d_tables(const double* v, const int d[3], const double lo[3], const double hi[3],
+                     double dtau, double mass, double* half_w, double* kin_w) {
+    const ses::Grid3D g = grid3(d, lo, hi);
+    std::vector<double> pot(v, v + g.size());
+    const ses::ImaginaryTimePropagator3D op{g, pot, dtau, mass};
+    std::memcpy(half_w, op.half_potential_weight().data(), g.size() * sizeof(double));
+    std::memcpy(kin_w, op.kinetic_weight().data(), g.size() * sizeof(double));
+}
+
+// ImaginaryTimePropagator3D::relax, or relax_deflated against `lower` states
+// (imaginary_time.ixx:100-134).
+int ref_it3d_relax(const double* v, const int d[3], const double lo[3], const double hi[3],
+                   double dtau, double mass, double* psi, const double* const* lower, int nlower,
+                   int nsteps) {
+    const ses::Grid3D g = grid3(d, lo, hi);
+    std::vector<double> pot(v, v + g.size());
+    const ses::ImaginaryTimePropagator3D op{g, pot, dtau, mass};
+    ses::Field3D f{g};
+    load(f.data(), psi);
+    std::vector<ses::Field3D> low;
+    low.reserve(static_cast<std::size_t>(nlower));
+    std::vector<const ses::Field3D*> ptrs;
+    for (int j = 0; j < nlower; ++j) {
+        low.emplace_back(g);
+        load(low.back().data(), lower[j]);
+    }
+    for (const ses::Field3D& q : low) ptrs.push_back(&q);
+    try {
+        if (nlower == 0) op.relax(f, nsteps);
+        else op.relax_deflated(f, ptrs, nsteps);
+    } catch (const std::invalid_argument&) {
+        return -1;
+    }
+    store(psi, f.data());
+    return 0;
+}
+
+// core/src/field.ixx:88-99
+void ref_inner_product(const double* a, const double* b, const int d[3], const double lo[3],
+                       const double hi[3], double* out) {
+    const ses::Grid3D g = grid3(d, lo, hi);
+    ses::Field3D fa{g}, fb{g};
+    load(fa.data(), a);
+    load(fb.data(), b);
+    const Cd c = ses::inner_product(fa, fb);
+    out[0] = c.real();
+    out[1] = c.imag();
+}
+
+void ref_soft_coulomb_1d(double xmin, double xmax, int n, double z, double a, double x0,
+                         double* v) {
+    const std::vector<double> r = ses::soft_coulomb_potential(ses::Grid1D{xmin, xmax, n}, z, a, x0);
+    std::memcpy(v, r.data(), r.size() * sizeof(double));
+}
+void ref_soft_coulomb_3d(const int d[3], const double lo[3], const double hi[3], double z, double a,
+                         const double c[3], double* v) {
+    const std::vector<double> r =
+        ses::soft_coulomb_potential(grid3(d, lo, hi), z, a, ses::Vec3d{c[0], c[1], c[2]});
+    std::memcpy(v, r.data(), r.size() * sizeof(double));
 }
 
 // Builders (core/src/potential.ixx, core/src/wavepacket.ixx).

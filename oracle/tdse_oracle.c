@@ -420,6 +420,148 @@ void orc_normalize(double* psi, size_t n, double cell_volume) {
     }
 }
 
+/* ---- imaginary-time relaxation (core/src/imaginary_time.ixx) ---------- */
+
+/* core/src/imaginary_time.ixx:24-40 */
+void orc_it1d_tables(const double* v, int n, double xmin, double xmax, double dtau,
+                     double* half_w, double* kin_w) {
+    for (int i = 0; i < n; ++i) half_w[i] = exp(-0.5 * v[i] * dtau);
+    double* k = (double*)malloc(sizeof(double) * (size_t)n);
+    orc_wavenumbers(n, xmin, xmax, k);
+    for (int j = 0; j < n; ++j) kin_w[j] = exp(-0.5 * k[j] * k[j] * dtau);
+    free(k);
+}
+
+/* core/src/imaginary_time.ixx:67-94: full N-entry kinetic weight */
+void orc_it3d_tables(const double* v, const int d[3], const double lo[3], const double hi[3],
+                     double dtau, double mass, double* half_w, double* kin_w) {
+    const long n = (long)d[0] * d[1] * d[2];
+    for (long i = 0; i < n; ++i) half_w[i] = exp(-0.5 * v[i] * dtau);
+    double* kx = (double*)malloc(sizeof(double) * (size_t)d[0]);
+    double* ky = (double*)malloc(sizeof(double) * (size_t)d[1]);
+    double* kz = (double*)malloc(sizeof(double) * (size_t)d[2]);
+    orc_wavenumbers(d[0], lo[0], hi[0], kx);
+    orc_wavenumbers(d[1], lo[1], hi[1], ky);
+    orc_wavenumbers(d[2], lo[2], hi[2], kz);
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < d[2]; ++k)
+        for (int j = 0; j < d[1]; ++j)
+            for (int i = 0; i < d[0]; ++i) {
+                const double kxx = kx[i], kyy = ky[j], kzz = kz[k];
+                kin_w[i + (long)d[0] * (j + (long)d[1] * k)] =
+                    exp(-0.5 * (kxx * kxx + kyy * kyy + kzz * kzz) / mass * dtau);
+            }
+    free(kx);
+    free(ky);
+    free(kz);
+}
+
+/* core/src/imaginary_time.ixx:57-62 and 136-141: a[i] = weight[i] * a[i] */
+static void apply_weight(const double* w, double* a, long n) {
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < n; ++i) {
+        a[2 * i] = w[i] * a[2 * i];
+        a[2 * i + 1] = w[i] * a[2 * i + 1];
+    }
+}
+
+/* core/src/imaginary_time.ixx:43-53 (1D normalize: field.ixx:41-46).
+ * last_norm (nullable) receives the norm of the last step before its
+ * renormalisation. */
+int orc_it1d_relax(double* psi, int n, double dx, const double* half_w, const double* kin_w,
+                   int nsteps, double* last_norm) {
+    if (!is_pow2((size_t)n)) return -1;
+    for (int s = 0; s < nsteps; ++s) {
+        apply_weight(half_w, psi, n);
+        orc_fft1d(psi, (size_t)n);
+        apply_weight(kin_w, psi, n);
+        orc_ifft1d(psi, (size_t)n);
+        apply_weight(half_w, psi, n);
+        const double n2 = orc_norm_sq(psi, (size_t)n, dx);
+        if (last_norm) *last_norm = n2;
+        const double inv = 1.0 / sqrt(n2);
+        for (int i = 0; i < 2 * n; ++i) psi[i] = inv * psi[i];
+    }
+    return 0;
+}
+
+/* core/src/field.ixx:88-99: <a|b> = sum conj(a) b dV, real and imaginary
+ * parts summed separately in index order */
+void orc_inner_product(const double* a, const double* b, size_t n, double dv, double* out) {
+    double re = 0.0, im = 0.0;
+    for (size_t i = 0; i < n; ++i) {
+        /* conj(a) * b with -fcx-limited-range: plain four-product form */
+        const double ar = a[2 * i], ai = -a[2 * i + 1], br = b[2 * i], bi = b[2 * i + 1];
+        re += ar * br - ai * bi;
+        im += ar * bi + ai * br;
+    }
+    out[0] = re * dv;
+    out[1] = im * dv;
+}
+
+/* core/src/imaginary_time.ixx:112-122 (relax) and 124-141 (relax_deflated;
+ * nlower = 0 is relax).  any_size: the mixed-radix FFT (no reference
+ * counterpart) for non-power-of-two axes. */
+int orc_it3d_relax(double* psi, const int d[3], double dv, const double* half_w,
+                   const double* kin_w, const double* const* lower, int nlower, int nsteps,
+                   int any_size, double* last_norm) {
+    const long n = (long)d[0] * d[1] * d[2];
+    if (!any_size &&
+        (!is_pow2((size_t)d[0]) || !is_pow2((size_t)d[1]) || !is_pow2((size_t)d[2])))
+        return -1;
+    const double inv_n = 1.0 / (double)n;
+    for (int s = 0; s < nsteps; ++s) {
+        apply_weight(half_w, psi, n);
+        if (any_size) orc_fft3d_any(psi, d[0], d[1], d[2], -1);
+        else orc_fft3d(psi, d[0], d[1], d[2]);
+        apply_weight(kin_w, psi, n);
+        if (any_size) {
+            orc_fft3d_any(psi, d[0], d[1], d[2], +1);
+            for (long i = 0; i < 2 * n; ++i) psi[i] = inv_n * psi[i];
+        } else {
+            orc_ifft3d(psi, d[0], d[1], d[2]);
+        }
+        apply_weight(half_w, psi, n);
+        for (int j = 0; j < nlower; ++j) {
+            double c[2];
+            orc_inner_product(lower[j], psi, (size_t)n, dv, c);
+            for (long i = 0; i < n; ++i) {
+                /* p[i] - c * q[i] */
+                const double qr = lower[j][2 * i], qi = lower[j][2 * i + 1];
+                psi[2 * i] = psi[2 * i] - (c[0] * qr - c[1] * qi);
+                psi[2 * i + 1] = psi[2 * i + 1] - (c[0] * qi + c[1] * qr);
+            }
+        }
+        const double n2 = orc_norm_sq(psi, (size_t)n, dv);
+        if (last_norm) *last_norm = n2;
+        orc_normalize(psi, (size_t)n, dv);
+    }
+    return 0;
+}
+
+/* core/src/potential.ixx:25-33 */
+void orc_soft_coulomb_1d(double xmin, double xmax, int n, double z, double a, double x0,
+                         double* v) {
+    for (int i = 0; i < n; ++i) {
+        const double dx = coord(xmin, xmax, n, i) - x0;
+        v[i] = -z / sqrt(dx * dx + a * a);
+    }
+}
+
+/* core/src/potential.ixx:51-65 */
+void orc_soft_coulomb_3d(const int d[3], const double lo[3], const double hi[3], double z,
+                         double a, const double c[3], double* v) {
+    for (int k = 0; k < d[2]; ++k)
+        for (int j = 0; j < d[1]; ++j)
+            for (int i = 0; i < d[0]; ++i) {
+                const double dx = coord(lo[0], hi[0], d[0], i) - c[0];
+                const double dy = coord(lo[1], hi[1], d[1], j) - c[1];
+                const double dz = coord(lo[2], hi[2], d[2], k) - c[2];
+                v[i + (long)d[0] * (j + (long)d[1] * k)] =
+                    -z / sqrt(dx * dx + dy * dy + dz * dz + a * a);
+            }
+}
+
 /* ---- builders (core/src/potential.ixx, core/src/wavepacket.ixx) --------- */
 
 static inline double cell_volume3(const double lo[3], const double hi[3], const int d[3]) {

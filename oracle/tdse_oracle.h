@@ -86,6 +86,23 @@ void orc_gaussian_3d(const int dims[3], const double lo[3], const double hi[3],
                      const double r0[3], const double sigma[3], const double k0[3],
                      double* psi);
 
+/* Imaginary-time relaxation (core/src/imaginary_time.ixx). */
+void orc_it1d_tables(const double* v, int n, double xmin, double xmax, double dtau,
+                     double* half_w, double* kin_w);
+void orc_it3d_tables(const double* v, const int dims[3], const double lo[3],
+                     const double hi[3], double dtau, double mass, double* half_w,
+                     double* kin_w);
+int orc_it1d_relax(double* psi, int n, double dx, const double* half_w, const double* kin_w,
+                   int nsteps, double* last_norm);
+int orc_it3d_relax(double* psi, const int dims[3], double dv, const double* half_w,
+                   const double* kin_w, const double* const* lower, int nlower, int nsteps,
+                   int any_size, double* last_norm);
+void orc_inner_product(const double* a, const double* b, size_t n, double dv, double* out);
+void orc_soft_coulomb_1d(double xmin, double xmax, int n, double z, double a, double x0,
+                         double* v);
+void orc_soft_coulomb_3d(const int dims[3], const double lo[3], const double hi[3], double z,
+                         double a, const double c[3], double* v);
+
 /* Observables (core/src/observables.ixx). */
 double orc_mean_position_1d(const double* psi, int n, double xmin, double xmax);
 double orc_mean_energy_1d(const double* psi, int n, double xmin, double xmax,

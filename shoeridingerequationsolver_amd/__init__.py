@@ -36,6 +36,7 @@ __all__ = [
     "SplitOperator3D", "WavepacketSimulation", "harmonic_potential", "barrier_potential",
     "regularized_coulomb_potential", "coulomb_floor_potential", "absorbing_mask",
     "gaussian_wavepacket", "TDSE_FP32", "TDSE_FP64", "TdseError", "Batch",
+    "ImaginaryTimePropagator1D", "ImaginaryTimePropagator3D", "soft_coulomb_potential",
 ]
 
 PRECISIONS = {"fp32": TDSE_FP32, "fp64": TDSE_FP64, TDSE_FP32: TDSE_FP32, TDSE_FP64: TDSE_FP64}
@@ -160,7 +161,8 @@ class Propagator:
     """Device-resident split-operator propagator for one grid (one C-ABI plan)."""
 
     def __init__(self, grid, potential, dt, mass=1.0, precision="fp64", mask=None,
-                 context: Context | None = None, unfused_baseline: bool = False):
+                 context: Context | None = None, unfused_baseline: bool = False,
+                 imaginary_time: bool = False):
         self.grid = grid
         self.ctx = context or default_context()
         self.precision = PRECISIONS[precision]
@@ -175,7 +177,8 @@ class Propagator:
             capi.check(L.tdse_plan_create(self.ctx.handle, capi.ivec(n), capi.dvec(lo),
                                           capi.dvec(hi), C.c_double(dt), C.c_double(mass),
                                           self.precision,
-                                          capi.TDSE_FLAG_UNFUSED_BASELINE if unfused_baseline else 0,
+                                          (capi.TDSE_FLAG_UNFUSED_BASELINE if unfused_baseline else 0)
+                                          | (capi.TDSE_FLAG_IMAGINARY_TIME if imaginary_time else 0),
                                           C.byref(h)))
         except TdseError as e:
             raise _map_error(e) from None
@@ -273,6 +276,25 @@ class Propagator:
         L = capi.lib()
         return {L.tdse_plan_sweep_name(self._h, i).decode(): (ms[i], cnt[i])
                 for i in range(ns.value)}
+
+    def relax_stats(self) -> dict:
+        """Imaginary time: the last step's norm before renormalisation, the
+        energy estimate -ln(norm)/(2 dtau) and the peak density."""
+        st = capi.RelaxStats()
+        try:
+            capi.check(capi.lib().tdse_plan_relax_stats(self._h, C.byref(st)))
+        except TdseError as e:
+            raise _map_error(e) from None
+        return {"norm_sq": st.norm_sq, "energy": st.energy, "peak": st.peak}
+
+    def relax_deflated(self, lower, nsteps):
+        """Imaginary time: relax with Gram-Schmidt deflation against the
+        current states of the ``lower`` propagators (same shape/precision)."""
+        arr = (C.c_void_p * max(1, len(lower)))(*[q.handle for q in lower])
+        try:
+            capi.check(capi.lib().tdse_relax_deflated(self._h, arr, len(lower), int(nsteps)))
+        except TdseError as e:
+            raise _map_error(e) from None
 
     @property
     def variants(self) -> dict:
@@ -429,6 +451,83 @@ class SplitOperator3D(_SplitOperatorBase):
         super().__init__(grid, potential, dt, mass, precision, context)
 
 
+# ---- imaginary time (core/src/imaginary_time.ixx) ----------------------------
+
+class _ImaginaryTimeBase:
+    def __init__(self, grid, potential, dtau, mass, precision, context):
+        self._grid = grid
+        self._dtau = float(dtau)
+        self._mass = float(mass)
+        self._v = np.ascontiguousarray(potential, dtype=np.float64).ravel()
+        self._ctx = context
+        self._prec = precision
+        self._prop = Propagator(grid, self._v, dtau, mass=mass, precision=precision,
+                                context=context, imaginary_time=True)
+
+    def _check(self, psi, nsteps):
+        if nsteps < 0:
+            raise ValueError("nsteps must be >= 0")
+        if psi.dtype != np.complex128 or not psi.flags.c_contiguous:
+            raise ValueError("psi must be a contiguous complex128 array")
+        if psi.size != self._prop.size:
+            raise ValueError("psi does not match the grid")
+
+    def half_potential_weight(self) -> np.ndarray:
+        """exp(-V dtau / 2) per cell (host copy of the weight the GPU forms)."""
+        return np.exp(-0.5 * self._v * self._dtau)
+
+    def relax(self, psi: np.ndarray, nsteps: int) -> None:
+        """``nsteps`` relax steps of ``psi`` in place, each ending normalised."""
+        self._check(psi, nsteps)
+        if nsteps == 0:
+            return
+        self._prop.upload(psi)
+        self._prop.step(nsteps)
+        self._prop.download(psi.reshape(self._prop.shape))
+
+    def stats(self) -> dict:
+        return self._prop.relax_stats()
+
+
+class ImaginaryTimePropagator1D(_ImaginaryTimeBase):
+    """``ses::ImaginaryTimePropagator1D(grid, V, dtau)`` (core/src/imaginary_time.ixx:21-63)."""
+
+    def __init__(self, grid: Grid1D, potential, dtau, precision="fp64", context=None):
+        super().__init__(grid, potential, dtau, 1.0, precision, context)
+
+
+class ImaginaryTimePropagator3D(_ImaginaryTimeBase):
+    """``ses::ImaginaryTimePropagator3D(grid, V, dtau, mass)``
+    (core/src/imaginary_time.ixx:65-144)."""
+
+    def __init__(self, grid: Grid3D, potential, dtau, mass=1.0, precision="fp64", context=None):
+        super().__init__(grid, potential, dtau, mass, precision, context)
+
+    def kinetic_weight(self) -> np.ndarray:
+        """exp(-|k|^2 dtau / 2m) over the full grid, x fastest."""
+        kx, ky, kz = (wavenumbers(a) for a in (self._grid.x, self._grid.y, self._grid.z))
+        k2 = kz[:, None, None] ** 2 + ky[None, :, None] ** 2 + kx[None, None, :] ** 2
+        return np.exp(-0.5 * k2 / self._mass * self._dtau).ravel()
+
+    def relax_deflated(self, psi: np.ndarray, lower, nsteps: int) -> None:
+        """Relax with Gram-Schmidt deflation against the fields in ``lower``
+        (core/src/imaginary_time.ixx:124-141): converges to the next state up."""
+        self._check(psi, nsteps)
+        holders = []
+        try:
+            for q in lower:
+                h = Propagator(self._grid, None, self._dtau, precision=self._prec,
+                               context=self._ctx)
+                h.upload(np.ascontiguousarray(q, dtype=np.complex128))
+                holders.append(h)
+            self._prop.upload(psi)
+            self._prop.relax_deflated(holders, nsteps)
+            self._prop.download(psi.reshape(self._prop.shape))
+        finally:
+            for h in holders:
+                h.close()
+
+
 # ---- builders (core/src/potential.ixx, core/src/wavepacket.ixx) -------------
 
 def _build(fn, grid, *scalars_and_vecs, out_complex=False):
@@ -463,6 +562,13 @@ def regularized_coulomb_potential(grid: Grid3D, Z, c):
 def coulomb_floor_potential(grid: Grid3D, Z, c):
     """-Z / max(r, h/2): the hydrogen-like workload's Coulomb (no reference builder)."""
     return _build(capi.lib().tdse_build_coulomb_floor, grid, C.c_double(Z), capi.dvec(c))
+
+
+def soft_coulomb_potential(grid, Z, a, c=0.0):
+    """-Z / sqrt(|r - c|^2 + a^2) (core/src/potential.ixx:25-33, 51-65); c is x0 for a
+    Grid1D."""
+    return _build(capi.lib().tdse_build_soft_coulomb, grid, C.c_double(Z), C.c_double(a),
+                  _center(grid, c))
 
 
 def absorbing_mask(grid, width):

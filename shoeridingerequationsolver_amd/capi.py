@@ -24,6 +24,7 @@ TDSE_FP32 = 0
 TDSE_FP64 = 1
 
 TDSE_FLAG_UNFUSED_BASELINE = 1
+TDSE_FLAG_IMAGINARY_TIME = 2
 
 TDSE_OBS_NORM = 1
 TDSE_OBS_POSITION = 2
@@ -43,9 +44,11 @@ EXPORTS = (
     "tdse_build_coulomb_floor", "tdse_build_absorbing_mask", "tdse_build_gaussian",
     "tdse_batch_create", "tdse_batch_destroy", "tdse_batch_step", "tdse_batch_sync",
     "tdse_batch_stream", "tdse_batch_launches_per_round", "tdse_plan_build",
+    "tdse_relax_deflated", "tdse_plan_relax_stats", "tdse_build_soft_coulomb",
 )
 
-TDSE_POT = {"none": 0, "harmonic": 1, "regularized_coulomb": 2, "coulomb_floor": 3, "barrier": 4}
+TDSE_POT = {"none": 0, "harmonic": 1, "regularized_coulomb": 2, "coulomb_floor": 3, "barrier": 4,
+            "soft_coulomb": 5}
 
 
 class Recipe(C.Structure):
@@ -59,7 +62,12 @@ class Recipe(C.Structure):
         ("r0", C.c_double * 3),
         ("sigma", C.c_double * 3),
         ("k0", C.c_double * 3),
+        ("softening", C.c_double),
     ]
+
+
+class RelaxStats(C.Structure):
+    _fields_ = [("norm_sq", C.c_double), ("energy", C.c_double), ("peak", C.c_double)]
 
 
 class TdseError(RuntimeError):
@@ -139,6 +147,9 @@ def lib() -> C.CDLL:
         L.tdse_batch_stream.argtypes = [C.c_void_p, C.POINTER(C.c_void_p)]
         L.tdse_batch_launches_per_round.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
         L.tdse_plan_build.argtypes = [C.c_void_p, C.POINTER(Recipe)]
+        L.tdse_relax_deflated.argtypes = [C.c_void_p, C.POINTER(C.c_void_p), C.c_int, C.c_int]
+        L.tdse_plan_relax_stats.argtypes = [C.c_void_p, C.POINTER(RelaxStats)]
+        L.tdse_build_soft_coulomb.argtypes = [_ip, _dp, _dp, C.c_double, C.c_double, _dp, _dp]
         _LIB = L
     return _LIB
 

@@ -127,6 +127,25 @@ int tdse_build_coulomb_floor(const int n[3], const double lo[3], const double hi
     return TDSE_OK;
 }
 
+// core/src/potential.ixx:25-33 (1D, n x 1 x 1) and 51-65 (3D): -Z / sqrt(r^2 + a^2)
+int tdse_build_soft_coulomb(const int n[3], const double lo[3], const double hi[3], double z,
+                            double a, const double c[3], double* v) {
+    if (!ok_grid(n, lo, hi) || !c || !v) return TDSE_E_INVALID;
+    for (int k = 0; k < n[2]; ++k)
+        for (int j = 0; j < n[1]; ++j)
+            for (int i = 0; i < n[0]; ++i) {
+                const double dx = coord(lo[0], hi[0], n[0], i) - c[0];
+                if (n[1] == 1 && n[2] == 1) {
+                    v[i] = -z / std::sqrt(dx * dx + a * a);  // the 1D builder's form
+                    continue;
+                }
+                const double dy = coord(lo[1], hi[1], n[1], j) - c[1];
+                const double dz = coord(lo[2], hi[2], n[2], k) - c[2];
+                v[flat(n, i, j, k)] = -z / std::sqrt(dx * dx + dy * dy + dz * dz + a * a);
+            }
+    return TDSE_OK;
+}
+
 // core/src/potential.ixx:194-233: separable sin^2 ramp, 1 on collapsed axes
 int tdse_build_absorbing_mask(const int n[3], const double lo[3], const double hi[3],
                               double width, double* m) {

@@ -73,11 +73,14 @@ build_cells(BuildArgs a) {
             case TDSE_POT_BARRIER:
                 pot = (x >= r.x_lo && x < r.x_hi) ? r.strength : 0.0;
                 break;
+            case TDSE_POT_SOFT_COULOMB:
+                pot = -r.strength / sqrt(dx * dx + dy * dy + dz * dz + r.softening * r.softening);
+                break;
             default:
                 pot = 0.0;
         }
         v[f] = static_cast<T>(pot);
-        phi[f] = static_cast<T>(reduce_2pi_dev(-0.5 * pot * a.dt));
+        phi[f] = static_cast<T>(a.imaginary ? -0.5 * pot * a.dt : reduce_2pi_dev(-0.5 * pot * a.dt));
         if (mask) {
             const double m = ramp(x, a.lo[0], a.hi[0], a.n[0], r.mask_width) *
                              ramp(y, a.lo[1], a.hi[1], a.n[1], r.mask_width) *

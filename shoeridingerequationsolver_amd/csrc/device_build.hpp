@@ -16,6 +16,7 @@ struct BuildArgs {
     int n[3];
     double lo[3], hi[3], h[3];
     double dt;
+    int imaginary;  // 1: phi = -V dt / 2 unreduced (imaginary time weights)
     tdse_recipe r;
 };
 

@@ -21,6 +21,8 @@ hipError_t launch_sweep(const SweepArgs& a, int max_blocks, hipStream_t stream) 
         block = S::BLOCK_S;
     }
     if (tiles <= 0) return hipSuccess;
+    // measuring sweeps keep one partial per block
+    if (mode_measures(MODE) && max_blocks > kRelaxMaxBlocks) max_blocks = kRelaxMaxBlocks;
     const int grid = static_cast<int>(tiles < max_blocks ? tiles : max_blocks);
     if constexpr (mode_contiguous(MODE)) {
         hipLaunchKernelGGL((contig_sweep<T, N, MODE, V>), dim3(grid), dim3(block), 0, stream, a);
@@ -79,6 +81,13 @@ LaunchFn pick_mode(int mode) {
         case S_KICK_MID_TW: return &launch_sweep<T, N, S_KICK_MID_TW>;
         case S_KICK_LAST: return &launch_sweep<T, N, S_KICK_LAST>;
         case S_FWD_TW: return &launch_sweep<T, N, S_FWD_TW>;
+        case C_RELAX_FIRST: return &launch_sweep<T, N, C_RELAX_FIRST>;
+        case C_RELAX_MID: return &launch_sweep<T, N, C_RELAX_MID>;
+        case C_RELAX_LAST: return &launch_sweep<T, N, C_RELAX_LAST>;
+        case C_RELAX_SINGLE: return &launch_sweep<T, N, C_RELAX_SINGLE>;
+        case S_RELAX_FIRST_TW: return &launch_sweep<T, N, S_RELAX_FIRST_TW>;
+        case S_RELAX_MID_TW: return &launch_sweep<T, N, S_RELAX_MID_TW>;
+        case S_RELAX_LAST: return &launch_sweep<T, N, S_RELAX_LAST>;
         default: return nullptr;
     }
 }

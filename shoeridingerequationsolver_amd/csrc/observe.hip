@@ -132,6 +132,74 @@ scale_state(void* psi_v, long long total, double s) {
     }
 }
 
+// psi *= s[0], the factor left in device memory by an earlier kernel
+// (imaginary time: the deferred 1 / sqrt(norm) of the last step).
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+scale_state_dev(void* psi_v, long long total, const double* s) {
+    cx<T>* psi = static_cast<cx<T>*>(psi_v);
+    const double f = *s;
+    const long long stride = static_cast<long long>(gridDim.x) * kBlock;
+    for (long long f_ = blockIdx.x * static_cast<long long>(kBlock) + threadIdx.x; f_ < total;
+         f_ += stride) {
+        const cx<T> z = psi[f_];
+        psi[f_] = {static_cast<T>(f * z.x), static_cast<T>(f * z.y)};
+    }
+}
+
+// <a|b> = sum conj(a) b (core/src/field.ixx:88-99): partial sums of the
+// real and imaginary parts per block, FP64.
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+reduce_inner(const void* a_v, const void* b_v, long long total, double* partials) {
+    const cx<T>* a = static_cast<const cx<T>*>(a_v);
+    const cx<T>* b = static_cast<const cx<T>*>(b_v);
+    double acc[kReduceQ] = {0, 0, 0, 0, 0, 0};
+    const long long stride = static_cast<long long>(gridDim.x) * kBlock;
+    for (long long f = blockIdx.x * static_cast<long long>(kBlock) + threadIdx.x; f < total;
+         f += stride) {
+        const double ar = a[f].x, ai = a[f].y, br = b[f].x, bi = b[f].y;
+        acc[0] += ar * br + ai * bi;
+        acc[1] += ar * bi - ai * br;
+    }
+    block_reduce_store<kReduceQ>(acc, partials + blockIdx.x * kReduceQ);
+}
+
+// psi -= c phi with c = (sum[0] + i sum[1]) dV from reduce_inner/reduce_final,
+// formed in FP64 and rounded once per component.
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+subtract_projection(void* psi_v, const void* phi_v, long long total, const double* sum, double dv) {
+    cx<T>* psi = static_cast<cx<T>*>(psi_v);
+    const cx<T>* phi = static_cast<const cx<T>*>(phi_v);
+    const double cr = sum[0] * dv, ci = sum[1] * dv;
+    const long long stride = static_cast<long long>(gridDim.x) * kBlock;
+    for (long long f = blockIdx.x * static_cast<long long>(kBlock) + threadIdx.x; f < total;
+         f += stride) {
+        const double pr = phi[f].x, pi = phi[f].y;
+        const double xr = static_cast<double>(psi[f].x) - (cr * pr - ci * pi);
+        const double xi = static_cast<double>(psi[f].y) - (cr * pi + ci * pr);
+        psi[f] = {static_cast<T>(xr), static_cast<T>(xi)};
+    }
+}
+
+// psi /= sqrt(sum[0] dV) (left as is for a zero field, core/src/field.ixx:
+// 78-87); the norm is also stored at *norm_out.
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+normalize_dev(void* psi_v, long long total, const double* sum, double dv, double* norm_out) {
+    cx<T>* psi = static_cast<cx<T>*>(psi_v);
+    const double n2 = sum[0] * dv;
+    const double inv = n2 > 0.0 ? 1.0 / sqrt(n2) : 1.0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *norm_out = n2;
+    const long long stride = static_cast<long long>(gridDim.x) * kBlock;
+    for (long long f = blockIdx.x * static_cast<long long>(kBlock) + threadIdx.x; f < total;
+         f += stride) {
+        const cx<T> z = psi[f];
+        psi[f] = {static_cast<T>(inv * z.x), static_cast<T>(inv * z.y)};
+    }
+}
+
 // Unfused FP64 elementwise passes of the reference data flow (the baseline
 // the fused path is compared with): psi *= table, psi = conj(psi) * s,
 // psi *= mask.
@@ -214,6 +282,62 @@ hipError_t launch_reduce(bool fp64, bool kspace, const ReduceArgs& a, double* pa
             hipLaunchKernelGGL(reduce_x<float>, dim3(blocks), dim3(kBlock), 0, stream, a, partials);
     }
     hipLaunchKernelGGL(reduce_final, dim3(1), dim3(kBlock), 0, stream, partials, blocks, out);
+    return hipGetLastError();
+}
+
+namespace {
+long long scale_blocks(long long total) {
+    long long b = (total + kBlock * 4 - 1) / (kBlock * 4);
+    if (b < 1) b = 1;
+    if (b > 8192) b = 8192;
+    return b;
+}
+}  // namespace
+
+hipError_t launch_scale_dev(bool fp64, void* psi, long long total, const double* s,
+                            hipStream_t stream) {
+    const long long b = scale_blocks(total);
+    if (fp64)
+        hipLaunchKernelGGL(scale_state_dev<double>, dim3(b), dim3(kBlock), 0, stream, psi, total, s);
+    else
+        hipLaunchKernelGGL(scale_state_dev<float>, dim3(b), dim3(kBlock), 0, stream, psi, total, s);
+    return hipGetLastError();
+}
+
+hipError_t launch_inner(bool fp64, const void* a, const void* b, long long total, double* partials,
+                        double* out, hipStream_t stream) {
+    const int blocks = reduce_blocks(total);
+    if (fp64)
+        hipLaunchKernelGGL(reduce_inner<double>, dim3(blocks), dim3(kBlock), 0, stream, a, b, total,
+                           partials);
+    else
+        hipLaunchKernelGGL(reduce_inner<float>, dim3(blocks), dim3(kBlock), 0, stream, a, b, total,
+                           partials);
+    hipLaunchKernelGGL(reduce_final, dim3(1), dim3(kBlock), 0, stream, partials, blocks, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_subtract_projection(bool fp64, void* psi, const void* phi, long long total,
+                                      const double* sum, double dv, hipStream_t stream) {
+    const long long b = scale_blocks(total);
+    if (fp64)
+        hipLaunchKernelGGL(subtract_projection<double>, dim3(b), dim3(kBlock), 0, stream, psi, phi,
+                           total, sum, dv);
+    else
+        hipLaunchKernelGGL(subtract_projection<float>, dim3(b), dim3(kBlock), 0, stream, psi, phi,
+                           total, sum, dv);
+    return hipGetLastError();
+}
+
+hipError_t launch_normalize_dev(bool fp64, void* psi, long long total, const double* sum, double dv,
+                                double* norm_out, hipStream_t stream) {
+    const long long b = scale_blocks(total);
+    if (fp64)
+        hipLaunchKernelGGL(normalize_dev<double>, dim3(b), dim3(kBlock), 0, stream, psi, total, sum,
+                           dv, norm_out);
+    else
+        hipLaunchKernelGGL(normalize_dev<float>, dim3(b), dim3(kBlock), 0, stream, psi, total, sum,
+                           dv, norm_out);
     return hipGetLastError();
 }
 

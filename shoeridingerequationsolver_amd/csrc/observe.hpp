@@ -29,6 +29,18 @@ int reduce_blocks(long long total);
 hipError_t launch_reduce(bool fp64, bool kspace, const ReduceArgs& a, double* partials,
                          double* out, hipStream_t stream);
 hipError_t launch_scale(bool fp64, void* psi, long long total, double s, hipStream_t stream);
+// psi *= *s (device scalar)
+hipError_t launch_scale_dev(bool fp64, void* psi, long long total, const double* s,
+                            hipStream_t stream);
+// out[0] + i out[1] = sum conj(a) b (device; partials as launch_reduce)
+hipError_t launch_inner(bool fp64, const void* a, const void* b, long long total, double* partials,
+                        double* out, hipStream_t stream);
+// psi -= (sum[0] + i sum[1]) dv * phi
+hipError_t launch_subtract_projection(bool fp64, void* psi, const void* phi, long long total,
+                                      const double* sum, double dv, hipStream_t stream);
+// psi /= sqrt(sum[0] dv) unless zero; *norm_out = sum[0] dv
+hipError_t launch_normalize_dev(bool fp64, void* psi, long long total, const double* sum, double dv,
+                                double* norm_out, hipStream_t stream);
 // FP64 elementwise passes of the unfused baseline
 hipError_t launch_table_mul(void* psi, const void* table, long long total, hipStream_t stream);
 hipError_t launch_conj_scale(void* psi, long long total, double s, hipStream_t stream);

@@ -58,10 +58,46 @@ enum SweepMode : int {
     S_KICK_LAST = 10,    // IFFT, kick_half*mask
     C_FWD = 11,          // FFT only (observables: psi -> k space)
     S_FWD_TW = 12,       // FFT, twiddle (observables, four-step 1D)
-    kNumModes = 13
+    // imaginary time: real weights w = exp(-V dtau / 2) and a norm per step
+    C_RELAX_FIRST = 13,     // w, FFT
+    C_RELAX_MID = 14,       // IFFT, scale * w, measure, w, FFT
+    C_RELAX_LAST = 15,      // IFFT, scale * w, measure
+    C_RELAX_SINGLE = 16,    // every relax step of a 1D line in one block
+    S_RELAX_FIRST_TW = 17,  // w, FFT, twiddle
+    S_RELAX_MID_TW = 18,    // IFFT, scale * w, measure, w, FFT, twiddle
+    S_RELAX_LAST = 19,      // IFFT, scale * w, measure
+    kNumModes = 20
 };
 
-constexpr bool mode_contiguous(int m) { return m <= C_SINGLE || m == C_FWD; }
+constexpr bool mode_contiguous(int m) {
+    return m <= C_SINGLE || m == C_FWD || (m >= C_RELAX_FIRST && m <= C_RELAX_SINGLE);
+}
+// modes that measure the norm across blocks (deferred renormalisation)
+constexpr bool mode_measures(int m) {
+    return m == C_RELAX_MID || m == C_RELAX_LAST || m == S_RELAX_MID_TW || m == S_RELAX_LAST;
+}
+
+// Device state of an imaginary-time plan.  A relax step ends with
+// psi /= sqrt(norm); the norm is only known once the sweep that measures it
+// has finished, so the division is deferred: the next measuring sweep
+// multiplies by the scale the previous one left (by step parity), and the
+// call ends with one scaling pass.
+struct RelaxSlots {
+    double scale[2];      // 1 / sqrt(norm) of the previous measured step, by parity
+    double norm_sq;       // sum |psi|^2 dV before renormalisation, last measured step
+    double peak;          // max |psi|^2 there (before renormalisation)
+    unsigned int ticket;  // blocks finished in the current measuring sweep
+    unsigned int pad;
+};
+
+// Blocks of a measuring sweep (partials hold two doubles per block).
+constexpr int kRelaxMaxBlocks = 2048;
+
+// Per-thread accumulation of a measuring sweep (over all tiles of a block).
+struct RelaxAcc {
+    double sum = 0.0;
+    double peak = 0.0;
+};
 
 struct SweepArgs {
     void* psi;            // cx<T>*: the state, x-fastest
@@ -81,7 +117,12 @@ struct SweepArgs {
     double scale;         // 1 / total points (inverse normalisation)
     int n_total;          // four-step twiddle modulus (1D), else 0
     int n_rows;           // four-step: number of rows N2 (contiguous K mapping)
-    int nsteps;           // C_SINGLE: steps to run
+    int nsteps;           // C_SINGLE / C_RELAX_SINGLE: steps to run
+    // imaginary time
+    RelaxSlots* relax;    // device slots, or null
+    double* partials;     // per-block (sum, max) of a measuring sweep
+    double cell;          // volume element dV
+    int parity;           // measuring sweeps: read scale[parity], write scale[parity ^ 1]
 };
 
 // ---- buffer access -----------------------------------------------------------
@@ -293,6 +334,35 @@ struct Line {
         load_kick(with_mask);
         apply_kick(f, with_mask);
     }
+    // Imaginary time.  kp holds phi = -V dtau / 2 (not reduced); the weight
+    // exp(phi) is formed in FP64.
+    __device__ __forceinline__ void relax_first() {
+        load_kick(false);
+        if (!valid) return;
+        for (int s = 0; s < E; ++s) {
+            const double e = exp(static_cast<double>(kp[s]));
+            v[s] = {static_cast<T>(e * v[s].x), static_cast<T>(e * v[s].y)};
+        }
+    }
+    // v <- v * pre * w, accumulated into (sum, peak) of |v|^2, then (second)
+    // times w again; one rounding per component.
+    __device__ __forceinline__ void relax_measure(double pre, bool second, RelaxAcc& ra) {
+        load_kick(false);
+        if (!valid) return;
+        for (int s = 0; s < E; ++s) {
+            const double e = exp(static_cast<double>(kp[s]));
+            const double f = pre * e;
+            double xr = f * v[s].x, xi = f * v[s].y;
+            const double a2 = xr * xr + xi * xi;
+            ra.sum += a2;
+            ra.peak = a2 > ra.peak ? a2 : ra.peak;
+            if (second) {
+                xr *= e;
+                xi *= e;
+            }
+            v[s] = {static_cast<T>(xr), static_cast<T>(xi)};
+        }
+    }
     __device__ __forceinline__ void fft() { Transform<T, N, E, -1, 1, Fwd>::run(v, t, tw, x); }
     __device__ __forceinline__ void ifft() { Transform<T, N, E, +1, 1, Inv>::run(v, t, tw, x); }
     // four-step twiddle after the column FFT: exp(-2 pi i n1 k2 / N), k2 = position
@@ -303,13 +373,75 @@ struct Line {
     }
 };
 
+// ---- imaginary-time norm ----------------------------------------------------------
+
+// Sum and max over the block, every thread receiving the same values: a
+// fixed xor-shuffle tree per wave64 (lane 0's result), then the waves in order.
+__device__ __forceinline__ void block_sum_max(double& s, double& m) {
+    __shared__ double red[2 * 16];
+    for (int off = 32; off > 0; off >>= 1) {
+        s += __shfl_xor(s, off, 64);
+        const double o = __shfl_xor(m, off, 64);
+        m = o > m ? o : m;
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int nw = (blockDim.x + 63) >> 6;
+    __syncthreads();  // red may still be read by a previous use
+    if (lane == 0) {
+        red[2 * wave] = s;
+        red[2 * wave + 1] = m;
+    }
+    __syncthreads();
+    s = 0.0;
+    m = 0.0;
+    for (int w = 0; w < nw; ++w) {
+        s += red[2 * w];
+        m = red[2 * w + 1] > m ? red[2 * w + 1] : m;
+    }
+}
+
+// End of a measuring sweep, per block: store this block's (sum, max); the
+// last block to finish (ticket) adds the partials in block order (fixed
+// order: bitwise repeatable) and leaves scale, norm and peak for the next
+// sweep.  Every block of the grid takes a ticket, so the last one exists.
+__device__ __forceinline__ void relax_block_done(const SweepArgs& a, RelaxAcc ra) {
+    double s = ra.sum, m = ra.peak;
+    block_sum_max(s, m);
+    __shared__ int last;
+    if (threadIdx.x == 0) {
+        a.partials[2 * blockIdx.x] = s;
+        a.partials[2 * blockIdx.x + 1] = m;
+        const unsigned t = __hip_atomic_fetch_add(&a.relax->ticket, 1u, __ATOMIC_ACQ_REL,
+                                                  __HIP_MEMORY_SCOPE_AGENT);
+        last = t == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!last) return;
+    double ts = 0.0, tm = 0.0;
+    for (unsigned b = threadIdx.x; b < gridDim.x; b += blockDim.x) {
+        ts += __hip_atomic_load(&a.partials[2 * b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const double q = __hip_atomic_load(&a.partials[2 * b + 1], __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT);
+        tm = q > tm ? q : tm;
+    }
+    block_sum_max(ts, tm);
+    if (threadIdx.x == 0) {
+        const double norm = ts * a.cell;
+        // a zero field is left as it is (core/src/field.ixx:78-81)
+        a.relax->scale[a.parity ^ 1] = norm > 0.0 ? 1.0 / sqrt(norm) : 1.0;
+        a.relax->norm_sq = norm;
+        a.relax->peak = tm;
+        __hip_atomic_store(&a.relax->ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
 // ---- per-tile bodies ---------------------------------------------------------------
 
 // One tile of a contiguous sweep: LPB whole lines.  `lds` holds LPB padded
 // lines, `tw` the staged FP64 twiddles of this line length.
 template <typename T, int N, int MODE, int V = 0>
 __device__ __forceinline__ void contig_tile(const SweepArgs& a, long long tile, cx<T>* lds,
-                                            const cx<double>* tw, int tid) {
+                                            const cx<double>* tw, int tid, RelaxAcc& ra) {
     using S = Shape<T, N, V>;
     using LT = Line<T, N, ContigX<T>, V>;
     constexpr int TT = S::TT, LPB = S::LPB, E = S::E;
@@ -370,6 +502,62 @@ __device__ __forceinline__ void contig_tile(const SweepArgs& a, long long tile, 
             L.apply_kick(step + 1 == a.nsteps ? 1.0 : 2.0, true);
         }
         L.template store<LT::R1>();
+    } else if constexpr (MODE == C_RELAX_FIRST) {
+        L.template load<LT::R1>();
+        L.relax_first();
+        L.fft();
+        L.template store<LT::RL>();
+    } else if constexpr (MODE == C_RELAX_MID || MODE == C_RELAX_LAST) {
+        const double c = a.relax->scale[a.parity];
+        L.template load<LT::RL>();
+        L.ifft();
+        L.relax_measure(c, MODE == C_RELAX_MID, ra);
+        if constexpr (MODE == C_RELAX_MID) {
+            L.fft();
+            L.template store<LT::RL>();
+        } else {
+            L.template store<LT::R1>();
+        }
+    } else if constexpr (MODE == C_RELAX_SINGLE) {
+        // the whole line is in this block: each step is renormalised exactly
+        // (core/src/imaginary_time.ixx:43-53)
+        const cx<double>* kt = static_cast<const cx<double>*>(a.kt[2]);
+        L.template load<LT::R1>();
+        L.load_kick(false);
+        double e[E];
+        for (int s = 0; s < E; ++s) e[s] = L.valid ? exp(static_cast<double>(L.kp[s])) : 0.0;
+        for (int s = 0; s < E; ++s)
+            L.v[s] = {static_cast<T>(e[s] * L.v[s].x), static_cast<T>(e[s] * L.v[s].y)};
+        double norm = 0.0, peak = 0.0;
+        for (int step = 0; step < a.nsteps; ++step) {
+            L.fft();
+            for_each_point<N, E, LT::RL>(t, [&](int s, int i, int, int) {
+                L.v[s] = mul_d(L.v[s], cscale(kt[i], a.scale));
+            });
+            L.ifft();
+            double sum = 0.0, pk = 0.0;
+            for (int s = 0; s < E; ++s) {
+                const double xr = e[s] * L.v[s].x, xi = e[s] * L.v[s].y;
+                const double a2 = xr * xr + xi * xi;
+                sum += a2;
+                pk = a2 > pk ? a2 : pk;
+                L.v[s] = {static_cast<T>(xr), static_cast<T>(xi)};
+            }
+            block_sum_max(sum, pk);
+            norm = sum * a.cell;
+            peak = pk;
+            const double c = norm > 0.0 ? 1.0 / sqrt(norm) : 1.0;
+            const bool last = step + 1 == a.nsteps;
+            for (int s = 0; s < E; ++s) {
+                const double f = last ? c : c * e[s];
+                L.v[s] = {static_cast<T>(f * L.v[s].x), static_cast<T>(f * L.v[s].y)};
+            }
+        }
+        if (a.relax && tid == 0 && a.nsteps > 0) {
+            a.relax->norm_sq = norm;
+            a.relax->peak = peak;
+        }
+        L.template store<LT::R1>();
     }
 }
 
@@ -377,7 +565,8 @@ __device__ __forceinline__ void contig_tile(const SweepArgs& a, long long tile, 
 // line-axis kinetic table (times 1/N) for S_KIN.
 template <typename T, int N, int MODE, int V = 0>
 __device__ __forceinline__ void strided_tile(const SweepArgs& a, long long tile, cx<T>* lds,
-                                             const cx<double>* tw, const cx<double>* kp, int tid) {
+                                             const cx<double>* tw, const cx<double>* kp, int tid,
+                                             RelaxAcc& ra) {
     using S = Shape<T, N, V>;
     constexpr int TX = S::TX, E = S::E;
     using LT = Line<T, N, StridedX<T, TX>, V>;
@@ -435,6 +624,24 @@ __device__ __forceinline__ void strided_tile(const SweepArgs& a, long long tile,
         L.fft();
         L.twiddle_out(xx);
         L.template store<LT::RL>();
+    } else if constexpr (MODE == S_RELAX_FIRST_TW) {
+        L.template load<LT::R1>();
+        L.relax_first();
+        L.fft();
+        L.twiddle_out(xx);
+        L.template store<LT::RL>();
+    } else if constexpr (MODE == S_RELAX_MID_TW || MODE == S_RELAX_LAST) {
+        const double c = a.relax->scale[a.parity];
+        L.template load<LT::RL>();
+        L.ifft();
+        L.relax_measure(c, MODE == S_RELAX_MID_TW, ra);
+        if constexpr (MODE == S_RELAX_MID_TW) {
+            L.fft();
+            L.twiddle_out(xx);
+            L.template store<LT::RL>();
+        } else {
+            L.template store<LT::R1>();
+        }
     }
 }
 
@@ -458,9 +665,15 @@ contig_sweep(SweepArgs a) {
         stage_table<N>(lds_tw, a.tw, tid, S::BLOCK_C);
         __syncthreads();
     }
+    if constexpr (MODE == C_RELAX_FIRST) {
+        // a call's first measuring sweep reads scale[0]: start it at 1
+        if (blockIdx.x == 0 && tid == 0) a.relax->scale[0] = 1.0;
+    }
+    RelaxAcc ra;
     const long long ntiles = (a.nlines + S::LPB - 1) / S::LPB;
     for (long long tile = blockIdx.x; tile < ntiles; tile += gridDim.x)
-        contig_tile<T, N, MODE, V>(a, tile, lds, lds_tw, tid);
+        contig_tile<T, N, MODE, V>(a, tile, lds, lds_tw, tid, ra);
+    if constexpr (mode_measures(MODE)) relax_block_done(a, ra);
 }
 
 template <typename T, int N, int MODE, int V = 0>
@@ -475,9 +688,14 @@ strided_sweep(SweepArgs a) {
     if constexpr (S::EXCHANGE) stage_table<N>(lds_tw, a.tw, tid, S::BLOCK_S);
     if constexpr (KIN) stage_kinetic<T, N>(lds_kp, a, tid, S::BLOCK_S);
     if constexpr (S::EXCHANGE || KIN) __syncthreads();
+    if constexpr (MODE == S_RELAX_FIRST_TW) {
+        if (blockIdx.x == 0 && tid == 0) a.relax->scale[0] = 1.0;
+    }
+    RelaxAcc ra;
     const long long ntiles = static_cast<long long>((a.nx + S::TX - 1) / S::TX) * a.nw;
     for (long long tile = blockIdx.x; tile < ntiles; tile += gridDim.x)
-        strided_tile<T, N, MODE, V>(a, tile, lds, lds_tw, lds_kp, tid);
+        strided_tile<T, N, MODE, V>(a, tile, lds, lds_tw, lds_kp, tid, ra);
+    if constexpr (mode_measures(MODE)) relax_block_done(a, ra);
 }
 
 // ---- batched kernels: one launch over the tiles of many plans ------------------------
@@ -520,12 +738,13 @@ contig_sweep_batch(BatchTable b) {
     const long long g1 = g0 + chunk < total ? g0 + chunk : total;
     for (long long g = g0; g < g1; ++g) {
         const int it = find_item(b, g);
+        RelaxAcc unused;
         if constexpr (MODE == C_SINGLE) {
             SweepArgs a = b.items[it];
             a.nsteps = b.nsteps;
-            contig_tile<T, N, MODE>(a, g - b.first_tile[it], lds, lds_tw, tid);
+            contig_tile<T, N, MODE>(a, g - b.first_tile[it], lds, lds_tw, tid, unused);
         } else {
-            contig_tile<T, N, MODE>(b.items[it], g - b.first_tile[it], lds, lds_tw, tid);
+            contig_tile<T, N, MODE>(b.items[it], g - b.first_tile[it], lds, lds_tw, tid, unused);
         }
     }
 }
@@ -558,7 +777,9 @@ strided_sweep_batch(BatchTable b) {
                 staged = it;
             }
         }
-        strided_tile<T, N, MODE>(b.items[it], g - b.first_tile[it], lds, lds_tw, lds_kp, tid);
+        RelaxAcc unused;
+        strided_tile<T, N, MODE>(b.items[it], g - b.first_tile[it], lds, lds_tw, lds_kp, tid,
+                                 unused);
     }
 }
 

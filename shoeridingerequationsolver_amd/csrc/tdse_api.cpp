@@ -110,6 +110,9 @@ struct tdse_plan {
     size_t csize() const { return precision == TDSE_FP64 ? 16 : 8; }
     size_t rsize() const { return precision == TDSE_FP64 ? 8 : 4; }
     bool fp64() const { return precision == TDSE_FP64; }
+    bool imag() const { return (flags & TDSE_FLAG_IMAGINARY_TIME) != 0; }
+    // imaginary time: device slots inside the reduction buffer (after red_out)
+    tdse::RelaxSlots* relax() const { return reinterpret_cast<tdse::RelaxSlots*>(red_out + 16); }
 };
 
 namespace {
@@ -288,8 +291,15 @@ int make_kinetic(tdse_plan* p, int slot, const Axis& ax) {
     for (int j = 0; j < ax.n; ++j) {
         const double kj = dk * ((2 * j < ax.n) ? j : j - ax.n);
         const double th = -0.5 * kj * kj / p->mass * p->dt;
-        k[2 * j] = std::cos(th);
-        k[2 * j + 1] = std::sin(th);
+        if (p->imag()) {
+            // imaginary time: the real weight exp(-k^2 dtau / 2m)
+            // (core/src/imaginary_time.ixx:35-39, 82-93)
+            k[2 * j] = std::exp(th);
+            k[2 * j + 1] = 0.0;
+        } else {
+            k[2 * j] = std::cos(th);
+            k[2 * j + 1] = std::sin(th);
+        }
     }
     return h2d(p, d, k.data(), k.size() * 8);
 }
@@ -325,10 +335,43 @@ SweepArgs base_args(const tdse_plan* p) {
     a.psi_bytes = static_cast<uint32_t>(p->total * p->csize());
     a.real_bytes = static_cast<uint32_t>(p->total * p->rsize());
     for (int i = 0; i < 3; ++i) a.kt[i] = nullptr;
+    if (p->imag()) {
+        a.relax = p->relax();
+        a.partials = p->partials;
+    }
+    a.cell = 1.0;
+    for (int i = 0; i < 3; ++i) a.cell *= (p->hi[i] - p->lo[i]) / p->n[i];
     return a;
 }
 
 double dk_of(const Axis& ax) { return 2.0 * 3.14159265358979323846 / ax.length(); }
+
+// Reduction arguments over the plan's state (x space, empty slab).
+tdse::ReduceArgs reduce_args(const tdse_plan* p) {
+    tdse::ReduceArgs r{};
+    r.psi = p->psi;
+    r.v = p->has_potential ? p->v : nullptr;
+    r.total = p->total;
+    for (int a = 0; a < 3; ++a) {
+        r.n[a] = p->n[a];
+        r.lo[a] = p->lo[a];
+        r.h[a] = (p->hi[a] - p->lo[a]) / p->n[a];
+        r.sq[a] = a < static_cast<int>(p->sq.size()) ? p->sq[a].n : 1;
+        r.dk[a] = a < static_cast<int>(p->sq.size()) ? dk_of(p->sq[a]) : 0.0;
+    }
+    r.slab_axis = 0;
+    r.slab_lo = 1.0;
+    r.slab_hi = 0.0;  // empty slab
+    r.mass = p->mass;
+    return r;
+}
+
+double cell_volume(const tdse_plan* p) {
+    double c = 1.0;
+    for (int a = 0; a < 3; ++a) c *= (p->hi[a] - p->lo[a]) / p->n[a];
+    return c;
+}
+
 
 // Sweep descriptors of the plan's geometry.  `target` is the array swept.
 struct Geometry {
@@ -408,7 +451,7 @@ struct SlotSweep {
     SweepArgs args;
 };
 
-SlotSweep slot_sweep(const tdse_plan* p, const Geometry& g, int slot) {
+SlotSweep slot_sweep_real(const tdse_plan* p, const Geometry& g, int slot) {
     if (p->kind == kKind3D) {
         const int n0 = p->sq[0].n, n1 = p->sq[1].n, n2 = p->sq[2].n;
         switch (slot) {
@@ -438,31 +481,54 @@ SlotSweep slot_sweep(const tdse_plan* p, const Geometry& g, int slot) {
     }
 }
 
+// Imaginary-time plans run the relax form of the position-space sweeps
+// (real weights, a measured norm per step); the other sweeps are shared.
+int relax_mode(int mode) {
+    switch (mode) {
+        case tdse::C_KICK_FIRST: return tdse::C_RELAX_FIRST;
+        case tdse::C_KICK_MID: return tdse::C_RELAX_MID;
+        case tdse::C_KICK_LAST: return tdse::C_RELAX_LAST;
+        case tdse::C_SINGLE: return tdse::C_RELAX_SINGLE;
+        case tdse::S_KICK_FIRST_TW: return tdse::S_RELAX_FIRST_TW;
+        case tdse::S_KICK_MID_TW: return tdse::S_RELAX_MID_TW;
+        case tdse::S_KICK_LAST: return tdse::S_RELAX_LAST;
+        default: return mode;
+    }
+}
+
+SlotSweep slot_sweep(const tdse_plan* p, const Geometry& g, int slot) {
+    SlotSweep w = slot_sweep_real(p, g, slot);
+    if (p->imag()) w.mode = relax_mode(w.mode);
+    return w;
+}
+
 // The launch list of `nsteps` steps (nsteps >= 1), each sweep in the kernel
-// variant chosen for its slot.
+// variant chosen for its slot.  Measuring sweeps (imaginary time) carry the
+// parity of their step.
 std::vector<Launch> schedule(const tdse_plan* p, int nsteps, void* target = nullptr) {
     std::vector<Launch> L;
     const Geometry g = geometry(p, target ? target : p->psi);
-    auto add = [&](int slot) {
+    auto add = [&](int slot, int step) {
         SlotSweep w = slot_sweep(p, g, slot);
-        if (w.mode == tdse::C_SINGLE) w.args.nsteps = nsteps;
+        if (w.mode == tdse::C_SINGLE || w.mode == tdse::C_RELAX_SINGLE) w.args.nsteps = nsteps;
+        w.args.parity = step & 1;
         L.push_back({w.mode, w.n, w.args, launcher(p, w.n, w.mode, p->variant[slot]), slot});
     };
     if (p->kind == kKind3D) {
-        add(0);
+        add(0, 0);
         for (int s = 0; s < nsteps; ++s) {
-            add(1);
-            add(2);
-            add(3);
-            add(s + 1 < nsteps ? 4 : 5);
+            add(1, s);
+            add(2, s);
+            add(3, s);
+            add(s + 1 < nsteps ? 4 : 5, s);
         }
     } else if (p->kind == kKindSingle) {
-        add(0);
+        add(0, 0);
     } else {  // 2D and four-step 1D: first, then (kinetic, mid | last) per step
-        add(0);
+        add(0, 0);
         for (int s = 0; s < nsteps; ++s) {
-            add(1);
-            add(s + 1 < nsteps ? 2 : 3);
+            add(1, s);
+            add(s + 1 < nsteps ? 2 : 3, s);
         }
     }
     return L;
@@ -856,9 +922,12 @@ int tdse_plan_bytes(const int n[3], int precision, int with_mask, uint64_t* out)
 int tdse_plan_create(tdse_ctx* ctx, const int n[3], const double lo[3], const double hi[3],
                      double dt, double mass, int precision, unsigned flags, tdse_plan** out) {
     if (!ctx || !n || !lo || !hi || !out) return fail(TDSE_E_INVALID, "null argument");
-    if (flags & ~TDSE_FLAG_UNFUSED_BASELINE) return fail(TDSE_E_INVALID, "unknown flags");
+    if (flags & ~(TDSE_FLAG_UNFUSED_BASELINE | TDSE_FLAG_IMAGINARY_TIME))
+        return fail(TDSE_E_INVALID, "unknown flags");
     if ((flags & TDSE_FLAG_UNFUSED_BASELINE) && precision != TDSE_FP64)
         return fail(TDSE_E_INVALID, "the unfused baseline is FP64 only");
+    if ((flags & TDSE_FLAG_UNFUSED_BASELINE) && (flags & TDSE_FLAG_IMAGINARY_TIME))
+        return fail(TDSE_E_INVALID, "the unfused baseline is real time only");
     *out = nullptr;
     if (precision != TDSE_FP32 && precision != TDSE_FP64)
         return fail(TDSE_E_INVALID, "precision must be TDSE_FP32 or TDSE_FP64");
@@ -914,6 +983,9 @@ int tdse_plan_create(tdse_ctx* ctx, const int n[3], const double lo[3], const do
         return bail(rc);
     p->partials = static_cast<double*>(part);
     p->red_out = p->partials + tdse::kMaxReduceBlocks * tdse::kReduceQ;
+    static_assert(2 * tdse::kRelaxMaxBlocks <= tdse::kMaxReduceBlocks * tdse::kReduceQ,
+                  "relax partials share the reduction partials");
+    static_assert(16 * 8 + sizeof(tdse::RelaxSlots) <= 64 * 8, "relax slots follow red_out");
     for (int N : lines)
         if ((rc = make_twiddles(p, N))) return bail(rc);
     for (size_t a = 0; a < p->sq.size(); ++a)
@@ -926,6 +998,9 @@ int tdse_plan_create(tdse_ctx* ctx, const int n[3], const double lo[3], const do
     if (hipMemsetAsync(p->psi, 0, total * p->csize(), p->stream) != hipSuccess ||
         hipMemsetAsync(p->phi, 0, total * p->rsize(), p->stream) != hipSuccess ||
         hipMemsetAsync(p->v, 0, total * p->rsize(), p->stream) != hipSuccess ||
+        hipMemsetAsync(p->partials, 0,
+                       static_cast<uint64_t>(tdse::kMaxReduceBlocks) * tdse::kReduceQ * 8 + 64 * 8,
+                       p->stream) != hipSuccess ||
         hipStreamSynchronize(p->stream) != hipSuccess) {
         (void)hipGetLastError();
         return bail(fail(TDSE_E_DEVICE, "hipMemset failed"));
@@ -966,9 +1041,10 @@ int tdse_plan_set_potential(tdse_plan* p, const double* v) {
         if (!std::isfinite(v[i])) return fail(TDSE_E_INVALID, "potential not finite");
         th[i] = -0.5 * v[i] * p->dt;  // core/src/propagator.ixx:29, 79
     }
-    // reduced modulo 2 pi in FP64 for both precisions (the kernels' sin/cos
-    // take |angle| <= 8)
-    if ((rc = upload_real(p, p->phi, th.data(), kPhase))) return rc;
+    // real time: reduced modulo 2 pi in FP64 for both precisions (the
+    // kernels' sin/cos take |angle| <= 8); imaginary time: the exponent of
+    // the weight exp(-V dtau / 2) (core/src/imaginary_time.ixx:30-33, 75-78)
+    if ((rc = upload_real(p, p->phi, th.data(), p->imag() ? kPlain : kPhase))) return rc;
     if ((rc = upload_real(p, p->v, v, kPlain))) return rc;
     if (p->base_half) {
         std::vector<double> h(2 * static_cast<size_t>(p->total));
@@ -985,6 +1061,7 @@ int tdse_plan_set_potential(tdse_plan* p, const double* v) {
 int tdse_plan_set_mask(tdse_plan* p, const double* mask) {
     int rc = check_plan(p);
     if (rc) return rc;
+    if (mask && p->imag()) return fail(TDSE_E_INVALID, "imaginary-time plans take no mask");
     if ((rc = set_device(p->ctx))) return rc;
     TDSE_HIP(hipStreamSynchronize(p->stream));
     if (!mask) {
@@ -1001,6 +1078,8 @@ int tdse_plan_build(tdse_plan* p, const tdse_recipe* r) {
     if (!r) return fail(TDSE_E_INVALID, "null recipe");
     if (p->flags & TDSE_FLAG_UNFUSED_BASELINE)
         return fail(TDSE_E_INVALID, "the unfused baseline takes host inputs");
+    if (p->imag() && r->mask_width > 0.0)
+        return fail(TDSE_E_INVALID, "imaginary-time plans take no mask");
     for (int a = 0; a < 3; ++a)
         if (!(r->sigma[a] > 0.0)) return fail(TDSE_E_INVALID, "sigma must be positive");
     if ((rc = set_device(p->ctx))) return rc;
@@ -1023,6 +1102,7 @@ int tdse_plan_build(tdse_plan* p, const tdse_recipe* r) {
         b.h[a] = (p->hi[a] - p->lo[a]) / p->n[a];
     }
     b.dt = p->dt;
+    b.imaginary = p->imag() ? 1 : 0;
     b.r = *r;
     TDSE_HIP(tdse::launch_build(p->fp64(), b, p->stream));
     p->has_potential = true;
@@ -1084,7 +1164,66 @@ int tdse_step(tdse_plan* p, int nsteps) {
     if (!p->has_potential) return fail(TDSE_E_STATE, "tdse_plan_set_potential not called");
     if ((rc = set_device(p->ctx))) return rc;
     if (p->flags & TDSE_FLAG_UNFUSED_BASELINE) return run_baseline(p, nsteps);
-    return run_launches(p, schedule(p, nsteps));
+    if ((rc = run_launches(p, schedule(p, nsteps)))) return rc;
+    if (p->imag() && p->kind != kKindSingle) {
+        // the last step's renormalisation, deferred by its measuring sweep
+        TDSE_HIP(tdse::launch_scale_dev(p->fp64(), p->psi, p->total, &p->relax()->scale[nsteps & 1],
+                                        p->stream));
+    }
+    return TDSE_OK;
+}
+
+int tdse_relax_deflated(tdse_plan* p, const tdse_plan* const* lower, int nlower, int nsteps) {
+    int rc = check_plan(p);
+    if (rc) return rc;
+    if (!p->imag()) return fail(TDSE_E_INVALID, "not an imaginary-time plan");
+    if (nsteps < 0 || nlower < 0 || (nlower > 0 && !lower))
+        return fail(TDSE_E_INVALID, "bad deflation arguments");
+    if (!p->has_potential) return fail(TDSE_E_STATE, "tdse_plan_set_potential not called");
+    for (int j = 0; j < nlower; ++j) {
+        const tdse_plan* q = lower[j];
+        if (!q || q == p) return fail(TDSE_E_INVALID, "null or self lower state");
+        if (q->ctx->device != p->ctx->device || q->precision != p->precision)
+            return fail(TDSE_E_INVALID, "lower state on another device or in another precision");
+        for (int a = 0; a < 3; ++a)
+            if (q->n[a] != p->n[a]) return fail(TDSE_E_INVALID, "lower state of another shape");
+    }
+    if ((rc = set_device(p->ctx))) return rc;
+    for (int j = 0; j < nlower; ++j) TDSE_HIP(hipStreamSynchronize(lower[j]->stream));
+    const double dv = cell_volume(p);
+    const tdse::ReduceArgs r = reduce_args(p);
+    const std::vector<Launch> one = schedule(p, 1);
+    // core/src/imaginary_time.ixx:112-134: relax step, Gram-Schmidt against
+    // each lower state in order, normalise
+    for (int s = 0; s < nsteps; ++s) {
+        if ((rc = run_launches(p, one))) return rc;
+        for (int j = 0; j < nlower; ++j) {
+            TDSE_HIP(tdse::launch_inner(p->fp64(), lower[j]->psi, p->psi, p->total, p->partials,
+                                        p->red_out, p->stream));
+            TDSE_HIP(tdse::launch_subtract_projection(p->fp64(), p->psi, lower[j]->psi, p->total,
+                                                      p->red_out, dv, p->stream));
+        }
+        TDSE_HIP(tdse::launch_reduce(p->fp64(), false, r, p->partials, p->red_out, p->stream));
+        TDSE_HIP(tdse::launch_normalize_dev(p->fp64(), p->psi, p->total, p->red_out, dv,
+                                            &p->relax()->norm_sq, p->stream));
+    }
+    return TDSE_OK;
+}
+
+int tdse_plan_relax_stats(tdse_plan* p, tdse_relax_stats* out) {
+    int rc = check_plan(p);
+    if (rc) return rc;
+    if (!out) return fail(TDSE_E_INVALID, "null out");
+    if (!p->imag()) return fail(TDSE_E_INVALID, "not an imaginary-time plan");
+    if ((rc = set_device(p->ctx))) return rc;
+    tdse::RelaxSlots h{};
+    if ((rc = d2h(p, &h, p->relax(), sizeof(h)))) return rc;
+    out->norm_sq = h.norm_sq;
+    // the norm before renormalisation decays as exp(-2 E dtau)
+    // (solver/src/vk_engine.ixx:3017-3023)
+    out->energy = (h.norm_sq > 0.0 && p->dt > 0.0) ? -std::log(h.norm_sq) / (2.0 * p->dt) : 0.0;
+    out->peak = h.norm_sq > 0.0 ? h.peak / h.norm_sq : 0.0;
+    return TDSE_OK;
 }
 
 int tdse_plan_sync(tdse_plan* p, int timeout_ms) {
@@ -1122,21 +1261,7 @@ int tdse_observe(tdse_plan* p, unsigned what, tdse_observables* out) {
     if (!out) return fail(TDSE_E_INVALID, "null out");
     if ((rc = set_device(p->ctx))) return rc;
     std::memset(out, 0, sizeof(*out));
-    tdse::ReduceArgs r{};
-    r.psi = p->psi;
-    r.v = p->has_potential ? p->v : nullptr;
-    r.total = p->total;
-    for (int a = 0; a < 3; ++a) {
-        r.n[a] = p->n[a];
-        r.lo[a] = p->lo[a];
-        r.h[a] = (p->hi[a] - p->lo[a]) / p->n[a];
-        r.sq[a] = a < static_cast<int>(p->sq.size()) ? p->sq[a].n : 1;
-        r.dk[a] = a < static_cast<int>(p->sq.size()) ? dk_of(p->sq[a]) : 0.0;
-    }
-    r.slab_axis = 0;
-    r.slab_lo = 1.0;
-    r.slab_hi = 0.0;  // empty slab
-    r.mass = p->mass;
+    tdse::ReduceArgs r = reduce_args(p);
     double host[tdse::kReduceQ];
     TDSE_HIP(tdse::launch_reduce(p->fp64(), false, r, p->partials, p->red_out, p->stream));
     TDSE_HIP(hipMemcpyAsync(host, p->red_out, sizeof(host), hipMemcpyDeviceToHost, p->stream));
@@ -1452,6 +1577,10 @@ int tdse_plan_autotune(tdse_plan* p, int reps, int max_slots, int* variants, int
     if (!p->scratch && (rc = alloc(p, &p->scratch, p->total * p->csize()))) return rc;
     int saved[kMaxSlots], chosen[kMaxSlots];
     for (int s = 0; s < kMaxSlots; ++s) saved[s] = p->variant[s];
+    // timing runs the measuring sweeps of an imaginary-time plan, which
+    // write its statistics: keep them
+    tdse::RelaxSlots kept{};
+    if (p->imag() && (rc = d2h(p, &kept, p->relax(), sizeof(kept)))) return rc;
     Tuner t{p};
     t.bytes = static_cast<size_t>(p->total) * p->csize();
     if (hipEventCreate(&t.e0) != hipSuccess || hipEventCreate(&t.e1) != hipSuccess) {
@@ -1461,6 +1590,11 @@ int tdse_plan_autotune(tdse_plan* p, int reps, int max_slots, int* variants, int
     rc = autotune_impl(p, t, reps, chosen);
     (void)hipEventDestroy(t.e0);
     (void)hipEventDestroy(t.e1);
+    if (p->imag()) {
+        kept.ticket = 0;
+        const int rc2 = h2d(p, p->relax(), &kept, sizeof(kept));
+        if (!rc) rc = rc2;
+    }
     if (rc) {
         for (int s = 0; s < kMaxSlots; ++s) p->variant[s] = saved[s];
         return rc;
@@ -1495,6 +1629,8 @@ int tdse_batch_create(tdse_ctx* ctx, tdse_plan* const* plans, int count, tdse_ba
         if (!plans[i]->has_potential) return fail(TDSE_E_STATE, "plan without potential");
         if (plans[i]->flags & TDSE_FLAG_UNFUSED_BASELINE)
             return fail(TDSE_E_INVALID, "the unfused baseline does not batch");
+        if (plans[i]->imag())
+            return fail(TDSE_E_UNSUPPORTED, "imaginary-time plans do not batch");
         for (int j = 0; j < i; ++j)
             if (plans[j] == plans[i]) return fail(TDSE_E_INVALID, "plan listed twice");
     }

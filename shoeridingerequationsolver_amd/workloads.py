@@ -65,7 +65,7 @@ class Solve:
     n: tuple            # (nx, ny, nz)
     lo: tuple
     hi: tuple
-    potential: str      # harmonic | coulomb_floor | regularized_coulomb | barrier | free
+    potential: str      # harmonic | coulomb_floor | regularized_coulomb | soft_coulomb | barrier | free
     params: dict
     r0: tuple
     sigma: tuple
@@ -92,7 +92,7 @@ class Solve:
 
     def build_potential(self) -> np.ndarray:
         from . import (barrier_potential, coulomb_floor_potential, harmonic_potential,
-                       regularized_coulomb_potential)
+                       regularized_coulomb_potential, soft_coulomb_potential)
         g = self.grid()
         p = self.params
         if self.potential == "harmonic":
@@ -104,6 +104,10 @@ class Solve:
             return regularized_coulomb_potential(g, p["Z"], p.get("center", (0.0, 0.0, 0.0)))
         if self.potential == "barrier":
             return barrier_potential(g, p["V0"], p["x_lo"], p["x_hi"])
+        if self.potential == "soft_coulomb":
+            c = p.get("center", (0.0, 0.0, 0.0))
+            return soft_coulomb_potential(g, p["Z"], p["a"],
+                                          c[0] if self.n[1] == self.n[2] == 1 else c)
         if self.potential == "free":
             return np.zeros(self.points)
         raise ValueError(f"unknown potential {self.potential}")
@@ -126,6 +130,8 @@ class Solve:
         elif self.potential == "barrier":
             r.potential, r.strength = TDSE_POT["barrier"], p["V0"]
             r.x_lo, r.x_hi = p["x_lo"], p["x_hi"]
+        elif self.potential == "soft_coulomb":
+            r.potential, r.strength, r.softening = TDSE_POT["soft_coulomb"], p["Z"], p["a"]
         elif self.potential == "free":
             r.potential = TDSE_POT["none"]
         else:

@@ -2,6 +2,7 @@
 // core/src/simulation.ixx (advance) and SplitOperator3D::step.  Writes the
 // stepped state as raw complex128 to argv[1]; exits 77 without a GPU (the
 // reference harness's skip code, solver/src/vkcheck_main.cpp:3824-3826).
+#include <cmath>
 #include <cstdio>
 #include <fstream>
 #include <stdexcept>
@@ -38,6 +39,21 @@ int main(int argc, char** argv) {
             return 1;
         } catch (const std::invalid_argument&) {
         }
+        // ImaginaryTimePropagator1D::relax (imaginary_time_test.cpp shape):
+        // the harmonic ground state, E0 = 0.5 from the norm decay of a step
+        const tdse::Grid1D g1{-16.0, 16.0, 512};
+        std::vector<double> v1(512);
+        for (int i = 0; i < 512; ++i) v1[static_cast<std::size_t>(i)] = 0.5 * g1.coord(i) * g1.coord(i);
+        tdse::Field1D p1{g1};
+        for (int i = 0; i < 512; ++i) {
+            const double x = g1.coord(i) - 1.0;
+            p1[i] = std::complex<double>{std::exp(-x * x / 16.0), 0.0};
+        }
+        tdse::ImaginaryTimePropagator1D relax{g1, v1, 0.005};
+        relax.relax(p1, 4000);
+        const double e0 = relax.stats().energy;
+        std::printf("facade: relaxed E0 = %.8f\n", e0);
+        if (std::abs(e0 - 0.5) > 1e-4) return 1;
         if (argc > 1) {
             std::ofstream out(argv[1], std::ios::binary);
             out.write(reinterpret_cast<const char*>(psi.data().data()),
