@@ -121,13 +121,18 @@ int tdse_plan_sync(tdse_plan* plan, int timeout_ms);
 int tdse_plan_stream(tdse_plan* plan, void** out);
 
 /* Observables of the resident state, FP64 results, fixed-order reductions.
+ * One read pass gives norm, position, width and <V> (every call); the k-space
+ * terms take one read-only line-FFT sweep per axis (no copy of the state).
  *   TDSE_OBS_NORM      norm = sum |psi|^2 * cell volume      (core/src/field.ixx:70-76)
- *   TDSE_OBS_POSITION  <x>, <y>, <z> (ratio form)            (core/src/observables.ixx:134-153)
+ *   TDSE_OBS_POSITION  <x>, <y>, <z> (ratio form) and the     (core/src/observables.ixx:134-175)
+ *                      widths sqrt(max(0, <x^2> - <x>^2))
  *   TDSE_OBS_ENERGY    <V> + <k^2/2m>, each term normalised  (core/src/observables.ixx:202-236)
+ *   TDSE_OBS_MOMENTUM  <k_x>, <k_y>, <k_z> (ratio form)      (core/src/observables.ixx:52-65, 177-200)
  */
 #define TDSE_OBS_NORM 1u
 #define TDSE_OBS_POSITION 2u
 #define TDSE_OBS_ENERGY 4u
+#define TDSE_OBS_MOMENTUM 8u
 typedef struct tdse_observables {
     double norm;
     double sum_abs2;      /* sum |psi|^2 without the cell volume */
@@ -135,6 +140,8 @@ typedef struct tdse_observables {
     double potential;     /* <V> */
     double kinetic;       /* <k^2 / 2m> */
     double energy;        /* potential + kinetic */
+    double sigma[3];      /* sqrt(max(0, <x_a^2> - <x_a>^2)) */
+    double momentum[3];   /* <k_a> */
 } tdse_observables;
 int tdse_observe(tdse_plan* plan, unsigned what, tdse_observables* out);
 /* Probability lo <= coord_axis < hi times the cell volume, not normalised

@@ -57,7 +57,8 @@ def orc() -> C.CDLL:
         lib = C.CDLL(ORC_PATH)
         for name in ("orc_norm_sq", "orc_mean_position_1d", "orc_mean_energy_1d",
                      "orc_probability_in_range_1d", "orc_mean_energy_3d",
-                     "orc_probability_in_slab_3d", "orc_spacing"):
+                     "orc_probability_in_slab_3d", "orc_spacing", "orc_sigma_x_1d",
+                     "orc_mean_momentum_1d"):
             getattr(lib, name).restype = C.c_double
         _ORC = lib
     return _ORC
@@ -72,7 +73,8 @@ def ref() -> C.CDLL:
     if _REF is None:
         lib = C.CDLL(REF_PATH)
         for name in ("ref_norm_sq_1d", "ref_norm_sq_3d", "ref_mean_position_1d",
-                     "ref_mean_energy_1d", "ref_probability_in_range_1d", "ref_mean_energy_3d"):
+                     "ref_mean_energy_1d", "ref_probability_in_range_1d", "ref_mean_energy_3d",
+                     "ref_sigma_x_1d", "ref_mean_momentum_1d"):
             getattr(lib, name).restype = C.c_double
         _REF = lib
     return _REF
@@ -195,6 +197,30 @@ def so3d_evolve(v, dims, lo, hi, dt, psi, nsteps, mass=1.0, mask=None, any_size=
                 raise ValueError("size must be a power of two")
             orc().orc_apply_mask(_p(psi), _p(mask), C.c_size_t(psi.size))
     return psi.reshape(shape)
+
+
+def sigma_x_1d(psi, xmin, xmax):
+    psi = _c128(psi)
+    return orc().orc_sigma_x_1d(_p(psi), psi.size, C.c_double(xmin), C.c_double(xmax))
+
+
+def mean_momentum_1d(psi, xmin, xmax):
+    psi = _c128(psi)
+    return orc().orc_mean_momentum_1d(_p(psi), psi.size, C.c_double(xmin), C.c_double(xmax))
+
+
+def sigma_position_3d(psi, dims, lo, hi):
+    psi = _c128(psi).ravel()
+    out = np.empty(3)
+    orc().orc_sigma_position_3d(_p(psi), _ivec(dims), _dvec(lo), _dvec(hi), _p(out))
+    return tuple(out)
+
+
+def mean_momentum_3d(psi, dims, lo, hi):
+    psi = _c128(psi).ravel()
+    out = np.empty(3)
+    orc().orc_mean_momentum_3d(_p(psi), _ivec(dims), _dvec(lo), _dvec(hi), _p(out))
+    return tuple(out)
 
 
 def it1d_tables(v, xmin, xmax, dtau):
@@ -610,3 +636,27 @@ class Ref:
         ref().ref_soft_coulomb_3d(_ivec(dims), _dvec(lo), _dvec(hi), C.c_double(z), C.c_double(a),
                                   _dvec(c), _p(v))
         return v
+
+    @staticmethod
+    def sigma_x_1d(psi, xmin, xmax):
+        psi = _c128(psi)
+        return ref().ref_sigma_x_1d(_p(psi), psi.size, C.c_double(xmin), C.c_double(xmax))
+
+    @staticmethod
+    def mean_momentum_1d(psi, xmin, xmax):
+        psi = _c128(psi)
+        return ref().ref_mean_momentum_1d(_p(psi), psi.size, C.c_double(xmin), C.c_double(xmax))
+
+    @staticmethod
+    def sigma_position_3d(psi, dims, lo, hi):
+        psi = _c128(psi).ravel()
+        out = np.empty(3)
+        ref().ref_sigma_position_3d(_p(psi), _ivec(dims), _dvec(lo), _dvec(hi), _p(out))
+        return tuple(out)
+
+    @staticmethod
+    def mean_momentum_3d(psi, dims, lo, hi):
+        psi = _c128(psi).ravel()
+        out = np.empty(3)
+        ref().ref_mean_momentum_3d(_p(psi), _ivec(dims), _dvec(lo), _dvec(hi), _p(out))
+        return tuple(out)

@@ -163,6 +163,36 @@ int ref_so3d_evolve(const double* v, const int d[3], const double lo[3], const d
     return 0;
 }
 
+// Widths and momenta (core/src/observables.ixx:40-65, 153-200).
+double ref_sigma_x_1d(const double* psi, int n, double xmin, double xmax) {
+    ses::Field1D f{ses::Grid1D{xmin, xmax, n}};
+    load(f.data(), psi);
+    return ses::sigma_x(f);
+}
+double ref_mean_momentum_1d(const double* psi, int n, double xmin, double xmax) {
+    ses::Field1D f{ses::Grid1D{xmin, xmax, n}};
+    load(f.data(), psi);
+    return ses::mean_momentum(f);
+}
+void ref_sigma_position_3d(const double* psi, const int d[3], const double lo[3],
+                           const double hi[3], double* out) {
+    ses::Field3D f{grid3(d, lo, hi)};
+    load(f.data(), psi);
+    const ses::Vec3d s = ses::sigma_position(f);
+    out[0] = s.x;
+    out[1] = s.y;
+    out[2] = s.z;
+}
+void ref_mean_momentum_3d(const double* psi, const int d[3], const double lo[3],
+                          const double hi[3], double* out) {
+    ses::Field3D f{grid3(d, lo, hi)};
+    load(f.data(), psi);
+    const ses::Vec3d s = ses::mean_momentum(f);
+    out[0] = s.x;
+    out[1] = s.y;
+    out[2] = s.z;
+}
+
 // ImaginaryTimePropagator1D::relax (core/src/imaginary_time.ixx:21-63).
 int ref_it1d_relax(const double* v, int n, double xmin, double xmax, double dtau, double* psi,
                    int nsteps) {

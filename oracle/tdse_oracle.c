@@ -795,6 +795,98 @@ void orc_mean_position_3d(const double* psi, const int d[3], const double lo[3],
     out[2] = obs_ratio(nz, den);
 }
 
+static inline double obs_sigma(double second, double mean) {
+    const double v = second - mean * mean;
+    return sqrt(v > 0.0 ? v : 0.0);
+}
+
+/* core/src/observables.ixx:40-50 */
+double orc_sigma_x_1d(const double* psi, int n, double xmin, double xmax) {
+    double num = 0.0, den = 0.0;
+    for (int i = 0; i < n; ++i) {
+        const double x = coord(xmin, xmax, n, i);
+        const double w = psi[2 * i] * psi[2 * i] + psi[2 * i + 1] * psi[2 * i + 1];
+        num += x * x * w;
+        den += w;
+    }
+    return obs_sigma(obs_ratio(num, den), orc_mean_position_1d(psi, n, xmin, xmax));
+}
+
+/* core/src/observables.ixx:52-65 (power-of-two n, like the reference) */
+double orc_mean_momentum_1d(const double* psi, int n, double xmin, double xmax) {
+    double* phi = (double*)malloc(sizeof(double) * 2 * (size_t)n);
+    memcpy(phi, psi, sizeof(double) * 2 * (size_t)n);
+    if (orc_fft1d(phi, (size_t)n) != 0) orc_fft_any(phi, (size_t)n, -1);
+    double* k = (double*)malloc(sizeof(double) * (size_t)n);
+    orc_wavenumbers(n, xmin, xmax, k);
+    double num = 0.0, den = 0.0;
+    for (int j = 0; j < n; ++j) {
+        const double w = phi[2 * j] * phi[2 * j] + phi[2 * j + 1] * phi[2 * j + 1];
+        num += k[j] * w;
+        den += w;
+    }
+    free(k);
+    free(phi);
+    return obs_ratio(num, den);
+}
+
+/* core/src/observables.ixx:153-175 */
+void orc_sigma_position_3d(const double* psi, const int d[3], const double lo[3],
+                           const double hi[3], double out[3]) {
+    double nx = 0.0, ny = 0.0, nz = 0.0, den = 0.0;
+    for (int k = 0; k < d[2]; ++k)
+        for (int j = 0; j < d[1]; ++j)
+            for (int i = 0; i < d[0]; ++i) {
+                const long f = i + (long)d[0] * (j + (long)d[1] * k);
+                const double w = psi[2 * f] * psi[2 * f] + psi[2 * f + 1] * psi[2 * f + 1];
+                const double x = coord(lo[0], hi[0], d[0], i);
+                const double y = coord(lo[1], hi[1], d[1], j);
+                const double z = coord(lo[2], hi[2], d[2], k);
+                nx += x * x * w;
+                ny += y * y * w;
+                nz += z * z * w;
+                den += w;
+            }
+    double m[3];
+    orc_mean_position_3d(psi, d, lo, hi, m);
+    out[0] = obs_sigma(obs_ratio(nx, den), m[0]);
+    out[1] = obs_sigma(obs_ratio(ny, den), m[1]);
+    out[2] = obs_sigma(obs_ratio(nz, den), m[2]);
+}
+
+/* core/src/observables.ixx:177-200 */
+void orc_mean_momentum_3d(const double* psi, const int d[3], const double lo[3],
+                          const double hi[3], double out[3]) {
+    const long n = (long)d[0] * d[1] * d[2];
+    double* phi = (double*)malloc(sizeof(double) * 2 * (size_t)n);
+    memcpy(phi, psi, sizeof(double) * 2 * (size_t)n);
+    if (orc_fft3d(phi, d[0], d[1], d[2]) != 0) orc_fft3d_any(phi, d[0], d[1], d[2], -1);
+    double* kx = (double*)malloc(sizeof(double) * (size_t)d[0]);
+    double* ky = (double*)malloc(sizeof(double) * (size_t)d[1]);
+    double* kz = (double*)malloc(sizeof(double) * (size_t)d[2]);
+    orc_wavenumbers(d[0], lo[0], hi[0], kx);
+    orc_wavenumbers(d[1], lo[1], hi[1], ky);
+    orc_wavenumbers(d[2], lo[2], hi[2], kz);
+    double nx = 0.0, ny = 0.0, nz = 0.0, den = 0.0;
+    for (int k = 0; k < d[2]; ++k)
+        for (int j = 0; j < d[1]; ++j)
+            for (int i = 0; i < d[0]; ++i) {
+                const long f = i + (long)d[0] * (j + (long)d[1] * k);
+                const double w = phi[2 * f] * phi[2 * f] + phi[2 * f + 1] * phi[2 * f + 1];
+                nx += kx[i] * w;
+                ny += ky[j] * w;
+                nz += kz[k] * w;
+                den += w;
+            }
+    free(kx);
+    free(ky);
+    free(kz);
+    free(phi);
+    out[0] = obs_ratio(nx, den);
+    out[1] = obs_ratio(ny, den);
+    out[2] = obs_ratio(nz, den);
+}
+
 /* core/src/observables.ixx:202-236 (power-of-two grids, like the reference) */
 double orc_mean_energy_3d(const double* psi, const int d[3], const double lo[3],
                           const double hi[3], const double* v, double mass) {

@@ -113,6 +113,12 @@ void orc_mean_position_3d(const double* psi, const int dims[3], const double lo[
                           const double hi[3], double out[3]);
 double orc_mean_energy_3d(const double* psi, const int dims[3], const double lo[3],
                           const double hi[3], const double* v, double mass);
+double orc_sigma_x_1d(const double* psi, int n, double xmin, double xmax);
+double orc_mean_momentum_1d(const double* psi, int n, double xmin, double xmax);
+void orc_sigma_position_3d(const double* psi, const int dims[3], const double lo[3],
+                           const double hi[3], double out[3]);
+void orc_mean_momentum_3d(const double* psi, const int dims[3], const double lo[3],
+                          const double hi[3], double out[3]);
 double orc_probability_in_slab_3d(const double* psi, const int dims[3], const double lo[3],
                                   const double hi[3], int axis, double a, double b);
 

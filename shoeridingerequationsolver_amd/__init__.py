@@ -28,8 +28,8 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import capi
-from .capi import (TDSE_FP32, TDSE_FP64, TDSE_OBS_ENERGY, TDSE_OBS_NORM, TDSE_OBS_POSITION,
-                   TdseError)
+from .capi import (TDSE_FP32, TDSE_FP64, TDSE_OBS_ENERGY, TDSE_OBS_MOMENTUM, TDSE_OBS_NORM,
+                   TDSE_OBS_POSITION, TdseError)
 
 __all__ = [
     "Grid1D", "Grid3D", "wavenumbers", "Context", "Propagator", "SplitOperator1D",
@@ -245,7 +245,8 @@ class Propagator:
         o = capi.Observables()
         capi.check(capi.lib().tdse_observe(self._h, what, C.byref(o)))
         return {"norm": o.norm, "sum_abs2": o.sum_abs2, "position": tuple(o.position),
-                "potential": o.potential, "kinetic": o.kinetic, "energy": o.energy}
+                "potential": o.potential, "kinetic": o.kinetic, "energy": o.energy,
+                "sigma": tuple(o.sigma), "momentum": tuple(o.momentum)}
 
     def probability_in_slab(self, axis, lo, hi):
         v = C.c_double()

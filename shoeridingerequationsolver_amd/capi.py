@@ -29,6 +29,7 @@ TDSE_FLAG_IMAGINARY_TIME = 2
 TDSE_OBS_NORM = 1
 TDSE_OBS_POSITION = 2
 TDSE_OBS_ENERGY = 4
+TDSE_OBS_MOMENTUM = 8
 
 # every symbol include/tdse.h declares (checked by the CPU test tier)
 EXPORTS = (
@@ -84,6 +85,8 @@ class Observables(C.Structure):
         ("potential", C.c_double),
         ("kinetic", C.c_double),
         ("energy", C.c_double),
+        ("sigma", C.c_double * 3),
+        ("momentum", C.c_double * 3),
     ]
 
 

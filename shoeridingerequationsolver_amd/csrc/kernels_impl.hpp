@@ -23,6 +23,7 @@ hipError_t launch_sweep(const SweepArgs& a, int max_blocks, hipStream_t stream) 
     if (tiles <= 0) return hipSuccess;
     // measuring sweeps keep one partial per block
     if (mode_measures(MODE) && max_blocks > kRelaxMaxBlocks) max_blocks = kRelaxMaxBlocks;
+    if (mode_observes(MODE) && max_blocks > kObsMaxBlocks) max_blocks = kObsMaxBlocks;
     const int grid = static_cast<int>(tiles < max_blocks ? tiles : max_blocks);
     if constexpr (mode_contiguous(MODE)) {
         hipLaunchKernelGGL((contig_sweep<T, N, MODE, V>), dim3(grid), dim3(block), 0, stream, a);
@@ -88,6 +89,8 @@ LaunchFn pick_mode(int mode) {
         case S_RELAX_FIRST_TW: return &launch_sweep<T, N, S_RELAX_FIRST_TW>;
         case S_RELAX_MID_TW: return &launch_sweep<T, N, S_RELAX_MID_TW>;
         case S_RELAX_LAST: return &launch_sweep<T, N, S_RELAX_LAST>;
+        case C_OBS: return &launch_sweep<T, N, C_OBS>;
+        case S_OBS: return &launch_sweep<T, N, S_OBS>;
         default: return nullptr;
     }
 }

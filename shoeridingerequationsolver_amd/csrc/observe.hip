@@ -43,13 +43,14 @@ __device__ __forceinline__ void block_reduce_store(double (&acc)[Q], double* out
 
 __device__ __forceinline__ int shift_k(int j, int n) { return (2 * j < n) ? j : j - n; }
 
-// x-space sums: |psi|^2, x|psi|^2, y|psi|^2, z|psi|^2, V|psi|^2, slab |psi|^2
+// x-space sums: |psi|^2, x|psi|^2, y|psi|^2, z|psi|^2, V|psi|^2, slab |psi|^2,
+// x^2|psi|^2, y^2|psi|^2, z^2|psi|^2
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
 reduce_x(const ReduceArgs a, double* partials) {
     const cx<T>* psi = static_cast<const cx<T>*>(a.psi);
     const T* v = static_cast<const T*>(a.v);
-    double acc[kReduceQ] = {0, 0, 0, 0, 0, 0};
+    double acc[kReduceQ] = {};
     const long long stride = static_cast<long long>(gridDim.x) * kBlock;
     for (long long f = blockIdx.x * static_cast<long long>(kBlock) + threadIdx.x; f < a.total;
          f += stride) {
@@ -69,18 +70,22 @@ reduce_x(const ReduceArgs a, double* partials) {
         if (v) acc[4] += static_cast<double>(v[f]) * w;
         const double c = a.slab_axis == 0 ? cxv : (a.slab_axis == 1 ? cyv : czv);
         if (c >= a.slab_lo && c < a.slab_hi) acc[5] += w;
+        acc[6] += cxv * cxv * w;
+        acc[7] += cyv * cyv * w;
+        acc[8] += czv * czv * w;
     }
     block_reduce_store<kReduceQ>(acc, partials + blockIdx.x * kReduceQ);
 }
 
-// k-space sums of the forward transform: |phi|^2 and 0.5 |k|^2 / m |phi|^2.
+// k-space sums of the forward transform: |phi|^2, 0.5 |k|^2 / m |phi|^2 and
+// the first moments k_a |phi|^2 (slots 2-4).
 // Natural order on the squeezed axes, or the four-step order of a 1D grid
 // (address k1 + N1*k2 holds bin k2 + N2*k1).
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
 reduce_k(const ReduceArgs a, double* partials) {
     const cx<T>* phi = static_cast<const cx<T>*>(a.psi);
-    double acc[kReduceQ] = {0, 0, 0, 0, 0, 0};
+    double acc[kReduceQ] = {};
     const long long stride = static_cast<long long>(gridDim.x) * kBlock;
     for (long long f = blockIdx.x * static_cast<long long>(kBlock) + threadIdx.x; f < a.total;
          f += stride) {
@@ -93,6 +98,7 @@ reduce_k(const ReduceArgs a, double* partials) {
             const int j = static_cast<int>(k2 + a.four_step_rows * k1);
             const double k = a.dk[0] * shift_k(j, static_cast<int>(a.total));
             k2sum = k * k;
+            acc[2] += k * w;
         } else {
             const int i0 = static_cast<int>(f % a.sq[0]);
             const long long r = f / a.sq[0];
@@ -102,6 +108,9 @@ reduce_k(const ReduceArgs a, double* partials) {
             const double k1 = a.dk[1] * shift_k(i1, a.sq[1]);
             const double k2 = a.dk[2] * shift_k(i2, a.sq[2]);
             k2sum = k0 * k0 + k1 * k1 + k2 * k2;
+            acc[2] += k0 * w;
+            acc[3] += k1 * w;
+            acc[4] += k2 * w;
         }
         acc[0] += w;
         acc[1] += 0.5 * k2sum / a.mass * w;
@@ -111,7 +120,7 @@ reduce_k(const ReduceArgs a, double* partials) {
 
 __global__ void __launch_bounds__(kBlock)
 reduce_final(const double* partials, int nparts, double* out) {
-    double acc[kReduceQ] = {0, 0, 0, 0, 0, 0};
+    double acc[kReduceQ] = {};
     for (int p = threadIdx.x; p < nparts; p += kBlock) {
 #pragma unroll
         for (int q = 0; q < kReduceQ; ++q) acc[q] += partials[p * kReduceQ + q];
@@ -130,6 +139,15 @@ scale_state(void* psi_v, long long total, double s) {
         cx<T> z = psi[f];
         psi[f] = {z.x * ss, z.y * ss};
     }
+}
+
+// Sum of per-block triples (observing sweeps), fixed order.
+__global__ void __launch_bounds__(kBlock)
+reduce_final3(const double* partials, int nparts, double* out) {
+    double acc[3] = {0, 0, 0};
+    for (int p = threadIdx.x; p < nparts; p += kBlock)
+        for (int q = 0; q < 3; ++q) acc[q] += partials[3 * p + q];
+    block_reduce_store<3>(acc, out);
 }
 
 // psi *= s[0], the factor left in device memory by an earlier kernel
@@ -154,7 +172,7 @@ __global__ void __launch_bounds__(kBlock)
 reduce_inner(const void* a_v, const void* b_v, long long total, double* partials) {
     const cx<T>* a = static_cast<const cx<T>*>(a_v);
     const cx<T>* b = static_cast<const cx<T>*>(b_v);
-    double acc[kReduceQ] = {0, 0, 0, 0, 0, 0};
+    double acc[kReduceQ] = {};
     const long long stride = static_cast<long long>(gridDim.x) * kBlock;
     for (long long f = blockIdx.x * static_cast<long long>(kBlock) + threadIdx.x; f < total;
          f += stride) {
@@ -293,6 +311,12 @@ long long scale_blocks(long long total) {
     return b;
 }
 }  // namespace
+
+hipError_t launch_reduce_final3(const double* partials, int nblocks, double* out,
+                                hipStream_t stream) {
+    hipLaunchKernelGGL(reduce_final3, dim3(1), dim3(kBlock), 0, stream, partials, nblocks, out);
+    return hipGetLastError();
+}
 
 hipError_t launch_scale_dev(bool fp64, void* psi, long long total, const double* s,
                             hipStream_t stream) {

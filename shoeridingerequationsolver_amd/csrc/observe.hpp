@@ -7,7 +7,7 @@
 
 namespace tdse {
 
-constexpr int kReduceQ = 6;
+constexpr int kReduceQ = 9;
 constexpr int kMaxReduceBlocks = 2048;
 
 struct ReduceArgs {
@@ -29,6 +29,9 @@ int reduce_blocks(long long total);
 hipError_t launch_reduce(bool fp64, bool kspace, const ReduceArgs& a, double* partials,
                          double* out, hipStream_t stream);
 hipError_t launch_scale(bool fp64, void* psi, long long total, double s, hipStream_t stream);
+// out[q] = sum over blocks b < nblocks of partials[3 b + q], q < 3, fixed order
+hipError_t launch_reduce_final3(const double* partials, int nblocks, double* out,
+                                hipStream_t stream);
 // psi *= *s (device scalar)
 hipError_t launch_scale_dev(bool fp64, void* psi, long long total, const double* s,
                             hipStream_t stream);

@@ -66,12 +66,17 @@ enum SweepMode : int {
     S_RELAX_FIRST_TW = 17,  // w, FFT, twiddle
     S_RELAX_MID_TW = 18,    // IFFT, scale * w, measure, w, FFT, twiddle
     S_RELAX_LAST = 19,      // IFFT, scale * w, measure
-    kNumModes = 20
+    // observables: read-only line spectra (no store)
+    C_OBS = 20,             // FFT, accumulate |psi_k|^2 (1, k, k^2)
+    S_OBS = 21,             // FFT, accumulate |psi_k|^2 (1, k, k^2)
+    kNumModes = 22
 };
 
 constexpr bool mode_contiguous(int m) {
-    return m <= C_SINGLE || m == C_FWD || (m >= C_RELAX_FIRST && m <= C_RELAX_SINGLE);
+    return m <= C_SINGLE || m == C_FWD || (m >= C_RELAX_FIRST && m <= C_RELAX_SINGLE) ||
+           m == C_OBS;
 }
+constexpr bool mode_observes(int m) { return m == C_OBS || m == S_OBS; }
 // modes that measure the norm across blocks (deferred renormalisation)
 constexpr bool mode_measures(int m) {
     return m == C_RELAX_MID || m == C_RELAX_LAST || m == S_RELAX_MID_TW || m == S_RELAX_LAST;
@@ -93,11 +98,17 @@ struct RelaxSlots {
 // Blocks of a measuring sweep (partials hold two doubles per block).
 constexpr int kRelaxMaxBlocks = 2048;
 
-// Per-thread accumulation of a measuring sweep (over all tiles of a block).
-struct RelaxAcc {
-    double sum = 0.0;
-    double peak = 0.0;
+// Per-thread accumulation over all tiles of a block: a measuring sweep keeps
+// (sum |psi|^2, max |psi|^2), an observing sweep (sum w, sum k w, sum k^2 w)
+// of its line spectra.
+struct SweepAcc {
+    double q0 = 0.0;
+    double q1 = 0.0;
+    double q2 = 0.0;
 };
+
+// Blocks of an observing sweep (partials hold three doubles per block).
+constexpr int kObsMaxBlocks = 2048;
 
 struct SweepArgs {
     void* psi;            // cx<T>*: the state, x-fastest
@@ -123,6 +134,7 @@ struct SweepArgs {
     double* partials;     // per-block (sum, max) of a measuring sweep
     double cell;          // volume element dV
     int parity;           // measuring sweeps: read scale[parity], write scale[parity ^ 1]
+    double dk;            // observing sweeps: 2 pi / L of the line axis
 };
 
 // ---- buffer access -----------------------------------------------------------
@@ -346,7 +358,7 @@ struct Line {
     }
     // v <- v * pre * w, accumulated into (sum, peak) of |v|^2, then (second)
     // times w again; one rounding per component.
-    __device__ __forceinline__ void relax_measure(double pre, bool second, RelaxAcc& ra) {
+    __device__ __forceinline__ void relax_measure(double pre, bool second, SweepAcc& ra) {
         load_kick(false);
         if (!valid) return;
         for (int s = 0; s < E; ++s) {
@@ -354,14 +366,27 @@ struct Line {
             const double f = pre * e;
             double xr = f * v[s].x, xi = f * v[s].y;
             const double a2 = xr * xr + xi * xi;
-            ra.sum += a2;
-            ra.peak = a2 > ra.peak ? a2 : ra.peak;
+            ra.q0 += a2;
+            ra.q1 = a2 > ra.q1 ? a2 : ra.q1;
             if (second) {
                 xr *= e;
                 xi *= e;
             }
             v[s] = {static_cast<T>(xr), static_cast<T>(xi)};
         }
+    }
+    // Spectrum moments of the transformed line (pattern RL, bin i):
+    // sum |v|^2, sum k |v|^2, sum k^2 |v|^2 with k the fftfreq wavenumber.
+    __device__ __forceinline__ void observe_spectrum(SweepAcc& acc) {
+        if (!valid) return;
+        for_each_point<N, E, RL>(t, [&](int s, int i, int, int) {
+            const double w = static_cast<double>(v[s].x) * v[s].x +
+                             static_cast<double>(v[s].y) * v[s].y;
+            const double k = a.dk * ((2 * i < N) ? i : i - N);
+            acc.q0 += w;
+            acc.q1 += k * w;
+            acc.q2 += k * k * w;
+        });
     }
     __device__ __forceinline__ void fft() { Transform<T, N, E, -1, 1, Fwd>::run(v, t, tw, x); }
     __device__ __forceinline__ void ifft() { Transform<T, N, E, +1, 1, Inv>::run(v, t, tw, x); }
@@ -404,8 +429,8 @@ __device__ __forceinline__ void block_sum_max(double& s, double& m) {
 // last block to finish (ticket) adds the partials in block order (fixed
 // order: bitwise repeatable) and leaves scale, norm and peak for the next
 // sweep.  Every block of the grid takes a ticket, so the last one exists.
-__device__ __forceinline__ void relax_block_done(const SweepArgs& a, RelaxAcc ra) {
-    double s = ra.sum, m = ra.peak;
+__device__ __forceinline__ void relax_block_done(const SweepArgs& a, SweepAcc ra) {
+    double s = ra.q0, m = ra.q1;
     block_sum_max(s, m);
     __shared__ int last;
     if (threadIdx.x == 0) {
@@ -435,13 +460,32 @@ __device__ __forceinline__ void relax_block_done(const SweepArgs& a, RelaxAcc ra
     }
 }
 
+// End of an observing sweep, per block: the three sums (fixed-order block
+// reduction) into partials[3 * block + q]; a single-block pass adds them.
+__device__ __forceinline__ void obs_block_done(const SweepArgs& a, SweepAcc acc) {
+    __shared__ double red[3 * 16];
+    double q[3] = {acc.q0, acc.q1, acc.q2};
+    for (int c = 0; c < 3; ++c)
+        for (int off = 32; off > 0; off >>= 1) q[c] += __shfl_xor(q[c], off, 64);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int nw = (blockDim.x + 63) >> 6;
+    if (lane == 0)
+        for (int c = 0; c < 3; ++c) red[3 * wave + c] = q[c];
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        double t = 0.0;
+        for (int w = 0; w < nw; ++w) t += red[3 * w + threadIdx.x];
+        a.partials[3 * blockIdx.x + threadIdx.x] = t;
+    }
+}
+
 // ---- per-tile bodies ---------------------------------------------------------------
 
 // One tile of a contiguous sweep: LPB whole lines.  `lds` holds LPB padded
 // lines, `tw` the staged FP64 twiddles of this line length.
 template <typename T, int N, int MODE, int V = 0>
 __device__ __forceinline__ void contig_tile(const SweepArgs& a, long long tile, cx<T>* lds,
-                                            const cx<double>* tw, int tid, RelaxAcc& ra) {
+                                            const cx<double>* tw, int tid, SweepAcc& ra) {
     using S = Shape<T, N, V>;
     using LT = Line<T, N, ContigX<T>, V>;
     constexpr int TT = S::TT, LPB = S::LPB, E = S::E;
@@ -502,6 +546,10 @@ __device__ __forceinline__ void contig_tile(const SweepArgs& a, long long tile, 
             L.apply_kick(step + 1 == a.nsteps ? 1.0 : 2.0, true);
         }
         L.template store<LT::R1>();
+    } else if constexpr (MODE == C_OBS) {
+        L.template load<LT::R1>();
+        L.fft();
+        L.observe_spectrum(ra);
     } else if constexpr (MODE == C_RELAX_FIRST) {
         L.template load<LT::R1>();
         L.relax_first();
@@ -566,7 +614,7 @@ __device__ __forceinline__ void contig_tile(const SweepArgs& a, long long tile, 
 template <typename T, int N, int MODE, int V = 0>
 __device__ __forceinline__ void strided_tile(const SweepArgs& a, long long tile, cx<T>* lds,
                                              const cx<double>* tw, const cx<double>* kp, int tid,
-                                             RelaxAcc& ra) {
+                                             SweepAcc& ra) {
     using S = Shape<T, N, V>;
     constexpr int TX = S::TX, E = S::E;
     using LT = Line<T, N, StridedX<T, TX>, V>;
@@ -624,6 +672,10 @@ __device__ __forceinline__ void strided_tile(const SweepArgs& a, long long tile,
         L.fft();
         L.twiddle_out(xx);
         L.template store<LT::RL>();
+    } else if constexpr (MODE == S_OBS) {
+        L.template load<LT::R1>();
+        L.fft();
+        L.observe_spectrum(ra);
     } else if constexpr (MODE == S_RELAX_FIRST_TW) {
         L.template load<LT::R1>();
         L.relax_first();
@@ -669,11 +721,12 @@ contig_sweep(SweepArgs a) {
         // a call's first measuring sweep reads scale[0]: start it at 1
         if (blockIdx.x == 0 && tid == 0) a.relax->scale[0] = 1.0;
     }
-    RelaxAcc ra;
+    SweepAcc ra;
     const long long ntiles = (a.nlines + S::LPB - 1) / S::LPB;
     for (long long tile = blockIdx.x; tile < ntiles; tile += gridDim.x)
         contig_tile<T, N, MODE, V>(a, tile, lds, lds_tw, tid, ra);
     if constexpr (mode_measures(MODE)) relax_block_done(a, ra);
+    if constexpr (mode_observes(MODE)) obs_block_done(a, ra);
 }
 
 template <typename T, int N, int MODE, int V = 0>
@@ -691,11 +744,12 @@ strided_sweep(SweepArgs a) {
     if constexpr (MODE == S_RELAX_FIRST_TW) {
         if (blockIdx.x == 0 && tid == 0) a.relax->scale[0] = 1.0;
     }
-    RelaxAcc ra;
+    SweepAcc ra;
     const long long ntiles = static_cast<long long>((a.nx + S::TX - 1) / S::TX) * a.nw;
     for (long long tile = blockIdx.x; tile < ntiles; tile += gridDim.x)
         strided_tile<T, N, MODE, V>(a, tile, lds, lds_tw, lds_kp, tid, ra);
     if constexpr (mode_measures(MODE)) relax_block_done(a, ra);
+    if constexpr (mode_observes(MODE)) obs_block_done(a, ra);
 }
 
 // ---- batched kernels: one launch over the tiles of many plans ------------------------
@@ -738,7 +792,7 @@ contig_sweep_batch(BatchTable b) {
     const long long g1 = g0 + chunk < total ? g0 + chunk : total;
     for (long long g = g0; g < g1; ++g) {
         const int it = find_item(b, g);
-        RelaxAcc unused;
+        SweepAcc unused;
         if constexpr (MODE == C_SINGLE) {
             SweepArgs a = b.items[it];
             a.nsteps = b.nsteps;
@@ -777,7 +831,7 @@ strided_sweep_batch(BatchTable b) {
                 staged = it;
             }
         }
-        RelaxAcc unused;
+        SweepAcc unused;
         strided_tile<T, N, MODE>(b.items[it], g - b.first_tile[it], lds, lds_tw, lds_kp, tid,
                                  unused);
     }

@@ -1274,44 +1274,80 @@ int tdse_observe(tdse_plan* p, unsigned what, tdse_observables* out) {
     out->position[1] = ratio(host[2], host[0]);
     out->position[2] = ratio(host[3], host[0]);
     out->potential = ratio(host[4], host[0]);
-    if (what & TDSE_OBS_ENERGY) {
+    for (int a = 0; a < 3; ++a) {
+        const double m = ratio(host[1 + a], host[0]);
+        out->sigma[a] = std::sqrt(std::max(0.0, ratio(host[6 + a], host[0]) - m * m));
+    }
+    if (!(what & (TDSE_OBS_ENERGY | TDSE_OBS_MOMENTUM))) return TDSE_OK;
+    // original axis of each squeezed axis
+    int orig[3] = {0, 0, 0};
+    for (int a = 0, q = 0; a < 3; ++a)
+        if (p->n[a] > 1) orig[q++] = a;
+    if (p->kind == kKindFourStep) {
+        // the 1D transform of a four-step grid, in a scratch copy
         if (!p->scratch && (rc = alloc(p, &p->scratch, p->total * p->csize()))) return rc;
         TDSE_HIP(hipMemcpyAsync(p->scratch, p->psi, p->total * p->csize(), hipMemcpyDeviceToDevice,
                                 p->stream));
         Geometry g = geometry(p, p->scratch);
         std::vector<Launch> L;
-        auto add = [&](int mode, int n, const SweepArgs& a) {
-            L.push_back({mode, n, a, launcher(p, n, mode), 0});
-        };
-        if (p->kind == kKind3D) {
-            add(tdse::C_FWD, p->sq[0].n, g.x);
-            add(tdse::S_FWD, p->sq[1].n, g.yf);
-            add(tdse::S_FWD, p->sq[2].n, g.z);
-        } else if (p->kind == kKind2D) {
-            add(tdse::C_FWD, p->sq[0].n, g.x);
-            add(tdse::S_FWD, p->sq[1].n, g.yf);
-        } else if (p->kind == kKindSingle) {
-            add(tdse::C_FWD, p->sq[0].n, g.single);
-        } else {
-            add(tdse::S_FWD_TW, p->n2, g.col);
-            add(tdse::C_FWD, p->n1, g.row);
-        }
+        L.push_back({tdse::S_FWD_TW, p->n2, g.col, launcher(p, p->n2, tdse::S_FWD_TW), 0});
+        L.push_back({tdse::C_FWD, p->n1, g.row, launcher(p, p->n1, tdse::C_FWD), 0});
         if ((rc = run_launches(p, L))) return rc;
         tdse::ReduceArgs rk = r;
         rk.psi = p->scratch;
         rk.v = nullptr;
-        if (p->kind == kKindFourStep) {
-            rk.sq[0] = p->n1;
-            rk.sq[1] = p->n2;
-            rk.four_step_rows = p->n2;
-        }
+        rk.sq[0] = p->n1;
+        rk.sq[1] = p->n2;
+        rk.four_step_rows = p->n2;
         double hk[tdse::kReduceQ];
         TDSE_HIP(tdse::launch_reduce(p->fp64(), true, rk, p->partials, p->red_out, p->stream));
         TDSE_HIP(hipMemcpyAsync(hk, p->red_out, sizeof(hk), hipMemcpyDeviceToHost, p->stream));
         TDSE_HIP(hipStreamSynchronize(p->stream));
         out->kinetic = ratio(hk[1], hk[0]);
-        out->energy = out->potential + out->kinetic;
+        out->momentum[orig[0]] = ratio(hk[2], hk[0]);
+    } else {
+        // Separable along axes: sum_k k_a^2 |psi_k|^2 / sum_k |psi_k|^2 equals
+        // the same ratio over the 1D spectra of the lines along axis a (the
+        // transforms along the other axes are unitary up to a constant), so
+        // one read-only line-FFT sweep per axis gives <k_a> and <k_a^2>.
+        const Geometry g = geometry(p, p->psi);
+        const int axes = static_cast<int>(p->sq.size());
+        double* dst = p->red_out + 24;
+        for (int a = 0; a < axes; ++a) {
+            SweepArgs args;
+            int mode, n = p->sq[a].n;
+            if (p->kind == kKindSingle) {
+                args = g.single;
+                mode = tdse::C_OBS;
+            } else if (a == 0) {
+                args = g.x;
+                mode = tdse::C_OBS;
+            } else if (a == 1) {
+                args = g.yf;
+                mode = tdse::S_OBS;
+            } else {
+                args = g.z;
+                mode = tdse::S_OBS;
+            }
+            args.dk = dk_of(p->sq[a]);
+            args.partials = p->partials;
+            const tdse::LaunchFn fn = launcher(p, n, mode);
+            if (!fn) return fail(TDSE_E_UNSUPPORTED, "no observing kernel for this length");
+            TDSE_HIP(fn(args, tdse::kObsMaxBlocks, p->stream));
+            const long long tiles = tiles_of(p->precision, n, mode, args);
+            const int blocks = static_cast<int>(std::min<long long>(tiles, tdse::kObsMaxBlocks));
+            TDSE_HIP(tdse::launch_reduce_final3(p->partials, blocks, dst + 3 * a, p->stream));
+        }
+        double hk[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        if ((rc = d2h(p, hk, dst, sizeof(double) * 3 * axes))) return rc;
+        double t = 0.0;
+        for (int a = 0; a < axes; ++a) {
+            t += 0.5 * ratio(hk[3 * a + 2], hk[3 * a]) / p->mass;
+            out->momentum[orig[a]] = ratio(hk[3 * a + 1], hk[3 * a]);
+        }
+        out->kinetic = t;
     }
+    out->energy = out->potential + out->kinetic;
     return TDSE_OK;
 }
 

@@ -1,0 +1,89 @@
+"""On-device observables against the oracle's restatement of
+core/src/observables.ixx (norm, position, width, momentum, energy), which
+tests/test_oracle.py and tests/test_oracle_relax.py pin to the reference.
+
+The k-space terms come from one read-only line-FFT sweep per axis (the 3D
+ratio sum k_a^2 |psi_k|^2 / sum |psi_k|^2 is the same over the 1D spectra of
+the lines along axis a); four-step 1D grids transform a scratch copy.
+Tolerances: FP64 states 1e-11 on widths and moments (absolute, unit-scale
+quantities) and 1e-10 on energies; FP32 states 2e-6 relative to the scale.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def grid3(t, n, lo, hi):
+    return t.Grid3D(t.Grid1D(lo[0], hi[0], n[0]), t.Grid1D(lo[1], hi[1], n[1]),
+                    t.Grid1D(lo[2], hi[2], n[2]))
+
+
+CASES = [
+    ((32, 16, 16), (-8, -4, -4), (8, 4, 4)),
+    ((64, 64, 64), (-10, -10, -10), (10, 10, 10)),
+    ((256, 128, 1), (-20, -10, 0), (20, 10, 1)),
+    ((96, 32, 16), (-12, -6, -4), (12, 6, 4)),
+    ((160, 192, 1), (-16, -16, 0), (16, 16, 1)),
+]
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("case", CASES, ids=lambda c: "x".join(map(str, c[0])))
+def test_observables_3d_match_oracle(orc, tdse, case, prec):
+    dims, lo, hi = case
+    print(f"case observe dims={dims} prec={prec}", flush=True)
+    v = orc.harmonic_3d(dims, lo, hi, 0.7, (0.5, 0, 0))
+    psi = orc.gaussian_3d(dims, lo, hi, (1.0, 0.5, 0.0), (1.2, 1.1, 0.9),
+                          (0.6, -0.4, 0.2)).reshape(dims[::-1])
+    p = tdse.Propagator(grid3(tdse, dims, lo, hi), v, 0.02, precision=prec)
+    p.upload(psi)
+    if prec == "fp32":
+        psi = p.download()  # the state as stored
+    o = p.observe(tdse.capi.TDSE_OBS_NORM | tdse.capi.TDSE_OBS_POSITION |
+                  tdse.capi.TDSE_OBS_ENERGY | tdse.capi.TDSE_OBS_MOMENTUM)
+    tol = 1e-11 if prec == "fp64" else 2e-6
+    cell = np.prod([(hi[a] - lo[a]) / dims[a] for a in range(3)])
+    assert abs(o["norm"] - orc.norm_sq(psi, cell)) < tol
+    np.testing.assert_allclose(o["position"], orc.mean_position_3d(psi, dims, lo, hi), atol=tol)
+    np.testing.assert_allclose(o["sigma"], orc.sigma_position_3d(psi, dims, lo, hi), atol=tol)
+    np.testing.assert_allclose(o["momentum"], orc.mean_momentum_3d(psi, dims, lo, hi), atol=tol)
+    e = orc.mean_energy_3d(psi, dims, lo, hi, v)
+    print(f"  energy {o['energy']:.14f} vs {e:.14f}", flush=True)
+    assert abs(o["energy"] - e) < (1e-10 if prec == "fp64" else 2e-6 * max(1.0, abs(e)))
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("n,lo,hi", [(512, -16.0, 16.0), (1024, -40.0, 40.0),
+                                     (65536, -100.0, 100.0)])
+def test_observables_1d_match_oracle(orc, tdse, n, lo, hi, prec):
+    print(f"case observe 1d n={n} prec={prec}", flush=True)
+    v = orc.harmonic_1d(lo, hi, n, 0.1)
+    psi = orc.gaussian_1d(lo, hi, n, 5.0, 2.0, 0.5)
+    p = tdse.Propagator(tdse.Grid1D(lo, hi, n), v, 0.01, precision=prec)
+    p.upload(psi)
+    if prec == "fp32":
+        psi = p.download()
+    o = p.observe(15)
+    tol = 1e-11 if prec == "fp64" else 2e-6
+    assert abs(o["norm"] - orc.norm_sq(psi, (hi - lo) / n)) < tol
+    assert abs(o["position"][0] - orc.mean_position_1d(psi, lo, hi)) < tol
+    assert abs(o["sigma"][0] - orc.sigma_x_1d(psi, lo, hi)) < tol
+    assert abs(o["momentum"][0] - orc.mean_momentum_1d(psi, lo, hi)) < tol
+    assert abs(o["energy"] - orc.mean_energy_1d(psi, lo, hi, v)) < (1e-10 if prec == "fp64" else 2e-6)
+
+
+def test_observables_of_relaxed_ground_state(orc, tdse):
+    """Closed forms: the harmonic ground state has <k> = 0, width 1/sqrt(2 w)
+    per axis and energy 3 w / 2 (to the grid and splitting accuracy)."""
+    dims, lo, hi = (64, 64, 64), (-8.0,) * 3, (8.0,) * 3
+    g = grid3(tdse, dims, lo, hi)
+    v = tdse.harmonic_potential(g, 1.0, (0.0, 0.0, 0.0))
+    p = tdse.Propagator(g, v, 0.01, imaginary_time=True)
+    p.upload(tdse.gaussian_wavepacket(g, (0.5, 0.2, 0.0), (1.0, 0.9, 0.8), (0.3, 0, 0)))
+    p.step(2000)
+    o = p.observe(15)
+    assert abs(o["energy"] - 1.5) < 1e-4
+    np.testing.assert_allclose(o["sigma"], [2 ** -0.5] * 3, atol=1e-4)
+    np.testing.assert_allclose(o["momentum"], [0, 0, 0], atol=1e-8)
+    np.testing.assert_allclose(o["position"], [0, 0, 0], atol=1e-8)

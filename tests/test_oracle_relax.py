@@ -152,3 +152,27 @@ def test_relax_restatement_bitwise_equals_reference(orc):
     x2, _ = orc.it3d_relax(v3, d, lo, hi, 0.05, g2, 40, lower=[x])
     assert np.array_equal(x2, R.it3d_relax(v3, d, lo, hi, 0.05, g2, 40, lower=[y]))
     assert orc.inner_product(x, g2, 1.0) == R.inner_product(x, g2, d, lo, hi)
+
+
+def test_width_and_momentum_closed_forms(orc):
+    """A Gaussian packet of density width s and wavenumber k0 has sigma = s
+    and <k> = k0 (resolved grid): pins orc_sigma_* and orc_mean_momentum_*."""
+    p = orc.gaussian_1d(-16.0, 16.0, 512, 1.3, 1.7, 0.6)
+    assert abs(orc.sigma_x_1d(p, -16.0, 16.0) - 1.7) < 1e-9
+    assert abs(orc.mean_momentum_1d(p, -16.0, 16.0) - 0.6) < 1e-9
+    d, lo, hi = (32, 32, 32), (-10.0,) * 3, (10.0,) * 3
+    g = orc.gaussian_3d(d, lo, hi, (0.5, 0.2, 0.0), (1.6, 1.4, 1.2), (0.4, -0.3, 0.2))
+    np.testing.assert_allclose(orc.sigma_position_3d(g, d, lo, hi), (1.6, 1.4, 1.2), atol=1e-6)
+    np.testing.assert_allclose(orc.mean_momentum_3d(g, d, lo, hi), (0.4, -0.3, 0.2), atol=1e-6)
+
+
+@needs_ref
+def test_width_and_momentum_bitwise_equal_reference(orc):
+    R = orc.Ref
+    p = orc.gaussian_1d(-16.0, 16.0, 512, 1.3, 1.7, 0.6)
+    assert orc.sigma_x_1d(p, -16.0, 16.0) == R.sigma_x_1d(p, -16.0, 16.0)
+    assert orc.mean_momentum_1d(p, -16.0, 16.0) == R.mean_momentum_1d(p, -16.0, 16.0)
+    d, lo, hi = (16, 8, 8), (-8.0, -4.0, -4.0), (8.0, 4.0, 4.0)
+    g = orc.gaussian_3d(d, lo, hi, (0.5, 0.2, 0.0), (1.2, 1.1, 1.0), (0.4, -0.3, 0.2))
+    assert orc.sigma_position_3d(g, d, lo, hi) == R.sigma_position_3d(g, d, lo, hi)
+    assert orc.mean_momentum_3d(g, d, lo, hi) == R.mean_momentum_3d(g, d, lo, hi)
