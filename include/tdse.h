@@ -161,6 +161,17 @@ const char* tdse_plan_sweep_name(const tdse_plan* plan, int slot);
  * sweep, kick phase and mask once). */
 int tdse_plan_step_bytes(const tdse_plan* plan, uint64_t* out);
 
+/* Dipole drive V(r, t) = amplitude (axis . r) cos(omega t) (core/src/drive.ixx:
+ * 19-66): with a drive set, tdse_step(plan, n) is driven_step(psi, prop, d,
+ * t0, n) with t0 the plan's clock.  The drive's half-kicks are diagonal like
+ * the potential's, so they merge into the same kick sweeps (at no extra
+ * memory traffic); amplitude 0 removes the drive (the static step, bit for
+ * bit).  The clock starts at 0, advances by dt per step of every tdse_step
+ * call and can be set.  Driven plans do not batch. */
+int tdse_plan_set_drive(tdse_plan* plan, const double axis[3], double amplitude, double omega);
+int tdse_plan_set_time(tdse_plan* plan, double t);
+int tdse_plan_get_time(const tdse_plan* plan, double* out);
+
 /* Imaginary time (plans created with TDSE_FLAG_IMAGINARY_TIME).
  * tdse_step(plan, n) = ImaginaryTimePropagator*::relax(psi, n)
  * (core/src/imaginary_time.ixx:43-53, 112-122).

@@ -296,6 +296,21 @@ def soft_coulomb_3d(dims, lo, hi, z, a, c):
     return v
 
 
+def driven_evolve(v, dims, lo, hi, dt, psi, nsteps, axis, amplitude, omega, t0=0.0, mass=1.0,
+                  any_size=False):
+    """ses::driven_step (core/src/drive.ixx:52-66) with SplitOperator3D tables."""
+    hv, kin = so3d_tables(v, dims, lo, hi, dt, mass)
+    psi = _c128(psi).copy()
+    shape = psi.shape
+    psi = psi.ravel()
+    if orc().orc_driven_step(_p(psi), _ivec(dims), _dvec(lo), _dvec(hi), _p(hv), _p(kin),
+                             _dvec(axis), C.c_double(amplitude), C.c_double(omega),
+                             C.c_double(t0), C.c_double(dt), nsteps,
+                             1 if any_size else 0) != 0:
+        raise ValueError("size must be a power of two")
+    return psi.reshape(shape)
+
+
 def norm_sq(psi, cell_volume):
     psi = _c128(psi)
     return orc().orc_norm_sq(_p(psi), C.c_size_t(psi.size), C.c_double(cell_volume))
@@ -660,3 +675,16 @@ class Ref:
         out = np.empty(3)
         ref().ref_mean_momentum_3d(_p(psi), _ivec(dims), _dvec(lo), _dvec(hi), _p(out))
         return tuple(out)
+
+    @staticmethod
+    def driven_evolve(v, dims, lo, hi, dt, psi, nsteps, axis, amplitude, omega, t0=0.0,
+                      mass=1.0):
+        v = np.ascontiguousarray(v, dtype=np.float64).ravel()
+        psi = _c128(psi).copy()
+        shape = psi.shape
+        psi = psi.ravel()
+        if ref().ref_driven_step(_p(v), _ivec(dims), _dvec(lo), _dvec(hi), C.c_double(dt),
+                                 C.c_double(mass), _p(psi), _dvec(axis), C.c_double(amplitude),
+                                 C.c_double(omega), C.c_double(t0), nsteps) != 0:
+            raise ValueError("size must be a power of two")
+        return psi.reshape(shape)

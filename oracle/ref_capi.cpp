@@ -26,6 +26,7 @@ import ses.wavepacket;
 import ses.observables;
 import ses.parallel;
 import ses.imaginary_time;
+import ses.drive;
 
 namespace {
 
@@ -156,6 +157,25 @@ int ref_so3d_evolve(const double* v, const int d[3], const double lo[3], const d
                 for (std::size_t i = 0; i < n; ++i) f.data()[i] *= mask[i];
             }
         }
+    } catch (const std::invalid_argument&) {
+        return -1;
+    }
+    store(psi, f.data());
+    return 0;
+}
+
+// ses::driven_step with a SplitOperator3D (core/src/drive.ixx:52-66).
+int ref_driven_step(const double* v, const int d[3], const double lo[3], const double hi[3],
+                    double dt, double mass, double* psi, const double axis[3], double amplitude,
+                    double omega, double t0, int nsteps) {
+    const ses::Grid3D g = grid3(d, lo, hi);
+    std::vector<double> pot(v, v + g.size());
+    const ses::SplitOperator3D op{g, pot, dt, mass};
+    ses::Field3D f{g};
+    load(f.data(), psi);
+    try {
+        ses::driven_step(f, op, ses::DipoleDrive{ses::Vec3d{axis[0], axis[1], axis[2]}, amplitude, omega},
+                         t0, nsteps);
     } catch (const std::invalid_argument&) {
         return -1;
     }

@@ -420,6 +420,47 @@ void orc_normalize(double* psi, size_t n, double cell_volume) {
     }
 }
 
+/* ---- dipole drive (core/src/drive.ixx) ---------------------------------- */
+
+/* core/src/drive.ixx:25-40: psi *= exp(-i theta (axis . r)),
+ * theta = amplitude cos(omega t) dt / 2 */
+void orc_dipole_halfkick(double* psi, const int d[3], const double lo[3], const double hi[3],
+                         const double axis[3], double amplitude, double omega, double t,
+                         double dt) {
+    const double theta = amplitude * cos(omega * t) * 0.5 * dt;
+    for (int k = 0; k < d[2]; ++k)
+        for (int j = 0; j < d[1]; ++j)
+            for (int i = 0; i < d[0]; ++i) {
+                const double u = axis[0] * coord(lo[0], hi[0], d[0], i) +
+                                 axis[1] * coord(lo[1], hi[1], d[1], j) +
+                                 axis[2] * coord(lo[2], hi[2], d[2], k);
+                const double ang = -theta * u;
+                const long f = i + (long)d[0] * (j + (long)d[1] * k);
+                double r, im;
+                cmul(psi[2 * f], psi[2 * f + 1], cos(ang), sin(ang), &r, &im);
+                psi[2 * f] = r;
+                psi[2 * f + 1] = im;
+            }
+}
+
+/* core/src/drive.ixx:52-66: each step is halfkick(t), the SplitOperator3D
+ * step with its tables, halfkick(t + dt), t = t0 + s dt.  any_size: the
+ * mixed-radix FFT (no reference counterpart). */
+int orc_driven_step(double* psi, const int d[3], const double lo[3], const double hi[3],
+                    const double* half_v, const double* kinetic, const double axis[3],
+                    double amplitude, double omega, double t0, double dt, int nsteps,
+                    int any_size) {
+    for (int s = 0; s < nsteps; ++s) {
+        const double t = t0 + s * dt;
+        orc_dipole_halfkick(psi, d, lo, hi, axis, amplitude, omega, t, dt);
+        const int rc = any_size ? orc_so3d_step_any(psi, d, half_v, kinetic, 1)
+                                : orc_so3d_step(psi, d, half_v, kinetic, 1);
+        if (rc != 0) return rc;
+        orc_dipole_halfkick(psi, d, lo, hi, axis, amplitude, omega, t + dt, dt);
+    }
+    return 0;
+}
+
 /* ---- imaginary-time relaxation (core/src/imaginary_time.ixx) ---------- */
 
 /* core/src/imaginary_time.ixx:24-40 */

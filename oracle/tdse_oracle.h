@@ -86,6 +86,15 @@ void orc_gaussian_3d(const int dims[3], const double lo[3], const double hi[3],
                      const double r0[3], const double sigma[3], const double k0[3],
                      double* psi);
 
+/* Dipole drive (core/src/drive.ixx). */
+void orc_dipole_halfkick(double* psi, const int dims[3], const double lo[3], const double hi[3],
+                         const double axis[3], double amplitude, double omega, double t,
+                         double dt);
+int orc_driven_step(double* psi, const int dims[3], const double lo[3], const double hi[3],
+                    const double* half_v, const double* kinetic, const double axis[3],
+                    double amplitude, double omega, double t0, double dt, int nsteps,
+                    int any_size);
+
 /* Imaginary-time relaxation (core/src/imaginary_time.ixx). */
 void orc_it1d_tables(const double* v, int n, double xmin, double xmax, double dtau,
                      double* half_w, double* kin_w);

@@ -278,6 +278,25 @@ class Propagator:
         return {L.tdse_plan_sweep_name(self._h, i).decode(): (ms[i], cnt[i])
                 for i in range(ns.value)}
 
+    def set_drive(self, axis, amplitude, omega):
+        """Dipole drive amplitude (axis . r) cos(omega t) on the following
+        steps (core/src/drive.ixx); amplitude 0 removes it."""
+        try:
+            capi.check(capi.lib().tdse_plan_set_drive(self._h, capi.dvec(axis), C.c_double(amplitude),
+                                                      C.c_double(omega)))
+        except TdseError as e:
+            raise _map_error(e) from None
+
+    @property
+    def time(self) -> float:
+        v = C.c_double()
+        capi.check(capi.lib().tdse_plan_get_time(self._h, C.byref(v)))
+        return v.value
+
+    @time.setter
+    def time(self, t):
+        capi.check(capi.lib().tdse_plan_set_time(self._h, C.c_double(t)))
+
     def relax_stats(self) -> dict:
         """Imaginary time: the last step's norm before renormalisation, the
         energy estimate -ln(norm)/(2 dtau) and the peak density."""

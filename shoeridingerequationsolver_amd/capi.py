@@ -46,6 +46,7 @@ EXPORTS = (
     "tdse_batch_create", "tdse_batch_destroy", "tdse_batch_step", "tdse_batch_sync",
     "tdse_batch_stream", "tdse_batch_launches_per_round", "tdse_plan_build",
     "tdse_relax_deflated", "tdse_plan_relax_stats", "tdse_build_soft_coulomb",
+    "tdse_plan_set_drive", "tdse_plan_set_time", "tdse_plan_get_time",
 )
 
 TDSE_POT = {"none": 0, "harmonic": 1, "regularized_coulomb": 2, "coulomb_floor": 3, "barrier": 4,
@@ -153,6 +154,9 @@ def lib() -> C.CDLL:
         L.tdse_relax_deflated.argtypes = [C.c_void_p, C.POINTER(C.c_void_p), C.c_int, C.c_int]
         L.tdse_plan_relax_stats.argtypes = [C.c_void_p, C.POINTER(RelaxStats)]
         L.tdse_build_soft_coulomb.argtypes = [_ip, _dp, _dp, C.c_double, C.c_double, _dp, _dp]
+        L.tdse_plan_set_drive.argtypes = [C.c_void_p, _dp, C.c_double, C.c_double]
+        L.tdse_plan_set_time.argtypes = [C.c_void_p, C.c_double]
+        L.tdse_plan_get_time.argtypes = [C.c_void_p, _dp]
         _LIB = L
     return _LIB
 

@@ -135,6 +135,14 @@ struct SweepArgs {
     double cell;          // volume element dV
     int parity;           // measuring sweeps: read scale[parity], write scale[parity ^ 1]
     double dk;            // observing sweeps: 2 pi / L of the line axis
+    // dipole drive, a time-dependent linear kick: angle += dg * (axis . r)
+    double dg;            // minus the drive angles theta merged into this kick (0: none)
+    double dax[3];        // polarisation axis
+    double dlo[3];        // grid origin per original axis
+    double dh[3];         // grid spacing per original axis
+    int dn1;              // contiguous: lines per z index (ny); four-step: row length n1
+    // C_SINGLE: theta(t) = damp cos(domega t) ddt / 2 with t = dt0 + s ddt
+    double damp, domega, dt0, ddt;
 };
 
 // ---- buffer access -----------------------------------------------------------
@@ -225,12 +233,18 @@ __device__ __forceinline__ double mask_value(T stored) {
 template <typename T>
 constexpr T kMaskOne = sizeof(T) == 8 ? T(1) : T(0);  // stored form of m = 1
 
-// kick factor m * exp(i * f * phi); f = 1 (half kick) or 2 (merged full kick)
+// kick factor m * exp(i * (f * phi + extra)); f = 1 (half kick) or 2 (merged
+// full kick), extra = the drive's linear angle (0 without one)
 template <typename T>
-__device__ __forceinline__ cx<double> kick_factor(T phi, double f, double m) {
+__device__ __forceinline__ cx<double> kick_factor(T phi, double f, double m, double extra = 0.0) {
     double s, c;
-    sincos_small(f * static_cast<double>(phi), &s, &c);
+    sincos_small(f * static_cast<double>(phi) + extra, &s, &c);
     return {c * m, s * m};
+}
+
+// theta(t) = E0 cos(w t) dt / 2 of a dipole drive (core/src/drive.ixx:25-26)
+__device__ __forceinline__ double drive_theta(const SweepArgs& a, double t) {
+    return a.damp * cos(a.domega * t) * 0.5 * a.ddt;
 }
 
 // four-step twiddle exp(DIR * 2 pi i (a*b mod n) / n) in FP64
@@ -322,16 +336,22 @@ struct Line {
             km[s] = m ? M.ld(vo, so) : kMaskOne<T>;
         });
     }
-    // v *= m exp(i f phi), the exponential in FP64.  Waves whose phases are
-    // all zero skip the exponential and multiply by the real m alone,
-    // rounding each component once.
-    __device__ __forceinline__ void apply_kick(double f, bool use_mask) {
+    // v *= m exp(i (f phi + g u)), the exponential in FP64; u = axis . r of
+    // the point (du0 + du1 * position along the line), g the drive angle of
+    // this kick (0: none).  Waves with no angle at all skip the exponential
+    // and multiply by the real m alone, rounding each component once.
+    double du0 = 0.0, du1 = 0.0;
+    __device__ __forceinline__ void apply_kick(double f, bool use_mask, double g = 0.0) {
         if (!valid) return;
-        bool nz = false;
+        const bool drive = g != 0.0;
+        bool nz = drive;
         for (int s = 0; s < E; ++s) nz |= kp[s] != T(0);
         if (__any(nz)) {
-            for (int s = 0; s < E; ++s)
-                v[s] = mul_d(v[s], kick_factor<T>(kp[s], f, use_mask ? mask_value(km[s]) : 1.0));
+            for_each_point<N, E, R1>(t, [&](int s, int, int j, int ro) {
+                const double extra = drive ? g * (du0 + du1 * (j + ro)) : 0.0;
+                v[s] = mul_d(v[s], kick_factor<T>(kp[s], f, use_mask ? mask_value(km[s]) : 1.0,
+                                                  extra));
+            });
         } else if (use_mask && a.mask) {
             if constexpr (sizeof(T) == 4) {
                 // v (1 - c) = v - v c with one rounding per component
@@ -342,9 +362,9 @@ struct Line {
             }
         }
     }
-    __device__ __forceinline__ void kick(double f, bool with_mask) {
+    __device__ __forceinline__ void kick(double f, bool with_mask, double g = 0.0) {
         load_kick(with_mask);
-        apply_kick(f, with_mask);
+        apply_kick(f, with_mask, g);
     }
     // Imaginary time.  kp holds phi = -V dtau / 2 (not reduced); the weight
     // exp(phi) is formed in FP64.
@@ -496,23 +516,34 @@ __device__ __forceinline__ void contig_tile(const SweepArgs& a, long long tile, 
     const Buf<T> M(a.mask, a.mask ? a.real_bytes : 0u);
     const long long line = tile * LPB + l;
     LT L{a, P, F, M, x, tw, t, static_cast<uint32_t>(line * N), 1u, line < a.nlines};
+    if constexpr (MODE == C_KICK_FIRST || MODE == C_KICK_MID || MODE == C_KICK_LAST ||
+                  MODE == C_SINGLE) {
+        if (a.dg != 0.0 || a.damp != 0.0) {
+            // x-line `line` of the grid: y index line % ny, z index line / ny
+            const int yj = a.dn1 > 0 ? static_cast<int>(line % a.dn1) : 0;
+            const int zk = a.dn1 > 0 ? static_cast<int>(line / a.dn1) : 0;
+            L.du0 = a.dax[0] * a.dlo[0] + a.dax[1] * (a.dlo[1] + a.dh[1] * yj) +
+                    a.dax[2] * (a.dlo[2] + a.dh[2] * zk);
+            L.du1 = a.dax[0] * a.dh[0];
+        }
+    }
     if constexpr (MODE == C_KICK_FIRST) {
         L.template load<LT::R1>();
-        L.kick(1.0, false);
+        L.kick(1.0, false, a.dg);
         L.fft();
         L.template store<LT::RL>();
     } else if constexpr (MODE == C_KICK_MID) {
         L.template load<LT::RL>();
         L.ifft();
         L.load_kick(true);
-        L.apply_kick(2.0, true);
+        L.apply_kick(2.0, true, a.dg);
         L.fft();
         L.template store<LT::RL>();
     } else if constexpr (MODE == C_KICK_LAST) {
         L.template load<LT::RL>();
         L.ifft();
         L.load_kick(true);
-        L.apply_kick(1.0, true);
+        L.apply_kick(1.0, true, a.dg);
         L.template store<LT::R1>();
     } else if constexpr (MODE == C_FWD) {
         L.template load<LT::R1>();
@@ -534,16 +565,21 @@ __device__ __forceinline__ void contig_tile(const SweepArgs& a, long long tile, 
         L.template store<LT::R1>();
     } else if constexpr (MODE == C_SINGLE) {
         const cx<double>* kt = static_cast<const cx<double>*>(a.kt[2]);
+        // the drive's merged angle between steps s and s + 1 is twice theta
+        // at their common time (core/src/drive.ixx:52-66)
+        const bool drive = a.damp != 0.0;
         L.template load<LT::R1>();
         L.load_kick(true);  // phases and mask stay in registers for every step
-        L.apply_kick(1.0, false);
+        L.apply_kick(1.0, false, drive ? -drive_theta(a, a.dt0) : 0.0);
         for (int step = 0; step < a.nsteps; ++step) {
             L.fft();
             for_each_point<N, E, LT::RL>(t, [&](int s, int i, int, int) {
                 L.v[s] = mul_d(L.v[s], cscale(kt[i], a.scale));
             });
             L.ifft();
-            L.apply_kick(step + 1 == a.nsteps ? 1.0 : 2.0, true);
+            const bool last = step + 1 == a.nsteps;
+            const double th = drive ? drive_theta(a, a.dt0 + (step + 1) * a.ddt) : 0.0;
+            L.apply_kick(last ? 1.0 : 2.0, true, last ? -th : -2.0 * th);
         }
         L.template store<LT::R1>();
     } else if constexpr (MODE == C_OBS) {
@@ -628,6 +664,13 @@ __device__ __forceinline__ void strided_tile(const SweepArgs& a, long long tile,
     const int xx = static_cast<int>(tile % xtiles) * TX + xi;
     LT L{a, P, F, M, x, tw, t, static_cast<uint32_t>(xx) + a.wstride * w, a.estride,
          xx < a.nx};
+    if constexpr (MODE == S_KICK_FIRST_TW || MODE == S_KICK_MID_TW || MODE == S_KICK_LAST) {
+        if (a.dg != 0.0) {
+            // four-step column xx: point p of the line is x index xx + n1 p
+            L.du0 = a.dax[0] * (a.dlo[0] + a.dh[0] * xx) + a.dax[1] * a.dlo[1] + a.dax[2] * a.dlo[2];
+            L.du1 = a.dax[0] * a.dh[0] * a.dn1;
+        }
+    }
     if constexpr (MODE == S_FWD) {
         L.template load<LT::R1>();
         L.fft();
@@ -649,7 +692,7 @@ __device__ __forceinline__ void strided_tile(const SweepArgs& a, long long tile,
         L.template store<LT::R1>();
     } else if constexpr (MODE == S_KICK_FIRST_TW) {
         L.template load<LT::R1>();
-        L.kick(1.0, false);
+        L.kick(1.0, false, a.dg);
         L.fft();
         L.twiddle_out(xx);
         L.template store<LT::RL>();
@@ -657,7 +700,7 @@ __device__ __forceinline__ void strided_tile(const SweepArgs& a, long long tile,
         L.template load<LT::RL>();
         L.ifft();
         L.load_kick(true);
-        L.apply_kick(2.0, true);
+        L.apply_kick(2.0, true, a.dg);
         L.fft();
         L.twiddle_out(xx);
         L.template store<LT::RL>();
@@ -665,7 +708,7 @@ __device__ __forceinline__ void strided_tile(const SweepArgs& a, long long tile,
         L.template load<LT::RL>();
         L.ifft();
         L.load_kick(true);
-        L.apply_kick(1.0, true);
+        L.apply_kick(1.0, true, a.dg);
         L.template store<LT::R1>();
     } else if constexpr (MODE == S_FWD_TW) {
         L.template load<LT::R1>();

@@ -103,6 +103,11 @@ struct tdse_plan {
     uint64_t bytes = 0;
     unsigned flags = 0;
     int variant[kMaxSlots] = {0, 0, 0, 0, 0, 0};  // kernel variant per schedule slot
+    // dipole drive V = amplitude (axis . r) cos(omega t) (core/src/drive.ixx:19-23)
+    bool drive = false;
+    double drive_axis[3] = {0, 0, 0};
+    double drive_amp = 0.0, drive_omega = 0.0;
+    double time = 0.0;  // plan clock: advanced by dt per step
     // unfused FP64 baseline: full complex128 tables of the reference propagator
     void* base_half = nullptr;  // exp(-i V dt / 2) per cell
     void* base_kin = nullptr;   // exp(-i |k|^2 dt / 2m) per cell
@@ -341,6 +346,13 @@ SweepArgs base_args(const tdse_plan* p) {
     }
     a.cell = 1.0;
     for (int i = 0; i < 3; ++i) a.cell *= (p->hi[i] - p->lo[i]) / p->n[i];
+    for (int i = 0; i < 3; ++i) {
+        a.dax[i] = p->drive_axis[i];
+        a.dlo[i] = p->lo[i];
+        a.dh[i] = (p->hi[i] - p->lo[i]) / p->n[i];
+    }
+    // x lines are ordered y fastest: line = j + ny k (1D four-step: rows of n1)
+    a.dn1 = p->kind == kKindFourStep ? p->n1 : p->n[1];
     return a;
 }
 
@@ -502,16 +514,48 @@ SlotSweep slot_sweep(const tdse_plan* p, const Geometry& g, int slot) {
     return w;
 }
 
+// Schedule slots (see kSlots* below) and which of them kick between steps.
+int slot_count(const tdse_plan* p) {
+    switch (p->kind) {
+        case kKind3D: return 6;
+        case kKindSingle: return 1;
+        default: return 4;
+    }
+}
+bool is_kick_slot(const tdse_plan* p, int slot) {
+    return p->kind == kKind3D ? slot == 4 : (p->kind != kKindSingle && slot == 2);
+}
+
 // The launch list of `nsteps` steps (nsteps >= 1), each sweep in the kernel
 // variant chosen for its slot.  Measuring sweeps (imaginary time) carry the
 // parity of their step.
 std::vector<Launch> schedule(const tdse_plan* p, int nsteps, void* target = nullptr) {
     std::vector<Launch> L;
     const Geometry g = geometry(p, target ? target : p->psi);
+    // dipole drive: theta(t) = E0 cos(w t) dt / 2; a call from t0 kicks by
+    // -theta(t0) first, -2 theta(t0 + (s + 1) dt) between steps s and s + 1
+    // and -theta(t0 + n dt) last (core/src/drive.ixx:52-66)
+    const double t0 = p->time;
+    auto theta = [&](double t) { return p->drive_amp * std::cos(p->drive_omega * t) * 0.5 * p->dt; };
     auto add = [&](int slot, int step) {
         SlotSweep w = slot_sweep(p, g, slot);
         if (w.mode == tdse::C_SINGLE || w.mode == tdse::C_RELAX_SINGLE) w.args.nsteps = nsteps;
         w.args.parity = step & 1;
+        if (p->drive) {
+            const bool first = L.empty(), last = slot == slot_count(p) - 1;
+            if (w.mode == tdse::C_SINGLE) {
+                w.args.damp = p->drive_amp;
+                w.args.domega = p->drive_omega;
+                w.args.dt0 = t0;
+                w.args.ddt = p->dt;
+            } else if (first) {
+                w.args.dg = -theta(t0);
+            } else if (last) {
+                w.args.dg = -theta(t0 + nsteps * p->dt);
+            } else if (is_kick_slot(p, slot)) {
+                w.args.dg = -2.0 * theta(t0 + (step + 1) * p->dt);
+            }
+        }
         L.push_back({w.mode, w.n, w.args, launcher(p, w.n, w.mode, p->variant[slot]), slot});
     };
     if (p->kind == kKind3D) {
@@ -538,15 +582,6 @@ const char* const kSlots3D[] = {"x_first", "y_fwd", "z_kin", "y_inv", "x_mid", "
 const char* const kSlots2D[] = {"x_first", "y_kin", "x_mid", "x_last"};
 const char* const kSlotsSingle[] = {"line_all_steps"};
 const char* const kSlotsFour[] = {"col_first", "row_kin", "col_mid", "col_last"};
-
-int slot_count(const tdse_plan* p) {
-    switch (p->kind) {
-        case kKind3D: return 6;
-        case kKind2D: return 4;
-        case kKindSingle: return 1;
-        default: return 4;
-    }
-}
 
 int run_launches(tdse_plan* p, const std::vector<Launch>& L) {
     for (const Launch& l : L) {
@@ -1165,11 +1200,41 @@ int tdse_step(tdse_plan* p, int nsteps) {
     if ((rc = set_device(p->ctx))) return rc;
     if (p->flags & TDSE_FLAG_UNFUSED_BASELINE) return run_baseline(p, nsteps);
     if ((rc = run_launches(p, schedule(p, nsteps)))) return rc;
+    p->time += nsteps * p->dt;
     if (p->imag() && p->kind != kKindSingle) {
         // the last step's renormalisation, deferred by its measuring sweep
         TDSE_HIP(tdse::launch_scale_dev(p->fp64(), p->psi, p->total, &p->relax()->scale[nsteps & 1],
                                         p->stream));
     }
+    return TDSE_OK;
+}
+
+int tdse_plan_set_drive(tdse_plan* p, const double axis[3], double amplitude, double omega) {
+    int rc = check_plan(p);
+    if (rc) return rc;
+    if (!axis) return fail(TDSE_E_INVALID, "null axis");
+    if (!std::isfinite(amplitude) || !std::isfinite(omega))
+        return fail(TDSE_E_INVALID, "drive amplitude and frequency must be finite");
+    if (p->imag() || (p->flags & TDSE_FLAG_UNFUSED_BASELINE))
+        return fail(TDSE_E_UNSUPPORTED, "the drive applies to fused real-time plans");
+    for (int a = 0; a < 3; ++a) p->drive_axis[a] = axis[a];
+    p->drive_amp = amplitude;
+    p->drive_omega = omega;
+    p->drive = amplitude != 0.0;
+    return TDSE_OK;
+}
+
+int tdse_plan_set_time(tdse_plan* p, double t) {
+    int rc = check_plan(p);
+    if (rc) return rc;
+    if (!std::isfinite(t)) return fail(TDSE_E_INVALID, "time must be finite");
+    p->time = t;
+    return TDSE_OK;
+}
+
+int tdse_plan_get_time(const tdse_plan* p, double* out) {
+    if (!p || !out) return fail(TDSE_E_INVALID, "null argument");
+    *out = p->time;
     return TDSE_OK;
 }
 
@@ -1667,6 +1732,8 @@ int tdse_batch_create(tdse_ctx* ctx, tdse_plan* const* plans, int count, tdse_ba
             return fail(TDSE_E_INVALID, "the unfused baseline does not batch");
         if (plans[i]->imag())
             return fail(TDSE_E_UNSUPPORTED, "imaginary-time plans do not batch");
+        if (plans[i]->drive)
+            return fail(TDSE_E_UNSUPPORTED, "driven plans do not batch");
         for (int j = 0; j < i; ++j)
             if (plans[j] == plans[i]) return fail(TDSE_E_INVALID, "plan listed twice");
     }

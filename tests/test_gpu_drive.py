@@ -1,0 +1,109 @@
+"""The dipole drive fused into the kick sweeps, through the C ABI, against the
+oracle's restatement of ses::driven_step (core/src/drive.ixx:52-66), pinned in
+tests/test_oracle_drive.py.  Tolerances as the parity tests: FP64 1e-11,
+FP32 2e-5 relative to max |psi|."""
+import math
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+TOL = {"fp64": 1e-11, "fp32": 2e-5}
+
+
+def rel(a, b):
+    return float(np.abs(a - b).max() / np.abs(b).max())
+
+
+def grid3(t, n, lo, hi):
+    return t.Grid3D(t.Grid1D(lo[0], hi[0], n[0]), t.Grid1D(lo[1], hi[1], n[1]),
+                    t.Grid1D(lo[2], hi[2], n[2]))
+
+
+CASES = [
+    ((32, 32, 32), (-8, -8, -8), (8, 8, 8), (0.3, 0.2, 1.0), 0.4, 0.9, 12),
+    ((64, 32, 16), (-10, -6, -4), (10, 6, 4), (1.0, -0.5, 0.25), 0.3, 1.3, 10),
+    ((128, 64, 1), (-16, -8, 0), (16, 8, 1), (0.6, 0.8, 0.0), 0.2, 0.5, 15),
+    ((96, 32, 16), (-12, -6, -4), (12, 6, 4), (0.0, 0.0, 1.0), 0.5, 0.0, 8),
+    ((1024, 1, 1), (-40, 0, 0), (40, 1, 1), (1.0, 0.0, 0.0), 0.05, 0.3, 40),
+    ((65536, 1, 1), (-100, 0, 0), (100, 1, 1), (1.0, 0.0, 0.0), 0.02, 0.2, 6),
+]
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("case", CASES, ids=lambda c: "x".join(map(str, c[0])))
+def test_driven_step_matches_oracle(orc, tdse, case, prec):
+    dims, lo, hi, axis, amp, omega, steps = case
+    print(f"case drive dims={dims} prec={prec}", flush=True)
+    v = orc.harmonic_3d(dims, lo, hi, 0.3, (0.2, 0.0, 0.0))
+    psi0 = orc.gaussian_3d(dims, lo, hi, (1.0, 0.0, 0.0), (1.3, 1.1, 0.9),
+                           (0.4, 0.0, 0.0)).reshape(dims[::-1])
+    pow2 = all(d & (d - 1) == 0 for d in dims)
+    t0 = 0.37
+    want = orc.driven_evolve(v, dims, lo, hi, 0.02, psi0, steps, axis, amp, omega, t0=t0,
+                             any_size=not pow2)
+    g = tdse.Grid1D(lo[0], hi[0], dims[0]) if dims[1] == dims[2] == 1 else grid3(tdse, dims, lo, hi)
+    p = tdse.Propagator(g, v, 0.02, precision=prec)
+    p.set_drive(axis, amp, omega)
+    p.time = t0
+    p.upload(psi0.reshape(p.shape))
+    p.step(steps)
+    err = rel(p.download().ravel(), want.ravel())
+    print(f"  rel err {err:.3e}", flush=True)
+    assert err < TOL[prec]
+    assert abs(p.time - (t0 + steps * 0.02)) < 1e-12
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_driven_split_calls_follow_the_clock(orc, tdse, prec):
+    dims, lo, hi = (32, 16, 16), (-8, -4, -4), (8, 4, 4)
+    v = orc.harmonic_3d(dims, lo, hi, 0.5, (0, 0, 0))
+    psi0 = orc.gaussian_3d(dims, lo, hi, (0.5, 0, 0), (1.2, 1.2, 1.2), (0, 0, 0))
+    want = orc.driven_evolve(v, dims, lo, hi, 0.02, psi0.reshape(dims[::-1]), 30,
+                             (1, 0.5, 0), 0.3, 1.1)
+    p = tdse.Propagator(grid3(tdse, dims, lo, hi), v, 0.02, precision=prec)
+    p.set_drive((1, 0.5, 0), 0.3, 1.1)
+    p.upload(psi0)
+    for k in (1, 4, 10, 15):
+        p.step(k)
+    assert rel(p.download(), want) < TOL[prec]
+
+
+def test_zero_amplitude_is_the_static_step_bitwise(orc, tdse):
+    dims, lo, hi = (32, 32, 16), (-8, -8, -4), (8, 8, 4)
+    v = orc.harmonic_3d(dims, lo, hi, 1.0, (0, 0, 0))
+    psi0 = orc.gaussian_3d(dims, lo, hi, (1.0, 0, 0), (1, 1, 1), (0, 0, 0))
+    g = grid3(tdse, dims, lo, hi)
+    a = tdse.Propagator(g, v, 0.02)
+    b = tdse.Propagator(g, v, 0.02)
+    b.set_drive((0, 0, 1), 0.0, 0.5)
+    a.upload(psi0)
+    b.upload(psi0)
+    a.step(10)
+    b.step(10)
+    assert np.array_equal(a.download(), b.download())
+
+
+def test_constant_field_ehrenfest_on_gpu(orc, tdse):
+    """tests/drive_test.cpp::ConstantFieldObeysEhrenfest through the GPU path."""
+    dims, lo, hi = (32, 32, 32), (-16.0,) * 3, (16.0,) * 3
+    g = grid3(tdse, dims, lo, hi)
+    dt, e0, steps = 0.04, 0.05, 50
+    p = tdse.Propagator(g, np.zeros(32 ** 3), dt)
+    p.set_drive((0, 0, 1), e0, 0.0)
+    p.upload(tdse.gaussian_wavepacket(g, (0, 0, 0), (1.5, 1.5, 1.5), (0, 0, 0)))
+    p.step(steps)
+    o = p.observe(15)
+    t = steps * dt
+    assert abs(o["momentum"][2] - (-e0 * t)) < 1e-8
+    assert abs(o["position"][2] - (-0.5 * e0 * t * t)) < 1e-6
+    assert abs(o["norm"] - 1.0) < 1e-12
+
+
+def test_driven_plans_do_not_batch(orc, tdse):
+    dims, lo, hi = (16, 16, 16), (-4.0,) * 3, (4.0,) * 3
+    p = tdse.Propagator(grid3(tdse, dims, lo, hi), orc.harmonic_3d(dims, lo, hi, 1, (0, 0, 0)), 0.02)
+    p.set_drive((1, 0, 0), 0.1, 1.0)
+    with pytest.raises(Exception):
+        tdse.Batch([p])
