@@ -170,6 +170,15 @@ int tdse_plan_step_bytes(const tdse_plan* plan, uint64_t* out);
  * call and can be set.  Driven plans do not batch. */
 int tdse_plan_set_drive(tdse_plan* plan, const double axis[3], double amplitude, double omega);
 int tdse_plan_set_time(tdse_plan* plan, double t);
+/* Uniform magnetic field `bfield` along `axis` (0 = x, 1 = y, 2 = z) in the
+ * symmetric gauge (MagneticPropagator3D, core/src/magnetic.ixx:19-80): the
+ * diamagnetic B^2/8 |r_perp|^2 joins the potential, and each step is
+ * R(a) . Strang step . R(a), a = (B/2)(dt/2), with R the exact three-shear
+ * rotation (core/src/rotation.ixx:73-84), one line-FFT sweep per shear; a
+ * mask follows each whole step.  Call before tdse_plan_set_potential (or
+ * tdse_plan_build); 3D grids, or 2D grids with axis 2.  0 removes the field.
+ * No drive, imaginary time or batching with a field. */
+int tdse_plan_set_magnetic(tdse_plan* plan, double bfield, int axis);
 int tdse_plan_get_time(const tdse_plan* plan, double* out);
 
 /* Imaginary time (plans created with TDSE_FLAG_IMAGINARY_TIME).

@@ -311,6 +311,41 @@ def driven_evolve(v, dims, lo, hi, dt, psi, nsteps, axis, amplitude, omega, t0=0
     return psi.reshape(shape)
 
 
+def rotate_axis(psi, dims, lo, hi, axis, theta):
+    psi = _c128(psi).copy()
+    if orc().orc_rotate_axis(_p(psi.ravel()), _ivec(dims), _dvec(lo), _dvec(hi), axis,
+                             C.c_double(theta)) != 0:
+        raise ValueError("size must be a power of two")
+    return psi
+
+
+def magnetic_veff(v, dims, lo, hi, bfield, axis):
+    v = np.ascontiguousarray(v, dtype=np.float64).ravel()
+    out = np.empty_like(v)
+    orc().orc_magnetic_veff(_p(v), _ivec(dims), _dvec(lo), _dvec(hi), C.c_double(bfield), axis,
+                            _p(out))
+    return out
+
+
+def magnetic_evolve(v, dims, lo, hi, dt, bfield, axis, psi, nsteps, mask=None):
+    """MagneticPropagator3D::step (core/src/magnetic.ixx:34-46); with ``mask``,
+    psi *= mask after each step."""
+    veff = magnetic_veff(v, dims, lo, hi, bfield, axis)
+    hv, kin = so3d_tables(veff, dims, lo, hi, dt, 1.0)
+    psi = _c128(psi).copy()
+    shape = psi.shape
+    psi = psi.ravel()
+    m = None if mask is None else np.ascontiguousarray(mask, dtype=np.float64).ravel()
+    for _ in range(nsteps if m is not None else 1):
+        k = 1 if m is not None else nsteps
+        if orc().orc_magnetic_step(_p(psi), _ivec(dims), _dvec(lo), _dvec(hi), _p(hv), _p(kin),
+                                   C.c_double(dt), C.c_double(bfield), axis, k) != 0:
+            raise ValueError("size must be a power of two")
+        if m is not None:
+            orc().orc_apply_mask(_p(psi), _p(m), C.c_size_t(psi.size))
+    return psi.reshape(shape)
+
+
 def norm_sq(psi, cell_volume):
     psi = _c128(psi)
     return orc().orc_norm_sq(_p(psi), C.c_size_t(psi.size), C.c_double(cell_volume))
@@ -688,3 +723,23 @@ class Ref:
                                  C.c_double(omega), C.c_double(t0), nsteps) != 0:
             raise ValueError("size must be a power of two")
         return psi.reshape(shape)
+
+    @staticmethod
+    def rotate_axis(psi, dims, lo, hi, axis, theta):
+        psi = _c128(psi).copy()
+        if ref().ref_rotate_axis(_p(psi.ravel()), _ivec(dims), _dvec(lo), _dvec(hi), axis,
+                                 C.c_double(theta)) != 0:
+            raise ValueError("size must be a power of two")
+        return psi
+
+    @staticmethod
+    def magnetic_evolve(v, dims, lo, hi, dt, bfield, axis, psi, nsteps):
+        v = np.ascontiguousarray(v, dtype=np.float64).ravel()
+        psi = _c128(psi).copy()
+        shape = psi.shape
+        psi = psi.ravel()
+        veff = np.empty_like(v)
+        if ref().ref_magnetic_evolve(_p(v), _ivec(dims), _dvec(lo), _dvec(hi), C.c_double(dt),
+                                     C.c_double(bfield), axis, _p(psi), nsteps, _p(veff)) != 0:
+            raise ValueError("size must be a power of two")
+        return psi.reshape(shape), veff

@@ -27,6 +27,8 @@ import ses.observables;
 import ses.parallel;
 import ses.imaginary_time;
 import ses.drive;
+import ses.rotation;
+import ses.magnetic;
 
 namespace {
 
@@ -176,6 +178,40 @@ int ref_driven_step(const double* v, const int d[3], const double lo[3], const d
     try {
         ses::driven_step(f, op, ses::DipoleDrive{ses::Vec3d{axis[0], axis[1], axis[2]}, amplitude, omega},
                          t0, nsteps);
+    } catch (const std::invalid_argument&) {
+        return -1;
+    }
+    store(psi, f.data());
+    return 0;
+}
+
+// ses::rotate_axis (core/src/rotation.ixx:73-84).
+int ref_rotate_axis(double* psi, const int d[3], const double lo[3], const double hi[3], int axis,
+                    double theta) {
+    ses::Field3D f{grid3(d, lo, hi)};
+    load(f.data(), psi);
+    try {
+        ses::rotate_axis(f, axis, theta);
+    } catch (const std::invalid_argument&) {
+        return -1;
+    }
+    store(psi, f.data());
+    return 0;
+}
+
+// MagneticPropagator3D::step and effective_potential (core/src/magnetic.ixx:19-80).
+int ref_magnetic_evolve(const double* v, const int d[3], const double lo[3], const double hi[3],
+                        double dt, double bfield, int axis, double* psi, int nsteps,
+                        double* veff_out) {
+    const ses::Grid3D g = grid3(d, lo, hi);
+    std::vector<double> pot(v, v + g.size());
+    const ses::MagneticPropagator3D op{g, pot, dt, bfield, axis};
+    if (veff_out)
+        std::memcpy(veff_out, op.effective_potential().data(), g.size() * sizeof(double));
+    ses::Field3D f{g};
+    load(f.data(), psi);
+    try {
+        op.step(f, nsteps);
     } catch (const std::invalid_argument&) {
         return -1;
     }

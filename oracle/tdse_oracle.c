@@ -461,6 +461,95 @@ int orc_driven_step(double* psi, const int d[3], const double lo[3], const doubl
     return 0;
 }
 
+/* ---- uniform magnetic field (core/src/magnetic.ixx, core/src/rotation.ixx) */
+
+/* core/src/rotation.ixx:25-66: Fourier-shift every line along freq_axis by
+ * coeff * coord(coord_axis of the line) */
+static int axis_shear(double* a, const int d[3], const double lo[3], const double hi[3],
+                      int freq_axis, int coord_axis, double coeff) {
+    const int s[3] = {1, d[0], d[0] * d[1]};
+    const int nf = d[freq_axis], sf = s[freq_axis];
+    if (!is_pow2((size_t)nf)) return -1;
+    int perp[2], pc = 0;
+    for (int q = 0; q < 3; ++q)
+        if (q != freq_axis) perp[pc++] = q;
+    double* kf = (double*)malloc(sizeof(double) * (size_t)nf);
+    orc_wavenumbers(nf, lo[freq_axis], hi[freq_axis], kf);
+    double* line = (double*)malloc(sizeof(double) * 2 * (size_t)nf);
+    for (int v = 0; v < d[perp[1]]; ++v) {
+        for (int u = 0; u < d[perp[0]]; ++u) {
+            const int base = u * s[perp[0]] + v * s[perp[1]];
+            const int cidx = (coord_axis == perp[0]) ? u : v;
+            const double dd = coeff * coord(lo[coord_axis], hi[coord_axis], d[coord_axis], cidx);
+            for (int t = 0; t < nf; ++t) {
+                line[2 * t] = a[2 * (size_t)(base + t * sf)];
+                line[2 * t + 1] = a[2 * (size_t)(base + t * sf) + 1];
+            }
+            orc_fft1d(line, (size_t)nf);
+            for (int t = 0; t < nf; ++t) {
+                const double ph = -kf[t] * dd;
+                double r, im;
+                cmul(line[2 * t], line[2 * t + 1], cos(ph), sin(ph), &r, &im);
+                line[2 * t] = r;
+                line[2 * t + 1] = im;
+            }
+            orc_ifft1d(line, (size_t)nf);
+            for (int t = 0; t < nf; ++t) {
+                a[2 * (size_t)(base + t * sf)] = line[2 * t];
+                a[2 * (size_t)(base + t * sf) + 1] = line[2 * t + 1];
+            }
+        }
+    }
+    free(line);
+    free(kf);
+    return 0;
+}
+
+/* core/src/rotation.ixx:73-84: Paeth three-shear rotation about `axis` */
+int orc_rotate_axis(double* psi, const int d[3], const double lo[3], const double hi[3],
+                    int axis, double theta) {
+    const int b = (axis + 1) % 3, c = (axis + 2) % 3;
+    const double t = tan(0.5 * theta);
+    const double s = sin(theta);
+    if (axis_shear(psi, d, lo, hi, b, c, -t)) return -1;
+    if (axis_shear(psi, d, lo, hi, c, b, s)) return -1;
+    return axis_shear(psi, d, lo, hi, b, c, -t);
+}
+
+/* core/src/magnetic.ixx:50-73: v + B^2/8 |r_perp|^2 */
+void orc_magnetic_veff(const double* v, const int d[3], const double lo[3], const double hi[3],
+                       double bfield, int axis, double* out) {
+    const double c = bfield * bfield / 8.0;
+    for (int k = 0; k < d[2]; ++k)
+        for (int j = 0; j < d[1]; ++j)
+            for (int i = 0; i < d[0]; ++i) {
+                const long f = i + (long)d[0] * (j + (long)d[1] * k);
+                out[f] = v[f];
+                if (c != 0.0) {
+                    const double co[3] = {coord(lo[0], hi[0], d[0], i), coord(lo[1], hi[1], d[1], j),
+                                          coord(lo[2], hi[2], d[2], k)};
+                    double perp2 = 0.0;
+                    for (int a = 0; a < 3; ++a)
+                        if (a != axis) perp2 += co[a] * co[a];
+                    out[f] += c * perp2;
+                }
+            }
+}
+
+/* core/src/magnetic.ixx:34-46: R(a), SplitOperator3D step (tables of the
+ * effective potential), R(a); a = (B/2)(dt/2) */
+int orc_magnetic_step(double* psi, const int d[3], const double lo[3], const double hi[3],
+                      const double* half_v, const double* kinetic, double dt, double bfield,
+                      int axis, int nsteps) {
+    const double half_angle = 0.5 * bfield * (0.5 * dt);
+    for (int s = 0; s < nsteps; ++s) {
+        if (half_angle != 0.0 && orc_rotate_axis(psi, d, lo, hi, axis, half_angle)) return -1;
+        if (orc_so3d_step(psi, d, half_v, kinetic, 1)) return -1;
+        if (half_angle != 0.0 && orc_rotate_axis(psi, d, lo, hi, axis, half_angle)) return -1;
+    }
+    return 0;
+}
+
 /* ---- imaginary-time relaxation (core/src/imaginary_time.ixx) ---------- */
 
 /* core/src/imaginary_time.ixx:24-40 */

@@ -95,6 +95,15 @@ int orc_driven_step(double* psi, const int dims[3], const double lo[3], const do
                     double amplitude, double omega, double t0, double dt, int nsteps,
                     int any_size);
 
+/* Uniform magnetic field (core/src/magnetic.ixx, core/src/rotation.ixx). */
+int orc_rotate_axis(double* psi, const int dims[3], const double lo[3], const double hi[3],
+                    int axis, double theta);
+void orc_magnetic_veff(const double* v, const int dims[3], const double lo[3],
+                       const double hi[3], double bfield, int axis, double* out);
+int orc_magnetic_step(double* psi, const int dims[3], const double lo[3], const double hi[3],
+                      const double* half_v, const double* kinetic, double dt, double bfield,
+                      int axis, int nsteps);
+
 /* Imaginary-time relaxation (core/src/imaginary_time.ixx). */
 void orc_it1d_tables(const double* v, int n, double xmin, double xmax, double dtau,
                      double* half_w, double* kin_w);

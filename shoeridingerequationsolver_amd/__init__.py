@@ -37,6 +37,7 @@ __all__ = [
     "regularized_coulomb_potential", "coulomb_floor_potential", "absorbing_mask",
     "gaussian_wavepacket", "TDSE_FP32", "TDSE_FP64", "TdseError", "Batch",
     "ImaginaryTimePropagator1D", "ImaginaryTimePropagator3D", "soft_coulomb_potential",
+    "MagneticPropagator3D",
 ]
 
 PRECISIONS = {"fp32": TDSE_FP32, "fp64": TDSE_FP64, TDSE_FP32: TDSE_FP32, TDSE_FP64: TDSE_FP64}
@@ -162,7 +163,7 @@ class Propagator:
 
     def __init__(self, grid, potential, dt, mass=1.0, precision="fp64", mask=None,
                  context: Context | None = None, unfused_baseline: bool = False,
-                 imaginary_time: bool = False):
+                 imaginary_time: bool = False, bfield: float = 0.0, b_axis: int = 2):
         self.grid = grid
         self.ctx = context or default_context()
         self.precision = PRECISIONS[precision]
@@ -184,6 +185,11 @@ class Propagator:
             raise _map_error(e) from None
         self._h = h
         self._has_mask = False
+        if bfield != 0.0:
+            try:
+                capi.check(L.tdse_plan_set_magnetic(self._h, C.c_double(bfield), int(b_axis)))
+            except TdseError as e:
+                raise _map_error(e) from None
         if potential is not None:
             self.set_potential(potential)
         if mask is not None:
@@ -469,6 +475,45 @@ class SplitOperator3D(_SplitOperatorBase):
 
     def __init__(self, grid: Grid3D, potential, dt, mass=1.0, precision="fp64", context=None):
         super().__init__(grid, potential, dt, mass, precision, context)
+
+
+class MagneticPropagator3D:
+    """``ses::MagneticPropagator3D(grid, V, dt, B, axis)`` (core/src/magnetic.ixx:19-80):
+    uniform field along ``axis`` in the symmetric gauge; each step is
+    R(a) . Strang(V + B^2/8 r_perp^2) . R(a) with an exact three-shear rotation."""
+
+    def __init__(self, grid: Grid3D, potential, dt, bfield, axis=2, precision="fp64",
+                 context=None):
+        self._grid = grid
+        self._dt = float(dt)
+        self._v = np.ascontiguousarray(potential, dtype=np.float64).ravel()
+        self._b = float(bfield)
+        self._axis = int(axis)
+        self._prop = Propagator(grid, self._v, dt, precision=precision, context=context,
+                                bfield=bfield, b_axis=axis)
+
+    def dt(self) -> float:
+        return self._dt
+
+    def effective_potential(self) -> np.ndarray:
+        """V + B^2/8 |r_perp|^2 (host copy of what the plan holds)."""
+        if self._b == 0.0:
+            return self._v.copy()
+        axes = [self._grid.x, self._grid.y, self._grid.z]
+        co = [a.xmin + np.arange(a.n) * a.spacing() for a in axes]
+        z, y, x = np.meshgrid(co[2], co[1], co[0], indexing="ij")
+        c = (x, y, z)
+        perp2 = sum(c[a] ** 2 for a in range(3) if a != self._axis)
+        return self._v + (self._b * self._b / 8.0 * perp2).ravel()
+
+    def step(self, psi: np.ndarray, nsteps: int = 1) -> None:
+        if psi.dtype != np.complex128 or not psi.flags.c_contiguous or psi.size != self._prop.size:
+            raise ValueError("psi must be a contiguous complex128 array of the grid's size")
+        if nsteps <= 0:
+            return
+        self._prop.upload(psi)
+        self._prop.step(nsteps)
+        self._prop.download(psi.reshape(self._prop.shape))
 
 
 # ---- imaginary time (core/src/imaginary_time.ixx) ----------------------------

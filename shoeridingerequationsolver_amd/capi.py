@@ -46,7 +46,7 @@ EXPORTS = (
     "tdse_batch_create", "tdse_batch_destroy", "tdse_batch_step", "tdse_batch_sync",
     "tdse_batch_stream", "tdse_batch_launches_per_round", "tdse_plan_build",
     "tdse_relax_deflated", "tdse_plan_relax_stats", "tdse_build_soft_coulomb",
-    "tdse_plan_set_drive", "tdse_plan_set_time", "tdse_plan_get_time",
+    "tdse_plan_set_drive", "tdse_plan_set_time", "tdse_plan_get_time", "tdse_plan_set_magnetic",
 )
 
 TDSE_POT = {"none": 0, "harmonic": 1, "regularized_coulomb": 2, "coulomb_floor": 3, "barrier": 4,
@@ -157,6 +157,7 @@ def lib() -> C.CDLL:
         L.tdse_plan_set_drive.argtypes = [C.c_void_p, _dp, C.c_double, C.c_double]
         L.tdse_plan_set_time.argtypes = [C.c_void_p, C.c_double]
         L.tdse_plan_get_time.argtypes = [C.c_void_p, _dp]
+        L.tdse_plan_set_magnetic.argtypes = [C.c_void_p, C.c_double, C.c_int]
         _LIB = L
     return _LIB
 

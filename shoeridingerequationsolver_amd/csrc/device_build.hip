@@ -79,6 +79,13 @@ build_cells(BuildArgs a) {
             default:
                 pot = 0.0;
         }
+        if (a.bfield != 0.0) {
+            const double c[3] = {x, y, z};
+            double perp2 = 0.0;
+            for (int q = 0; q < 3; ++q)
+                if (q != a.b_axis) perp2 += c[q] * c[q];
+            pot += a.bfield * a.bfield / 8.0 * perp2;
+        }
         v[f] = static_cast<T>(pot);
         phi[f] = static_cast<T>(a.imaginary ? -0.5 * pot * a.dt : reduce_2pi_dev(-0.5 * pot * a.dt));
         if (mask) {

@@ -17,6 +17,8 @@ struct BuildArgs {
     double lo[3], hi[3], h[3];
     double dt;
     int imaginary;  // 1: phi = -V dt / 2 unreduced (imaginary time weights)
+    double bfield;  // uniform field: V += B^2/8 |r_perp|^2 (0: none)
+    int b_axis;
     tdse_recipe r;
 };
 

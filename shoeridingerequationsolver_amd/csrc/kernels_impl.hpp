@@ -91,6 +91,8 @@ LaunchFn pick_mode(int mode) {
         case S_RELAX_LAST: return &launch_sweep<T, N, S_RELAX_LAST>;
         case C_OBS: return &launch_sweep<T, N, C_OBS>;
         case S_OBS: return &launch_sweep<T, N, S_OBS>;
+        case C_SHEAR: return &launch_sweep<T, N, C_SHEAR>;
+        case S_SHEAR: return &launch_sweep<T, N, S_SHEAR>;
         default: return nullptr;
     }
 }

@@ -69,12 +69,15 @@ enum SweepMode : int {
     // observables: read-only line spectra (no store)
     C_OBS = 20,             // FFT, accumulate |psi_k|^2 (1, k, k^2)
     S_OBS = 21,             // FFT, accumulate |psi_k|^2 (1, k, k^2)
-    kNumModes = 22
+    // shears of a three-shear rotation: FFT, exp(-i k d(line)) / N, IFFT (, mask)
+    C_SHEAR = 22,
+    S_SHEAR = 23,
+    kNumModes = 24
 };
 
 constexpr bool mode_contiguous(int m) {
     return m <= C_SINGLE || m == C_FWD || (m >= C_RELAX_FIRST && m <= C_RELAX_SINGLE) ||
-           m == C_OBS;
+           m == C_OBS || m == C_SHEAR;
 }
 constexpr bool mode_observes(int m) { return m == C_OBS || m == S_OBS; }
 // modes that measure the norm across blocks (deferred renormalisation)
@@ -143,6 +146,11 @@ struct SweepArgs {
     int dn1;              // contiguous: lines per z index (ny); four-step: row length n1
     // C_SINGLE: theta(t) = damp cos(domega t) ddt / 2 with t = dt0 + s ddt
     double damp, domega, dt0, ddt;
+    // shear sweeps: the line's shift d = sh_d0 + sh_d1 * idx, idx its index
+    // along the coordinate axis (sh_sel 0: y of an x line / x of a strided
+    // line; 1: z of an x line / the outer index w of a strided line)
+    double sh_d0, sh_d1;
+    int sh_sel;
 };
 
 // ---- buffer access -----------------------------------------------------------
@@ -408,6 +416,28 @@ struct Line {
             acc.q2 += k * k * w;
         });
     }
+    // Shear: the transformed line (pattern RL, bin i) times exp(-i k_i d) / N
+    // (core/src/rotation.ixx:25-66), in FP64.
+    __device__ __forceinline__ void shear_phase(double d) {
+        for_each_point<N, E, RL>(t, [&](int s, int i, int, int) {
+            const double k = a.dk * ((2 * i < N) ? i : i - N);
+            double sn, cs;
+            sincos_small(-k * d, &sn, &cs);
+            v[s] = mul_d(v[s], cx<double>{cs * a.scale, sn * a.scale});
+        });
+    }
+    // v *= m (absorbing mask only), pattern R1
+    __device__ __forceinline__ void mask_only() {
+        if (!valid || !a.mask) return;
+        for_each_point<N, E, R1>(t, [&](int s, int, int j, int ro) {
+            const uint32_t vo = (base + j * es) * RS, so = ro * es * RS;
+            const T c = M.ld(vo, so);
+            if constexpr (sizeof(T) == 4)
+                v[s] = {__builtin_fmaf(-v[s].x, c, v[s].x), __builtin_fmaf(-v[s].y, c, v[s].y)};
+            else
+                v[s] = cscale(v[s], c);
+        });
+    }
     __device__ __forceinline__ void fft() { Transform<T, N, E, -1, 1, Fwd>::run(v, t, tw, x); }
     __device__ __forceinline__ void ifft() { Transform<T, N, E, +1, 1, Inv>::run(v, t, tw, x); }
     // four-step twiddle after the column FFT: exp(-2 pi i n1 k2 / N), k2 = position
@@ -586,6 +616,14 @@ __device__ __forceinline__ void contig_tile(const SweepArgs& a, long long tile, 
         L.template load<LT::R1>();
         L.fft();
         L.observe_spectrum(ra);
+    } else if constexpr (MODE == C_SHEAR) {
+        const long long idx = a.sh_sel ? line / a.dn1 : line % a.dn1;
+        L.template load<LT::R1>();
+        L.fft();
+        L.shear_phase(a.sh_d0 + a.sh_d1 * static_cast<double>(idx));
+        L.ifft();
+        L.mask_only();
+        L.template store<LT::R1>();
     } else if constexpr (MODE == C_RELAX_FIRST) {
         L.template load<LT::R1>();
         L.relax_first();
@@ -719,6 +757,14 @@ __device__ __forceinline__ void strided_tile(const SweepArgs& a, long long tile,
         L.template load<LT::R1>();
         L.fft();
         L.observe_spectrum(ra);
+    } else if constexpr (MODE == S_SHEAR) {
+        const int idx = a.sh_sel ? w : xx;
+        L.template load<LT::R1>();
+        L.fft();
+        L.shear_phase(a.sh_d0 + a.sh_d1 * static_cast<double>(idx));
+        L.ifft();
+        L.mask_only();
+        L.template store<LT::R1>();
     } else if constexpr (MODE == S_RELAX_FIRST_TW) {
         L.template load<LT::R1>();
         L.relax_first();

@@ -108,6 +108,9 @@ struct tdse_plan {
     double drive_axis[3] = {0, 0, 0};
     double drive_amp = 0.0, drive_omega = 0.0;
     double time = 0.0;  // plan clock: advanced by dt per step
+    // uniform magnetic field along axis b_axis (core/src/magnetic.ixx:19-80)
+    double bfield = 0.0;
+    int b_axis = 2;
     // unfused FP64 baseline: full complex128 tables of the reference propagator
     void* base_half = nullptr;  // exp(-i V dt / 2) per cell
     void* base_kin = nullptr;   // exp(-i |k|^2 dt / 2m) per cell
@@ -621,6 +624,58 @@ int run_baseline(tdse_plan* p, int nsteps) {
     return TDSE_OK;
 }
 
+// core/src/magnetic.ixx:34-46: each step is R(a), the static Strang step
+// (potential + diamagnetic term), R(a), with a = (B/2)(dt/2); R is the
+// three-shear rotation of core/src/rotation.ixx:73-84.  The caller's mask
+// follows each whole step, so it rides on the step's last shear.
+int run_magnetic(tdse_plan* p, int nsteps) {
+    const Geometry g = geometry(p, p->psi);
+    const double half_angle = 0.5 * p->bfield * (0.5 * p->dt);
+    const int b = (p->b_axis + 1) % 3, c = (p->b_axis + 2) % 3;
+    // Shear of the lines along `fa` by coeff * coord along `ca`.
+    auto shear = [&](int fa, int ca, double coeff, bool masked) -> Launch {
+        SweepArgs a;
+        int mode;
+        if (fa == 0) {
+            a = g.x;
+            mode = tdse::C_SHEAR;
+            a.sh_sel = ca == 2 ? 1 : 0;  // x lines: y = line % ny, z = line / ny
+        } else if (fa == 1) {
+            a = g.yf;
+            mode = tdse::S_SHEAR;
+            a.sh_sel = ca == 2 ? 1 : 0;  // y lines: x = xx, z = w
+        } else {
+            a = g.z;
+            mode = tdse::S_SHEAR;
+            a.sh_sel = ca == 1 ? 1 : 0;  // z lines: x = xx, y = w
+        }
+        const double h = (p->hi[ca] - p->lo[ca]) / p->n[ca];
+        a.sh_d0 = coeff * p->lo[ca];
+        a.sh_d1 = coeff * h;
+        a.dk = 2.0 * 3.14159265358979323846 / (p->hi[fa] - p->lo[fa]);
+        a.scale = 1.0 / static_cast<double>(p->n[fa]);
+        a.kt[0] = a.kt[1] = a.kt[2] = nullptr;
+        a.mask = masked ? p->mask : nullptr;
+        const int n = p->n[fa];
+        return {mode, n, a, launcher(p, n, mode), 0};
+    };
+    const double t = std::tan(0.5 * half_angle), sn = std::sin(half_angle);
+    std::vector<Launch> L;
+    auto rotate = [&](bool mask_last) {
+        L.push_back(shear(b, c, -t, false));
+        L.push_back(shear(c, b, sn, false));
+        L.push_back(shear(b, c, -t, mask_last));
+    };
+    std::vector<Launch> core = schedule(p, 1);
+    core.back().args.mask = nullptr;  // the mask follows the closing rotation
+    for (int s = 0; s < nsteps; ++s) {
+        rotate(false);
+        for (const Launch& l : core) L.push_back(l);
+        rotate(p->mask != nullptr);
+    }
+    return run_launches(p, L);
+}
+
 int check_plan(const tdse_plan* p) {
     if (!p) return fail(TDSE_E_INVALID, "null plan");
     return TDSE_OK;
@@ -1071,6 +1126,26 @@ int tdse_plan_set_potential(tdse_plan* p, const double* v) {
     if (!v) return fail(TDSE_E_INVALID, "null potential");
     if ((rc = set_device(p->ctx))) return rc;
     TDSE_HIP(hipStreamSynchronize(p->stream));
+    std::vector<double> veff;
+    if (p->bfield != 0.0) {
+        // the diamagnetic term joins the potential (core/src/magnetic.ixx:50-73)
+        veff.assign(v, v + p->total);
+        const double c = p->bfield * p->bfield / 8.0;
+        for (int k = 0; k < p->n[2]; ++k)
+            for (int j = 0; j < p->n[1]; ++j)
+                for (int i = 0; i < p->n[0]; ++i) {
+                    const int idx[3] = {i, j, k};
+                    double perp2 = 0.0;
+                    for (int a = 0; a < 3; ++a) {
+                        if (a == p->b_axis) continue;
+                        const double x = p->lo[a] + idx[a] * ((p->hi[a] - p->lo[a]) / p->n[a]);
+                        perp2 += x * x;
+                    }
+                    veff[i + static_cast<size_t>(p->n[0]) * (j + static_cast<size_t>(p->n[1]) * k)] +=
+                        c * perp2;
+                }
+        v = veff.data();
+    }
     std::vector<double> th(static_cast<size_t>(p->total));
     for (long long i = 0; i < p->total; ++i) {
         if (!std::isfinite(v[i])) return fail(TDSE_E_INVALID, "potential not finite");
@@ -1138,6 +1213,8 @@ int tdse_plan_build(tdse_plan* p, const tdse_recipe* r) {
     }
     b.dt = p->dt;
     b.imaginary = p->imag() ? 1 : 0;
+    b.bfield = p->bfield;
+    b.b_axis = p->b_axis;
     b.r = *r;
     TDSE_HIP(tdse::launch_build(p->fp64(), b, p->stream));
     p->has_potential = true;
@@ -1199,6 +1276,11 @@ int tdse_step(tdse_plan* p, int nsteps) {
     if (!p->has_potential) return fail(TDSE_E_STATE, "tdse_plan_set_potential not called");
     if ((rc = set_device(p->ctx))) return rc;
     if (p->flags & TDSE_FLAG_UNFUSED_BASELINE) return run_baseline(p, nsteps);
+    if (p->bfield != 0.0) {
+        if ((rc = run_magnetic(p, nsteps))) return rc;
+        p->time += nsteps * p->dt;
+        return TDSE_OK;
+    }
     if ((rc = run_launches(p, schedule(p, nsteps)))) return rc;
     p->time += nsteps * p->dt;
     if (p->imag() && p->kind != kKindSingle) {
@@ -1215,12 +1297,29 @@ int tdse_plan_set_drive(tdse_plan* p, const double axis[3], double amplitude, do
     if (!axis) return fail(TDSE_E_INVALID, "null axis");
     if (!std::isfinite(amplitude) || !std::isfinite(omega))
         return fail(TDSE_E_INVALID, "drive amplitude and frequency must be finite");
-    if (p->imag() || (p->flags & TDSE_FLAG_UNFUSED_BASELINE))
-        return fail(TDSE_E_UNSUPPORTED, "the drive applies to fused real-time plans");
+    if (p->imag() || (p->flags & TDSE_FLAG_UNFUSED_BASELINE) || p->bfield != 0.0)
+        return fail(TDSE_E_UNSUPPORTED, "the drive applies to fused real-time plans without a field");
     for (int a = 0; a < 3; ++a) p->drive_axis[a] = axis[a];
     p->drive_amp = amplitude;
     p->drive_omega = omega;
     p->drive = amplitude != 0.0;
+    return TDSE_OK;
+}
+
+int tdse_plan_set_magnetic(tdse_plan* p, double bfield, int axis) {
+    int rc = check_plan(p);
+    if (rc) return rc;
+    if (!std::isfinite(bfield) || axis < 0 || axis > 2)
+        return fail(TDSE_E_INVALID, "field must be finite, axis 0, 1 or 2");
+    if (p->has_potential) return fail(TDSE_E_STATE, "set the field before the potential");
+    if (bfield != 0.0) {
+        if (p->imag() || (p->flags & TDSE_FLAG_UNFUSED_BASELINE) || p->drive)
+            return fail(TDSE_E_UNSUPPORTED, "the field applies to plain real-time plans");
+        if (p->kind != kKind3D && !(p->kind == kKind2D && axis == 2 && p->n[2] == 1))
+            return fail(TDSE_E_UNSUPPORTED, "the field needs a 3D grid (or a 2D one rotated about z)");
+    }
+    p->bfield = bfield;
+    p->b_axis = axis;
     return TDSE_OK;
 }
 
@@ -1732,8 +1831,8 @@ int tdse_batch_create(tdse_ctx* ctx, tdse_plan* const* plans, int count, tdse_ba
             return fail(TDSE_E_INVALID, "the unfused baseline does not batch");
         if (plans[i]->imag())
             return fail(TDSE_E_UNSUPPORTED, "imaginary-time plans do not batch");
-        if (plans[i]->drive)
-            return fail(TDSE_E_UNSUPPORTED, "driven plans do not batch");
+        if (plans[i]->drive || plans[i]->bfield != 0.0)
+            return fail(TDSE_E_UNSUPPORTED, "driven and magnetic plans do not batch");
         for (int j = 0; j < i; ++j)
             if (plans[j] == plans[i]) return fail(TDSE_E_INVALID, "plan listed twice");
     }
