@@ -68,6 +68,14 @@ extern "C" {
  * zero field, core/src/field.ixx:78-87).  No mask, no batching. */
 #define TDSE_FLAG_UNFUSED_BASELINE 1u
 #define TDSE_FLAG_IMAGINARY_TIME 2u
+/* TDSE_FLAG_LATTICE: the 2D finite-difference propagator with Peierls link
+ * phases (PeierlsLattice2D, core/src/lattice2d.ixx:31-275) on an n x m x 1 grid
+ * (any n <= 2048, m; open boundaries, hopping 1/(2 h^2) per axis, onsite
+ * V + 2 tx + 2 ty).  tdse_step = PeierlsLattice2D::step; with
+ * TDSE_FLAG_IMAGINARY_TIME, tdse_step = relax (renormalised each step).  The
+ * energy observable is the lattice <H>; no momentum, drive, field of
+ * tdse_plan_set_magnetic, batching or kernel variants. */
+#define TDSE_FLAG_LATTICE 4u
 
 typedef struct tdse_ctx tdse_ctx;
 typedef struct tdse_plan tdse_plan;
@@ -91,7 +99,8 @@ int tdse_plan_bytes(const int n[3], int precision, int with_mask, uint64_t* out)
 /* A propagator for one grid: replaces the SplitOperator1D/3D constructor
  * (core/src/propagator.ixx:23, 72).  lo/hi are the periodic box bounds per
  * axis (Grid1D::xmin/xmax), mass the effective mass (default 1.0 in the
- * reference), flags 0, TDSE_FLAG_UNFUSED_BASELINE or TDSE_FLAG_IMAGINARY_TIME. */
+ * reference), flags 0, TDSE_FLAG_UNFUSED_BASELINE, TDSE_FLAG_IMAGINARY_TIME and/or
+ * TDSE_FLAG_LATTICE. */
 int tdse_plan_create(tdse_ctx* ctx, const int n[3], const double lo[3], const double hi[3],
                      double dt, double mass, int precision, unsigned flags, tdse_plan** out);
 int tdse_plan_destroy(tdse_plan* plan);
@@ -179,6 +188,15 @@ int tdse_plan_set_time(tdse_plan* plan, double t);
  * tdse_plan_build); 3D grids, or 2D grids with axis 2.  0 removes the field.
  * No drive, imaginary time or batching with a field. */
 int tdse_plan_set_magnetic(tdse_plan* plan, double bfield, int axis);
+
+/* Lattice links (TDSE_FLAG_LATTICE plans).  A uniform field B along z in the
+ * Landau gauge: x links of row j carry exp(-i B hx y_j)
+ * (core/src/lattice2d.ixx:93-105).  A solenoid of flux phi in the cell
+ * holding (xs, ys), string gauge: the x links of that column above it carry
+ * exp(-i phi) (cut_up) or those below and at it exp(+i phi)
+ * (core/src/lattice2d.ixx:60-91).  Each call replaces the links. */
+int tdse_lattice_set_uniform_field(tdse_plan* plan, double b);
+int tdse_lattice_set_solenoid(tdse_plan* plan, double phi, double xs, double ys, int cut_up);
 int tdse_plan_get_time(const tdse_plan* plan, double* out);
 
 /* Imaginary time (plans created with TDSE_FLAG_IMAGINARY_TIME).

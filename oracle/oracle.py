@@ -58,7 +58,7 @@ def orc() -> C.CDLL:
         for name in ("orc_norm_sq", "orc_mean_position_1d", "orc_mean_energy_1d",
                      "orc_probability_in_range_1d", "orc_mean_energy_3d",
                      "orc_probability_in_slab_3d", "orc_spacing", "orc_sigma_x_1d",
-                     "orc_mean_momentum_1d"):
+                     "orc_mean_momentum_1d", "orc_lat2d_energy"):
             getattr(lib, name).restype = C.c_double
         _ORC = lib
     return _ORC
@@ -344,6 +344,46 @@ def magnetic_evolve(v, dims, lo, hi, dt, bfield, axis, psi, nsteps, mask=None):
         if m is not None:
             orc().orc_apply_mask(_p(psi), _p(m), C.c_size_t(psi.size))
     return psi.reshape(shape)
+
+
+def lat2d_links(dims, lo, hi, links=0, a=0.0, xs=0.0, ys=0.0, cut_up=True):
+    """x-link phases of PeierlsLattice2D (1 everywhere, a uniform field or a
+    solenoid), interleaved complex, row-major (j, i)."""
+    nx, ny = dims[0], dims[1]
+    lx = np.empty(nx * ny, np.complex128)
+    if links == 1:
+        orc().orc_lat2d_uniform_field(_p(lx), nx, ny, C.c_double(lo[0]), C.c_double(hi[0]),
+                                      C.c_double(lo[1]), C.c_double(hi[1]), C.c_double(a))
+    else:
+        orc().orc_lat2d_solenoid(_p(lx), nx, ny, C.c_double(lo[0]), C.c_double(hi[0]),
+                                 C.c_double(lo[1]), C.c_double(hi[1]),
+                                 C.c_double(a if links == 2 else 0.0),
+                                 C.c_double(xs if links == 2 else 1e300),
+                                 C.c_double(ys), 1 if cut_up else 0)
+    return lx
+
+
+def lat2d_evolve(v, dims, lo, hi, dt, psi, nsteps, relax=False, links=0, a=0.0, xs=0.0, ys=0.0,
+                 cut_up=True):
+    """PeierlsLattice2D::step / relax; returns (psi, energy of the result)."""
+    nx, ny = dims[0], dims[1]
+    v = np.ascontiguousarray(v, dtype=np.float64).ravel()
+    hx = (hi[0] - lo[0]) / nx
+    hy = (hi[1] - lo[1]) / ny
+    consts = np.empty(14)
+    hv = np.empty(nx * ny, np.complex128)
+    rhv = np.empty(nx * ny)
+    orc().orc_lat2d_tables(_p(v), nx, ny, C.c_double(hx), C.c_double(hy), C.c_double(dt),
+                           _p(consts), _p(hv), _p(rhv))
+    lx = lat2d_links(dims, lo, hi, links, a, xs, ys, cut_up)
+    psi = _c128(psi).copy()
+    shape = psi.shape
+    psi = psi.ravel()
+    dv = hx * hy * (hi[2] - lo[2]) / dims[2]
+    orc().orc_lat2d_evolve(_p(psi), nx, ny, _p(consts), _p(hv), _p(rhv), _p(lx),
+                           1 if relax else 0, C.c_double(dv), nsteps)
+    e = orc().orc_lat2d_energy(_p(psi), nx, ny, _p(v), _p(consts), _p(lx))
+    return psi.reshape(shape), e
 
 
 def norm_sq(psi, cell_volume):
@@ -743,3 +783,17 @@ class Ref:
                                      C.c_double(bfield), axis, _p(psi), nsteps, _p(veff)) != 0:
             raise ValueError("size must be a power of two")
         return psi.reshape(shape), veff
+
+    @staticmethod
+    def lat2d_evolve(v, dims, lo, hi, dt, psi, nsteps, relax=False, links=0, a=0.0, xs=0.0,
+                     ys=0.0, cut_up=True):
+        v = np.ascontiguousarray(v, dtype=np.float64).ravel()
+        psi = _c128(psi).copy()
+        shape = psi.shape
+        psi = psi.ravel()
+        lx = np.empty(dims[0] * dims[1], np.complex128)
+        e = C.c_double()
+        ref().ref_lat2d_evolve(_p(v), _ivec(dims), _dvec(lo), _dvec(hi), C.c_double(dt), _p(psi),
+                               nsteps, 1 if relax else 0, links, C.c_double(a), C.c_double(xs),
+                               C.c_double(ys), 1 if cut_up else 0, _p(lx), C.byref(e))
+        return psi.reshape(shape), e.value, lx

@@ -29,6 +29,7 @@ import ses.imaginary_time;
 import ses.drive;
 import ses.rotation;
 import ses.magnetic;
+import ses.lattice2d;
 
 namespace {
 
@@ -216,6 +217,35 @@ int ref_magnetic_evolve(const double* v, const int d[3], const double lo[3], con
         return -1;
     }
     store(psi, f.data());
+    return 0;
+}
+
+// PeierlsLattice2D (core/src/lattice2d.ixx:31-275).  links: 0 none,
+// 1 uniform field a, 2 solenoid (flux a at (xs, ys), cut_up).  Writes the
+// x-links (link_x(i, j), interleaved) when links_out is given and returns
+// the energy of the final state in *energy when given.
+int ref_lat2d_evolve(const double* v, const int d[3], const double lo[3], const double hi[3],
+                     double dt, double* psi, int nsteps, int relax, int links, double a, double xs,
+                     double ys, int cut_up, double* links_out, double* energy) {
+    const ses::Grid3D g = grid3(d, lo, hi);
+    std::vector<double> pot(v, v + g.size());
+    ses::PeierlsLattice2D prop{g, pot, dt};
+    if (links == 1) prop.set_uniform_field(a);
+    if (links == 2) prop.set_solenoid(a, xs, ys, cut_up != 0);
+    ses::Field3D f{g};
+    load(f.data(), psi);
+    if (relax) prop.relax(f, nsteps);
+    else prop.step(f, nsteps);
+    store(psi, f.data());
+    if (links_out) {
+        for (int j = 0; j < d[1]; ++j)
+            for (int i = 0; i < d[0]; ++i) {
+                const Cd u = prop.link_x(i, j);
+                links_out[2 * (static_cast<std::size_t>(j) * d[0] + i)] = u.real();
+                links_out[2 * (static_cast<std::size_t>(j) * d[0] + i) + 1] = u.imag();
+            }
+    }
+    if (energy) *energy = prop.energy(f);
     return 0;
 }
 

@@ -550,6 +550,213 @@ int orc_magnetic_step(double* psi, const int d[3], const double lo[3], const dou
     return 0;
 }
 
+/* ---- Peierls lattice (core/src/lattice2d.ixx) ------------------------- */
+
+/* Complex helpers with the reference's -fcx-limited-range products. */
+static inline void cm(const double* a, const double* b, double* out) {
+    const double r = a[0] * b[0] - a[1] * b[1];
+    const double i = a[0] * b[1] + a[1] * b[0];
+    out[0] = r;
+    out[1] = i;
+}
+
+/* core/src/lattice2d.ixx:33-58: hopping t = 1/(2 h^2) per axis, the bond
+ * mixing constants (x: dt/2, y-even: dt/2, y-odd: dt) and the onsite tables
+ * (FD diagonal 2 tx + 2 ty added to V).  consts = {tx, ty, cx, sx, cy, sy,
+ * cy2, sy2, rcx, rsx, rcy, rsy, rcy2, rsy2}. */
+void orc_lat2d_tables(const double* v, int nx, int ny, double hx, double hy, double dt,
+                      double* consts, double* half_v, double* relax_half_v) {
+    const double tx = 0.5 / (hx * hx), ty = 0.5 / (hy * hy);
+    consts[0] = tx;
+    consts[1] = ty;
+    consts[2] = cos(tx * 0.5 * dt);
+    consts[3] = sin(tx * 0.5 * dt);
+    consts[4] = cos(ty * 0.5 * dt);
+    consts[5] = sin(ty * 0.5 * dt);
+    consts[6] = cos(ty * dt);
+    consts[7] = sin(ty * dt);
+    consts[8] = cosh(tx * 0.5 * dt);
+    consts[9] = sinh(tx * 0.5 * dt);
+    consts[10] = cosh(ty * 0.5 * dt);
+    consts[11] = sinh(ty * 0.5 * dt);
+    consts[12] = cosh(ty * dt);
+    consts[13] = sinh(ty * dt);
+    for (long i = 0; i < (long)nx * ny; ++i) {
+        const double e0 = v[i] + 2.0 * tx + 2.0 * ty;
+        const double th = -0.5 * e0 * dt;
+        half_v[2 * i] = cos(th);
+        half_v[2 * i + 1] = sin(th);
+        relax_half_v[i] = exp(-0.5 * e0 * dt);
+    }
+}
+
+/* core/src/lattice2d.ixx:63-91: string-gauge solenoid on the x-links;
+ * link_x is reset to 1 first.  core/src/lattice2d.ixx:96-105: uniform field,
+ * Landau gauge A_x = B y. */
+void orc_lat2d_solenoid(double* link_x, int nx, int ny, double xmin, double xmax, double ymin,
+                        double ymax, double phi, double xs, double ys, int cut_up) {
+    for (long i = 0; i < (long)nx * ny; ++i) {
+        link_x[2 * i] = 1.0;
+        link_x[2 * i + 1] = 0.0;
+    }
+    int is = -1, js = -1;
+    for (int i = 0; i + 1 < nx; ++i)
+        if (coord(xmin, xmax, nx, i) <= xs && xs < coord(xmin, xmax, nx, i + 1)) is = i;
+    for (int j = 0; j + 1 < ny; ++j)
+        if (coord(ymin, ymax, ny, j) <= ys && ys < coord(ymin, ymax, ny, j + 1)) js = j;
+    if (is < 0 || js < 0) return;
+    const double up[2] = {cos(phi), -sin(phi)};
+    const double dn[2] = {cos(phi), sin(phi)};
+    if (cut_up) {
+        for (int j = js + 1; j < ny; ++j) {
+            link_x[2 * ((long)j * nx + is)] = up[0];
+            link_x[2 * ((long)j * nx + is) + 1] = up[1];
+        }
+    } else {
+        for (int j = 0; j <= js; ++j) {
+            link_x[2 * ((long)j * nx + is)] = dn[0];
+            link_x[2 * ((long)j * nx + is) + 1] = dn[1];
+        }
+    }
+}
+
+void orc_lat2d_uniform_field(double* link_x, int nx, int ny, double xmin, double xmax,
+                             double ymin, double ymax, double b) {
+    const double bh = b * orc_spacing(xmin, xmax, nx);
+    for (int j = 0; j < ny; ++j) {
+        const double th = -bh * coord(ymin, ymax, ny, j);
+        const double c = cos(th), s = sin(th);
+        for (int i = 0; i < nx; ++i) {
+            link_x[2 * ((long)j * nx + i)] = c;
+            link_x[2 * ((long)j * nx + i) + 1] = s;
+        }
+    }
+}
+
+/* core/src/lattice2d.ixx:178-197 (x) and 199-216 (y): one bond parity group,
+ * a_i = c a_i + mix u a_k, a_k = mix conj(u) a_i + c a_k */
+static void lat_bond(double* a, long i, long k, const double* u, double c, const double* mix) {
+    const double pa[2] = {a[2 * i], a[2 * i + 1]};
+    const double pb[2] = {a[2 * k], a[2 * k + 1]};
+    double mu[2], t[2], mcu[2], t2[2];
+    cm(mix, u, mu);
+    cm(mu, pb, t);
+    a[2 * i] = c * pa[0] + t[0];
+    a[2 * i + 1] = c * pa[1] + t[1];
+    const double uc[2] = {u[0], -u[1]};
+    cm(mix, uc, mcu);
+    cm(mcu, pa, t2);
+    a[2 * k] = t2[0] + c * pb[0];
+    a[2 * k + 1] = t2[1] + c * pb[1];
+}
+
+static void lat_sweep_x(double* a, int nx, int ny, const double* link_x, int parity, double c,
+                        const double* mix) {
+    for (int j = 0; j < ny; ++j) {
+        const long row = (long)j * nx;
+        for (int i = parity; i + 1 < nx; i += 2)
+            lat_bond(a, row + i, row + i + 1, link_x + 2 * (row + i), c, mix);
+    }
+}
+
+static void lat_sweep_y(double* a, int nx, int ny, const double* link_y, int parity, double c,
+                        const double* mix) {
+    for (int j = parity; j + 1 < ny; j += 2) {
+        const long lo = (long)j * nx, hi = (long)(j + 1) * nx;
+        for (int i = 0; i < nx; ++i) lat_bond(a, lo + i, hi + i, link_y + 2 * (lo + i), c, mix);
+    }
+}
+
+static void lat_phase(double* a, long n, const double* table) {
+    for (long i = 0; i < n; ++i) {
+        double t[2];
+        cm(a + 2 * i, table + 2 * i, t);
+        a[2 * i] = t[0];
+        a[2 * i + 1] = t[1];
+    }
+}
+
+/* core/src/lattice2d.ixx:218-232 (step) and 234-249 (relax: cosh/sinh
+ * mixing, real onsite decay as a complex (w, 0) factor, then normalise
+ * with cell volume dv).  relax_half_v is real; link_y is all ones. */
+int orc_lat2d_evolve(double* psi, int nx, int ny, const double* consts, const double* half_v,
+                     const double* relax_half_v, const double* link_x, int relax, double dv,
+                     int nsteps) {
+    const long n = (long)nx * ny;
+    double* link_y = (double*)malloc(sizeof(double) * 2 * (size_t)n);
+    for (long i = 0; i < n; ++i) {
+        link_y[2 * i] = 1.0;
+        link_y[2 * i + 1] = 0.0;
+    }
+    double* rtab = NULL;
+    if (relax) {
+        rtab = (double*)malloc(sizeof(double) * 2 * (size_t)n);
+        for (long i = 0; i < n; ++i) {
+            rtab[2 * i] = relax_half_v[i];
+            rtab[2 * i + 1] = 0.0;
+        }
+    }
+    const double* tab = relax ? rtab : half_v;
+    const double cx = consts[relax ? 8 : 2], cy = consts[relax ? 10 : 4],
+                 cy2 = consts[relax ? 12 : 6];
+    double mx[2], my[2], my2[2];
+    if (relax) {
+        mx[0] = consts[9]; mx[1] = 0.0;
+        my[0] = consts[11]; my[1] = 0.0;
+        my2[0] = consts[13]; my2[1] = 0.0;
+    } else {
+        mx[0] = 0.0; mx[1] = consts[3];
+        my[0] = 0.0; my[1] = consts[5];
+        my2[0] = 0.0; my2[1] = consts[7];
+    }
+    for (int s = 0; s < nsteps; ++s) {
+        lat_phase(psi, n, tab);
+        lat_sweep_x(psi, nx, ny, link_x, 0, cx, mx);
+        lat_sweep_x(psi, nx, ny, link_x, 1, cx, mx);
+        lat_sweep_y(psi, nx, ny, link_y, 0, cy, my);
+        lat_sweep_y(psi, nx, ny, link_y, 1, cy2, my2);
+        lat_sweep_y(psi, nx, ny, link_y, 0, cy, my);
+        lat_sweep_x(psi, nx, ny, link_x, 1, cx, mx);
+        lat_sweep_x(psi, nx, ny, link_x, 0, cx, mx);
+        lat_phase(psi, n, tab);
+        if (relax) orc_normalize(psi, (size_t)n, dv);
+    }
+    free(link_y);
+    free(rtab);
+    return 0;
+}
+
+/* core/src/lattice2d.ixx:251-274: <H> = hops + onsite, normalised */
+double orc_lat2d_energy(const double* psi, int nx, int ny, const double* v, const double* consts,
+                        const double* link_x) {
+    const double tx = consts[0], ty = consts[1];
+    double e = 0.0, den = 0.0;
+    for (int j = 0; j < ny; ++j) {
+        for (int i = 0; i < nx; ++i) {
+            const long f = (long)j * nx + i;
+            const double z[2] = {psi[2 * f], psi[2 * f + 1]};
+            const double w = z[0] * z[0] + z[1] * z[1];
+            den += w;
+            e += (v[f] + 2.0 * tx + 2.0 * ty) * w;
+            const double zc[2] = {z[0], -z[1]};
+            if (i + 1 < nx) {
+                double t[2], t2[2];
+                cm(zc, link_x + 2 * f, t);
+                cm(t, psi + 2 * (f + 1), t2);
+                e += -tx * 2.0 * t2[0];
+            }
+            if (j + 1 < ny) {
+                const double one[2] = {1.0, 0.0};
+                double t[2], t2[2];
+                cm(zc, one, t);
+                cm(t, psi + 2 * (f + nx), t2);
+                e += -ty * 2.0 * t2[0];
+            }
+        }
+    }
+    return e / den;
+}
+
 /* ---- imaginary-time relaxation (core/src/imaginary_time.ixx) ---------- */
 
 /* core/src/imaginary_time.ixx:24-40 */

@@ -104,6 +104,19 @@ int orc_magnetic_step(double* psi, const int dims[3], const double lo[3], const 
                       const double* half_v, const double* kinetic, double dt, double bfield,
                       int axis, int nsteps);
 
+/* Peierls lattice (core/src/lattice2d.ixx). */
+void orc_lat2d_tables(const double* v, int nx, int ny, double hx, double hy, double dt,
+                      double* consts, double* half_v, double* relax_half_v);
+void orc_lat2d_solenoid(double* link_x, int nx, int ny, double xmin, double xmax, double ymin,
+                        double ymax, double phi, double xs, double ys, int cut_up);
+void orc_lat2d_uniform_field(double* link_x, int nx, int ny, double xmin, double xmax,
+                             double ymin, double ymax, double b);
+int orc_lat2d_evolve(double* psi, int nx, int ny, const double* consts, const double* half_v,
+                     const double* relax_half_v, const double* link_x, int relax, double dv,
+                     int nsteps);
+double orc_lat2d_energy(const double* psi, int nx, int ny, const double* v, const double* consts,
+                        const double* link_x);
+
 /* Imaginary-time relaxation (core/src/imaginary_time.ixx). */
 void orc_it1d_tables(const double* v, int n, double xmin, double xmax, double dtau,
                      double* half_w, double* kin_w);

@@ -37,7 +37,7 @@ __all__ = [
     "regularized_coulomb_potential", "coulomb_floor_potential", "absorbing_mask",
     "gaussian_wavepacket", "TDSE_FP32", "TDSE_FP64", "TdseError", "Batch",
     "ImaginaryTimePropagator1D", "ImaginaryTimePropagator3D", "soft_coulomb_potential",
-    "MagneticPropagator3D",
+    "MagneticPropagator3D", "PeierlsLattice2D",
 ]
 
 PRECISIONS = {"fp32": TDSE_FP32, "fp64": TDSE_FP64, TDSE_FP32: TDSE_FP32, TDSE_FP64: TDSE_FP64}
@@ -163,7 +163,8 @@ class Propagator:
 
     def __init__(self, grid, potential, dt, mass=1.0, precision="fp64", mask=None,
                  context: Context | None = None, unfused_baseline: bool = False,
-                 imaginary_time: bool = False, bfield: float = 0.0, b_axis: int = 2):
+                 imaginary_time: bool = False, bfield: float = 0.0, b_axis: int = 2,
+                 lattice: bool = False):
         self.grid = grid
         self.ctx = context or default_context()
         self.precision = PRECISIONS[precision]
@@ -179,7 +180,8 @@ class Propagator:
                                           capi.dvec(hi), C.c_double(dt), C.c_double(mass),
                                           self.precision,
                                           (capi.TDSE_FLAG_UNFUSED_BASELINE if unfused_baseline else 0)
-                                          | (capi.TDSE_FLAG_IMAGINARY_TIME if imaginary_time else 0),
+                                          | (capi.TDSE_FLAG_IMAGINARY_TIME if imaginary_time else 0)
+                                          | (capi.TDSE_FLAG_LATTICE if lattice else 0),
                                           C.byref(h)))
         except TdseError as e:
             raise _map_error(e) from None
@@ -302,6 +304,21 @@ class Propagator:
     @time.setter
     def time(self, t):
         capi.check(capi.lib().tdse_plan_set_time(self._h, C.c_double(t)))
+
+    def set_uniform_field(self, b):
+        """Lattice plans: uniform field along z in the Landau gauge."""
+        try:
+            capi.check(capi.lib().tdse_lattice_set_uniform_field(self._h, C.c_double(b)))
+        except TdseError as e:
+            raise _map_error(e) from None
+
+    def set_solenoid(self, phi, xs, ys, cut_up=True):
+        """Lattice plans: a solenoid of flux phi at (xs, ys), string gauge."""
+        try:
+            capi.check(capi.lib().tdse_lattice_set_solenoid(self._h, C.c_double(phi), C.c_double(xs),
+                                                            C.c_double(ys), 1 if cut_up else 0))
+        except TdseError as e:
+            raise _map_error(e) from None
 
     def relax_stats(self) -> dict:
         """Imaginary time: the last step's norm before renormalisation, the
@@ -475,6 +492,92 @@ class SplitOperator3D(_SplitOperatorBase):
 
     def __init__(self, grid: Grid3D, potential, dt, mass=1.0, precision="fp64", context=None):
         super().__init__(grid, potential, dt, mass, precision, context)
+
+
+class PeierlsLattice2D:
+    """``ses::PeierlsLattice2D(grid, V, dt)`` (core/src/lattice2d.ixx:31-275): 2D
+    finite differences with Peierls link phases on an n x m x 1 grid, open
+    boundaries.  ``step`` propagates, ``relax`` relaxes in imaginary time (a
+    second plan with the same links), ``energy`` is the lattice <H>."""
+
+    def __init__(self, grid: Grid3D, potential, dt, precision="fp64", context=None):
+        self._grid = grid
+        self._dt = float(dt)
+        self._v = np.ascontiguousarray(potential, dtype=np.float64).ravel()
+        self._prec = precision
+        self._ctx = context
+        self._links = ("none",)
+        self._prop = Propagator(grid, self._v, dt, precision=precision, context=context,
+                                lattice=True)
+        self._relaxer = None
+
+    def dt(self) -> float:
+        return self._dt
+
+    def _apply_links(self, p):
+        kind = self._links[0]
+        if kind == "uniform":
+            p.set_uniform_field(self._links[1])
+        elif kind == "solenoid":
+            p.set_solenoid(*self._links[1:])
+        else:
+            p.set_solenoid(0.0, 1e300, 1e300)  # outside the lattice: every link 1
+
+    def set_solenoid(self, phi, xs, ys, cut_up=True):
+        self._links = ("solenoid", phi, xs, ys, cut_up)
+        self._apply_links(self._prop)
+        if self._relaxer:
+            self._apply_links(self._relaxer)
+
+    def set_uniform_field(self, b):
+        self._links = ("uniform", b)
+        self._apply_links(self._prop)
+        if self._relaxer:
+            self._apply_links(self._relaxer)
+
+    def link_x(self, i, j) -> complex:
+        """Directed x link (i, j) -> (i + 1, j) (host copy of the plan's rule)."""
+        kind = self._links[0]
+        if kind == "uniform":
+            th = -self._links[1] * self._grid.x.spacing() * self._grid.y.coord(j)
+            return complex(math.cos(th), math.sin(th))
+        if kind == "solenoid":
+            phi, xs, ys, cut_up = self._links[1:]
+            gx, gy = self._grid.x, self._grid.y
+            isol = max([i_ for i_ in range(gx.n - 1) if gx.coord(i_) <= xs < gx.coord(i_ + 1)],
+                       default=-1)
+            jsol = max([j_ for j_ in range(gy.n - 1) if gy.coord(j_) <= ys < gy.coord(j_ + 1)],
+                       default=-1)
+            if isol >= 0 and jsol >= 0 and i == isol and ((cut_up and j > jsol) or
+                                                          (not cut_up and j <= jsol)):
+                return complex(math.cos(phi), -math.sin(phi) if cut_up else math.sin(phi))
+        return 1.0 + 0.0j
+
+    def link_y(self, i, j) -> complex:
+        return 1.0 + 0.0j
+
+    def _run(self, p, psi, nsteps):
+        if psi.dtype != np.complex128 or not psi.flags.c_contiguous or psi.size != p.size:
+            raise ValueError("psi must be a contiguous complex128 array of the grid's size")
+        if nsteps <= 0:
+            return
+        p.upload(psi)
+        p.step(nsteps)
+        p.download(psi.reshape(p.shape))
+
+    def step(self, psi: np.ndarray, nsteps: int = 1) -> None:
+        self._run(self._prop, psi, nsteps)
+
+    def relax(self, psi: np.ndarray, nsteps: int = 1) -> None:
+        if self._relaxer is None:
+            self._relaxer = Propagator(self._grid, self._v, self._dt, precision=self._prec,
+                                       context=self._ctx, lattice=True, imaginary_time=True)
+            self._apply_links(self._relaxer)
+        self._run(self._relaxer, psi, nsteps)
+
+    def energy(self, psi: np.ndarray) -> float:
+        self._prop.upload(np.ascontiguousarray(psi, dtype=np.complex128))
+        return self._prop.observe(TDSE_OBS_NORM | TDSE_OBS_ENERGY)["energy"]
 
 
 class MagneticPropagator3D:

@@ -37,6 +37,7 @@ UNITS = [
     ("kernels_f64_v3.hip", DEVICE),
     ("observe.hip", DEVICE),
     ("device_build.hip", DEVICE),
+    ("lattice.hip", DEVICE),
     ("tdse_api.cpp", ["-x", "hip"] + DEVICE),
     ("builders.cpp", ["-ffp-contract=off"]),
 ]

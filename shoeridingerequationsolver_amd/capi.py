@@ -25,6 +25,7 @@ TDSE_FP64 = 1
 
 TDSE_FLAG_UNFUSED_BASELINE = 1
 TDSE_FLAG_IMAGINARY_TIME = 2
+TDSE_FLAG_LATTICE = 4
 
 TDSE_OBS_NORM = 1
 TDSE_OBS_POSITION = 2
@@ -47,6 +48,7 @@ EXPORTS = (
     "tdse_batch_stream", "tdse_batch_launches_per_round", "tdse_plan_build",
     "tdse_relax_deflated", "tdse_plan_relax_stats", "tdse_build_soft_coulomb",
     "tdse_plan_set_drive", "tdse_plan_set_time", "tdse_plan_get_time", "tdse_plan_set_magnetic",
+    "tdse_lattice_set_uniform_field", "tdse_lattice_set_solenoid",
 )
 
 TDSE_POT = {"none": 0, "harmonic": 1, "regularized_coulomb": 2, "coulomb_floor": 3, "barrier": 4,
@@ -158,6 +160,9 @@ def lib() -> C.CDLL:
         L.tdse_plan_set_time.argtypes = [C.c_void_p, C.c_double]
         L.tdse_plan_get_time.argtypes = [C.c_void_p, _dp]
         L.tdse_plan_set_magnetic.argtypes = [C.c_void_p, C.c_double, C.c_int]
+        L.tdse_lattice_set_uniform_field.argtypes = [C.c_void_p, C.c_double]
+        L.tdse_lattice_set_solenoid.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double,
+                                                C.c_int]
         _LIB = L
     return _LIB
 

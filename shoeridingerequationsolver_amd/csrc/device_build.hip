@@ -87,7 +87,8 @@ build_cells(BuildArgs a) {
             pot += a.bfield * a.bfield / 8.0 * perp2;
         }
         v[f] = static_cast<T>(pot);
-        phi[f] = static_cast<T>(a.imaginary ? -0.5 * pot * a.dt : reduce_2pi_dev(-0.5 * pot * a.dt));
+        const double e0 = pot + a.onsite_shift;
+        phi[f] = static_cast<T>(a.imaginary ? -0.5 * e0 * a.dt : reduce_2pi_dev(-0.5 * e0 * a.dt));
         if (mask) {
             const double m = ramp(x, a.lo[0], a.hi[0], a.n[0], r.mask_width) *
                              ramp(y, a.lo[1], a.hi[1], a.n[1], r.mask_width) *

@@ -18,6 +18,7 @@ struct BuildArgs {
     double dt;
     int imaginary;  // 1: phi = -V dt / 2 unreduced (imaginary time weights)
     double bfield;  // uniform field: V += B^2/8 |r_perp|^2 (0: none)
+    double onsite_shift;  // added to V in the kick phase only (lattice: 2 tx + 2 ty)
     int b_axis;
     tdse_recipe r;
 };

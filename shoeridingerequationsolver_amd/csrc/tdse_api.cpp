@@ -22,6 +22,7 @@
 #include "launch.hpp"
 #include "observe.hpp"
 #include "device_build.hpp"
+#include "lattice.hpp"
 
 using tdse::SweepArgs;
 
@@ -56,7 +57,7 @@ struct Axis {
     double length() const { return hi - lo; }
 };
 
-enum Kind { kKind3D, kKind2D, kKindSingle, kKindFourStep };
+enum Kind { kKind3D, kKind2D, kKindSingle, kKindFourStep, kKindLattice };
 
 constexpr int kMaxSlots = 6;
 
@@ -111,6 +112,16 @@ struct tdse_plan {
     // uniform magnetic field along axis b_axis (core/src/magnetic.ixx:19-80)
     double bfield = 0.0;
     int b_axis = 2;
+    // Peierls lattice x links (core/src/lattice2d.ixx:63-105): 0 none,
+    // 1 uniform field (angle lat_a per unit y), 2 solenoid (flux lat_a on
+    // column lat_col, rows [lat_j0, lat_j1), sign lat_sign)
+    int lat_mode = 0;
+    double lat_a = 0.0, lat_sign = -1.0;
+    int lat_col = -1, lat_j0 = 0, lat_j1 = 0;
+    double hop(int axis) const {
+        const double h = (hi[axis] - lo[axis]) / n[axis];
+        return 0.5 / (h * h);
+    }
     // unfused FP64 baseline: full complex128 tables of the reference propagator
     void* base_half = nullptr;  // exp(-i V dt / 2) per cell
     void* base_kin = nullptr;   // exp(-i |k|^2 dt / 2m) per cell
@@ -522,6 +533,7 @@ int slot_count(const tdse_plan* p) {
     switch (p->kind) {
         case kKind3D: return 6;
         case kKindSingle: return 1;
+        case kKindLattice: return 0;
         default: return 4;
     }
 }
@@ -674,6 +686,87 @@ int run_magnetic(tdse_plan* p, int nsteps) {
         rotate(p->mask != nullptr);
     }
     return run_launches(p, L);
+}
+
+// The Peierls lattice arguments of a plan (core/src/lattice2d.ixx:33-58):
+// mixing constants of the bond groups, real time (cos, i sin) or imaginary
+// time (cosh, sinh).
+tdse::LatticeArgs lattice_args(const tdse_plan* p) {
+    tdse::LatticeArgs a{};
+    a.psi = p->psi;
+    a.phi = p->phi;
+    a.mask = p->mask;
+    a.v = p->v;
+    a.nx = p->n[0];
+    a.ny = p->n[1];
+    a.tx = p->hop(0);
+    a.ty = p->hop(1);
+    const double dt = p->dt;
+    a.relax = p->imag() ? 1 : 0;
+    if (p->imag()) {
+        a.cx = std::cosh(a.tx * 0.5 * dt);
+        a.mxr = std::sinh(a.tx * 0.5 * dt);
+        a.cy = std::cosh(a.ty * 0.5 * dt);
+        a.myr = std::sinh(a.ty * 0.5 * dt);
+        a.cy2 = std::cosh(a.ty * dt);
+        a.my2r = std::sinh(a.ty * dt);
+    } else {
+        a.cx = std::cos(a.tx * 0.5 * dt);
+        a.mxi = std::sin(a.tx * 0.5 * dt);
+        a.cy = std::cos(a.ty * 0.5 * dt);
+        a.myi = std::sin(a.ty * 0.5 * dt);
+        a.cy2 = std::cos(a.ty * dt);
+        a.my2i = std::sin(a.ty * dt);
+    }
+    a.link_mode = p->lat_mode;
+    a.link_a = p->lat_a;
+    a.ylo = p->lo[1];
+    a.hy = (p->hi[1] - p->lo[1]) / p->n[1];
+    a.link_col = p->lat_col;
+    a.link_j0 = p->lat_j0;
+    a.link_j1 = p->lat_j1;
+    a.link_sign = p->lat_sign;
+    return a;
+}
+
+// core/src/lattice2d.ixx:218-249.  Real time: one row sweep per step
+// boundary (x groups, merged onsite factor, mask) and one column sweep (y
+// groups) per step.  Imaginary time: each step runs its own row sweeps and is
+// renormalised (fixed-order reduction, then a scaling pass).
+int run_lattice(tdse_plan* p, int nsteps) {
+    int rc;
+    // the column sweep writes the scratch array, the row sweep reads it back
+    if (!p->scratch && (rc = alloc(p, &p->scratch, p->total * p->csize()))) return rc;
+    tdse::LatticeArgs a = lattice_args(p);
+    const bool fp64 = p->fp64();
+    auto rows = [&](int stage, bool from_scratch) {
+        a.stage = stage;
+        a.other = from_scratch ? p->scratch : nullptr;
+        return tdse::launch_lattice_rows(fp64, a, p->stream);
+    };
+    auto cols = [&]() {
+        a.other = p->scratch;
+        return tdse::launch_lattice_cols(fp64, a, p->stream);
+    };
+    if (!p->imag()) {
+        TDSE_HIP(rows(0, false));
+        for (int s = 0; s < nsteps; ++s) {
+            TDSE_HIP(cols());
+            TDSE_HIP(rows(s + 1 < nsteps ? 1 : 2, true));
+        }
+        return TDSE_OK;
+    }
+    const tdse::ReduceArgs r = reduce_args(p);
+    const double dv = cell_volume(p);
+    for (int s = 0; s < nsteps; ++s) {
+        TDSE_HIP(rows(0, false));
+        TDSE_HIP(cols());
+        TDSE_HIP(rows(2, true));
+        TDSE_HIP(tdse::launch_reduce(fp64, false, r, p->partials, p->red_out, p->stream));
+        TDSE_HIP(tdse::launch_normalize_dev(fp64, p->psi, p->total, p->red_out, dv,
+                                            &p->relax()->norm_sq, p->stream));
+    }
+    return TDSE_OK;
 }
 
 int check_plan(const tdse_plan* p) {
@@ -1012,8 +1105,15 @@ int tdse_plan_bytes(const int n[3], int precision, int with_mask, uint64_t* out)
 int tdse_plan_create(tdse_ctx* ctx, const int n[3], const double lo[3], const double hi[3],
                      double dt, double mass, int precision, unsigned flags, tdse_plan** out) {
     if (!ctx || !n || !lo || !hi || !out) return fail(TDSE_E_INVALID, "null argument");
-    if (flags & ~(TDSE_FLAG_UNFUSED_BASELINE | TDSE_FLAG_IMAGINARY_TIME))
+    if (flags & ~(TDSE_FLAG_UNFUSED_BASELINE | TDSE_FLAG_IMAGINARY_TIME | TDSE_FLAG_LATTICE))
         return fail(TDSE_E_INVALID, "unknown flags");
+    if (flags & TDSE_FLAG_LATTICE) {
+        if ((flags & TDSE_FLAG_UNFUSED_BASELINE) || !n)
+            return fail(TDSE_E_INVALID, "the lattice has no unfused baseline");
+        if (n[2] != 1 || n[0] < 2 || n[1] < 2)
+            return fail(TDSE_E_INVALID, "the lattice is a 2D grid n x m x 1 with n, m >= 2");
+        if (n[0] > tdse::kLatMaxNx) return fail(TDSE_E_UNSUPPORTED, "lattice rows hold at most 2048 sites");
+    }
     if ((flags & TDSE_FLAG_UNFUSED_BASELINE) && precision != TDSE_FP64)
         return fail(TDSE_E_INVALID, "the unfused baseline is FP64 only");
     if ((flags & TDSE_FLAG_UNFUSED_BASELINE) && (flags & TDSE_FLAG_IMAGINARY_TIME))
@@ -1046,10 +1146,16 @@ int tdse_plan_create(tdse_ctx* ctx, const int n[3], const double lo[3], const do
     p->total = total;
     p->flags = flags;
     std::vector<int> lines;
-    rc = classify(n, p->sq, lo, hi, &p->kind, &p->n1, &p->n2, &lines);
-    if (rc) {
-        delete p;
-        return rc;
+    if (flags & TDSE_FLAG_LATTICE) {
+        // finite differences: any extents, no line FFTs
+        p->kind = kKindLattice;
+        for (int a = 0; a < 2; ++a) p->sq.push_back({n[a], lo[a], hi[a]});
+    } else {
+        rc = classify(n, p->sq, lo, hi, &p->kind, &p->n1, &p->n2, &lines);
+        if (rc) {
+            delete p;
+            return rc;
+        }
     }
     if ((flags & TDSE_FLAG_UNFUSED_BASELINE) && p->kind != kKind3D && p->kind != kKind2D) {
         delete p;
@@ -1078,8 +1184,9 @@ int tdse_plan_create(tdse_ctx* ctx, const int n[3], const double lo[3], const do
     static_assert(16 * 8 + sizeof(tdse::RelaxSlots) <= 64 * 8, "relax slots follow red_out");
     for (int N : lines)
         if ((rc = make_twiddles(p, N))) return bail(rc);
-    for (size_t a = 0; a < p->sq.size(); ++a)
-        if ((rc = make_kinetic(p, static_cast<int>(a), p->sq[a]))) return bail(rc);
+    if (p->kind != kKindLattice)
+        for (size_t a = 0; a < p->sq.size(); ++a)
+            if ((rc = make_kinetic(p, static_cast<int>(a), p->sq[a]))) return bail(rc);
     if (flags & TDSE_FLAG_UNFUSED_BASELINE) {
         if ((rc = alloc(p, &p->base_half, total * 16))) return bail(rc);
         if ((rc = alloc(p, &p->base_kin, total * 16))) return bail(rc);
@@ -1147,9 +1254,12 @@ int tdse_plan_set_potential(tdse_plan* p, const double* v) {
         v = veff.data();
     }
     std::vector<double> th(static_cast<size_t>(p->total));
+    // the lattice onsite energy carries the finite-difference diagonal
+    // 2 tx + 2 ty (core/src/lattice2d.ixx:50-56)
+    const double shift = p->kind == kKindLattice ? 2.0 * p->hop(0) + 2.0 * p->hop(1) : 0.0;
     for (long long i = 0; i < p->total; ++i) {
         if (!std::isfinite(v[i])) return fail(TDSE_E_INVALID, "potential not finite");
-        th[i] = -0.5 * v[i] * p->dt;  // core/src/propagator.ixx:29, 79
+        th[i] = -0.5 * (v[i] + shift) * p->dt;  // core/src/propagator.ixx:29, 79
     }
     // real time: reduced modulo 2 pi in FP64 for both precisions (the
     // kernels' sin/cos take |angle| <= 8); imaginary time: the exponent of
@@ -1215,6 +1325,7 @@ int tdse_plan_build(tdse_plan* p, const tdse_recipe* r) {
     b.imaginary = p->imag() ? 1 : 0;
     b.bfield = p->bfield;
     b.b_axis = p->b_axis;
+    b.onsite_shift = p->kind == kKindLattice ? 2.0 * p->hop(0) + 2.0 * p->hop(1) : 0.0;
     b.r = *r;
     TDSE_HIP(tdse::launch_build(p->fp64(), b, p->stream));
     p->has_potential = true;
@@ -1276,6 +1387,11 @@ int tdse_step(tdse_plan* p, int nsteps) {
     if (!p->has_potential) return fail(TDSE_E_STATE, "tdse_plan_set_potential not called");
     if ((rc = set_device(p->ctx))) return rc;
     if (p->flags & TDSE_FLAG_UNFUSED_BASELINE) return run_baseline(p, nsteps);
+    if (p->kind == kKindLattice) {
+        if ((rc = run_lattice(p, nsteps))) return rc;
+        p->time += nsteps * p->dt;
+        return TDSE_OK;
+    }
     if (p->bfield != 0.0) {
         if ((rc = run_magnetic(p, nsteps))) return rc;
         p->time += nsteps * p->dt;
@@ -1297,7 +1413,8 @@ int tdse_plan_set_drive(tdse_plan* p, const double axis[3], double amplitude, do
     if (!axis) return fail(TDSE_E_INVALID, "null axis");
     if (!std::isfinite(amplitude) || !std::isfinite(omega))
         return fail(TDSE_E_INVALID, "drive amplitude and frequency must be finite");
-    if (p->imag() || (p->flags & TDSE_FLAG_UNFUSED_BASELINE) || p->bfield != 0.0)
+    if (p->imag() || (p->flags & TDSE_FLAG_UNFUSED_BASELINE) || p->bfield != 0.0 ||
+        p->kind == kKindLattice)
         return fail(TDSE_E_UNSUPPORTED, "the drive applies to fused real-time plans without a field");
     for (int a = 0; a < 3; ++a) p->drive_axis[a] = axis[a];
     p->drive_amp = amplitude;
@@ -1313,13 +1430,56 @@ int tdse_plan_set_magnetic(tdse_plan* p, double bfield, int axis) {
         return fail(TDSE_E_INVALID, "field must be finite, axis 0, 1 or 2");
     if (p->has_potential) return fail(TDSE_E_STATE, "set the field before the potential");
     if (bfield != 0.0) {
-        if (p->imag() || (p->flags & TDSE_FLAG_UNFUSED_BASELINE) || p->drive)
+        if (p->imag() || (p->flags & TDSE_FLAG_UNFUSED_BASELINE) || p->drive ||
+            p->kind == kKindLattice)
             return fail(TDSE_E_UNSUPPORTED, "the field applies to plain real-time plans");
         if (p->kind != kKind3D && !(p->kind == kKind2D && axis == 2 && p->n[2] == 1))
             return fail(TDSE_E_UNSUPPORTED, "the field needs a 3D grid (or a 2D one rotated about z)");
     }
     p->bfield = bfield;
     p->b_axis = axis;
+    return TDSE_OK;
+}
+
+int tdse_lattice_set_uniform_field(tdse_plan* p, double b) {
+    int rc = check_plan(p);
+    if (rc) return rc;
+    if (p->kind != kKindLattice) return fail(TDSE_E_INVALID, "not a lattice plan");
+    if (!std::isfinite(b)) return fail(TDSE_E_INVALID, "field must be finite");
+    // x links of row j: exp(-i B hx y_j) (core/src/lattice2d.ixx:96-105)
+    p->lat_mode = 1;
+    p->lat_a = -b * ((p->hi[0] - p->lo[0]) / p->n[0]);
+    return TDSE_OK;
+}
+
+int tdse_lattice_set_solenoid(tdse_plan* p, double phi, double xs, double ys, int cut_up) {
+    int rc = check_plan(p);
+    if (rc) return rc;
+    if (p->kind != kKindLattice) return fail(TDSE_E_INVALID, "not a lattice plan");
+    if (!std::isfinite(phi) || !std::isfinite(xs) || !std::isfinite(ys))
+        return fail(TDSE_E_INVALID, "solenoid arguments must be finite");
+    // core/src/lattice2d.ixx:63-91: the cell holding (xs, ys); the string cut
+    // runs up (rows above it, exp(-i phi)) or down (rows up to it, exp(+i phi))
+    auto coord = [&](int a, int i) { return p->lo[a] + i * ((p->hi[a] - p->lo[a]) / p->n[a]); };
+    int is = -1, js = -1;
+    for (int i = 0; i + 1 < p->n[0]; ++i)
+        if (coord(0, i) <= xs && xs < coord(0, i + 1)) is = i;
+    for (int j = 0; j + 1 < p->n[1]; ++j)
+        if (coord(1, j) <= ys && ys < coord(1, j + 1)) js = j;
+    p->lat_mode = 0;  // links reset to 1 (a solenoid outside the lattice leaves them so)
+    if (is < 0 || js < 0) return TDSE_OK;
+    p->lat_mode = 2;
+    p->lat_a = phi;
+    p->lat_col = is;
+    if (cut_up) {
+        p->lat_j0 = js + 1;
+        p->lat_j1 = p->n[1];
+        p->lat_sign = -1.0;
+    } else {
+        p->lat_j0 = 0;
+        p->lat_j1 = js + 1;
+        p->lat_sign = 1.0;
+    }
     return TDSE_OK;
 }
 
@@ -1341,6 +1501,7 @@ int tdse_relax_deflated(tdse_plan* p, const tdse_plan* const* lower, int nlower,
     int rc = check_plan(p);
     if (rc) return rc;
     if (!p->imag()) return fail(TDSE_E_INVALID, "not an imaginary-time plan");
+    if (p->kind == kKindLattice) return fail(TDSE_E_UNSUPPORTED, "no deflation on the lattice");
     if (nsteps < 0 || nlower < 0 || (nlower > 0 && !lower))
         return fail(TDSE_E_INVALID, "bad deflation arguments");
     if (!p->has_potential) return fail(TDSE_E_STATE, "tdse_plan_set_potential not called");
@@ -1443,6 +1604,18 @@ int tdse_observe(tdse_plan* p, unsigned what, tdse_observables* out) {
         out->sigma[a] = std::sqrt(std::max(0.0, ratio(host[6 + a], host[0]) - m * m));
     }
     if (!(what & (TDSE_OBS_ENERGY | TDSE_OBS_MOMENTUM))) return TDSE_OK;
+    if (p->kind == kKindLattice) {
+        // <H> of the lattice Hamiltonian (hops + onsite, core/src/lattice2d.ixx:251-274)
+        if (what & TDSE_OBS_MOMENTUM)
+            return fail(TDSE_E_UNSUPPORTED, "no momentum observable on the lattice");
+        double* dst = p->red_out + 24;
+        TDSE_HIP(tdse::launch_lattice_energy(p->fp64(), lattice_args(p), p->partials, dst, p->stream));
+        double hk[3];
+        if ((rc = d2h(p, hk, dst, sizeof(hk)))) return rc;
+        out->energy = ratio(hk[0], hk[1]);
+        out->kinetic = out->energy - out->potential;
+        return TDSE_OK;
+    }
     // original axis of each squeezed axis
     int orig[3] = {0, 0, 0};
     for (int a = 0, q = 0; a < 3; ++a)
@@ -1553,6 +1726,7 @@ int tdse_profile_step(tdse_plan* p, int nsteps, int max_slots, double* ms_per_sw
                       int* n_slots) {
     int rc = check_plan(p);
     if (rc) return rc;
+    if (p->kind == kKindLattice) return fail(TDSE_E_UNSUPPORTED, "the lattice has no sweep slots");
     if (nsteps < 1 || !ms_per_sweep || !launches || !n_slots)
         return fail(TDSE_E_INVALID, "bad profiling arguments");
     if (p->flags & TDSE_FLAG_UNFUSED_BASELINE)
@@ -1615,6 +1789,7 @@ int tdse_plan_get_variants(const tdse_plan* p, int max_slots, int* variants, int
 
 int tdse_plan_set_variants(tdse_plan* p, const int* variants, int n_slots) {
     if (!p || !variants) return fail(TDSE_E_INVALID, "null argument");
+    if (p->kind == kKindLattice) return fail(TDSE_E_UNSUPPORTED, "the lattice has no sweep slots");
     const int slots = slot_count(p);
     if (n_slots != slots) return fail(TDSE_E_INVALID, "expected one variant per schedule slot");
     const Geometry g = geometry(p, p->psi);
@@ -1766,6 +1941,7 @@ int autotune_impl(tdse_plan* p, Tuner& t, int reps, int* chosen) {
 int tdse_plan_autotune(tdse_plan* p, int reps, int max_slots, int* variants, int* n_slots) {
     int rc = check_plan(p);
     if (rc) return rc;
+    if (p->kind == kKindLattice) return fail(TDSE_E_UNSUPPORTED, "the lattice has no sweep slots");
     if (reps < 1 || reps > 64 || !variants || !n_slots || max_slots < 0)
         return fail(TDSE_E_INVALID, "bad autotune arguments");
     if (!p->has_potential) return fail(TDSE_E_STATE, "tdse_plan_set_potential not called");
@@ -1809,6 +1985,13 @@ int tdse_plan_step_bytes(const tdse_plan* p, uint64_t* out) {
     if (p->kind == kKind3D) sweeps = 4;
     else if (p->kind == kKind2D || p->kind == kKindFourStep) sweeps = 2;
     const uint64_t N = static_cast<uint64_t>(p->total);
+    if (p->kind == kKindLattice) {
+        // row sweep (state in and out, onsite phase, mask) + column sweep
+        // (state in and out; its 3-row halos read 6 of every 58 rows again)
+        *out = N * (2 * p->csize() + p->rsize() + (p->mask ? p->rsize() : 0)) +
+               N * p->csize() * 64 / 58 + N * p->csize();
+        return TDSE_OK;
+    }
     if (p->flags & TDSE_FLAG_UNFUSED_BASELINE) {
         // 3 table passes (48 B), 2 x (axes) FFT sweeps (32 B), 2 conj passes (32 B), mask (40 B)
         const int axes = p->kind == kKind3D ? 3 : 2;
@@ -1831,8 +2014,8 @@ int tdse_batch_create(tdse_ctx* ctx, tdse_plan* const* plans, int count, tdse_ba
             return fail(TDSE_E_INVALID, "the unfused baseline does not batch");
         if (plans[i]->imag())
             return fail(TDSE_E_UNSUPPORTED, "imaginary-time plans do not batch");
-        if (plans[i]->drive || plans[i]->bfield != 0.0)
-            return fail(TDSE_E_UNSUPPORTED, "driven and magnetic plans do not batch");
+        if (plans[i]->drive || plans[i]->bfield != 0.0 || plans[i]->kind == kKindLattice)
+            return fail(TDSE_E_UNSUPPORTED, "driven, magnetic and lattice plans do not batch");
         for (int j = 0; j < i; ++j)
             if (plans[j] == plans[i]) return fail(TDSE_E_INVALID, "plan listed twice");
     }
