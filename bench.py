@@ -167,6 +167,53 @@ def batch_leg(t, workloads, steps):
     return out
 
 
+def paths_leg(t, hip, workloads, steps):
+    """Throughput of the other propagator paths on the headline grid (256^3,
+    inputs resident, each timed over `steps` steps with HIP events): imaginary
+    time, the dipole drive, the magnetic field (shear sweeps), the Peierls
+    lattice on 1024^2, and one full observables call."""
+    solve = workloads.barrier(seed=3)
+    grid = solve.grid()
+    out = {}
+
+    def timed(p, n):
+        gpu_s, _ = time_steps(hip, p, n, max(2, n // 10))
+        sb = p.step_bytes()
+        return {"steps_per_s": round(n / gpu_s, 1), "ms_per_step": round(1e3 * gpu_s / n, 4),
+                "step_bytes": sb, "achieved_gbs": round(sb * n / gpu_s / 1e9, 1)}
+
+    for prec in ("fp32", "fp64"):
+        p = t.Propagator(grid, None, solve.dt, precision=prec, imaginary_time=True)
+        r = solve.recipe()
+        r.mask_width = 0.0
+        p.build(r)
+        out[f"relax_{prec}"] = timed(p, steps)
+        p.close()
+    p = t.Propagator(grid, None, solve.dt, precision="fp32")
+    p.build(solve.recipe())
+    p.set_drive((1.0, 0.0, 0.0), 0.05, 0.5)
+    out["driven_fp32"] = timed(p, steps)
+    o_ms = []
+    for _ in range(5):
+        t0 = time.perf_counter()
+        p.observe(15)
+        o_ms.append(1e3 * (time.perf_counter() - t0))
+    out["observe_all_fp32_ms"] = round(min(o_ms), 3)
+    p.close()
+    p = t.Propagator(grid, None, solve.dt, precision="fp32", bfield=0.5)
+    p.build(solve.recipe())
+    out["magnetic_fp32"] = timed(p, max(10, steps // 4))
+    p.close()
+    lg = t.Grid3D(t.Grid1D(-60.0, 60.0, 1024), t.Grid1D(-60.0, 60.0, 1024), t.Grid1D(-1.0, 1.0, 1))
+    for prec in ("fp32", "fp64"):
+        p = t.Propagator(lg, np.zeros(1024 * 1024), 0.01, precision=prec, lattice=True)
+        p.set_uniform_field(0.3)
+        p.upload(t.gaussian_wavepacket(lg, (-10.0, 0.0, 0.0), (3.0, 3.0, 1.0), (1.0, 0.0, 0.0)))
+        out[f"lattice1024_{prec}"] = timed(p, steps)
+        p.close()
+    return out
+
+
 def serving_leg(rates, count, budget):
     """Poisson request streams (mixed 1D/2D/3D up to 256^3, harmonic / Coulomb /
     barrier+mask, 100-2000 steps, FP32 or FP64), continuous batching under an
@@ -235,6 +282,8 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--cpu-steps-1core", type=int, default=4)
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-paths", action="store_true")
+    ap.add_argument("--path-steps", type=int, default=200)
     ap.add_argument("--batch-steps", type=int, default=500)
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=30)
@@ -341,6 +390,10 @@ def main():
                     "transmitted_err_fp32": outs["fp32"][2] - outs["fp64"][2]}
         log(f"accuracy after {full} steps: max|dpsi| {d:.3e}, transmitted {outs['fp64'][2]:.6f}")
 
+    paths = None
+    if not args.no_paths:
+        paths = paths_leg(t, hip, workloads, args.path_steps)
+        log(f"paths: {paths}")
     batch = None
     if not args.no_batch:
         batch = batch_leg(t, workloads, args.batch_steps)
@@ -390,6 +443,7 @@ def main():
         "cpu_baseline": cpu,
         "batch": batch,
         "serving": serve,
+        "paths": paths,
     }
     print(json.dumps(line), flush=True)
 

@@ -1992,6 +1992,12 @@ int tdse_plan_step_bytes(const tdse_plan* p, uint64_t* out) {
                N * p->csize() * 64 / 58 + N * p->csize();
         return TDSE_OK;
     }
+    if (p->bfield != 0.0) {
+        // six shear sweeps and the five sweeps of one unmerged Strang step
+        // (two kick sweeps read the phase), mask with the last shear
+        *out = N * (11 * 2 * p->csize() + 2 * p->rsize() + (p->mask ? p->rsize() : 0));
+        return TDSE_OK;
+    }
     if (p->flags & TDSE_FLAG_UNFUSED_BASELINE) {
         // 3 table passes (48 B), 2 x (axes) FFT sweeps (32 B), 2 conj passes (32 B), mask (40 B)
         const int axes = p->kind == kKind3D ? 3 : 2;
