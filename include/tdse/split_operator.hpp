@@ -8,6 +8,9 @@
 //   ses::SplitOperator3D(g, V, dt, m)    core/src/propagator.ixx:66-134
 //   ses::WavepacketSimulation            core/src/simulation.ixx:22-88
 //   ses::ImaginaryTimePropagator1D/3D    core/src/imaginary_time.ixx:21-144
+//   ses::DipoleDrive, ses::driven_step   core/src/drive.ixx:19-66
+//   ses::MagneticPropagator3D            core/src/magnetic.ixx:19-80
+//   ses::PeierlsLattice2D                core/src/lattice2d.ixx:27-275
 //
 // step(Field&, n) uploads the field, runs n Strang steps on the GPU and reads
 // it back (the reference's in-place contract).  Device-resident stepping is
@@ -117,9 +120,11 @@ enum class Precision : int { FP32 = TDSE_FP32, FP64 = TDSE_FP64 };
 // Device-resident propagator (one plan): the state lives in HBM between calls.
 class Propagator {
 public:
+    // bfield != 0: a uniform field along b_axis (set before the potential,
+    // which then carries the diamagnetic term)
     Propagator(const Grid3D& g, const std::vector<double>& potential, double dt, double mass = 1.0,
                Precision prec = Precision::FP64, Context& ctx = Context::instance(),
-               unsigned flags = 0u) {
+               unsigned flags = 0u, double bfield = 0.0, int b_axis = 2) {
         const int n[3] = {g.x.n, g.y.n, g.z.n};
         const double lo[3] = {g.x.xmin, g.y.xmin, g.z.xmin};
         const double hi[3] = {g.x.xmax, g.y.xmax, g.z.xmax};
@@ -129,6 +134,7 @@ public:
         check(tdse_plan_create(ctx.get(), n, lo, hi, dt, mass, static_cast<int>(prec), flags, &p));
         plan_.reset(p);
         size_ = g.size();
+        if (bfield != 0.0) check(tdse_plan_set_magnetic(plan_.get(), bfield, b_axis));
         check(tdse_plan_set_potential(plan_.get(), potential.data()));
     }
     void set_mask(const std::vector<double>& m) { check(tdse_plan_set_mask(plan_.get(), m.data())); }
@@ -191,10 +197,136 @@ public:
         prop_->step(nsteps);
         prop_->download(psi.data());
     }
+    // the device plan behind this propagator (driven_step sets its drive)
+    Propagator& device() const noexcept { return *prop_; }
 
 private:
     double dt_;
     std::unique_ptr<Propagator> prop_;
+};
+
+// ses::DipoleDrive (core/src/drive.ixx:19-23): V = amplitude (axis . r) cos(omega t)
+struct DipoleDrive {
+    Vec3d axis;
+    double amplitude{};
+    double omega{};
+};
+
+// ses::driven_step (core/src/drive.ixx:52-66): nsteps driven Strang steps of
+// `prop` from time t0; the drive's half kicks ride in the kick sweeps.  The
+// propagator's plan is left without a drive afterwards.
+inline void driven_step(Field3D& psi, const SplitOperator3D& prop, const DipoleDrive& d, double t0,
+                        int nsteps) {
+    if (nsteps <= 0) return;
+    Propagator& p = prop.device();
+    const double axis[3] = {d.axis.x, d.axis.y, d.axis.z};
+    check(tdse_plan_set_drive(p.get(), axis, d.amplitude, d.omega));
+    check(tdse_plan_set_time(p.get(), t0));
+    p.upload(psi.data());
+    p.step(nsteps);
+    p.download(psi.data());
+    check(tdse_plan_set_drive(p.get(), axis, 0.0, 0.0));
+}
+
+// ses::MagneticPropagator3D (core/src/magnetic.ixx:19-80): uniform field
+// `bfield` along `axis` (0 = x, 1 = y, 2 = z), symmetric gauge; each step is
+// R(a) . Strang(V + B^2/8 |r_perp|^2) . R(a), a = (B/2)(dt/2).
+class MagneticPropagator3D {
+public:
+    MagneticPropagator3D(const Grid3D& g, const std::vector<double>& v, double dt, double bfield,
+                         int axis = 2, Precision prec = Precision::FP64)
+        : dt_(dt), veff_(v),
+          prop_(std::make_unique<Propagator>(g, v, dt, 1.0, prec, Context::instance(), 0u, bfield,
+                                             axis)) {
+        // host copy of the diamagnetic term the plan folded into its potential
+        const double c = bfield * bfield / 8.0;
+        if (c != 0.0)
+            for (int k = 0; k < g.z.n; ++k)
+                for (int j = 0; j < g.y.n; ++j)
+                    for (int i = 0; i < g.x.n; ++i) {
+                        const double r[3] = {g.x.coord(i), g.y.coord(j), g.z.coord(k)};
+                        double perp = 0.0;
+                        for (int a = 0; a < 3; ++a)
+                            if (a != axis) perp += r[a] * r[a];
+                        veff_[static_cast<std::size_t>(g.flat(i, j, k))] += c * perp;
+                    }
+    }
+    double dt() const noexcept { return dt_; }
+    const std::vector<double>& effective_potential() const noexcept { return veff_; }
+    void step(Field3D& psi, int nsteps = 1) const {
+        if (nsteps <= 0) return;
+        prop_->upload(psi.data());
+        prop_->step(nsteps);
+        prop_->download(psi.data());
+    }
+
+private:
+    double dt_;
+    std::vector<double> veff_;
+    std::unique_ptr<Propagator> prop_;
+};
+
+// ses::PeierlsLattice2D (core/src/lattice2d.ixx:27-275): finite-difference
+// hopping with Peierls link phases on an n x m x 1 grid.  step propagates,
+// relax relaxes in imaginary time (a second plan with the same links),
+// energy is the lattice <H>.
+class PeierlsLattice2D {
+public:
+    PeierlsLattice2D(const Grid3D& g, const std::vector<double>& potential, double dt,
+                     Precision prec = Precision::FP64)
+        : grid_(g), v_(potential), dt_(dt), prec_(prec),
+          prop_(std::make_unique<Propagator>(g, potential, dt, 1.0, prec, Context::instance(),
+                                             TDSE_FLAG_LATTICE)) {}
+    double dt() const noexcept { return dt_; }
+    void set_solenoid(double phi, double xs, double ys, bool cut_up = true) {
+        links_ = {2, phi, xs, ys, cut_up};
+        apply(*prop_);
+        if (relaxer_) apply(*relaxer_);
+    }
+    void set_uniform_field(double b) {
+        links_ = {1, b, 0.0, 0.0, true};
+        apply(*prop_);
+        if (relaxer_) apply(*relaxer_);
+    }
+    void step(Field3D& psi, int nsteps = 1) const { run(*prop_, psi, nsteps); }
+    void relax(Field3D& psi, int nsteps = 1) const {
+        if (!relaxer_) {
+            relaxer_ = std::make_unique<Propagator>(grid_, v_, dt_, 1.0, prec_, Context::instance(),
+                                                    TDSE_FLAG_LATTICE | TDSE_FLAG_IMAGINARY_TIME);
+            apply(*relaxer_);
+        }
+        run(*relaxer_, psi, nsteps);
+    }
+    double energy(const Field3D& psi) const {
+        prop_->upload(psi.data());
+        return prop_->observe(TDSE_OBS_NORM | TDSE_OBS_ENERGY).energy;
+    }
+
+private:
+    struct Links {
+        int kind;  // 0 none, 1 uniform field, 2 solenoid
+        double a, xs, ys;
+        bool cut_up;
+    };
+    void apply(Propagator& p) const {
+        if (links_.kind == 1) check(tdse_lattice_set_uniform_field(p.get(), links_.a));
+        if (links_.kind == 2)
+            check(tdse_lattice_set_solenoid(p.get(), links_.a, links_.xs, links_.ys,
+                                            links_.cut_up ? 1 : 0));
+    }
+    static void run(Propagator& p, Field3D& psi, int nsteps) {
+        if (nsteps <= 0) return;
+        p.upload(psi.data());
+        p.step(nsteps);
+        p.download(psi.data());
+    }
+    Grid3D grid_;
+    std::vector<double> v_;
+    double dt_;
+    Precision prec_;
+    Links links_{0, 0.0, 0.0, 0.0, true};
+    std::unique_ptr<Propagator> prop_;
+    mutable std::unique_ptr<Propagator> relaxer_;
 };
 
 // ses::ImaginaryTimePropagator1D (core/src/imaginary_time.ixx:21-63): relax

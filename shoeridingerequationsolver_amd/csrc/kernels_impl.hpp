@@ -93,6 +93,8 @@ LaunchFn pick_mode(int mode) {
         case S_OBS: return &launch_sweep<T, N, S_OBS>;
         case C_SHEAR: return &launch_sweep<T, N, C_SHEAR>;
         case S_SHEAR: return &launch_sweep<T, N, S_SHEAR>;
+        case C_SHEAR_KICK_FIRST: return &launch_sweep<T, N, C_SHEAR_KICK_FIRST>;
+        case C_KICK_LAST_SHEAR: return &launch_sweep<T, N, C_KICK_LAST_SHEAR>;
         default: return nullptr;
     }
 }

@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include "observe.hpp"
+#include "sweeps.hpp"
 
 namespace tdse {
 namespace {
@@ -373,6 +374,33 @@ hipError_t launch_scale(bool fp64, void* psi, long long total, double s, hipStre
         hipLaunchKernelGGL(scale_state<double>, dim3(b), dim3(kBlock), 0, stream, psi, total, s);
     else
         hipLaunchKernelGGL(scale_state<float>, dim3(b), dim3(kBlock), 0, stream, psi, total, s);
+    return hipGetLastError();
+}
+
+// Imaginary time: the norm of a measuring sweep from its per-block partials
+// (block order, fixed-order tree), then scale[parity ^ 1] = 1 / sqrt(norm)
+// (a zero field is left as it is, core/src/field.ixx:78-81), norm and peak.
+__global__ void __launch_bounds__(256) relax_finish(const double* partials, RelaxSlots* slots,
+                                                    int parity, double cell) {
+    const unsigned nb = slots->nblocks;
+    double ts = 0.0, tm = 0.0;
+    for (unsigned b = threadIdx.x; b < nb; b += blockDim.x) {
+        ts += partials[2 * b];
+        const double q = partials[2 * b + 1];
+        tm = q > tm ? q : tm;
+    }
+    block_sum_max(ts, tm);
+    if (threadIdx.x == 0) {
+        const double norm = ts * cell;
+        slots->scale[parity ^ 1] = norm > 0.0 ? 1.0 / sqrt(norm) : 1.0;
+        slots->norm_sq = norm;
+        slots->peak = tm;
+    }
+}
+
+hipError_t launch_relax_finish(const SweepArgs& a, int, hipStream_t stream) {
+    hipLaunchKernelGGL(relax_finish, dim3(1), dim3(256), 0, stream, a.partials, a.relax, a.parity,
+                       a.cell);
     return hipGetLastError();
 }
 

@@ -44,6 +44,10 @@ hipError_t launch_subtract_projection(bool fp64, void* psi, const void* phi, lon
 // psi /= sqrt(sum[0] dv) unless zero; *norm_out = sum[0] dv
 hipError_t launch_normalize_dev(bool fp64, void* psi, long long total, const double* sum, double dv,
                                 double* norm_out, hipStream_t stream);
+// Imaginary time: after a measuring sweep with args `a`, turn its per-block
+// partials into the next scale, the norm and the peak (a LaunchFn)
+struct SweepArgs;
+hipError_t launch_relax_finish(const SweepArgs& a, int max_blocks, hipStream_t stream);
 // FP64 elementwise passes of the unfused baseline
 hipError_t launch_table_mul(void* psi, const void* table, long long total, hipStream_t stream);
 hipError_t launch_conj_scale(void* psi, long long total, double s, hipStream_t stream);

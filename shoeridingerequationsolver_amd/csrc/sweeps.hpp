@@ -72,12 +72,16 @@ enum SweepMode : int {
     // shears of a three-shear rotation: FFT, exp(-i k d(line)) / N, IFFT (, mask)
     C_SHEAR = 22,
     S_SHEAR = 23,
-    kNumModes = 24
+    // the magnetic step's first and last x sweeps fused with the adjacent x
+    // shear of the rotation around it
+    C_SHEAR_KICK_FIRST = 24,  // FFT, shear, IFFT, kick_half, FFT
+    C_KICK_LAST_SHEAR = 25,   // IFFT, kick_half, FFT, shear, IFFT (, mask)
+    kNumModes = 26
 };
 
 constexpr bool mode_contiguous(int m) {
     return m <= C_SINGLE || m == C_FWD || (m >= C_RELAX_FIRST && m <= C_RELAX_SINGLE) ||
-           m == C_OBS || m == C_SHEAR;
+           m == C_OBS || m == C_SHEAR || m == C_SHEAR_KICK_FIRST || m == C_KICK_LAST_SHEAR;
 }
 constexpr bool mode_observes(int m) { return m == C_OBS || m == S_OBS; }
 // modes that measure the norm across blocks (deferred renormalisation)
@@ -94,7 +98,7 @@ struct RelaxSlots {
     double scale[2];      // 1 / sqrt(norm) of the previous measured step, by parity
     double norm_sq;       // sum |psi|^2 dV before renormalisation, last measured step
     double peak;          // max |psi|^2 there (before renormalisation)
-    unsigned int ticket;  // blocks finished in the current measuring sweep
+    unsigned int nblocks;  // blocks (partials) of the last measuring sweep
     unsigned int pad;
 };
 
@@ -475,38 +479,20 @@ __device__ __forceinline__ void block_sum_max(double& s, double& m) {
     }
 }
 
-// End of a measuring sweep, per block: store this block's (sum, max); the
-// last block to finish (ticket) adds the partials in block order (fixed
-// order: bitwise repeatable) and leaves scale, norm and peak for the next
-// sweep.  Every block of the grid takes a ticket, so the last one exists.
+// End of a measuring sweep, per block: this block's (sum, max) of |psi|^2
+// (fixed-order block reduction) into partials[2 b], partials[2 b + 1]; block 0
+// records the grid size.  launch_relax_finish, queued right after the sweep,
+// adds the partials in block order (bitwise repeatable) and leaves scale,
+// norm and peak for the next sweep.  (A last-block-done ticket here needed an
+// agent-scope release/acquire per block, i.e. an L2 writeback and invalidate
+// per block on this multi-L2 part: it tripled the sweep's time.)
 __device__ __forceinline__ void relax_block_done(const SweepArgs& a, SweepAcc ra) {
     double s = ra.q0, m = ra.q1;
     block_sum_max(s, m);
-    __shared__ int last;
     if (threadIdx.x == 0) {
         a.partials[2 * blockIdx.x] = s;
         a.partials[2 * blockIdx.x + 1] = m;
-        const unsigned t = __hip_atomic_fetch_add(&a.relax->ticket, 1u, __ATOMIC_ACQ_REL,
-                                                  __HIP_MEMORY_SCOPE_AGENT);
-        last = t == gridDim.x - 1;
-    }
-    __syncthreads();
-    if (!last) return;
-    double ts = 0.0, tm = 0.0;
-    for (unsigned b = threadIdx.x; b < gridDim.x; b += blockDim.x) {
-        ts += __hip_atomic_load(&a.partials[2 * b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        const double q = __hip_atomic_load(&a.partials[2 * b + 1], __ATOMIC_RELAXED,
-                                           __HIP_MEMORY_SCOPE_AGENT);
-        tm = q > tm ? q : tm;
-    }
-    block_sum_max(ts, tm);
-    if (threadIdx.x == 0) {
-        const double norm = ts * a.cell;
-        // a zero field is left as it is (core/src/field.ixx:78-81)
-        a.relax->scale[a.parity ^ 1] = norm > 0.0 ? 1.0 / sqrt(norm) : 1.0;
-        a.relax->norm_sq = norm;
-        a.relax->peak = tm;
-        __hip_atomic_store(&a.relax->ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (blockIdx.x == 0) a.relax->nblocks = gridDim.x;
     }
 }
 
@@ -619,6 +605,26 @@ __device__ __forceinline__ void contig_tile(const SweepArgs& a, long long tile, 
     } else if constexpr (MODE == C_SHEAR) {
         const long long idx = a.sh_sel ? line / a.dn1 : line % a.dn1;
         L.template load<LT::R1>();
+        L.fft();
+        L.shear_phase(a.sh_d0 + a.sh_d1 * static_cast<double>(idx));
+        L.ifft();
+        L.mask_only();
+        L.template store<LT::R1>();
+    } else if constexpr (MODE == C_SHEAR_KICK_FIRST) {
+        const long long idx = a.sh_sel ? line / a.dn1 : line % a.dn1;
+        L.template load<LT::R1>();
+        L.fft();
+        L.shear_phase(a.sh_d0 + a.sh_d1 * static_cast<double>(idx));
+        L.ifft();
+        L.kick(1.0, false, 0.0);
+        L.fft();
+        L.template store<LT::RL>();
+    } else if constexpr (MODE == C_KICK_LAST_SHEAR) {
+        const long long idx = a.sh_sel ? line / a.dn1 : line % a.dn1;
+        L.template load<LT::RL>();
+        L.ifft();
+        L.load_kick(false);
+        L.apply_kick(1.0, false, 0.0);
         L.fft();
         L.shear_phase(a.sh_d0 + a.sh_d1 * static_cast<double>(idx));
         L.ifft();

@@ -61,6 +61,9 @@ enum Kind { kKind3D, kKind2D, kKindSingle, kKindFourStep, kKindLattice };
 
 constexpr int kMaxSlots = 6;
 
+// Launch::mode of the small kernel that ends a measuring sweep
+constexpr int kFinishMode = -1;
+
 struct Launch {
     int mode;
     int n;          // line length
@@ -572,6 +575,9 @@ std::vector<Launch> schedule(const tdse_plan* p, int nsteps, void* target = null
             }
         }
         L.push_back({w.mode, w.n, w.args, launcher(p, w.n, w.mode, p->variant[slot]), slot});
+        // a measuring sweep leaves per-block partials: its norm and the next
+        // scale come from one small kernel right after it (timed with its slot)
+        if (tdse::mode_measures(w.mode)) L.push_back({kFinishMode, w.n, w.args, &tdse::launch_relax_finish, slot});
     };
     if (p->kind == kKind3D) {
         add(0, 0);
@@ -680,10 +686,44 @@ int run_magnetic(tdse_plan* p, int nsteps) {
     };
     std::vector<Launch> core = schedule(p, 1);
     core.back().args.mask = nullptr;  // the mask follows the closing rotation
+    const bool masked = p->mask != nullptr;
+    if (b != 0) {
+        for (int s = 0; s < nsteps; ++s) {
+            rotate(false);
+            for (const Launch& l : core) L.push_back(l);
+            rotate(masked);
+        }
+        return run_launches(p, L);
+    }
+    // Field along z: the rotation's outer shears run along x, next to the
+    // core step's first and last x sweeps, so each joins that sweep (one
+    // read and write of the state instead of two); without a mask the
+    // closing shear of step s and the opening shear of step s + 1 are one
+    // shear by the summed coefficient.
+    auto fused = [&](Launch l, int mode, double coeff) {
+        const Launch sh = shear(b, c, coeff, false);
+        l.mode = mode;
+        l.args.sh_sel = sh.args.sh_sel;
+        l.args.sh_d0 = sh.args.sh_d0;
+        l.args.sh_d1 = sh.args.sh_d1;
+        l.args.dk = sh.args.dk;
+        l.args.scale = sh.args.scale;
+        l.fn = launcher(p, l.n, mode);
+        return l;
+    };
+    const Launch first = fused(core.front(), tdse::C_SHEAR_KICK_FIRST, -t);
+    const Launch last = fused(core.back(), tdse::C_KICK_LAST_SHEAR, -t);
     for (int s = 0; s < nsteps; ++s) {
-        rotate(false);
-        for (const Launch& l : core) L.push_back(l);
-        rotate(p->mask != nullptr);
+        if (s == 0 || masked) L.push_back(shear(b, c, -t, false));
+        L.push_back(shear(c, b, sn, false));
+        L.push_back(first);
+        for (size_t i = 1; i + 1 < core.size(); ++i) L.push_back(core[i]);
+        L.push_back(last);
+        L.push_back(shear(c, b, sn, false));
+        if (masked)
+            L.push_back(shear(b, c, -t, true));
+        else
+            L.push_back(shear(b, c, s + 1 < nsteps ? -2.0 * t : -t, false));
     }
     return run_launches(p, L);
 }
@@ -1759,9 +1799,9 @@ int tdse_profile_step(tdse_plan* p, int nsteps, int max_slots, double* ms_per_sw
         for (size_t i = 0; i < L.size(); ++i) {
             float ms = 0.f;
             if (hipEventElapsedTime(&ms, ev[2 * i], ev[2 * i + 1]) != hipSuccess) continue;
-            if (L[i].slot < max_slots) {
+            if (L[i].slot >= 0 && L[i].slot < max_slots) {
                 ms_per_sweep[L[i].slot] += ms;
-                launches[L[i].slot] += 1;
+                if (L[i].mode != kFinishMode) launches[L[i].slot] += 1;
             }
         }
     }
@@ -1822,13 +1862,13 @@ struct Tuner {
 
     // Milliseconds per steady-state step of the current variants: a schedule
     // of `k` steps, timed from after its first sweep to before its last; the
-    // best of three runs.
-    int per_step(int k, double* out) {
+    // best of `rounds` runs.
+    int per_step(int k, double* out, int rounds = 3) {
         const std::vector<Launch> L = schedule(p, k, p->scratch);
         for (const Launch& l : L)
             if (!l.fn) return fail(TDSE_E_UNSUPPORTED, "no kernel for a tuned sweep");
         double best = 1e30;
-        for (int round = 0; round < 3; ++round) {
+        for (int round = 0; round < rounds; ++round) {
             int rc = refresh();
             if (rc) return rc;
             TDSE_HIP(L.front().fn(L.front().args, p->ctx->max_blocks, p->stream));
@@ -1843,6 +1883,25 @@ struct Tuner {
             best = std::min(best, static_cast<double>(ms) / k);
         }
         *out = best;
+        return TDSE_OK;
+    }
+
+    // Per-step times of variant sets `a` and `b`, measured in alternation
+    // (a b a b ...) so that a drift of clocks or temperature during tuning
+    // falls on both; the best of `pairs` runs each.
+    int compare(const int* a, const int* b, int k, int pairs, double* ta, double* tb) {
+        *ta = *tb = 1e30;
+        for (int i = 0; i < pairs; ++i) {
+            for (int side = 0; side < 2; ++side) {
+                const int* set = side ? b : a;
+                for (int s = 0; s < kMaxSlots; ++s) p->variant[s] = set[s];
+                double t = 0.0;
+                const int rc = per_step(k, &t, 1);
+                if (rc) return rc;
+                double* dst = side ? tb : ta;
+                *dst = std::min(*dst, t);
+            }
+        }
         return TDSE_OK;
     }
 
@@ -1897,24 +1956,31 @@ int autotune_impl(tdse_plan* p, Tuner& t, int reps, int* chosen) {
         double ms = 0.0;
         if ((rc = t.per_step(k, &ms))) return rc;
         if (ms * k < 2.0) k = static_cast<int>(std::min(256.0, std::ceil(2.0 / std::max(ms, 1e-4))));
-        double best = 0.0;
-        if ((rc = t.per_step(k, &best))) return rc;
-        // one pass of coordinate descent over the per-step slots
+        // one pass of coordinate descent over the per-step slots; each
+        // candidate is timed in alternation with the current choice and
+        // must win by 2%
+        int cur[kMaxSlots] = {};
         for (int s : per_step) {
             const SlotSweep w = slot_sweep(p, g, s);
-            int pick = p->variant[s];
             for (int v = 1; v < tdse::kVariants; ++v) {
                 if (!launcher(p, w.n, w.mode, v)) continue;
-                p->variant[s] = v;
-                double t_v = 0.0;
-                if ((rc = t.per_step(k, &t_v))) return rc;
-                if (t_v < 0.98 * best) {
-                    best = t_v;
-                    pick = v;
-                }
-                p->variant[s] = pick;
+                int cand[kMaxSlots];
+                std::copy(cur, cur + kMaxSlots, cand);
+                cand[s] = v;
+                double t_cur = 0.0, t_cand = 0.0;
+                if ((rc = t.compare(cur, cand, k, 3, &t_cur, &t_cand))) return rc;
+                if (t_cand < 0.98 * t_cur) std::copy(cand, cand + kMaxSlots, cur);
             }
         }
+        // the set found must still beat the default kernels by 2% in a
+        // longer alternation, else the default is kept
+        const int zero[kMaxSlots] = {};
+        if (!std::equal(cur, cur + kMaxSlots, zero)) {
+            double t0 = 0.0, t1 = 0.0;
+            if ((rc = t.compare(zero, cur, k, 5, &t0, &t1))) return rc;
+            if (!(t1 < 0.98 * t0)) std::fill(cur, cur + kMaxSlots, 0);
+        }
+        for (int s = 0; s < kMaxSlots; ++s) p->variant[s] = cur[s];
     }
     for (int s : per_call) {
         const SlotSweep w = slot_sweep(p, g, s);
@@ -1967,7 +2033,6 @@ int tdse_plan_autotune(tdse_plan* p, int reps, int max_slots, int* variants, int
     (void)hipEventDestroy(t.e0);
     (void)hipEventDestroy(t.e1);
     if (p->imag()) {
-        kept.ticket = 0;
         const int rc2 = h2d(p, p->relax(), &kept, sizeof(kept));
         if (!rc) rc = rc2;
     }
@@ -1993,9 +2058,15 @@ int tdse_plan_step_bytes(const tdse_plan* p, uint64_t* out) {
         return TDSE_OK;
     }
     if (p->bfield != 0.0) {
-        // six shear sweeps and the five sweeps of one unmerged Strang step
-        // (two kick sweeps read the phase), mask with the last shear
-        *out = N * (11 * 2 * p->csize() + 2 * p->rsize() + (p->mask ? p->rsize() : 0));
+        // the sweeps of one unmerged Strang step (two kick sweeps read the
+        // phase) and six shear sweeps, mask with the last shear.  Field along
+        // z: the x shears next to the core step join its x sweeps, and
+        // without a mask the closing and opening x shears of neighbouring
+        // steps are one sweep.
+        const int core = p->kind == kKind3D ? 5 : 3;
+        int sweeps = core + 6;
+        if (p->b_axis == 2) sweeps = core + (p->mask ? 4 : 3);
+        *out = N * (sweeps * 2 * p->csize() + 2 * p->rsize() + (p->mask ? p->rsize() : 0));
         return TDSE_OK;
     }
     if (p->flags & TDSE_FLAG_UNFUSED_BASELINE) {

@@ -54,6 +54,82 @@ int main(int argc, char** argv) {
         const double e0 = relax.stats().energy;
         std::printf("facade: relaxed E0 = %.8f\n", e0);
         if (std::abs(e0 - 0.5) > 1e-4) return 1;
+
+        // ses::driven_step (drive.ixx:52-66): zero amplitude is the static
+        // step of one 12-step call (f above) bit for bit, a drive changes the
+        // state, and the plan is left without a drive (a later step(Field&)
+        // is static again)
+        {
+            tdse::SplitOperator3D dop{g, v, 0.02};
+            tdse::Field3D a = tdse::gaussian_wavepacket(g, {1.0, 0.0, 0.0}, {1.0, 1.0, 1.0}, {0.0, 0.5, 0.0});
+            tdse::Field3D b = a, c = a;
+            tdse::driven_step(a, dop, tdse::DipoleDrive{{1.0, 0.0, 0.0}, 0.0, 0.3}, 0.7, 12);
+            tdse::driven_step(b, dop, tdse::DipoleDrive{{1.0, 0.0, 0.0}, 0.2, 0.3}, 0.7, 12);
+            dop.step(c, 12);
+            double same = 0.0, moved = 0.0;
+            for (std::size_t i = 0; i < a.data().size(); ++i) {
+                same = std::max(same, std::abs(a.data()[i] - f.data()[i]));
+                moved = std::max(moved, std::abs(b.data()[i] - f.data()[i]));
+                same = std::max(same, std::abs(c.data()[i] - f.data()[i]));
+            }
+            std::printf("facade: driven zero-amplitude diff %.3e, driven change %.3e\n", same, moved);
+            if (same != 0.0 || moved < 1e-6) return 1;
+        }
+        // ses::MagneticPropagator3D (magnetic.ixx:19-80): the diamagnetic
+        // term in effective_potential, and a unitary step
+        {
+            const double bf = 0.4;
+            tdse::MagneticPropagator3D mp{g, v, 0.02, bf, 2};
+            const int i = 5, j = 20, k = 3;
+            const double x = g.x.coord(i), y = g.y.coord(j);
+            const double want = v[static_cast<std::size_t>(g.flat(i, j, k))] + bf * bf / 8.0 * (x * x + y * y);
+            const double got = mp.effective_potential()[static_cast<std::size_t>(g.flat(i, j, k))];
+            tdse::Field3D m = tdse::gaussian_wavepacket(g, {1.0, 0.0, 0.0}, {1.0, 1.0, 1.0}, {0.0, 0.5, 0.0});
+            mp.step(m, 20);
+            double nrm = 0.0;
+            for (const auto& z : m.data()) nrm += std::norm(z);
+            nrm *= g.cell_volume();
+            std::printf("facade: magnetic veff %.3e off, norm after 20 steps %.15f\n", got - want, nrm);
+            if (std::abs(got - want) > 1e-15 * std::abs(want) || std::abs(nrm - 1.0) > 1e-12) return 1;
+        }
+        // ses::PeierlsLattice2D (lattice2d.ixx): with V = 0 and no field the
+        // open-boundary ground state is sin(pi(i+1)/(n+1)) sin(pi(j+1)/(m+1)),
+        // energy 2 tx (1 - cos(pi/(n+1))) + 2 ty (1 - cos(pi/(m+1))); a field
+        // step is unitary
+        {
+            const tdse::Grid3D g2{tdse::Grid1D{-8.0, 8.0, 32}, tdse::Grid1D{-6.0, 6.0, 24},
+                                  tdse::Grid1D{0.0, 1.0, 1}};
+            const std::vector<double> v0(static_cast<std::size_t>(g2.size()), 0.0);
+            tdse::PeierlsLattice2D lat{g2, v0, 0.01};
+            const double pi = 3.14159265358979323846;
+            tdse::Field3D s{g2};
+            double nrm = 0.0;
+            for (int j = 0; j < 24; ++j)
+                for (int i = 0; i < 32; ++i) {
+                    const double a = std::sin(pi * (i + 1) / 33.0) * std::sin(pi * (j + 1) / 25.0);
+                    s(i, j, 0) = {a, 0.0};
+                    nrm += a * a;
+                }
+            for (auto& z : s.data()) z /= std::sqrt(nrm * g2.cell_volume());
+            const double hx = g2.x.spacing(), hy = g2.y.spacing();
+            const double tx = 0.5 / (hx * hx), ty = 0.5 / (hy * hy);
+            const double e_want = 2 * tx * (1 - std::cos(pi / 33.0)) + 2 * ty * (1 - std::cos(pi / 25.0));
+            const double e_got = lat.energy(s);
+            tdse::Field3D r = s;
+            lat.relax(r, 200);
+            const double e_relaxed = lat.energy(r);
+            lat.set_uniform_field(0.3);
+            tdse::Field3D w = s;
+            lat.step(w, 50);
+            double n2 = 0.0;
+            for (const auto& z : w.data()) n2 += std::norm(z);
+            n2 *= g2.cell_volume();
+            std::printf("facade: lattice E0 %.15f (want %.15f), after relax %.12f, field-step norm %.15f\n",
+                        e_got, e_want, e_relaxed, n2);
+            if (std::abs(e_got - e_want) > 1e-12 * e_want || std::abs(e_relaxed - e_want) > 1e-6 * e_want ||
+                std::abs(n2 - 1.0) > 1e-12)
+                return 1;
+        }
         if (argc > 1) {
             std::ofstream out(argv[1], std::ios::binary);
             out.write(reinterpret_cast<const char*>(psi.data().data()),

@@ -24,6 +24,7 @@ CASES = [
     ((32, 32, 32), (-12.0,) * 3, (12.0,) * 3, 0.4, 0, 15),
     ((64, 32, 16), (-12, -8, -4), (12, 8, 4), 0.6, 1, 10),
     ((128, 64, 1), (-16, -12, 0), (16, 12, 1), 0.3, 2, 25),
+    ((64, 64, 32), (-12, -12, -8), (12, 12, 8), 0.8, 2, 1),
 ]
 
 
@@ -56,6 +57,28 @@ def test_magnetic_with_mask_matches_oracle(orc, tdse, prec):
     p.upload(psi0)
     p.step(25)
     assert rel(p.download(), want) < TOL[prec]
+
+
+@pytest.mark.parametrize("masked", [False, True])
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_magnetic_split_calls_match_oracle(orc, tdse, prec, masked):
+    # a field along z fuses the rotation's x shears into the core x sweeps
+    # and merges the shears of neighbouring steps inside one call; calls of
+    # 1, 6 and 13 steps must still give the oracle's 20 steps
+    dims, lo, hi = (32, 64, 16), (-12, -12, -6), (12, 12, 6)
+    v = orc.harmonic_3d(dims, lo, hi, 0.3, (0.5, 0, 0))
+    m = orc.absorbing_mask_3d(dims, lo, hi, 3.0) if masked else None
+    psi0 = orc.gaussian_3d(dims, lo, hi, (3.0, 0.0, 0.0), (1.5, 1.5, 1.5),
+                           (0.0, 0.8, 0.0)).reshape(dims[::-1])
+    print(f"case magnetic split prec={prec} masked={masked}", flush=True)
+    want = orc.magnetic_evolve(v, dims, lo, hi, 0.02, 0.9, 2, psi0, 20, mask=m)
+    p = tdse.Propagator(grid3(tdse, dims, lo, hi), v, 0.02, precision=prec, mask=m, bfield=0.9)
+    p.upload(psi0)
+    for n in (1, 6, 13):
+        p.step(n)
+    err = rel(p.download(), want)
+    print(f"  rel err {err:.3e}", flush=True)
+    assert err < TOL[prec]
 
 
 def test_device_build_carries_the_diamagnetic_term(orc, tdse):
