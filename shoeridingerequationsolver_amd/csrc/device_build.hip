@@ -88,7 +88,9 @@ build_cells(BuildArgs a) {
         }
         v[f] = static_cast<T>(pot);
         const double e0 = pot + a.onsite_shift;
-        phi[f] = static_cast<T>(a.imaginary ? -0.5 * e0 * a.dt : reduce_2pi_dev(-0.5 * e0 * a.dt));
+        double kick = a.imaginary ? -0.5 * e0 * a.dt : reduce_2pi_dev(-0.5 * e0 * a.dt);
+        if (a.tan_half) kick = tan(0.5 * kick);
+        phi[f] = static_cast<T>(kick);
         if (mask) {
             const double m = ramp(x, a.lo[0], a.hi[0], a.n[0], r.mask_width) *
                              ramp(y, a.lo[1], a.hi[1], a.n[1], r.mask_width) *

@@ -17,6 +17,7 @@ struct BuildArgs {
     double lo[3], hi[3], h[3];
     double dt;
     int imaginary;  // 1: phi = -V dt / 2 unreduced (imaginary time weights)
+    int tan_half;   // 1: phi stored as tan(phi / 2) (FP32 real-time kick sweeps)
     double bfield;  // uniform field: V += B^2/8 |r_perp|^2 (0: none)
     double onsite_shift;  // added to V in the kick phase only (lattice: 2 tx + 2 ty)
     int b_axis;

@@ -95,6 +95,13 @@ LaunchFn pick_mode(int mode) {
         case S_SHEAR: return &launch_sweep<T, N, S_SHEAR>;
         case C_SHEAR_KICK_FIRST: return &launch_sweep<T, N, C_SHEAR_KICK_FIRST>;
         case C_KICK_LAST_SHEAR: return &launch_sweep<T, N, C_KICK_LAST_SHEAR>;
+        case C_KICK_FIRST | kDrive: return &launch_sweep<T, N, C_KICK_FIRST | kDrive>;
+        case C_KICK_MID | kDrive: return &launch_sweep<T, N, C_KICK_MID | kDrive>;
+        case C_KICK_LAST | kDrive: return &launch_sweep<T, N, C_KICK_LAST | kDrive>;
+        case C_SINGLE | kDrive: return &launch_sweep<T, N, C_SINGLE | kDrive>;
+        case S_KICK_FIRST_TW | kDrive: return &launch_sweep<T, N, S_KICK_FIRST_TW | kDrive>;
+        case S_KICK_MID_TW | kDrive: return &launch_sweep<T, N, S_KICK_MID_TW | kDrive>;
+        case S_KICK_LAST | kDrive: return &launch_sweep<T, N, S_KICK_LAST | kDrive>;
         default: return nullptr;
     }
 }

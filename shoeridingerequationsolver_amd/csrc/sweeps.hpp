@@ -79,7 +79,19 @@ enum SweepMode : int {
     kNumModes = 26
 };
 
-constexpr bool mode_contiguous(int m) {
+// A kick mode with the dipole drive (driven plans): the mode | kDrive.  The
+// drive's extra exponential lives only in these instantiations, so the
+// static kick sweeps keep their register budget.
+constexpr int kDrive = 32;
+constexpr int base_mode(int m) { return m & ~kDrive; }
+constexpr bool mode_driven(int m) { return (m & kDrive) != 0; }
+constexpr bool mode_kicks(int m) {
+    return m == C_KICK_FIRST || m == C_KICK_MID || m == C_KICK_LAST || m == C_SINGLE ||
+           m == S_KICK_FIRST_TW || m == S_KICK_MID_TW || m == S_KICK_LAST;
+}
+
+constexpr bool mode_contiguous(int mode) {
+    const int m = base_mode(mode);
     return m <= C_SINGLE || m == C_FWD || (m >= C_RELAX_FIRST && m <= C_RELAX_SINGLE) ||
            m == C_OBS || m == C_SHEAR || m == C_SHEAR_KICK_FIRST || m == C_KICK_LAST_SHEAR;
 }
@@ -245,13 +257,40 @@ __device__ __forceinline__ double mask_value(T stored) {
 template <typename T>
 constexpr T kMaskOne = sizeof(T) == 8 ? T(1) : T(0);  // stored form of m = 1
 
+// exp(i f phi) from t = tan(phi / 2), which FP32 plans store in place of
+// phi: exp(i phi) = ((1 - t^2) + 2 i t) / (1 + t^2), of modulus 1 up to a few
+// FP64 roundings and with no sine or cosine (the reciprocal is the hardware
+// estimate refined by two Newton steps); f = 2 squares it.
+__device__ __forceinline__ cx<double> tan_rotation(double t, double f) {
+    const double t2 = t * t;
+    const double den = 1.0 + t2;
+    double r = __builtin_amdgcn_rcp(den);
+    r = fma(r, fma(-den, r, 1.0), r);
+    r = fma(r, fma(-den, r, 1.0), r);
+    const double c = (1.0 - t2) * r, s = (t + t) * r;
+    if (f == 2.0) return {fma(c, c, -s * s), (c + c) * s};
+    return {c, s};
+}
+
 // kick factor m * exp(i * (f * phi + extra)); f = 1 (half kick) or 2 (merged
-// full kick), extra = the drive's linear angle (0 without one)
+// full kick), extra = the drive's linear angle (0 without one).  FP32 plans
+// store tan(phi / 2) (tan_rotation), FP64 plans phi.
 template <typename T>
-__device__ __forceinline__ cx<double> kick_factor(T phi, double f, double m, double extra = 0.0) {
-    double s, c;
-    sincos_small(f * static_cast<double>(phi) + extra, &s, &c);
-    return {c * m, s * m};
+__device__ __forceinline__ cx<double> kick_factor(T stored, double f, double m, double extra,
+                                                  bool drive) {
+    if constexpr (sizeof(T) == 4) {
+        cx<double> k = tan_rotation(static_cast<double>(stored), f);
+        if (drive) {
+            double s, c;
+            sincos_small(extra, &s, &c);
+            k = {k.x * c - k.y * s, k.x * s + k.y * c};
+        }
+        return {k.x * m, k.y * m};
+    } else {
+        double s, c;
+        sincos_small(f * static_cast<double>(stored) + extra, &s, &c);
+        return {c * m, s * m};
+    }
 }
 
 // theta(t) = E0 cos(w t) dt / 2 of a dipole drive (core/src/drive.ixx:25-26)
@@ -337,7 +376,7 @@ struct Line {
     // Kick inputs in pattern R1 (position space after IFFT / before FFT):
     // all loads are issued together, so their latency is paid once per line.
     // (Issuing them before the inverse transform measured no faster.)
-    T kp[E];  // phase
+    T kp[E];  // kick phase (FP32 real-time plans: its half-angle tangent)
     T km[E];  // mask, stored form (see mask_value)
     __device__ __forceinline__ void load_kick(bool with_mask) {
         if (!valid) return;
@@ -353,17 +392,21 @@ struct Line {
     // this kick (0: none).  Waves with no angle at all skip the exponential
     // and multiply by the real m alone, rounding each component once.
     double du0 = 0.0, du1 = 0.0;
-    __device__ __forceinline__ void apply_kick(double f, bool use_mask, double g = 0.0) {
+    template <bool DRIVE>
+    __device__ __forceinline__ void kick_points(double f, bool use_mask, double g) {
+        for_each_point<N, E, R1>(t, [&](int s, int, int j, int ro) {
+            const double extra = DRIVE ? g * (du0 + du1 * (j + ro)) : 0.0;
+            v[s] = mul_d(v[s], kick_factor<T>(kp[s], f, use_mask ? mask_value(km[s]) : 1.0, extra,
+                                              DRIVE));
+        });
+    }
+    template <bool DRIVE>
+    __device__ __forceinline__ void apply_kick(double f, bool use_mask, double g) {
         if (!valid) return;
-        const bool drive = g != 0.0;
-        bool nz = drive;
+        bool nz = DRIVE;
         for (int s = 0; s < E; ++s) nz |= kp[s] != T(0);
         if (__any(nz)) {
-            for_each_point<N, E, R1>(t, [&](int s, int, int j, int ro) {
-                const double extra = drive ? g * (du0 + du1 * (j + ro)) : 0.0;
-                v[s] = mul_d(v[s], kick_factor<T>(kp[s], f, use_mask ? mask_value(km[s]) : 1.0,
-                                                  extra));
-            });
+            kick_points<DRIVE>(f, use_mask, g);
         } else if (use_mask && a.mask) {
             if constexpr (sizeof(T) == 4) {
                 // v (1 - c) = v - v c with one rounding per component
@@ -374,9 +417,10 @@ struct Line {
             }
         }
     }
-    __device__ __forceinline__ void kick(double f, bool with_mask, double g = 0.0) {
+    template <bool DRIVE>
+    __device__ __forceinline__ void kick(double f, bool with_mask, double g) {
         load_kick(with_mask);
-        apply_kick(f, with_mask, g);
+        apply_kick<DRIVE>(f, with_mask, g);
     }
     // Imaginary time.  kp holds phi = -V dtau / 2 (not reduced); the weight
     // exp(phi) is formed in FP64.
@@ -523,6 +567,8 @@ template <typename T, int N, int MODE, int V = 0>
 __device__ __forceinline__ void contig_tile(const SweepArgs& a, long long tile, cx<T>* lds,
                                             const cx<double>* tw, int tid, SweepAcc& ra) {
     using S = Shape<T, N, V>;
+    constexpr int BM = base_mode(MODE);
+    constexpr bool DRV = mode_driven(MODE);
     using LT = Line<T, N, ContigX<T>, V>;
     constexpr int TT = S::TT, LPB = S::LPB, E = S::E;
     const int l = tid / TT, t = tid % TT;
@@ -532,40 +578,37 @@ __device__ __forceinline__ void contig_tile(const SweepArgs& a, long long tile, 
     const Buf<T> M(a.mask, a.mask ? a.real_bytes : 0u);
     const long long line = tile * LPB + l;
     LT L{a, P, F, M, x, tw, t, static_cast<uint32_t>(line * N), 1u, line < a.nlines};
-    if constexpr (MODE == C_KICK_FIRST || MODE == C_KICK_MID || MODE == C_KICK_LAST ||
-                  MODE == C_SINGLE) {
-        if (a.dg != 0.0 || a.damp != 0.0) {
-            // x-line `line` of the grid: y index line % ny, z index line / ny
-            const int yj = a.dn1 > 0 ? static_cast<int>(line % a.dn1) : 0;
-            const int zk = a.dn1 > 0 ? static_cast<int>(line / a.dn1) : 0;
-            L.du0 = a.dax[0] * a.dlo[0] + a.dax[1] * (a.dlo[1] + a.dh[1] * yj) +
-                    a.dax[2] * (a.dlo[2] + a.dh[2] * zk);
-            L.du1 = a.dax[0] * a.dh[0];
-        }
+    if constexpr (DRV) {
+        // x-line `line` of the grid: y index line % ny, z index line / ny
+        const int yj = a.dn1 > 0 ? static_cast<int>(line % a.dn1) : 0;
+        const int zk = a.dn1 > 0 ? static_cast<int>(line / a.dn1) : 0;
+        L.du0 = a.dax[0] * a.dlo[0] + a.dax[1] * (a.dlo[1] + a.dh[1] * yj) +
+                a.dax[2] * (a.dlo[2] + a.dh[2] * zk);
+        L.du1 = a.dax[0] * a.dh[0];
     }
-    if constexpr (MODE == C_KICK_FIRST) {
+    if constexpr (BM == C_KICK_FIRST) {
         L.template load<LT::R1>();
-        L.kick(1.0, false, a.dg);
+        L.template kick<DRV>(1.0, false, a.dg);
         L.fft();
         L.template store<LT::RL>();
-    } else if constexpr (MODE == C_KICK_MID) {
+    } else if constexpr (BM == C_KICK_MID) {
         L.template load<LT::RL>();
         L.ifft();
         L.load_kick(true);
-        L.apply_kick(2.0, true, a.dg);
+        L.template apply_kick<DRV>(2.0, true, a.dg);
         L.fft();
         L.template store<LT::RL>();
-    } else if constexpr (MODE == C_KICK_LAST) {
+    } else if constexpr (BM == C_KICK_LAST) {
         L.template load<LT::RL>();
         L.ifft();
         L.load_kick(true);
-        L.apply_kick(1.0, true, a.dg);
+        L.template apply_kick<DRV>(1.0, true, a.dg);
         L.template store<LT::R1>();
-    } else if constexpr (MODE == C_FWD) {
+    } else if constexpr (BM == C_FWD) {
         L.template load<LT::R1>();
         L.fft();
         L.template store<LT::RL>();
-    } else if constexpr (MODE == C_KIN_TW) {
+    } else if constexpr (BM == C_KIN_TW) {
         // row k2 = line; after the row FFT position k1 holds bin k2 + N2*k1
         const cx<double>* kt = static_cast<const cx<double>*>(a.kt[2]);
         L.template load<LT::R1>();
@@ -579,14 +622,14 @@ __device__ __forceinline__ void contig_tile(const SweepArgs& a, long long tile, 
             L.v[s] = mul_d(L.v[s], step_twiddle<+1>(i, line, a.n_total));
         });
         L.template store<LT::R1>();
-    } else if constexpr (MODE == C_SINGLE) {
+    } else if constexpr (BM == C_SINGLE) {
         const cx<double>* kt = static_cast<const cx<double>*>(a.kt[2]);
         // the drive's merged angle between steps s and s + 1 is twice theta
         // at their common time (core/src/drive.ixx:52-66)
-        const bool drive = a.damp != 0.0;
+        constexpr bool drive = DRV;
         L.template load<LT::R1>();
         L.load_kick(true);  // phases and mask stay in registers for every step
-        L.apply_kick(1.0, false, drive ? -drive_theta(a, a.dt0) : 0.0);
+        L.template apply_kick<DRV>(1.0, false, drive ? -drive_theta(a, a.dt0) : 0.0);
         for (int step = 0; step < a.nsteps; ++step) {
             L.fft();
             for_each_point<N, E, LT::RL>(t, [&](int s, int i, int, int) {
@@ -595,14 +638,14 @@ __device__ __forceinline__ void contig_tile(const SweepArgs& a, long long tile, 
             L.ifft();
             const bool last = step + 1 == a.nsteps;
             const double th = drive ? drive_theta(a, a.dt0 + (step + 1) * a.ddt) : 0.0;
-            L.apply_kick(last ? 1.0 : 2.0, true, last ? -th : -2.0 * th);
+            L.template apply_kick<DRV>(last ? 1.0 : 2.0, true, last ? -th : -2.0 * th);
         }
         L.template store<LT::R1>();
-    } else if constexpr (MODE == C_OBS) {
+    } else if constexpr (BM == C_OBS) {
         L.template load<LT::R1>();
         L.fft();
         L.observe_spectrum(ra);
-    } else if constexpr (MODE == C_SHEAR) {
+    } else if constexpr (BM == C_SHEAR) {
         const long long idx = a.sh_sel ? line / a.dn1 : line % a.dn1;
         L.template load<LT::R1>();
         L.fft();
@@ -610,43 +653,43 @@ __device__ __forceinline__ void contig_tile(const SweepArgs& a, long long tile, 
         L.ifft();
         L.mask_only();
         L.template store<LT::R1>();
-    } else if constexpr (MODE == C_SHEAR_KICK_FIRST) {
+    } else if constexpr (BM == C_SHEAR_KICK_FIRST) {
         const long long idx = a.sh_sel ? line / a.dn1 : line % a.dn1;
         L.template load<LT::R1>();
         L.fft();
         L.shear_phase(a.sh_d0 + a.sh_d1 * static_cast<double>(idx));
         L.ifft();
-        L.kick(1.0, false, 0.0);
+        L.template kick<DRV>(1.0, false, 0.0);
         L.fft();
         L.template store<LT::RL>();
-    } else if constexpr (MODE == C_KICK_LAST_SHEAR) {
+    } else if constexpr (BM == C_KICK_LAST_SHEAR) {
         const long long idx = a.sh_sel ? line / a.dn1 : line % a.dn1;
         L.template load<LT::RL>();
         L.ifft();
         L.load_kick(false);
-        L.apply_kick(1.0, false, 0.0);
+        L.template apply_kick<DRV>(1.0, false, 0.0);
         L.fft();
         L.shear_phase(a.sh_d0 + a.sh_d1 * static_cast<double>(idx));
         L.ifft();
         L.mask_only();
         L.template store<LT::R1>();
-    } else if constexpr (MODE == C_RELAX_FIRST) {
+    } else if constexpr (BM == C_RELAX_FIRST) {
         L.template load<LT::R1>();
         L.relax_first();
         L.fft();
         L.template store<LT::RL>();
-    } else if constexpr (MODE == C_RELAX_MID || MODE == C_RELAX_LAST) {
+    } else if constexpr (BM == C_RELAX_MID || BM == C_RELAX_LAST) {
         const double c = a.relax->scale[a.parity];
         L.template load<LT::RL>();
         L.ifft();
-        L.relax_measure(c, MODE == C_RELAX_MID, ra);
-        if constexpr (MODE == C_RELAX_MID) {
+        L.relax_measure(c, BM == C_RELAX_MID, ra);
+        if constexpr (BM == C_RELAX_MID) {
             L.fft();
             L.template store<LT::RL>();
         } else {
             L.template store<LT::R1>();
         }
-    } else if constexpr (MODE == C_RELAX_SINGLE) {
+    } else if constexpr (BM == C_RELAX_SINGLE) {
         // the whole line is in this block: each step is renormalised exactly
         // (core/src/imaginary_time.ixx:43-53)
         const cx<double>* kt = static_cast<const cx<double>*>(a.kt[2]);
@@ -696,6 +739,8 @@ __device__ __forceinline__ void strided_tile(const SweepArgs& a, long long tile,
                                              const cx<double>* tw, const cx<double>* kp, int tid,
                                              SweepAcc& ra) {
     using S = Shape<T, N, V>;
+    constexpr int BM = base_mode(MODE);
+    constexpr bool DRV = mode_driven(MODE);
     constexpr int TX = S::TX, E = S::E;
     using LT = Line<T, N, StridedX<T, TX>, V>;
     const int xi = tid % TX, t = tid / TX;
@@ -708,22 +753,20 @@ __device__ __forceinline__ void strided_tile(const SweepArgs& a, long long tile,
     const int xx = static_cast<int>(tile % xtiles) * TX + xi;
     LT L{a, P, F, M, x, tw, t, static_cast<uint32_t>(xx) + a.wstride * w, a.estride,
          xx < a.nx};
-    if constexpr (MODE == S_KICK_FIRST_TW || MODE == S_KICK_MID_TW || MODE == S_KICK_LAST) {
-        if (a.dg != 0.0) {
-            // four-step column xx: point p of the line is x index xx + n1 p
-            L.du0 = a.dax[0] * (a.dlo[0] + a.dh[0] * xx) + a.dax[1] * a.dlo[1] + a.dax[2] * a.dlo[2];
-            L.du1 = a.dax[0] * a.dh[0] * a.dn1;
-        }
+    if constexpr (DRV) {
+        // four-step column xx: point p of the line is x index xx + n1 p
+        L.du0 = a.dax[0] * (a.dlo[0] + a.dh[0] * xx) + a.dax[1] * a.dlo[1] + a.dax[2] * a.dlo[2];
+        L.du1 = a.dax[0] * a.dh[0] * a.dn1;
     }
-    if constexpr (MODE == S_FWD) {
+    if constexpr (BM == S_FWD) {
         L.template load<LT::R1>();
         L.fft();
         L.template store<LT::RL>();
-    } else if constexpr (MODE == S_INV) {
+    } else if constexpr (BM == S_INV) {
         L.template load<LT::RL>();
         L.ifft();
         L.template store<LT::R1>();
-    } else if constexpr (MODE == S_KIN) {
+    } else if constexpr (BM == S_KIN) {
         cx<double> kxw = {1.0, 0.0};
         if (a.kt[0] && L.valid) kxw = static_cast<const cx<double>*>(a.kt[0])[xx];
         if (a.kt[1]) kxw = cmul(kxw, static_cast<const cx<double>*>(a.kt[1])[w]);
@@ -734,36 +777,36 @@ __device__ __forceinline__ void strided_tile(const SweepArgs& a, long long tile,
         });
         L.ifft();
         L.template store<LT::R1>();
-    } else if constexpr (MODE == S_KICK_FIRST_TW) {
+    } else if constexpr (BM == S_KICK_FIRST_TW) {
         L.template load<LT::R1>();
-        L.kick(1.0, false, a.dg);
+        L.template kick<DRV>(1.0, false, a.dg);
         L.fft();
         L.twiddle_out(xx);
         L.template store<LT::RL>();
-    } else if constexpr (MODE == S_KICK_MID_TW) {
+    } else if constexpr (BM == S_KICK_MID_TW) {
         L.template load<LT::RL>();
         L.ifft();
         L.load_kick(true);
-        L.apply_kick(2.0, true, a.dg);
+        L.template apply_kick<DRV>(2.0, true, a.dg);
         L.fft();
         L.twiddle_out(xx);
         L.template store<LT::RL>();
-    } else if constexpr (MODE == S_KICK_LAST) {
+    } else if constexpr (BM == S_KICK_LAST) {
         L.template load<LT::RL>();
         L.ifft();
         L.load_kick(true);
-        L.apply_kick(1.0, true, a.dg);
+        L.template apply_kick<DRV>(1.0, true, a.dg);
         L.template store<LT::R1>();
-    } else if constexpr (MODE == S_FWD_TW) {
+    } else if constexpr (BM == S_FWD_TW) {
         L.template load<LT::R1>();
         L.fft();
         L.twiddle_out(xx);
         L.template store<LT::RL>();
-    } else if constexpr (MODE == S_OBS) {
+    } else if constexpr (BM == S_OBS) {
         L.template load<LT::R1>();
         L.fft();
         L.observe_spectrum(ra);
-    } else if constexpr (MODE == S_SHEAR) {
+    } else if constexpr (BM == S_SHEAR) {
         const int idx = a.sh_sel ? w : xx;
         L.template load<LT::R1>();
         L.fft();
@@ -771,18 +814,18 @@ __device__ __forceinline__ void strided_tile(const SweepArgs& a, long long tile,
         L.ifft();
         L.mask_only();
         L.template store<LT::R1>();
-    } else if constexpr (MODE == S_RELAX_FIRST_TW) {
+    } else if constexpr (BM == S_RELAX_FIRST_TW) {
         L.template load<LT::R1>();
         L.relax_first();
         L.fft();
         L.twiddle_out(xx);
         L.template store<LT::RL>();
-    } else if constexpr (MODE == S_RELAX_MID_TW || MODE == S_RELAX_LAST) {
+    } else if constexpr (BM == S_RELAX_MID_TW || BM == S_RELAX_LAST) {
         const double c = a.relax->scale[a.parity];
         L.template load<LT::RL>();
         L.ifft();
-        L.relax_measure(c, MODE == S_RELAX_MID_TW, ra);
-        if constexpr (MODE == S_RELAX_MID_TW) {
+        L.relax_measure(c, BM == S_RELAX_MID_TW, ra);
+        if constexpr (BM == S_RELAX_MID_TW) {
             L.fft();
             L.twiddle_out(xx);
             L.template store<LT::RL>();

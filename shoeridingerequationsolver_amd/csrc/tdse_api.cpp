@@ -135,6 +135,10 @@ struct tdse_plan {
     bool imag() const { return (flags & TDSE_FLAG_IMAGINARY_TIME) != 0; }
     // imaginary time: device slots inside the reduction buffer (after red_out)
     tdse::RelaxSlots* relax() const { return reinterpret_cast<tdse::RelaxSlots*>(red_out + 16); }
+    // FP32 real-time sweep plans store the kick phase phi as tan(phi / 2):
+    // the kernels form exp(i phi) = (1 - t^2 + 2 i t) / (1 + t^2) without a
+    // sine or cosine (the lattice keeps phi)
+    bool tan_kick() const { return !fp64() && !imag() && kind != kKindLattice; }
 };
 
 namespace {
@@ -229,17 +233,19 @@ int d2h(const tdse_plan* p, void* dst, const void* src, size_t bytes) {
     return TDSE_OK;
 }
 
-enum RealForm { kPlain, kPhase, kMask };
+enum RealForm { kPlain, kPhase, kTanHalf, kMask };
 
 // A per-cell real array in the plan's precision: plain values, kick phases
-// reduced to [-pi, pi], or an absorbing mask (stored as 1 - m in FP32 plans,
-// m in FP64 plans; the kernels undo it in FP64).
+// reduced to [-pi, pi], the tangent of half such a phase, or an absorbing
+// mask (stored as 1 - m in FP32 plans, m in FP64 plans; the kernels undo it
+// in FP64).
 template <typename T>
 std::vector<T> to_real(const double* src, long long n, RealForm form) {
     std::vector<T> out(static_cast<size_t>(n));
     for (long long i = 0; i < n; ++i) {
         double x = src[i];
         if (form == kPhase) x = reduce_2pi_host(x);
+        else if (form == kTanHalf) x = std::tan(0.5 * reduce_2pi_host(x));
         else if (form == kMask && sizeof(T) == 4) x = 1.0 - x;
         out[i] = static_cast<T>(x);
     }
@@ -574,7 +580,13 @@ std::vector<Launch> schedule(const tdse_plan* p, int nsteps, void* target = null
                 w.args.dg = -2.0 * theta(t0 + (step + 1) * p->dt);
             }
         }
-        L.push_back({w.mode, w.n, w.args, launcher(p, w.n, w.mode, p->variant[slot]), slot});
+        if (p->drive && tdse::mode_kicks(w.mode)) {
+            // the driven kick sweeps are compiled once (no variants)
+            w.mode |= tdse::kDrive;
+            L.push_back({w.mode, w.n, w.args, launcher(p, w.n, w.mode), slot});
+        } else {
+            L.push_back({w.mode, w.n, w.args, launcher(p, w.n, w.mode, p->variant[slot]), slot});
+        }
         // a measuring sweep leaves per-block partials: its norm and the next
         // scale come from one small kernel right after it (timed with its slot)
         if (tdse::mode_measures(w.mode)) L.push_back({kFinishMode, w.n, w.args, &tdse::launch_relax_finish, slot});
@@ -1304,7 +1316,8 @@ int tdse_plan_set_potential(tdse_plan* p, const double* v) {
     // real time: reduced modulo 2 pi in FP64 for both precisions (the
     // kernels' sin/cos take |angle| <= 8); imaginary time: the exponent of
     // the weight exp(-V dtau / 2) (core/src/imaginary_time.ixx:30-33, 75-78)
-    if ((rc = upload_real(p, p->phi, th.data(), p->imag() ? kPlain : kPhase))) return rc;
+    if ((rc = upload_real(p, p->phi, th.data(), p->imag() ? kPlain : p->tan_kick() ? kTanHalf : kPhase)))
+        return rc;
     if ((rc = upload_real(p, p->v, v, kPlain))) return rc;
     if (p->base_half) {
         std::vector<double> h(2 * static_cast<size_t>(p->total));
@@ -1363,6 +1376,7 @@ int tdse_plan_build(tdse_plan* p, const tdse_recipe* r) {
     }
     b.dt = p->dt;
     b.imaginary = p->imag() ? 1 : 0;
+    b.tan_half = p->tan_kick() ? 1 : 0;
     b.bfield = p->bfield;
     b.b_axis = p->b_axis;
     b.onsite_shift = p->kind == kKindLattice ? 2.0 * p->hop(0) + 2.0 * p->hop(1) : 0.0;
