@@ -345,15 +345,21 @@ def main():
         log(f"fp64 unfused baseline: {bsteps / gpu_s:.1f} steps/s ({1e3 * gpu_s / bsteps:.4f} ms/step)")
         pb.close()
 
-    # roofline of the dominant sweep (the x sweep: IFFT, kick*mask, FFT)
+    # roofline of the dominant sweep (most time per step among the interior
+    # x sweep -- IFFT, kick*mask, FFT -- and the z sweep -- FFT, K, IFFT)
     p32 = props["fp32"]
     prof = p32.profile(max(8, min(args.steps, 200)))
     log("per-sweep ms: " + ", ".join(f"{k}={ms / max(c, 1):.4f}x{c}" for k, (ms, c) in prof.items()))
     N = solve.points
     csz, rsz = 8, 4
-    sweep_bytes = {"x_mid": N * (2 * csz + 2 * rsz), "y_fwd": N * 2 * csz, "z_kin": N * 2 * csz,
-                   "y_inv": N * 2 * csz}
-    dom = "x_mid"
+    # the mask array the x sweep reads: none when the plan applies the
+    # absorbing mask from its per-axis ramps (step bytes tell)
+    mask_bytes = p32.step_bytes() - N * (4 * 2 * csz + rsz)
+    sweep_bytes = {"x_mid": N * (2 * csz + rsz) + mask_bytes, "y_fwd": N * 2 * csz,
+                   "z_kin": N * 2 * csz, "y_inv": N * 2 * csz}
+    names = {"x_mid": "contig_sweep<float,256,C_KICK_MID,{}> (x sweep)",
+             "z_kin": "strided_sweep<float,256,S_KIN,{}> (z sweep)"}
+    dom = max(names, key=lambda k: prof[k][0] / max(prof[k][1], 1))
     ms, cnt = prof[dom]
     avg_s = ms / cnt / 1e3
     achieved = sweep_bytes[dom] / avg_s / 1e9
@@ -362,7 +368,7 @@ def main():
         with open(TRAFFIC_FILE) as f:
             traffic = json.load(f).get(dom)
     xv = p32.variants.get(dom, 0)
-    roofline = {"bound": "hbm", "kernel": f"contig_sweep<float,256,C_KICK_MID,{xv}> (x sweep)",
+    roofline = {"bound": "hbm", "kernel": names[dom].format(xv), "mask_array_bytes": mask_bytes,
                 "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "bytes_per_launch": sweep_bytes[dom], "avg_launch_ms": round(avg_s * 1e3, 5),

@@ -167,6 +167,14 @@ struct SweepArgs {
     // line; 1: z of an x line / the outer index w of a strided line)
     double sh_d0, sh_d1;
     int sh_sel;
+    // absorbing mask as the product of per-axis ramps, (mx * my) * mz, the
+    // way absorbing_mask builds it (core/src/potential.ixx:217-233): with
+    // mask_sep, `mask` is the ramp along the line (stored form, one entry per
+    // point of the line) and mask_yz the ramps of the line's other two axes
+    // (dn1 entries, then the rest); the x sweep then reads no mask array
+    const double* mask_yz;
+    uint32_t mask_bytes;  // byte extent of mask
+    int mask_sep;
 };
 
 // ---- buffer access -----------------------------------------------------------
@@ -384,7 +392,7 @@ struct Line {
         for_each_point<N, E, R1>(t, [&](int s, int, int j, int ro) {
             const uint32_t vo = (base + j * es) * RS, so = ro * es * RS;
             kp[s] = F.ld(vo, so);
-            km[s] = m ? M.ld(vo, so) : kMaskOne<T>;
+            km[s] = m ? M.ld(a.mask_sep ? j * es * RS : vo, so) : kMaskOne<T>;
         });
     }
     // v *= m exp(i (f phi + g u)), the exponential in FP64; u = axis . r of
@@ -392,12 +400,14 @@ struct Line {
     // this kick (0: none).  Waves with no angle at all skip the exponential
     // and multiply by the real m alone, rounding each component once.
     double du0 = 0.0, du1 = 0.0;
+    // separable mask: the ramps of the line's other two axes (1 otherwise)
+    double my = 1.0, mz = 1.0;
     template <bool DRIVE>
     __device__ __forceinline__ void kick_points(double f, bool use_mask, double g) {
         for_each_point<N, E, R1>(t, [&](int s, int, int j, int ro) {
             const double extra = DRIVE ? g * (du0 + du1 * (j + ro)) : 0.0;
-            v[s] = mul_d(v[s], kick_factor<T>(kp[s], f, use_mask ? mask_value(km[s]) : 1.0, extra,
-                                              DRIVE));
+            const double m = use_mask ? (mask_value(km[s]) * my) * mz : 1.0;
+            v[s] = mul_d(v[s], kick_factor<T>(kp[s], f, m, extra, DRIVE));
         });
     }
     template <bool DRIVE>
@@ -407,6 +417,12 @@ struct Line {
         for (int s = 0; s < E; ++s) nz |= kp[s] != T(0);
         if (__any(nz)) {
             kick_points<DRIVE>(f, use_mask, g);
+        } else if (use_mask && a.mask && a.mask_sep) {
+            for (int s = 0; s < E; ++s) {
+                const double m = (mask_value(km[s]) * my) * mz;
+                v[s] = {static_cast<T>(m * static_cast<double>(v[s].x)),
+                        static_cast<T>(m * static_cast<double>(v[s].y))};
+            }
         } else if (use_mask && a.mask) {
             if constexpr (sizeof(T) == 4) {
                 // v (1 - c) = v - v c with one rounding per component
@@ -575,9 +591,13 @@ __device__ __forceinline__ void contig_tile(const SweepArgs& a, long long tile, 
     ContigX<T> x{lds + l * S::LSTR};
     const Buf<cx<T>> P(a.psi, a.psi_bytes);
     const Buf<T> F(a.phi, a.real_bytes);
-    const Buf<T> M(a.mask, a.mask ? a.real_bytes : 0u);
+    const Buf<T> M(a.mask, a.mask ? a.mask_bytes : 0u);
     const long long line = tile * LPB + l;
     LT L{a, P, F, M, x, tw, t, static_cast<uint32_t>(line * N), 1u, line < a.nlines};
+    if (L.valid && a.mask && a.mask_sep) {
+        L.my = a.mask_yz[line % a.dn1];
+        L.mz = a.mask_yz[a.dn1 + line / a.dn1];
+    }
     if constexpr (DRV) {
         // x-line `line` of the grid: y index line % ny, z index line / ny
         const int yj = a.dn1 > 0 ? static_cast<int>(line % a.dn1) : 0;
@@ -747,7 +767,7 @@ __device__ __forceinline__ void strided_tile(const SweepArgs& a, long long tile,
     StridedX<T, TX> x{lds + xi};
     const Buf<cx<T>> P(a.psi, a.psi_bytes);
     const Buf<T> F(a.phi, a.real_bytes);
-    const Buf<T> M(a.mask, a.mask ? a.real_bytes : 0u);
+    const Buf<T> M(a.mask, a.mask ? a.mask_bytes : 0u);
     const int xtiles = (a.nx + TX - 1) / TX;
     const int w = static_cast<int>(tile / xtiles);
     const int xx = static_cast<int>(tile % xtiles) * TX + xi;

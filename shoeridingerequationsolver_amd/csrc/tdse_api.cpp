@@ -97,6 +97,11 @@ struct tdse_plan {
     void* phi = nullptr;
     void* v = nullptr;
     void* mask = nullptr;
+    // a mask that is a product of per-axis ramps (the reference's absorbing
+    // mask): the x ramp in stored form and the y, z ramps in FP64, read by
+    // the x sweeps instead of the full array (which the other sweeps keep)
+    void* mask_x = nullptr;
+    double* mask_yz = nullptr;
     void* scratch = nullptr;  // forward transform for <T>
     double* partials = nullptr;
     double* red_out = nullptr;
@@ -354,6 +359,65 @@ int make_baseline_kinetic(tdse_plan* p) {
     return h2d(p, p->base_kin, t.data(), t.size() * 8);
 }
 
+void release_mask_axes(tdse_plan* p) {
+    release(p, p->mask_x, static_cast<uint64_t>(p->n[0]) * p->rsize());
+    void* yz = p->mask_yz;
+    release(p, yz, static_cast<uint64_t>(p->n[1] + p->n[2]) * 8);
+    p->mask_yz = nullptr;
+}
+
+// Can the x sweeps use per-axis ramps?  3D and 2D sweep plans whose x lines
+// run along the first axis, no magnetic field (its shears take the full
+// array).
+bool mask_axes_apply(const tdse_plan* p) {
+    return (p->kind == kKind3D || (p->kind == kKind2D && p->n[2] == 1)) && p->n[0] > 1 &&
+           p->bfield == 0.0 && !(p->flags & TDSE_FLAG_UNFUSED_BASELINE);
+}
+
+// A mask equal, bit for bit, to (mx[i] * my[j]) * mz[k] (absorbing_mask,
+// core/src/potential.ixx:217-233): the ramps, read off the lines through the
+// centre cell (which must be 1), then every cell checked.
+bool separable_mask(const tdse_plan* p, const double* m, std::vector<double>* rx,
+                    std::vector<double>* ryz) {
+    const int n0 = p->n[0], n1 = p->n[1], n2 = p->n[2];
+    auto at = [&](int i, int j, int k) {
+        return m[i + static_cast<size_t>(n0) * (j + static_cast<size_t>(n1) * k)];
+    };
+    const int ic = n0 / 2, jc = n1 / 2, kc = n2 / 2;
+    if (at(ic, jc, kc) != 1.0) return false;
+    rx->assign(static_cast<size_t>(n0), 0.0);
+    ryz->assign(static_cast<size_t>(n1 + n2), 0.0);
+    for (int i = 0; i < n0; ++i) (*rx)[i] = at(i, jc, kc);
+    for (int j = 0; j < n1; ++j) (*ryz)[j] = at(ic, j, kc);
+    for (int k = 0; k < n2; ++k) (*ryz)[n1 + k] = at(ic, jc, k);
+    for (int k = 0; k < n2; ++k)
+        for (int j = 0; j < n1; ++j) {
+            const double myz_j = (*ryz)[j], mz_k = (*ryz)[n1 + k];
+            for (int i = 0; i < n0; ++i)
+                if (at(i, j, k) != ((*rx)[i] * myz_j) * mz_k) return false;
+        }
+    return true;
+}
+
+// Upload the ramps (x in the plan's stored mask form).
+int upload_mask_axes(tdse_plan* p, const std::vector<double>& rx, const std::vector<double>& ryz) {
+    int rc;
+    if (!p->mask_x && (rc = alloc(p, &p->mask_x, static_cast<uint64_t>(p->n[0]) * p->rsize())))
+        return rc;
+    if (!p->mask_yz) {
+        void* yz = nullptr;
+        if ((rc = alloc(p, &yz, static_cast<uint64_t>(p->n[1] + p->n[2]) * 8))) return rc;
+        p->mask_yz = static_cast<double*>(yz);
+    }
+    if (p->fp64()) {
+        if ((rc = h2d(p, p->mask_x, rx.data(), rx.size() * 8))) return rc;
+    } else {
+        std::vector<float> f = to_real<float>(rx.data(), static_cast<long long>(rx.size()), kMask);
+        if ((rc = h2d(p, p->mask_x, f.data(), f.size() * 4))) return rc;
+    }
+    return h2d(p, p->mask_yz, ryz.data(), ryz.size() * 8);
+}
+
 SweepArgs base_args(const tdse_plan* p) {
     SweepArgs a{};
     a.psi = p->psi;
@@ -362,6 +426,7 @@ SweepArgs base_args(const tdse_plan* p) {
     a.scale = 1.0 / static_cast<double>(p->total);
     a.psi_bytes = static_cast<uint32_t>(p->total * p->csize());
     a.real_bytes = static_cast<uint32_t>(p->total * p->rsize());
+    a.mask_bytes = a.real_bytes;
     for (int i = 0; i < 3; ++i) a.kt[i] = nullptr;
     if (p->imag()) {
         a.relax = p->relax();
@@ -415,6 +480,15 @@ struct Geometry {
     SweepArgs single;           // 1D in one block
 };
 
+// x sweeps of a plan with per-axis mask ramps read them instead of the array
+void use_mask_axes(const tdse_plan* p, SweepArgs* x) {
+    if (!p->mask || !p->mask_x || !mask_axes_apply(p)) return;
+    x->mask = p->mask_x;
+    x->mask_yz = p->mask_yz;
+    x->mask_bytes = static_cast<uint32_t>(p->n[0] * p->rsize());
+    x->mask_sep = 1;
+}
+
 Geometry geometry(const tdse_plan* p, void* target) {
     Geometry g{};
     SweepArgs a = base_args(p);
@@ -440,6 +514,7 @@ Geometry geometry(const tdse_plan* p, void* target) {
         g.z.kt[0] = p->kt[0];
         g.z.kt[1] = p->kt[1];
         g.z.kt[2] = p->kt[2];
+        use_mask_axes(p, &g.x);
     } else if (p->kind == kKind2D) {
         const int n0 = p->sq[0].n, n1 = p->sq[1].n;
         g.x = a;
@@ -454,6 +529,7 @@ Geometry geometry(const tdse_plan* p, void* target) {
         g.yf.kt[0] = p->kt[0];
         g.yf.kt[2] = p->kt[1];
         g.yi = g.yf;
+        use_mask_axes(p, &g.x);
     } else if (p->kind == kKindSingle) {
         g.single = a;
         g.single.nlines = 1;
@@ -1267,6 +1343,7 @@ int tdse_plan_destroy(tdse_plan* p) {
     release(p, p->phi, p->total * rs);
     release(p, p->v, p->total * rs);
     release(p, p->mask, p->total * rs);
+    release_mask_axes(p);
     release(p, p->scratch, p->total * cs);
     release(p, p->base_half, p->total * 16);
     release(p, p->base_kin, p->total * 16);
@@ -1337,12 +1414,16 @@ int tdse_plan_set_mask(tdse_plan* p, const double* mask) {
     if (mask && p->imag()) return fail(TDSE_E_INVALID, "imaginary-time plans take no mask");
     if ((rc = set_device(p->ctx))) return rc;
     TDSE_HIP(hipStreamSynchronize(p->stream));
+    release_mask_axes(p);
     if (!mask) {
         release(p, p->mask, p->total * p->rsize());
         return TDSE_OK;
     }
     if (!p->mask && (rc = alloc(p, &p->mask, p->total * p->rsize()))) return rc;
-    return upload_real(p, p->mask, mask, kMask);
+    if ((rc = upload_real(p, p->mask, mask, kMask))) return rc;
+    std::vector<double> rx, ryz;
+    if (mask_axes_apply(p) && separable_mask(p, mask, &rx, &ryz)) return upload_mask_axes(p, rx, ryz);
+    return TDSE_OK;
 }
 
 int tdse_plan_build(tdse_plan* p, const tdse_recipe* r) {
@@ -1357,8 +1438,22 @@ int tdse_plan_build(tdse_plan* p, const tdse_recipe* r) {
         if (!(r->sigma[a] > 0.0)) return fail(TDSE_E_INVALID, "sigma must be positive");
     if ((rc = set_device(p->ctx))) return rc;
     TDSE_HIP(hipStreamSynchronize(p->stream));
+    release_mask_axes(p);
     if (r->mask_width > 0.0) {
         if (!p->mask && (rc = alloc(p, &p->mask, p->total * p->rsize()))) return rc;
+        if (mask_axes_apply(p)) {
+            // the ramps of absorbing_mask (core/src/potential.ixx:194-215)
+            // from the host builder, one axis at a time
+            std::vector<double> rx(static_cast<size_t>(p->n[0])), ryz(static_cast<size_t>(p->n[1] + p->n[2]));
+            for (int a = 0; a < 3; ++a) {
+                const int n1d[3] = {p->n[a], 1, 1};
+                const double lo1d[3] = {p->lo[a], 0.0, 0.0}, hi1d[3] = {p->hi[a], 1.0, 1.0};
+                double* dst = a == 0 ? rx.data() : ryz.data() + (a == 1 ? 0 : p->n[1]);
+                if (tdse_build_absorbing_mask(n1d, lo1d, hi1d, r->mask_width, dst) != TDSE_OK)
+                    return fail(TDSE_E_INVALID, "bad mask width");
+            }
+            if ((rc = upload_mask_axes(p, rx, ryz))) return rc;
+        }
     } else {
         release(p, p->mask, p->total * p->rsize());
     }
@@ -2089,7 +2184,9 @@ int tdse_plan_step_bytes(const tdse_plan* p, uint64_t* out) {
         *out = N * (3 * 48 + 2 * axes * 32 + 2 * 32 + (p->mask ? 40 : 0));
         return TDSE_OK;
     }
-    *out = N * (sweeps * 2 * p->csize() + p->rsize() + (p->mask ? p->rsize() : 0));
+    // the x sweeps of a plan with mask ramps read no mask array
+    const bool mask_array = p->mask && !(p->mask_x && mask_axes_apply(p));
+    *out = N * (sweeps * 2 * p->csize() + p->rsize() + (mask_array ? p->rsize() : 0));
     return TDSE_OK;
 }
 
