@@ -342,6 +342,10 @@ __device__ __forceinline__ void pass_compute(cx<T> (&v)[E], int t,
             for (int r = 1; r < R; ++r) {
                 v[b * R + r] = mul_d(v[b * R + r], w);
                 if (r + 1 < R) w = cmul(w, w1);
+                // apply each twiddle as it is formed: left free, the
+                // scheduler forms all R - 1 first (FP64, 4 VGPRs each) and
+                // the strided two-transform sweeps drop to 2 waves per SIMD
+                __builtin_amdgcn_sched_barrier(0);
             }
         }
         Dft<R, DIR>::run(&v[b * R]);

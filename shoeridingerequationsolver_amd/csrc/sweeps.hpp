@@ -156,9 +156,8 @@ struct SweepArgs {
     double dk;            // observing sweeps: 2 pi / L of the line axis
     // dipole drive, a time-dependent linear kick: angle += dg * (axis . r)
     double dg;            // minus the drive angles theta merged into this kick (0: none)
-    double dax[3];        // polarisation axis
-    double dlo[3];        // grid origin per original axis
-    double dh[3];         // grid spacing per original axis
+    // axis . r of grid point (i, j, k) = du_c + du_x i + du_y j + du_z k
+    double du_c, du_x, du_y, du_z;
     int dn1;              // contiguous: lines per z index (ny); four-step: row length n1
     // C_SINGLE: theta(t) = damp cos(domega t) ddt / 2 with t = dt0 + s ddt
     double damp, domega, dt0, ddt;
@@ -404,11 +403,37 @@ struct Line {
     double my = 1.0, mz = 1.0;
     template <bool DRIVE>
     __device__ __forceinline__ void kick_points(double f, bool use_mask, double g) {
-        for_each_point<N, E, R1>(t, [&](int s, int, int j, int ro) {
-            const double extra = DRIVE ? g * (du0 + du1 * (j + ro)) : 0.0;
-            const double m = use_mask ? (mask_value(km[s]) * my) * mz : 1.0;
-            v[s] = mul_d(v[s], kick_factor<T>(kp[s], f, m, extra, DRIVE));
-        });
+        if constexpr (DRIVE && sizeof(T) == 4) {
+            // the static kick, then the drive's exp(i g u) along the line by
+            // recurrence: the points of butterfly b sit at j_b + r N / R1
+            // (r < R1), so exp(i g u) = exp(i g (du0 + du1 j_b))
+            // exp(i g du1 N / R1)^r (FP64, at most 32 products: relative
+            // error below 1e-14).  Two passes keep the registers of one.
+            kick_points<false>(f, use_mask, 0.0);
+            __builtin_amdgcn_sched_barrier(0);
+            constexpr int TT = N / E;
+            double ss, cs;
+            sincos_small(g * du1 * static_cast<double>(N / R1), &ss, &cs);
+#pragma unroll
+            for (int b = 0; b < E / R1; ++b) {
+                const int j = t + b * TT;
+                double s0, c0;
+                sincos_small(g * (du0 + du1 * j), &s0, &c0);
+                cx<double> w = {c0, s0};
+#pragma unroll
+                for (int r = 0; r < R1; ++r) {
+                    const int s = b * R1 + r;
+                    v[s] = mul_d(v[s], w);
+                    w = {w.x * cs - w.y * ss, w.x * ss + w.y * cs};
+                }
+            }
+        } else {
+            for_each_point<N, E, R1>(t, [&](int s, int, int j, int ro) {
+                const double extra = DRIVE ? g * (du0 + du1 * (j + ro)) : 0.0;
+                const double m = use_mask ? (mask_value(km[s]) * my) * mz : 1.0;
+                v[s] = mul_d(v[s], kick_factor<T>(kp[s], f, m, extra, DRIVE));
+            });
+        }
     }
     template <bool DRIVE>
     __device__ __forceinline__ void apply_kick(double f, bool use_mask, double g) {
@@ -602,9 +627,8 @@ __device__ __forceinline__ void contig_tile(const SweepArgs& a, long long tile, 
         // x-line `line` of the grid: y index line % ny, z index line / ny
         const int yj = a.dn1 > 0 ? static_cast<int>(line % a.dn1) : 0;
         const int zk = a.dn1 > 0 ? static_cast<int>(line / a.dn1) : 0;
-        L.du0 = a.dax[0] * a.dlo[0] + a.dax[1] * (a.dlo[1] + a.dh[1] * yj) +
-                a.dax[2] * (a.dlo[2] + a.dh[2] * zk);
-        L.du1 = a.dax[0] * a.dh[0];
+        L.du0 = a.du_c + a.du_y * yj + a.du_z * zk;
+        L.du1 = a.du_x;
     }
     if constexpr (BM == C_KICK_FIRST) {
         L.template load<LT::R1>();
@@ -775,8 +799,8 @@ __device__ __forceinline__ void strided_tile(const SweepArgs& a, long long tile,
          xx < a.nx};
     if constexpr (DRV) {
         // four-step column xx: point p of the line is x index xx + n1 p
-        L.du0 = a.dax[0] * (a.dlo[0] + a.dh[0] * xx) + a.dax[1] * a.dlo[1] + a.dax[2] * a.dlo[2];
-        L.du1 = a.dax[0] * a.dh[0] * a.dn1;
+        L.du0 = a.du_c + a.du_x * xx;
+        L.du1 = a.du_x * a.dn1;
     }
     if constexpr (BM == S_FWD) {
         L.template load<LT::R1>();
@@ -794,6 +818,7 @@ __device__ __forceinline__ void strided_tile(const SweepArgs& a, long long tile,
         L.fft();
         for_each_point<N, E, LT::RL>(t, [&](int s, int i, int, int) {
             L.v[s] = mul_d(L.v[s], cmul(kxw, kp[i]));
+            __builtin_amdgcn_sched_barrier(0);
         });
         L.ifft();
         L.template store<LT::R1>();

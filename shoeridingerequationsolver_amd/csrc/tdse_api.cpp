@@ -434,10 +434,12 @@ SweepArgs base_args(const tdse_plan* p) {
     }
     a.cell = 1.0;
     for (int i = 0; i < 3; ++i) a.cell *= (p->hi[i] - p->lo[i]) / p->n[i];
+    // the drive's axis . r = sum_a axis_a (lo_a + h_a i_a)
+    a.du_c = 0.0;
+    double* per[3] = {&a.du_x, &a.du_y, &a.du_z};
     for (int i = 0; i < 3; ++i) {
-        a.dax[i] = p->drive_axis[i];
-        a.dlo[i] = p->lo[i];
-        a.dh[i] = (p->hi[i] - p->lo[i]) / p->n[i];
+        a.du_c += p->drive_axis[i] * p->lo[i];
+        *per[i] = p->drive_axis[i] * ((p->hi[i] - p->lo[i]) / p->n[i]);
     }
     // x lines are ordered y fastest: line = j + ny k (1D four-step: rows of n1)
     a.dn1 = p->kind == kKindFourStep ? p->n1 : p->n[1];
