@@ -197,3 +197,38 @@ def test_unfused_baseline_matches_oracle(orc, tdse):
     assert rel(p.download(), want) < FP64_TOL
     with pytest.raises(ValueError):
         tdse.Propagator(grid3(tdse, dims, lo, hi), v, 0.02, precision="fp32", unfused_baseline=True)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("dims", [(32, 16, 8), (64, 64, 1), (1024, 1, 1), (256 * 64, 1, 1)])
+def test_large_kick_angles_match_oracle(orc, tdse, dims, prec):
+    """Potentials whose half-kick angle -V dt/2 spans many turns, with cells at
+    and next to +-pi, zero and tiny angles: FP32 plans keep tan(phi / 2) of
+    the reduced angle, which grows without bound near phi = +-pi."""
+    lo = tuple(-6.0 if n > 1 else 0.0 for n in dims)
+    hi = tuple(6.0 if n > 1 else 1.0 for n in dims)
+    n = int(np.prod(dims))
+    rng = np.random.default_rng(11)
+    dt = 0.02
+    v = rng.uniform(-2000.0, 2000.0, n)
+    # angles exactly pi, just inside and outside +-pi, 0 and tiny
+    special = np.array([np.pi, -np.pi, np.pi - 1e-9, -np.pi + 1e-9, np.pi + 1e-7, 0.0, 1e-12,
+                        3 * np.pi, -5 * np.pi])
+    idx = rng.choice(n, size=special.size, replace=False)
+    v[idx] = -2.0 * special / dt
+    if dims[1] == 1 and dims[2] == 1:
+        psi0 = orc.gaussian_1d(lo[0], hi[0], dims[0], 0.5, 1.0, 1.5)
+        want = orc.so1d_evolve(v, lo[0], hi[0], dt, psi0, 7)
+        g = tdse.Grid1D(lo[0], hi[0], dims[0])
+    else:
+        psi0 = packet3(orc, dims, lo, hi, (0.5, -0.3, 0.0), (1.2, 1.1, 0.9), (0.7, -0.4, 0.3))
+        want = orc.so3d_evolve(v, dims, lo, hi, dt, psi0, 7)
+        g = grid3(tdse, dims, lo, hi)
+    print(f"case large angles dims={dims} prec={prec}", flush=True)
+    p = tdse.Propagator(g, v, dt, precision=prec)
+    p.upload(psi0)
+    p.step(3)
+    p.step(4)
+    err = rel(p.download(), want)
+    print(f"  rel err {err:.3e}", flush=True)
+    assert err < (FP64_TOL if prec == "fp64" else FP32_TOL)
