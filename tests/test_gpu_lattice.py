@@ -38,6 +38,14 @@ CASES = [
     ((96, 59, 1), (-12.0, -8.0, -1.0), (12.0, 8.0, 1.0),
      dict(links=2, a=2.4, xs=0.31, ys=-0.17, cut_up=False), 20),
     ((1024, 40, 1), (-60.0, -8.0, -1.0), (60.0, 8.0, 1.0), dict(links=1, a=0.2), 10),
+    # the widest row (2048), odd and tiny widths, fewer rows than one column
+    # tile, one row past a tile, and solenoid cuts across tile edges
+    ((2048, 20, 1), (-60.0, -6.0, -1.0), (60.0, 6.0, 1.0), dict(links=1, a=0.1), 6),
+    ((257, 33, 1), (-16.0, -8.0, -1.0), (16.0, 8.0, 1.0),
+     dict(links=2, a=1.3, xs=0.9, ys=-1.1), 12),
+    ((6, 7, 1), (-3.0, -3.0, -1.0), (3.0, 3.0, 1.0), dict(links=1, a=0.7), 15),
+    ((130, 17, 1), (-12.0, -6.0, -1.0), (12.0, 6.0, 1.0),
+     dict(links=2, a=2.0, xs=-0.4, ys=0.2, cut_up=False), 9),
 ]
 
 
