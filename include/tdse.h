@@ -110,7 +110,10 @@ int tdse_plan_destroy(tdse_plan* plan);
  * modulo 2*pi in FP64 (then rounded to float on the FP32 path). */
 int tdse_plan_set_potential(tdse_plan* plan, const double* v);
 /* Real absorbing mask multiplied into psi after every step (null clears it).
- * core/src/potential.ixx:194-233 builds such masks. */
+ * core/src/potential.ixx:194-233 builds such masks: the product of per-axis
+ * ramps.  A 3D or 2D mask equal bit for bit to such a product is applied by
+ * the x sweeps from its ramps (no per-cell mask read per step); any other
+ * mask is read per cell (4 or 8 more bytes per cell and step). */
 int tdse_plan_set_mask(tdse_plan* plan, const double* mask);
 
 /* State transfer, interleaved complex128 on the host. */

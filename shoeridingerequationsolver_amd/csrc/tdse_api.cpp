@@ -1392,9 +1392,10 @@ int tdse_plan_set_potential(tdse_plan* p, const double* v) {
         if (!std::isfinite(v[i])) return fail(TDSE_E_INVALID, "potential not finite");
         th[i] = -0.5 * (v[i] + shift) * p->dt;  // core/src/propagator.ixx:29, 79
     }
-    // real time: reduced modulo 2 pi in FP64 for both precisions (the
-    // kernels' sin/cos take |angle| <= 8); imaginary time: the exponent of
-    // the weight exp(-V dtau / 2) (core/src/imaginary_time.ixx:30-33, 75-78)
+    // real time: reduced modulo 2 pi in FP64 for both precisions (the FP64
+    // kernels' sin/cos take |angle| <= 8), FP32 sweep plans then keep its
+    // half-angle tangent; imaginary time: the exponent of the weight
+    // exp(-V dtau / 2) (core/src/imaginary_time.ixx:30-33, 75-78)
     if ((rc = upload_real(p, p->phi, th.data(), p->imag() ? kPlain : p->tan_kick() ? kTanHalf : kPhase)))
         return rc;
     if ((rc = upload_real(p, p->v, v, kPlain))) return rc;
