@@ -128,6 +128,55 @@ def cpu_baseline_subprocess(args, threads, sample_steps):
     return json.loads(res.stdout.strip().splitlines()[-1])
 
 
+def cpu_paths(threads):
+    """The reference's own CPU implementations of the other paths (oracle/_ref:
+    ImaginaryTimePropagator3D, driven_step, MagneticPropagator3D on the 256^3
+    grid, PeierlsLattice2D on 1024^2), each on a bounded sample after a
+    one-step call whose time (table build) is subtracted."""
+    from oracle import oracle as orc
+    from shoeridingerequationsolver_amd import workloads
+    if not orc.ref_available():
+        return None
+    s = workloads.barrier(seed=3)
+    dims, lo, hi, dt = s.n, s.lo, s.hi, s.dt
+    v = s.build_potential()
+    psi = np.ascontiguousarray(s.build_psi0()).reshape(dims[::-1])
+    lg_dims, lg_lo, lg_hi = (1024, 1024, 1), (-60.0, -60.0, -1.0), (60.0, 60.0, 1.0)
+    lat_v = np.zeros(1024 * 1024)
+    lat_psi = np.ones(1024 * 1024, np.complex128) / 1024.0
+
+    def rate(run, k):
+        t0 = time.perf_counter()
+        run(1)
+        t1 = time.perf_counter()
+        run(1 + k)
+        t2 = time.perf_counter()
+        return round(k / ((t2 - t1) - (t1 - t0)), 3)
+
+    out = {"cores": orc.ref().ref_parallel_workers(), "kind": "reference"}
+    out["relax_256"] = rate(lambda n: orc.Ref.it3d_relax(v, dims, lo, hi, dt, psi, n), 3)
+    out["driven_256"] = rate(lambda n: orc.Ref.driven_evolve(v, dims, lo, hi, dt, psi, n,
+                                                             (1.0, 0.0, 0.0), 0.05, 0.5), 3)
+    out["magnetic_256"] = rate(lambda n: orc.Ref.magnetic_evolve(v, dims, lo, hi, dt, 0.5, 2,
+                                                                 psi, n), 2)
+    out["lattice_1024"] = rate(lambda n: orc.Ref.lat2d_evolve(lat_v, lg_dims, lg_lo, lg_hi, 0.01,
+                                                              lat_psi, n, links=1, a=0.3), 10)
+    out["unit"] = "steps/s (FP64)"
+    return out
+
+
+def cpu_paths_subprocess(threads):
+    import subprocess
+    env = dict(os.environ, SES_PARALLEL_WORKERS=str(threads), OMP_NUM_THREADS=str(threads))
+    code = ("import json, sys; sys.path.insert(0, %r); import bench; "
+            "print(json.dumps(bench.cpu_paths(%d)))") % (ROOT, threads)
+    res = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True,
+                         timeout=600)
+    if res.returncode != 0:
+        raise RuntimeError(res.stderr[-500:])
+    return json.loads(res.stdout.strip().splitlines()[-1])
+
+
 def batch_leg(t, workloads, steps):
     """48 uneven instances (seeded shapes 64^3..160^3, 512^2, 1024^2, 65536),
     complex64, inputs built in HBM; one batch vs the same plans stepped
@@ -419,6 +468,12 @@ def main():
             log(f"cpu baseline, one core: {one}")
         except Exception as e:  # the baseline is reported, never required
             log(f"cpu baseline unavailable: {e}")
+        if paths is not None:
+            try:
+                paths["cpu_reference"] = cpu_paths_subprocess(args.cpu_threads)
+                log(f"cpu reference, other paths: {paths['cpu_reference']}")
+            except Exception as e:
+                log(f"cpu reference for the other paths unavailable: {e}")
 
     line = {
         "metric": "256^3 split-op steps/s FP32 (FP64 beside it) @ max|dpsi|<=1e-5 vs FP64",
