@@ -177,11 +177,15 @@ def cpu_paths_subprocess(threads):
     return json.loads(res.stdout.strip().splitlines()[-1])
 
 
-def batch_leg(t, workloads, steps):
+def batch_leg(t, workloads, steps, small=False):
     """48 uneven instances (seeded shapes 64^3..160^3, 512^2, 1024^2, 65536),
     complex64, inputs built in HBM; one batch vs the same plans stepped
-    independently (one stream each)."""
-    solves = workloads.batch_instances(seed=4, count=48, steps=steps)
+    independently (one stream each).  small: 256 instances of 32^3 instead
+    (launch-bound when stepped one plan at a time)."""
+    if small:
+        solves = workloads.batch_instances(seed=5, count=256, steps=steps, shapes=[(32, 32, 32)])
+    else:
+        solves = workloads.batch_instances(seed=4, count=48, steps=steps)
     props = []
     for s in solves:
         p = t.Propagator(s.grid(), None, s.dt, precision=s.precision)
@@ -453,6 +457,8 @@ def main():
     if not args.no_batch:
         batch = batch_leg(t, workloads, args.batch_steps)
         log(f"batch: {batch}")
+        batch["small_32cubed"] = batch_leg(t, workloads, 200, small=True)
+        log(f"batch of 256 x 32^3: {batch['small_32cubed']}")
     serve = None
     if not args.no_serve:
         serve = serving_leg([5.0, 10.0, 20.0, 50.0, 100.0, 200.0], args.serve_requests,
