@@ -8,6 +8,10 @@ Reference interface mirrored here (names, argument meaning, error cases):
 * ``SplitOperator1D(grid, V, dt)``         core/src/propagator.ixx:21-64
 * ``SplitOperator3D(grid, V, dt, mass)``   core/src/propagator.ixx:66-134
 * ``WavepacketSimulation``                 core/src/simulation.ixx:22-88
+* ``ImaginaryTimePropagator1D/3D``         core/src/imaginary_time.ixx:21-144
+* ``DipoleDrive``, ``driven_step``         core/src/drive.ixx:19-66
+* ``MagneticPropagator3D``                 core/src/magnetic.ixx:19-80
+* ``PeierlsLattice2D``                     core/src/lattice2d.ixx:27-275
 * potential / mask / packet builders       core/src/potential.ixx, wavepacket.ixx
 
 Fields are numpy ``complex128`` arrays: shape ``(n,)`` for a ``Grid1D`` and
@@ -37,7 +41,7 @@ __all__ = [
     "regularized_coulomb_potential", "coulomb_floor_potential", "absorbing_mask",
     "gaussian_wavepacket", "TDSE_FP32", "TDSE_FP64", "TdseError", "Batch",
     "ImaginaryTimePropagator1D", "ImaginaryTimePropagator3D", "soft_coulomb_potential",
-    "MagneticPropagator3D", "PeierlsLattice2D",
+    "MagneticPropagator3D", "PeierlsLattice2D", "DipoleDrive", "driven_step",
 ]
 
 PRECISIONS = {"fp32": TDSE_FP32, "fp64": TDSE_FP64, TDSE_FP32: TDSE_FP32, TDSE_FP64: TDSE_FP64}
@@ -492,6 +496,34 @@ class SplitOperator3D(_SplitOperatorBase):
 
     def __init__(self, grid: Grid3D, potential, dt, mass=1.0, precision="fp64", context=None):
         super().__init__(grid, potential, dt, mass, precision, context)
+
+
+@dataclass
+class DipoleDrive:
+    """``ses::DipoleDrive`` (core/src/drive.ixx:19-23): V = amplitude (axis . r)
+    cos(omega t); the axis need not be a unit vector (it scales E0)."""
+    axis: tuple = (0.0, 0.0, 1.0)
+    amplitude: float = 0.0
+    omega: float = 0.0
+
+
+def driven_step(psi: np.ndarray, prop: SplitOperator3D, d: DipoleDrive, t0: float,
+                nsteps: int) -> None:
+    """``ses::driven_step(psi, prop, d, t0, nsteps)`` (core/src/drive.ixx:52-66):
+    ``nsteps`` Strang steps of ``prop`` from time ``t0`` with the dipole
+    half-kicks around each, in place.  The drive's kicks ride in the kick
+    sweeps; ``prop`` is left without a drive afterwards."""
+    if nsteps < 0:
+        raise ValueError("nsteps must be >= 0")
+    if nsteps == 0:
+        return
+    p = prop._prop
+    p.set_drive(tuple(d.axis), d.amplitude, d.omega)
+    try:
+        p.time = t0
+        prop.step(psi, nsteps)
+    finally:
+        p.set_drive(tuple(d.axis), 0.0, 0.0)
 
 
 class PeierlsLattice2D:

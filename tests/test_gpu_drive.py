@@ -107,3 +107,27 @@ def test_driven_plans_do_not_batch(orc, tdse):
     p.set_drive((1, 0, 0), 0.1, 1.0)
     with pytest.raises(Exception):
         tdse.Batch([p])
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_driven_step_facade_matches_oracle(orc, tdse, prec):
+    """ses::driven_step(psi, prop, d, t0, n) through the Python mirror: the
+    oracle's driven steps from t0, and the propagator static again after."""
+    dims, lo, hi = (32, 32, 16), (-8.0, -8.0, -4.0), (8.0, 8.0, 4.0)
+    g = tdse.Grid3D(tdse.Grid1D(lo[0], hi[0], dims[0]), tdse.Grid1D(lo[1], hi[1], dims[1]),
+                    tdse.Grid1D(lo[2], hi[2], dims[2]))
+    v = orc.harmonic_3d(dims, lo, hi, 0.4, (0.0, 0.0, 0.0))
+    psi0 = orc.gaussian_3d(dims, lo, hi, (1.0, 0.0, 0.0), (1.0, 1.0, 1.0),
+                           (0.0, 0.5, 0.0)).reshape(dims[::-1])
+    d = tdse.DipoleDrive((1.0, 0.5, 0.0), 0.3, 0.7)
+    print(f"case driven_step facade prec={prec}", flush=True)
+    want = orc.driven_evolve(v, dims, lo, hi, 0.02, psi0, 9, d.axis, d.amplitude, d.omega,
+                             t0=1.3)
+    prop = tdse.SplitOperator3D(g, v, 0.02, precision=prec)
+    psi = psi0.copy()
+    tdse.driven_step(psi, prop, d, 1.3, 9)
+    tol = 1e-11 if prec == "fp64" else 2e-5
+    assert rel(psi, want) < tol
+    psi = psi0.copy()
+    prop.step(psi, 9)
+    assert rel(psi, orc.so3d_evolve(v, dims, lo, hi, 0.02, psi0, 9)) < tol
