@@ -131,7 +131,7 @@ def cpu_baseline_subprocess(args, threads, sample_steps):
 def cpu_paths(threads):
     """The reference's own CPU implementations of the other paths (oracle/_ref:
     ImaginaryTimePropagator3D, driven_step, MagneticPropagator3D on the 256^3
-    grid, PeierlsLattice2D on 1024^2), each on a bounded sample after a
+    grid, PeierlsLattice2D on 512^2 and 1024^2), each on a bounded sample after a
     one-step call whose time (table build) is subtracted."""
     from oracle import oracle as orc
     from shoeridingerequationsolver_amd import workloads
@@ -141,9 +141,6 @@ def cpu_paths(threads):
     dims, lo, hi, dt = s.n, s.lo, s.hi, s.dt
     v = s.build_potential()
     psi = np.ascontiguousarray(s.build_psi0()).reshape(dims[::-1])
-    lg_dims, lg_lo, lg_hi = (1024, 1024, 1), (-60.0, -60.0, -1.0), (60.0, 60.0, 1.0)
-    lat_v = np.zeros(1024 * 1024)
-    lat_psi = np.ones(1024 * 1024, np.complex128) / 1024.0
 
     def rate(run, k):
         t0 = time.perf_counter()
@@ -159,8 +156,12 @@ def cpu_paths(threads):
                                                              (1.0, 0.0, 0.0), 0.05, 0.5), 3)
     out["magnetic_256"] = rate(lambda n: orc.Ref.magnetic_evolve(v, dims, lo, hi, dt, 0.5, 2,
                                                                  psi, n), 2)
-    out["lattice_1024"] = rate(lambda n: orc.Ref.lat2d_evolve(lat_v, lg_dims, lg_lo, lg_hi, 0.01,
-                                                              lat_psi, n, links=1, a=0.3), 10)
+    for m, k in ((512, 40), (1024, 10)):
+        lat_v = np.zeros(m * m)
+        lat_psi = np.ones(m * m, np.complex128) / m
+        out[f"lattice_{m}"] = rate(
+            lambda n: orc.Ref.lat2d_evolve(lat_v, (m, m, 1), (-60.0, -60.0, -1.0),
+                                           (60.0, 60.0, 1.0), 0.01, lat_psi, n, links=1, a=0.3), k)
     out["unit"] = "steps/s (FP64)"
     return out
 
@@ -224,7 +225,7 @@ def paths_leg(t, hip, workloads, steps):
     """Throughput of the other propagator paths on the headline grid (256^3,
     inputs resident, each timed over `steps` steps with HIP events): imaginary
     time, the dipole drive, the magnetic field (shear sweeps), the Peierls
-    lattice on 1024^2, and one full observables call."""
+    lattice on 512^2 and 1024^2, and one full observables call."""
     solve = workloads.barrier(seed=3)
     grid = solve.grid()
     out = {}
@@ -257,13 +258,15 @@ def paths_leg(t, hip, workloads, steps):
     p.build(solve.recipe())
     out["magnetic_fp32"] = timed(p, max(10, steps // 4))
     p.close()
-    lg = t.Grid3D(t.Grid1D(-60.0, 60.0, 1024), t.Grid1D(-60.0, 60.0, 1024), t.Grid1D(-1.0, 1.0, 1))
-    for prec in ("fp32", "fp64"):
-        p = t.Propagator(lg, np.zeros(1024 * 1024), 0.01, precision=prec, lattice=True)
-        p.set_uniform_field(0.3)
-        p.upload(t.gaussian_wavepacket(lg, (-10.0, 0.0, 0.0), (3.0, 3.0, 1.0), (1.0, 0.0, 0.0)))
-        out[f"lattice1024_{prec}"] = timed(p, steps)
-        p.close()
+    for n in (512, 1024):  # 512^2: the reference's lattice scenes
+        lg = t.Grid3D(t.Grid1D(-60.0, 60.0, n), t.Grid1D(-60.0, 60.0, n), t.Grid1D(-1.0, 1.0, 1))
+        for prec in ("fp32", "fp64"):
+            p = t.Propagator(lg, np.zeros(n * n), 0.01, precision=prec, lattice=True)
+            p.set_uniform_field(0.3)
+            p.upload(t.gaussian_wavepacket(lg, (-10.0, 0.0, 0.0), (3.0, 3.0, 1.0),
+                                           (1.0, 0.0, 0.0)))
+            out[f"lattice{n}_{prec}"] = timed(p, steps)
+            p.close()
     return out
 
 

@@ -1,16 +1,13 @@
-// lattice.hip -- the Peierls lattice propagator (core/src/lattice2d.ixx) as
-// two sweeps per step.
+// lattice.hip -- the Peierls lattice propagator (core/src/lattice2d.ixx).
 //
 // The reference step is  P, X0, X1, Y0, Y1', Y0, X1, X0, P  (P the onsite
 // half factor, Xp / Yp the bond groups of parity p, each pair of sites an
 // exact 2x2 rotation; Y1' over a full dt).  Consecutive steps meet in
-// X1, X0, P, P, X0, X1, so a step boundary is ONE row sweep (a row in LDS,
-// both x groups, the merged onsite factor and the caller's mask) and the Y
-// groups are ONE column sweep (a tile of columns with a 3-row halo: a final
-// row value depends on rows j-3 .. j+3 through the three groups).  Tiles
-// read each other's halo rows, so the column sweep writes a second array
-// and the next row sweep reads it back.  Bond rotations and onsite factors
-// are formed in FP64 for both state types.
+// X1, X0, P, P, X0, X1.  A call opens with one row sweep (P, X0, X1: a row
+// in LDS); after that every step is the same unit, Y0 Y1' Y0 X1 X0 P P X0 X1
+// (the last step stops after its P), and whole units run on 2D tiles, two
+// steps per launch.  Bond rotations and onsite factors are formed in FP64
+// for both state types.
 #include <hip/hip_runtime.h>
 
 #include "fft_engine.hpp"
@@ -22,23 +19,30 @@ namespace tdse {
 namespace {
 
 constexpr int kBlock = 256;
-constexpr int kCols = 32;   // columns per column tile
-constexpr int kRows = 58;   // rows written per column tile
-constexpr int kHalo = 3;
-constexpr int kWin = kRows + 2 * kHalo;
 
 __device__ __forceinline__ cx<double> cmuld(cx<double> a, cx<double> b) {
     return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
 }
 
-// pa <- c pa + (mix u) pb,  pb <- (mix conj(u)) pa + c pb
-__device__ __forceinline__ void bond(cx<double>& pa, cx<double>& pb, double c, cx<double> mix,
-                                     cx<double> u) {
+// The 2x2 rotation of one bond with link u:
+//     pa <- c pa + (mix u) pb,  pb <- (mix conj(u)) pa + c pb.
+// mix u and mix conj(u) are the same for every bond of a group that shares a
+// link, so they are formed once per group, not per bond.
+struct BondCoef {
+    double c;
+    cx<double> mu, mv;  // mix u, mix conj(u)
+};
+
+__device__ __forceinline__ BondCoef bond_coef(double c, cx<double> mix, cx<double> u) {
+    return {c, cmuld(mix, u), cmuld(mix, cx<double>{u.x, -u.y})};
+}
+
+__device__ __forceinline__ void bond(cx<double>& pa, cx<double>& pb, const BondCoef& k) {
     const cx<double> a0 = pa, b0 = pb;
-    const cx<double> t = cmuld(cmuld(mix, u), b0);
-    const cx<double> t2 = cmuld(cmuld(mix, cx<double>{u.x, -u.y}), a0);
-    pa = {c * a0.x + t.x, c * a0.y + t.y};
-    pb = {t2.x + c * b0.x, t2.y + c * b0.y};
+    const cx<double> t = cmuld(k.mu, b0);
+    const cx<double> t2 = cmuld(k.mv, a0);
+    pa = {k.c * a0.x + t.x, k.c * a0.y + t.y};
+    pb = {t2.x + k.c * b0.x, t2.y + k.c * b0.y};
 }
 
 // x link of bond (i, j) -> (i + 1, j)
@@ -48,129 +52,255 @@ __device__ __forceinline__ cx<double> link_x(const LatticeArgs& a, int i, int j,
     return {1.0, 0.0};
 }
 
+// x link of row j: exp(-i B hx y_j) for a uniform field (the host keeps
+// |B hx y| below 1e5, inside the short reduction of sincos_small), the
+// host-formed exp(+-i phi) on a solenoid's cut
 __device__ __forceinline__ cx<double> row_link(const LatticeArgs& a, int j) {
-    double s, c;
     if (a.link_mode == 1) {
-        sincos(a.link_a * (a.ylo + a.hy * j), &s, &c);
+        double s, c;
+        sincos_small(a.link_a * (a.ylo + a.hy * j), &s, &c);
         return {c, s};
     }
-    if (a.link_mode == 2) {
-        sincos(a.link_a, &s, &c);
-        return {c, a.link_sign * s};
-    }
+    if (a.link_mode == 2) return {a.link_c, a.link_s};
     return {1.0, 0.0};
 }
 
+// m exp(i f phi) (real time) or m exp(f phi) (imaginary time) of one site
 template <typename T>
-__device__ __forceinline__ void onsite(const LatticeArgs& a, cx<double>* row, long base, double f,
-                                       bool with_mask) {
-    const T* phi = static_cast<const T*>(a.phi);
-    const T* mask = static_cast<const T*>(a.mask);
-    for (int i = threadIdx.x; i < a.nx; i += blockDim.x) {
-        const double p = f * static_cast<double>(phi[base + i]);
-        double m = 1.0;
-        if (with_mask && mask) m = mask_value(mask[base + i]);
-        cx<double> fac;
-        if (a.relax) {
-            fac = {m * exp(p), 0.0};
-        } else {
-            double s, c;
-            sincos_small(p, &s, &c);
-            fac = {m * c, m * s};
-        }
-        row[i] = cmuld(row[i], fac);
-    }
+__device__ __forceinline__ cx<double> onsite_factor(const LatticeArgs& a, T phi, double m,
+                                                    double f) {
+    const double p = f * static_cast<double>(phi);
+    if (a.relax) return {m * exp(p), 0.0};
+    double s, c;
+    sincos_small(p, &s, &c);
+    return {m * c, m * s};
 }
 
-__device__ __forceinline__ void x_group(const LatticeArgs& a, cx<double>* row, int j, int parity,
-                                        cx<double> urow) {
-    const cx<double> mix = {a.mxr, a.mxi};
+// x bonds (i, i + 1), i = parity mod 2, of one row with coefficients k; the
+// bond (cut, cut + 1) (a solenoid's branch cut, cut >= 0) takes kcut instead
+// and is applied by thread 0 once the others are done with the group's
+// disjoint pairs.
+__device__ __forceinline__ void x_group(const LatticeArgs& a, cx<double>* row, int parity,
+                                        BondCoef k, int cut, BondCoef kcut) {
     for (int b = threadIdx.x;; b += blockDim.x) {
         const int i = 2 * b + parity;
         if (i + 1 >= a.nx) break;
-        bond(row[i], row[i + 1], a.cx, mix, link_x(a, i, j, urow));
+        if (i != cut) bond(row[i], row[i + 1], k);
     }
+    if (threadIdx.x == 0 && cut >= 0 && (cut & 1) == parity && cut + 1 < a.nx)
+        bond(row[cut], row[cut + 1], kcut);
     __syncthreads();
 }
 
-template <typename T>
-__global__ void __launch_bounds__(kBlock) lattice_rows(LatticeArgs a) {
+// The opening half step of a call, P X0 X1, one row per block, src -> dst
+// (the same array or not).  Thread t owns sites t + u kBlock, u < PER, of a
+// row (PER kBlock >= nx); every load of a row is issued before the first use.
+template <typename T, int PER>
+__global__ void __launch_bounds__(kBlock) lattice_rows(LatticeArgs a, const void* src_,
+                                                       void* dst_) {
     __shared__ cx<double> row[kLatMaxNx];
-    cx<T>* psi = static_cast<cx<T>*>(a.psi);
-    const cx<T>* src = static_cast<const cx<T>*>(a.other ? a.other : a.psi);
+    const cx<T>* src = static_cast<const cx<T>*>(src_);
+    cx<T>* dst = static_cast<cx<T>*>(dst_);
+    const T* phi = static_cast<const T*>(a.phi);
     for (int j = blockIdx.x; j < a.ny; j += gridDim.x) {
         const long base = static_cast<long>(j) * a.nx;
-        const cx<double> urow = row_link(a, j);
-        for (int i = threadIdx.x; i < a.nx; i += blockDim.x) {
-            const cx<T> z = src[base + i];
-            row[i] = {static_cast<double>(z.x), static_cast<double>(z.y)};
-        }
-        __syncthreads();
-        if (a.stage == 0) {
-            onsite<T>(a, row, base, 1.0, false);
-            __syncthreads();
-            x_group(a, row, j, 0, urow);
-            x_group(a, row, j, 1, urow);
-        } else {
-            x_group(a, row, j, 1, urow);
-            x_group(a, row, j, 0, urow);
-            // end of one step (and its mask), start of the next: merged
-            onsite<T>(a, row, base, a.stage == 1 ? 2.0 : 1.0, true);
-            __syncthreads();
-            if (a.stage == 1) {
-                x_group(a, row, j, 0, urow);
-                x_group(a, row, j, 1, urow);
+        cx<T> z[PER];
+        T ph[PER];
+#pragma unroll
+        for (int u = 0; u < PER; ++u) {
+            const int i = threadIdx.x + u * kBlock;
+            if (i < a.nx) {
+                z[u] = src[base + i];
+                ph[u] = phi[base + i];
             }
         }
+        // x links: every bond of the row (uniform field), or one bond of the
+        // rows a solenoid's cut crosses; u = 1 elsewhere
+        const cx<double> mix = {a.mxr, a.mxi};
+        const bool cut_row = a.link_mode == 2 && j >= a.link_j0 && j < a.link_j1;
+        const int cut = cut_row ? a.link_col : -1;
+        const cx<double> u = row_link(a, j);
+        const BondCoef kx = bond_coef(a.cx, mix, a.link_mode == 1 ? u : cx<double>{1.0, 0.0});
+        const BondCoef kcut = bond_coef(a.cx, mix, u);
+#pragma unroll
+        for (int u = 0; u < PER; ++u) {
+            const int i = threadIdx.x + u * kBlock;
+            if (i < a.nx) {
+                const cx<double> w = {static_cast<double>(z[u].x), static_cast<double>(z[u].y)};
+                row[i] = cmuld(w, onsite_factor(a, ph[u], 1.0, 1.0));
+            }
+            __builtin_amdgcn_sched_barrier(0);  // one site's sine and cosine at a time
+        }
+        __syncthreads();
+        x_group(a, row, 0, kx, cut, kcut);
+        x_group(a, row, 1, kx, cut, kcut);
         for (int i = threadIdx.x; i < a.nx; i += blockDim.x)
-            psi[base + i] = {static_cast<T>(row[i].x), static_cast<T>(row[i].y)};
+            dst[base + i] = {static_cast<T>(row[i].x), static_cast<T>(row[i].y)};
         __syncthreads();
     }
 }
 
-// y bonds (j, j + 1) of rows j = parity mod 2 inside the window [wlo, whi)
-__device__ __forceinline__ void y_group(cx<double> (*tile)[kCols + 1], int wlo, int whi, int parity,
-                                        double c, cx<double> mix) {
-    const int first = wlo + ((wlo & 1) != parity ? 1 : 0);
-    const int nb = first + 1 < whi ? (whi - 1 - first + 1) / 2 : 0;  // bonds j = first, first+2, ..
-    for (int q = threadIdx.x; q < nb * kCols; q += blockDim.x) {
-        const int col = q % kCols, jb = first + 2 * (q / kCols);
-        if (jb + 1 >= whi) continue;
-        bond(tile[jb - wlo][col], tile[jb + 1 - wlo][col], c, mix, cx<double>{1.0, 0.0});
+// ---- whole steps on 2D tiles ----------------------------------------------------
+//
+// One launch advances K steps past a step boundary: per step the three y
+// groups, then X1, X0, the merged onsite factor and mask, and (unless it is
+// the call's last step) X0, X1 of the next step.  A tile writes kTileW x
+// kTileH sites and holds a window with 3K halo rows and 4K halo columns in
+// LDS: each bond group leaves at most one more row (y) or column (x) at a
+// window edge stale, and stale sites only spread along their own row or
+// column.  Windows start on even columns, so a bond's parity in the window is
+// its parity in the lattice.  Windows are clipped at the lattice edges, where
+// the open boundary has no bond to miss.  Out of place: tiles read each
+// other's halos.
+
+constexpr int kTileW = 32, kTileH = 32;
+
+// K steps per launch, BS threads per tile: 1024 when there are few tiles (4
+// waves per SIMD of one tile hide a phase's LDS and FP64 latencies), 512 when
+// two tiles per CU fit (their LDS and 128 VGPRs per lane allow it)
+template <int K, int BS>
+struct StepTile {
+    static constexpr int HX = 4 * K, HY = 3 * K;  // halo columns, rows
+    static constexpr int W = kTileW + 2 * HX, H = kTileH + 2 * HY;
+    static constexpr int PER = (W * H + BS - 1) / BS;             // window sites per thread
+    static constexpr int YB = (H / 2 * W + BS - 1) / BS;          // y bonds per thread
+    static constexpr int XB = (H * (W / 2) + BS - 1) / BS;        // x bonds per thread
+};
+
+// threadIdx.x as a value the compiler cannot see through: each phase forms
+// its own LDS indices instead of hoisting all phases' indices out of the step
+// loop and keeping them live (which spills).  The asm emits no instruction.
+__device__ __forceinline__ int phase_tid() {
+    int t = static_cast<int>(threadIdx.x);
+    asm volatile("" : "+v"(t));
+    return t;
+}
+
+// y bonds (j, j + 1), j = parity mod 2, of the window rows [0, wyn) (row 0
+// is lattice row wy0) over columns [0, wxn)
+template <int W, int YB, int BS>
+__device__ __forceinline__ void y_group_tile(cx<double> (*w)[W], int wxn, int wy0, int wyn,
+                                             int parity, const BondCoef& k) {
+    const int first = (wy0 & 1) != parity ? 1 : 0;
+    const int nb = (wyn - first) / 2;
+    const int tid = phase_tid();
+#pragma unroll
+    for (int u = 0; u < YB; ++u) {
+        const int q = tid + u * BS;
+        const int lc = q % W, lr = first + 2 * (q / W);
+        if (q < nb * W && lc < wxn) bond(w[lr][lc], w[lr + 1][lc], k);
+        __builtin_amdgcn_sched_barrier(0);  // one bond in registers at a time
     }
     __syncthreads();
 }
 
-template <typename T>
-__global__ void __launch_bounds__(kBlock) lattice_cols(LatticeArgs a) {
-    __shared__ cx<double> tile[kWin][kCols + 1];
-    const cx<T>* psi = static_cast<const cx<T>*>(a.psi);
-    cx<T>* out = static_cast<cx<T>*>(a.other);
-    const int xt = (a.nx + kCols - 1) / kCols, yt = (a.ny + kRows - 1) / kRows;
+// x bonds (i, i + 1), i = parity mod 2, of every window row: xk[row] on the
+// row's bonds, kc on a solenoid's cut bond
+template <int W, int XB, int BS>
+__device__ __forceinline__ void x_group_tile(const LatticeArgs& a, cx<double> (*w)[W],
+                                             const BondCoef* xk, const BondCoef* kc, int wx0,
+                                             int wxn, int wy0, int wyn, int parity) {
+    constexpr int NB = W / 2;
+    const int tid = phase_tid();
+#pragma unroll
+    for (int u = 0; u < XB; ++u) {
+        const int q = tid + u * BS;
+        const int lr = q / NB, lc = 2 * (q % NB) + parity;
+        if (lr < wyn && lc + 1 < wxn) {
+            const int j = wy0 + lr;
+            const bool cut = a.link_mode == 2 && wx0 + lc == a.link_col && j >= a.link_j0 &&
+                             j < a.link_j1;
+            bond(w[lr][lc], w[lr][lc + 1], *(cut ? kc : &xk[lr]));
+        }
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    __syncthreads();
+}
+
+template <typename T, int K, int BS>
+__global__ void __launch_bounds__(BS, 4) lattice_steps(LatticeArgs a, const void* src_, void* dst_,
+                                                    int units, int last_stage) {
+    using S = StepTile<K, BS>;
+    constexpr int W = S::W, H = S::H;
+    __shared__ cx<double> w[H][W];
+    __shared__ T sph[H][W];
+    __shared__ T smk[H][W];
+    __shared__ BondCoef xk[H];
+    __shared__ BondCoef kc, ky[2];  // the cut bond; y bonds of even and odd rows
+    const cx<T>* src = static_cast<const cx<T>*>(src_);
+    cx<T>* dst = static_cast<cx<T>*>(dst_);
+    const T* phi = static_cast<const T*>(a.phi);
+    const T* mask = static_cast<const T*>(a.mask);
+    const int tid = threadIdx.x;
+    const cx<double> mix = {a.mxr, a.mxi};
+    if (tid == 0) {
+        kc = bond_coef(a.cx, mix, cx<double>{a.link_c, a.link_s});
+        ky[0] = bond_coef(a.cy, cx<double>{a.myr, a.myi}, cx<double>{1.0, 0.0});
+        ky[1] = bond_coef(a.cy2, cx<double>{a.my2r, a.my2i}, cx<double>{1.0, 0.0});
+    }
+    const int xt = (a.nx + kTileW - 1) / kTileW, yt = (a.ny + kTileH - 1) / kTileH;
     for (long t = blockIdx.x; t < static_cast<long>(xt) * yt; t += gridDim.x) {
-        const int c0 = static_cast<int>(t % xt) * kCols, r0 = static_cast<int>(t / xt) * kRows;
-        const int wlo = r0 - kHalo > 0 ? r0 - kHalo : 0;
-        const int whi = r0 + kRows + kHalo < a.ny ? r0 + kRows + kHalo : a.ny;
-        for (int q = threadIdx.x; q < (whi - wlo) * kCols; q += blockDim.x) {
-            const int col = q % kCols, r = wlo + q / kCols;
-            cx<double> z = {0.0, 0.0};
-            if (c0 + col < a.nx) {
-                const cx<T> v = psi[static_cast<long>(r) * a.nx + c0 + col];
-                z = {static_cast<double>(v.x), static_cast<double>(v.y)};
+        const int c0 = static_cast<int>(t % xt) * kTileW, r0 = static_cast<int>(t / xt) * kTileH;
+        const int wx0 = c0 - S::HX > 0 ? c0 - S::HX : 0;
+        const int wy0 = r0 - S::HY > 0 ? r0 - S::HY : 0;
+        const int wxn = (c0 + kTileW + S::HX < a.nx ? c0 + kTileW + S::HX : a.nx) - wx0;
+        const int wyn = (r0 + kTileH + S::HY < a.ny ? r0 + kTileH + S::HY : a.ny) - wy0;
+        // every load of the window first (state, phases, mask), then LDS
+        cx<T> z[S::PER];
+        T ph[S::PER], mk[S::PER];
+#pragma unroll
+        for (int u = 0; u < S::PER; ++u) {
+            const int q = tid + u * BS, lr = q / W, lc = q % W;
+            if (lr < wyn && lc < wxn) {
+                const long f = static_cast<long>(wy0 + lr) * a.nx + wx0 + lc;
+                z[u] = src[f];
+                ph[u] = phi[f];
+                mk[u] = mask ? mask[f] : kMaskOne<T>;
             }
-            tile[r - wlo][col] = z;
+        }
+        if (tid < wyn) {
+            const int j = wy0 + tid;
+            xk[tid] = bond_coef(a.cx, mix, a.link_mode == 1 ? row_link(a, j) : cx<double>{1.0, 0.0});
+        }
+#pragma unroll
+        for (int u = 0; u < S::PER; ++u) {
+            const int q = tid + u * BS, lr = q / W, lc = q % W;
+            if (lr < wyn && lc < wxn) {
+                w[lr][lc] = {static_cast<double>(z[u].x), static_cast<double>(z[u].y)};
+                sph[lr][lc] = ph[u];
+                smk[lr][lc] = mk[u];
+            }
         }
         __syncthreads();
-        y_group(tile, wlo, whi, 0, a.cy, cx<double>{a.myr, a.myi});
-        y_group(tile, wlo, whi, 1, a.cy2, cx<double>{a.my2r, a.my2i});
-        y_group(tile, wlo, whi, 0, a.cy, cx<double>{a.myr, a.myi});
-        const int rend = r0 + kRows < a.ny ? r0 + kRows : a.ny;
-        for (int q = threadIdx.x; q < (rend - r0) * kCols; q += blockDim.x) {
-            const int col = q % kCols, r = r0 + q / kCols;
-            if (c0 + col < a.nx) {
-                const cx<double> z = tile[r - wlo][col];
-                out[static_cast<long>(r) * a.nx + c0 + col] = {static_cast<T>(z.x),
-                                                               static_cast<T>(z.y)};
+        for (int s = 0; s < units; ++s) {
+            const int stage = s + 1 < units ? 1 : last_stage;
+            y_group_tile<W, S::YB, BS>(w, wxn, wy0, wyn, 0, ky[0]);
+            y_group_tile<W, S::YB, BS>(w, wxn, wy0, wyn, 1, ky[1]);
+            y_group_tile<W, S::YB, BS>(w, wxn, wy0, wyn, 0, ky[0]);
+            x_group_tile<W, S::XB, BS>(a, w, xk, &kc, wx0, wxn, wy0, wyn, 1);
+            x_group_tile<W, S::XB, BS>(a, w, xk, &kc, wx0, wxn, wy0, wyn, 0);
+            // end of one step (and its mask), start of the next: merged
+            const double f = stage == 1 ? 2.0 : 1.0;
+            const int ptid = phase_tid();
+#pragma unroll
+            for (int u = 0; u < S::PER; ++u) {
+                const int q = ptid + u * BS, lr = q / W, lc = q % W;
+                if (lr < wyn && lc < wxn)
+                    w[lr][lc] = cmuld(w[lr][lc], onsite_factor(a, sph[lr][lc],
+                                                               mask_value(smk[lr][lc]), f));
+                __builtin_amdgcn_sched_barrier(0);
+            }
+            __syncthreads();
+            if (stage == 1) {
+                x_group_tile<W, S::XB, BS>(a, w, xk, &kc, wx0, wxn, wy0, wyn, 0);
+                x_group_tile<W, S::XB, BS>(a, w, xk, &kc, wx0, wxn, wy0, wyn, 1);
+            }
+        }
+        for (int q = tid; q < kTileW * kTileH; q += BS) {
+            const int r = r0 + q / kTileW, c = c0 + q % kTileW;
+            if (r < a.ny && c < a.nx) {
+                const cx<double> v = w[r - wy0][c - wx0];
+                dst[static_cast<long>(r) * a.nx + c] = {static_cast<T>(v.x), static_cast<T>(v.y)};
             }
         }
         __syncthreads();
@@ -229,23 +359,50 @@ int blocks_for(long long work, int cap) {
 
 }  // namespace
 
-hipError_t launch_lattice_rows(bool fp64, const LatticeArgs& a, hipStream_t stream) {
+hipError_t launch_lattice_rows(bool fp64, const LatticeArgs& a, const void* src, void* dst,
+                               hipStream_t stream) {
+    if (a.nx < 2 || a.nx > kLatMaxNx) return hipErrorInvalidValue;
     const int blocks = blocks_for(a.ny, 8192);
-    if (fp64)
-        hipLaunchKernelGGL(lattice_rows<double>, dim3(blocks), dim3(kBlock), 0, stream, a);
-    else
-        hipLaunchKernelGGL(lattice_rows<float>, dim3(blocks), dim3(kBlock), 0, stream, a);
+    const int per = (a.nx + kBlock - 1) / kBlock;
+    auto go = [&](auto kernel) {
+        hipLaunchKernelGGL(kernel, dim3(blocks), dim3(kBlock), 0, stream, a, src, dst);
+    };
+    if (fp64) {
+        if (per == 1) go(lattice_rows<double, 1>);
+        else if (per == 2) go(lattice_rows<double, 2>);
+        else if (per <= 4) go(lattice_rows<double, 4>);
+        else go(lattice_rows<double, kLatMaxNx / kBlock>);
+    } else {
+        if (per == 1) go(lattice_rows<float, 1>);
+        else if (per == 2) go(lattice_rows<float, 2>);
+        else if (per <= 4) go(lattice_rows<float, 4>);
+        else go(lattice_rows<float, kLatMaxNx / kBlock>);
+    }
     return hipGetLastError();
 }
 
-hipError_t launch_lattice_cols(bool fp64, const LatticeArgs& a, hipStream_t stream) {
-    const long long tiles = static_cast<long long>((a.nx + kCols - 1) / kCols) *
-                            ((a.ny + kRows - 1) / kRows);
+hipError_t launch_lattice_steps(bool fp64, const LatticeArgs& a, const void* src, void* dst,
+                                int units, int last_stage, hipStream_t stream) {
+    if (units < 1 || units > kLatStepsPerLaunch || (last_stage != 1 && last_stage != 2))
+        return hipErrorInvalidValue;
+    const long long tiles = static_cast<long long>((a.nx + kTileW - 1) / kTileW) *
+                            ((a.ny + kTileH - 1) / kTileH);
     const int blocks = blocks_for(tiles, 8192);
-    if (fp64)
-        hipLaunchKernelGGL(lattice_cols<double>, dim3(blocks), dim3(kBlock), 0, stream, a);
-    else
-        hipLaunchKernelGGL(lattice_cols<float>, dim3(blocks), dim3(kBlock), 0, stream, a);
+    auto go = [&](auto kernel, int bs) {
+        hipLaunchKernelGGL(kernel, dim3(blocks), dim3(bs), 0, stream, a, src, dst, units,
+                           last_stage);
+    };
+    constexpr int KL = kLatStepsPerLaunch;
+    if (fp64) {  // the FP64 window does not fit 512 threads in 128 VGPRs without spilling
+        if (units == 1) go(lattice_steps<double, 1, 1024>, 1024);
+        else go(lattice_steps<double, KL, 1024>, 1024);
+    } else if (tiles >= 512) {  // two tiles per CU on 256 CUs
+        if (units == 1) go(lattice_steps<float, 1, 512>, 512);
+        else go(lattice_steps<float, KL, 512>, 512);
+    } else {
+        if (units == 1) go(lattice_steps<float, 1, 1024>, 1024);
+        else go(lattice_steps<float, KL, 1024>, 1024);
+    }
     return hipGetLastError();
 }
 

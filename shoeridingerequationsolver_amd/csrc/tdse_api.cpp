@@ -855,43 +855,44 @@ tdse::LatticeArgs lattice_args(const tdse_plan* p) {
     a.link_col = p->lat_col;
     a.link_j0 = p->lat_j0;
     a.link_j1 = p->lat_j1;
-    a.link_sign = p->lat_sign;
+    if (p->lat_mode == 2) {  // the cut's link exp(lat_sign i phi), as std::polar forms it
+        a.link_c = std::cos(p->lat_a);
+        a.link_s = p->lat_sign * std::sin(p->lat_a);
+    }
     return a;
 }
 
-// core/src/lattice2d.ixx:218-249.  Real time: one row sweep per step
-// boundary (x groups, merged onsite factor, mask) and one column sweep (y
-// groups) per step.  Imaginary time: each step runs its own row sweeps and is
-// renormalised (fixed-order reduction, then a scaling pass).
+// core/src/lattice2d.ixx:218-249.  A call opens with the half step P X0 X1
+// (one row sweep), then runs whole steps on 2D tiles, two per launch, out of
+// place between psi and a scratch array; the last step ends with its P and
+// the mask.  Imaginary time: each step is a call of one step, renormalised
+// after it (fixed-order reduction, then a scaling pass).
 int run_lattice(tdse_plan* p, int nsteps) {
     int rc;
-    // the column sweep writes the scratch array, the row sweep reads it back
     if (!p->scratch && (rc = alloc(p, &p->scratch, p->total * p->csize()))) return rc;
     tdse::LatticeArgs a = lattice_args(p);
     const bool fp64 = p->fp64();
-    auto rows = [&](int stage, bool from_scratch) {
-        a.stage = stage;
-        a.other = from_scratch ? p->scratch : nullptr;
-        return tdse::launch_lattice_rows(fp64, a, p->stream);
-    };
-    auto cols = [&]() {
-        a.other = p->scratch;
-        return tdse::launch_lattice_cols(fp64, a, p->stream);
-    };
-    if (!p->imag()) {
-        TDSE_HIP(rows(0, false));
-        for (int s = 0; s < nsteps; ++s) {
-            TDSE_HIP(cols());
-            TDSE_HIP(rows(s + 1 < nsteps ? 1 : 2, true));
+    // the opening row sweep writes where the tile launches must start so that
+    // the last one writes psi
+    auto steps = [&](int n) {
+        const int per = tdse::kLatStepsPerLaunch;
+        const int launches = (n + per - 1) / per;
+        void* cur = (launches & 1) ? p->scratch : p->psi;
+        TDSE_HIP(tdse::launch_lattice_rows(fp64, a, p->psi, cur, p->stream));
+        for (int l = 0; l < launches; ++l) {
+            void* next = cur == p->psi ? p->scratch : p->psi;
+            const int units = std::min(per, n - l * per);
+            TDSE_HIP(tdse::launch_lattice_steps(fp64, a, cur, next, units,
+                                                l + 1 < launches ? 1 : 2, p->stream));
+            cur = next;
         }
         return TDSE_OK;
-    }
+    };
+    if (!p->imag()) return steps(nsteps);
     const tdse::ReduceArgs r = reduce_args(p);
     const double dv = cell_volume(p);
     for (int s = 0; s < nsteps; ++s) {
-        TDSE_HIP(rows(0, false));
-        TDSE_HIP(cols());
-        TDSE_HIP(rows(2, true));
+        if ((rc = steps(1))) return rc;
         TDSE_HIP(tdse::launch_reduce(fp64, false, r, p->partials, p->red_out, p->stream));
         TDSE_HIP(tdse::launch_normalize_dev(fp64, p->psi, p->total, p->red_out, dv,
                                             &p->relax()->norm_sq, p->stream));
@@ -1599,8 +1600,13 @@ int tdse_lattice_set_uniform_field(tdse_plan* p, double b) {
     if (p->kind != kKindLattice) return fail(TDSE_E_INVALID, "not a lattice plan");
     if (!std::isfinite(b)) return fail(TDSE_E_INVALID, "field must be finite");
     // x links of row j: exp(-i B hx y_j) (core/src/lattice2d.ixx:96-105)
+    const double a = -b * ((p->hi[0] - p->lo[0]) / p->n[0]);
+    const double ymax = std::max(std::fabs(p->lo[1]), std::fabs(p->hi[1]));
+    // the kernels reduce link angles by a short (Cody-Waite) reduction
+    if (std::fabs(a) * ymax >= 1.0e5)
+        return fail(TDSE_E_UNSUPPORTED, "link angles B hx y must stay below 1e5 rad");
     p->lat_mode = 1;
-    p->lat_a = -b * ((p->hi[0] - p->lo[0]) / p->n[0]);
+    p->lat_a = a;
     return TDSE_OK;
 }
 
@@ -2163,10 +2169,11 @@ int tdse_plan_step_bytes(const tdse_plan* p, uint64_t* out) {
     else if (p->kind == kKind2D || p->kind == kKindFourStep) sweeps = 2;
     const uint64_t N = static_cast<uint64_t>(p->total);
     if (p->kind == kKindLattice) {
-        // row sweep (state in and out, onsite phase, mask) + column sweep
-        // (state in and out; its 3-row halos read 6 of every 58 rows again)
-        *out = N * (2 * p->csize() + p->rsize() + (p->mask ? p->rsize() : 0)) +
-               N * p->csize() * 64 / 58 + N * p->csize();
+        // step tiles, two steps per launch: a 48 x 44 window (state, onsite
+        // phase, mask) read per 32 x 32 sites written; a call's opening row
+        // sweep is not counted
+        const uint64_t in = p->csize() + p->rsize() + (p->mask ? p->rsize() : 0);
+        *out = N * (33 * in + 16 * p->csize()) / 32;
         return TDSE_OK;
     }
     if (p->bfield != 0.0) {

@@ -1,6 +1,7 @@
-"""The Peierls lattice on the GPU (row sweep + column sweep per step) against
-the oracle's restatement of PeierlsLattice2D (core/src/lattice2d.ixx), pinned
-in tests/test_oracle_lattice.py.  FP64 1e-11, FP32 2e-5 relative to max |psi|;
+"""The Peierls lattice on the GPU (an opening row sweep, then whole steps on
+2D tiles, two per launch) against the oracle's restatement of
+PeierlsLattice2D (core/src/lattice2d.ixx), pinned in
+tests/test_oracle_lattice.py.  FP64 1e-11, FP32 2e-5 relative to max |psi|;
 lattice energies 1e-11 (FP64) / 2e-6 (FP32)."""
 import math
 
@@ -38,7 +39,7 @@ CASES = [
     ((96, 59, 1), (-12.0, -8.0, -1.0), (12.0, 8.0, 1.0),
      dict(links=2, a=2.4, xs=0.31, ys=-0.17, cut_up=False), 20),
     ((1024, 40, 1), (-60.0, -8.0, -1.0), (60.0, 8.0, 1.0), dict(links=1, a=0.2), 10),
-    # the widest row (2048), odd and tiny widths, fewer rows than one column
+    # the widest row (2048), odd and tiny widths, fewer rows than one step
     # tile, one row past a tile, and solenoid cuts across tile edges
     ((2048, 20, 1), (-60.0, -6.0, -1.0), (60.0, 6.0, 1.0), dict(links=1, a=0.1), 6),
     ((257, 33, 1), (-16.0, -8.0, -1.0), (16.0, 8.0, 1.0),
@@ -46,6 +47,12 @@ CASES = [
     ((6, 7, 1), (-3.0, -3.0, -1.0), (3.0, 3.0, 1.0), dict(links=1, a=0.7), 15),
     ((130, 17, 1), (-12.0, -6.0, -1.0), (12.0, 6.0, 1.0),
      dict(links=2, a=2.0, xs=-0.4, ys=0.2, cut_up=False), 9),
+    # 512 step tiles and more (the FP32 tiles then take 512 threads): a
+    # solenoid cut through tile edges, an odd step count (a one-step launch
+    # last), and a uniform field
+    ((768, 704, 1), (-30.0, -28.0, -1.0), (30.0, 28.0, 1.0),
+     dict(links=2, a=1.7, xs=0.9, ys=-1.1), 5),
+    ((1024, 512, 1), (-40.0, -20.0, -1.0), (40.0, 20.0, 1.0), dict(links=1, a=0.15), 8),
 ]
 
 
@@ -74,6 +81,23 @@ def test_lattice_step_matches_oracle(orc, tdse, case, prec):
     print(f"  rel err {err:.3e}; energy {e:.14f} vs {e_want:.14f}", flush=True)
     assert err < TOL[prec]
     assert abs(e - e_want) < (1e-11 if prec == "fp64" else 2e-6) * max(1.0, abs(e_want))
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_lattice_split_calls(orc, tdse, prec):
+    """Calls of 1, 2 and 7 steps (each opens with its own half step) equal
+    one run of 10 steps."""
+    dims, lo, hi = (160, 96, 1), (-16.0, -10.0, -1.0), (16.0, 10.0, 1.0)
+    kw = dict(links=1, a=0.4)
+    v = orc.harmonic_3d(dims, lo, hi, 0.3, (0.5, 0, 0))
+    psi0 = packet(dims, lo, hi, -2.0, 1.0, 2.0, 1.2)
+    want, _ = orc.lat2d_evolve(v, dims, lo, hi, 0.02, psi0, 10, **kw)
+    p = tdse.Propagator(grid3(tdse, dims, lo, hi), v, 0.02, precision=prec, lattice=True)
+    _set_links(p, kw)
+    p.upload(psi0)
+    for k in (1, 2, 7):
+        p.step(k)
+    assert rel(p.download(), want) < TOL[prec]
 
 
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
@@ -130,6 +154,8 @@ def test_lattice_rejections(tdse):
     p = tdse.Propagator(g, np.zeros(1024), 0.02, lattice=True)
     with pytest.raises(ValueError):
         p.set_drive((1, 0, 0), 0.1, 1.0)
+    with pytest.raises(ValueError):  # link angles B hx y of 1e5 rad and beyond
+        p.set_uniform_field(1e5)
     with pytest.raises(Exception):
         tdse.Batch([p])
     with pytest.raises(ValueError):
