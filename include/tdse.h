@@ -197,7 +197,8 @@ int tdse_plan_set_magnetic(tdse_plan* plan, double bfield, int axis);
  * (core/src/lattice2d.ixx:93-105).  A solenoid of flux phi in the cell
  * holding (xs, ys), string gauge: the x links of that column above it carry
  * exp(-i phi) (cut_up) or those below and at it exp(+i phi)
- * (core/src/lattice2d.ixx:60-91).  Each call replaces the links. */
+ * (core/src/lattice2d.ixx:60-91).  Each call replaces the links.  A uniform
+ * field whose link angles |B hx y| reach 1e5 rad returns TDSE_E_UNSUPPORTED. */
 int tdse_lattice_set_uniform_field(tdse_plan* plan, double b);
 int tdse_lattice_set_solenoid(tdse_plan* plan, double phi, double xs, double ys, int cut_up);
 int tdse_plan_get_time(const tdse_plan* plan, double* out);
