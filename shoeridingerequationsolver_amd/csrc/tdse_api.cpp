@@ -13,6 +13,7 @@
 #include <cstdint>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <thread>
@@ -74,12 +75,24 @@ struct Launch {
 
 }  // namespace
 
+// Two pinned host buffers per context for state uploads and downloads: the
+// host conversion of one chunk overlaps the DMA of the other.  Allocated on
+// the first transfer; without them transfers go through pageable memory.
+struct PinnedStage {
+    static constexpr size_t kBytes = size_t(32) << 20;
+    void* buf[2] = {nullptr, nullptr};
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    bool tried = false;
+    std::mutex mu;  // one transfer at a time per context
+};
+
 struct tdse_ctx {
     int device = 0;
     uint64_t budget = 0;
     uint64_t used = 0;
     int max_blocks = 1 << 20;
     std::mutex mu;
+    PinnedStage pin;
 };
 
 struct tdse_plan {
@@ -238,6 +251,33 @@ int d2h(const tdse_plan* p, void* dst, const void* src, size_t bytes) {
     return TDSE_OK;
 }
 
+// f(i0, i1) over [0, n) in contiguous chunks on up to 16 host threads (one
+// per 2^18 elements): the element-wise conversions between the
+// caller's FP64 arrays and the plan's storage, each element independent, so
+// the result does not depend on the split.
+template <class F>
+void parallel_chunks(long long n, F&& f) {
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    const long long want = std::min<long long>(std::min<unsigned>(hw, 16u), n >> 18);
+    if (want <= 1) {
+        f(0LL, n);
+        return;
+    }
+    std::vector<std::thread> pool;
+    const long long per = (n + want - 1) / want;
+    for (long long t = 1; t < want; ++t) {
+        const long long a = t * per, b = std::min(n, a + per);
+        if (a >= b) continue;
+        try {
+            pool.emplace_back([&f, a, b] { f(a, b); });
+        } catch (...) {  // no thread to be had: this one does the chunk
+            f(a, b);
+        }
+    }
+    f(0LL, std::min(n, per));
+    for (auto& th : pool) th.join();
+}
+
 enum RealForm { kPlain, kPhase, kTanHalf, kMask };
 
 // A per-cell real array in the plan's precision: plain values, kick phases
@@ -247,14 +287,88 @@ enum RealForm { kPlain, kPhase, kTanHalf, kMask };
 template <typename T>
 std::vector<T> to_real(const double* src, long long n, RealForm form) {
     std::vector<T> out(static_cast<size_t>(n));
-    for (long long i = 0; i < n; ++i) {
-        double x = src[i];
-        if (form == kPhase) x = reduce_2pi_host(x);
-        else if (form == kTanHalf) x = std::tan(0.5 * reduce_2pi_host(x));
-        else if (form == kMask && sizeof(T) == 4) x = 1.0 - x;
-        out[i] = static_cast<T>(x);
-    }
+    parallel_chunks(n, [&](long long i0, long long i1) {
+        for (long long i = i0; i < i1; ++i) {
+            double x = src[i];
+            if (form == kPhase) x = reduce_2pi_host(x);
+            else if (form == kTanHalf) x = std::tan(0.5 * reduce_2pi_host(x));
+            else if (form == kMask && sizeof(T) == 4) x = 1.0 - x;
+            out[i] = static_cast<T>(x);
+        }
+    });
     return out;
+}
+
+// The context's pinned buffers, allocated on first use (false: none to be had).
+bool pinned_stage(tdse_ctx* ctx) {
+    PinnedStage& st = ctx->pin;
+    if (!st.tried) {
+        st.tried = true;
+        bool ok = true;
+        for (int b = 0; b < 2 && ok; ++b)
+            ok = hipHostMalloc(&st.buf[b], PinnedStage::kBytes, hipHostMallocDefault) == hipSuccess &&
+                 hipEventCreateWithFlags(&st.ev[b], hipEventDisableTiming) == hipSuccess;
+        if (!ok) {
+            (void)hipGetLastError();
+            for (int b = 0; b < 2; ++b) {
+                if (st.buf[b]) (void)hipHostFree(st.buf[b]);
+                if (st.ev[b]) (void)hipEventDestroy(st.ev[b]);
+                st.buf[b] = nullptr;
+                st.ev[b] = nullptr;
+            }
+        }
+    }
+    return st.buf[0] != nullptr;
+}
+
+// n FP64 host values -> device array of T, through the pinned buffers in
+// chunks: chunk k + 1 is converted while chunk k is in flight.
+template <typename T>
+int upload_staged(tdse_plan* p, void* dst, const double* src, long long n) {
+    PinnedStage& st = p->ctx->pin;
+    const long long per = static_cast<long long>(PinnedStage::kBytes / sizeof(T));
+    for (long long off = 0, k = 0; off < n; off += per, ++k) {
+        const long long len = std::min(per, n - off);
+        const int b = static_cast<int>(k & 1);
+        TDSE_HIP(hipEventSynchronize(st.ev[b]));  // the buffer's previous DMA is done
+        T* h = static_cast<T*>(st.buf[b]);
+        parallel_chunks(len, [&](long long i0, long long i1) {
+            for (long long i = i0; i < i1; ++i) h[i] = static_cast<T>(src[off + i]);
+        });
+        TDSE_HIP(hipMemcpyAsync(static_cast<T*>(dst) + off, h, len * sizeof(T),
+                                hipMemcpyHostToDevice, p->stream));
+        TDSE_HIP(hipEventRecord(st.ev[b], p->stream));
+    }
+    TDSE_HIP(hipStreamSynchronize(p->stream));
+    return TDSE_OK;
+}
+
+// Device array of T (n values) -> FP64 host values: the DMA of chunk k + 1 is
+// in flight while chunk k is converted.
+template <typename T>
+int download_staged(tdse_plan* p, double* dst, const void* src, long long n) {
+    PinnedStage& st = p->ctx->pin;
+    const long long per = static_cast<long long>(PinnedStage::kBytes / sizeof(T));
+    const long long chunks = (n + per - 1) / per;
+    auto issue = [&](long long k) {
+        const long long off = k * per, len = std::min(per, n - off);
+        TDSE_HIP(hipMemcpyAsync(st.buf[k & 1], static_cast<const T*>(src) + off, len * sizeof(T),
+                                hipMemcpyDeviceToHost, p->stream));
+        TDSE_HIP(hipEventRecord(st.ev[k & 1], p->stream));
+        return TDSE_OK;
+    };
+    int rc = chunks > 0 ? issue(0) : TDSE_OK;
+    for (long long k = 0; k < chunks && !rc; ++k) {
+        // chunk k + 1 goes into the buffer chunk k - 1 was converted from
+        if (k + 1 < chunks && (rc = issue(k + 1))) break;
+        TDSE_HIP(hipEventSynchronize(st.ev[k & 1]));
+        const long long off = k * per, len = std::min(per, n - off);
+        const T* h = static_cast<const T*>(st.buf[k & 1]);
+        parallel_chunks(len, [&](long long i0, long long i1) {
+            for (long long i = i0; i < i1; ++i) dst[off + i] = static_cast<double>(h[i]);
+        });
+    }
+    return rc;
 }
 
 int upload_real(tdse_plan* p, void* dst, const double* src, RealForm form) {
@@ -1204,6 +1318,11 @@ int tdse_ctx_create(int device_ordinal, uint64_t hbm_budget_bytes, tdse_ctx** ou
 
 int tdse_ctx_destroy(tdse_ctx* ctx) {
     if (!ctx) return fail(TDSE_E_INVALID, "null ctx");
+    (void)hipSetDevice(ctx->device);
+    for (int b = 0; b < 2; ++b) {
+        if (ctx->pin.buf[b]) (void)hipHostFree(ctx->pin.buf[b]);
+        if (ctx->pin.ev[b]) (void)hipEventDestroy(ctx->pin.ev[b]);
+    }
     delete ctx;
     return TDSE_OK;
 }
@@ -1494,10 +1613,19 @@ int tdse_state_upload(tdse_plan* p, const double* psi) {
     if (rc) return rc;
     if (!psi) return fail(TDSE_E_INVALID, "null state");
     if ((rc = set_device(p->ctx))) return rc;
+    {
+        std::lock_guard<std::mutex> lk(p->ctx->pin.mu);
+        if (pinned_stage(p->ctx))
+            return p->fp64() ? upload_staged<double>(p, p->psi, psi, 2 * p->total)
+                             : upload_staged<float>(p, p->psi, psi, 2 * p->total);
+    }
     if (p->fp64()) return h2d(p, p->psi, psi, p->total * 16);
-    std::vector<float> f(2 * static_cast<size_t>(p->total));
-    for (long long i = 0; i < 2 * p->total; ++i) f[i] = static_cast<float>(psi[i]);
-    return h2d(p, p->psi, f.data(), f.size() * 4);
+    // left uninitialised: the converting threads touch its pages first
+    std::unique_ptr<float[]> f(new float[2 * static_cast<size_t>(p->total)]);
+    parallel_chunks(2 * p->total, [&](long long i0, long long i1) {
+        for (long long i = i0; i < i1; ++i) f[i] = static_cast<float>(psi[i]);
+    });
+    return h2d(p, p->psi, f.get(), 2 * static_cast<size_t>(p->total) * 4);
 }
 
 int tdse_state_download(tdse_plan* p, double* psi) {
@@ -1505,10 +1633,18 @@ int tdse_state_download(tdse_plan* p, double* psi) {
     if (rc) return rc;
     if (!psi) return fail(TDSE_E_INVALID, "null state");
     if ((rc = set_device(p->ctx))) return rc;
+    {
+        std::lock_guard<std::mutex> lk(p->ctx->pin.mu);
+        if (pinned_stage(p->ctx))
+            return p->fp64() ? download_staged<double>(p, psi, p->psi, 2 * p->total)
+                             : download_staged<float>(p, psi, p->psi, 2 * p->total);
+    }
     if (p->fp64()) return d2h(p, psi, p->psi, p->total * 16);
-    std::vector<float> f(2 * static_cast<size_t>(p->total));
-    if ((rc = d2h(p, f.data(), p->psi, f.size() * 4))) return rc;
-    for (long long i = 0; i < 2 * p->total; ++i) psi[i] = f[i];
+    std::unique_ptr<float[]> f(new float[2 * static_cast<size_t>(p->total)]);
+    if ((rc = d2h(p, f.get(), p->psi, 2 * static_cast<size_t>(p->total) * 4))) return rc;
+    parallel_chunks(2 * p->total, [&](long long i0, long long i1) {
+        for (long long i = i0; i < i1; ++i) psi[i] = f[i];
+    });
     return TDSE_OK;
 }
 
