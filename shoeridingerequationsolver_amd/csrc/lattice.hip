@@ -245,30 +245,39 @@ __global__ void __launch_bounds__(BS, 4) lattice_steps(LatticeArgs a, const void
         const int wy0 = r0 - S::HY > 0 ? r0 - S::HY : 0;
         const int wxn = (c0 + kTileW + S::HX < a.nx ? c0 + kTileW + S::HX : a.nx) - wx0;
         const int wyn = (r0 + kTileH + S::HY < a.ny ? r0 + kTileH + S::HY : a.ny) - wy0;
-        // every load of the window first (state, phases, mask), then LDS
-        cx<T> z[S::PER];
-        T ph[S::PER], mk[S::PER];
+        // the window in G rounds: every load of a round (state, phases,
+        // mask) is issued before its LDS writes; FP64 at 512 threads takes
+        // two rounds to stay within 128 VGPRs
+        constexpr int G = sizeof(T) == 8 && BS == 512 ? 2 : 1;
+        constexpr int PG = (S::PER + G - 1) / G;
+        const int ltid = phase_tid();
 #pragma unroll
-        for (int u = 0; u < S::PER; ++u) {
-            const int q = tid + u * BS, lr = q / W, lc = q % W;
-            if (lr < wyn && lc < wxn) {
-                const long f = static_cast<long>(wy0 + lr) * a.nx + wx0 + lc;
-                z[u] = src[f];
-                ph[u] = phi[f];
-                mk[u] = mask ? mask[f] : kMaskOne<T>;
+        for (int g = 0; g < G; ++g) {
+            cx<T> z[PG];
+            T ph[PG], mk[PG];
+#pragma unroll
+            for (int v = 0; v < PG; ++v) {
+                const int q = ltid + (g * PG + v) * BS, lr = q / W, lc = q % W;
+                if (g * PG + v < S::PER && lr < wyn && lc < wxn) {
+                    const long f = static_cast<long>(wy0 + lr) * a.nx + wx0 + lc;
+                    z[v] = src[f];
+                    ph[v] = phi[f];
+                    mk[v] = mask ? mask[f] : kMaskOne<T>;
+                }
             }
-        }
-        if (tid < wyn) {
-            const int j = wy0 + tid;
-            xk[tid] = bond_coef(a.cx, mix, a.link_mode == 1 ? row_link(a, j) : cx<double>{1.0, 0.0});
-        }
+            if (g == 0 && ltid < wyn) {
+                const int j = wy0 + ltid;
+                xk[ltid] = bond_coef(a.cx, mix,
+                                    a.link_mode == 1 ? row_link(a, j) : cx<double>{1.0, 0.0});
+            }
 #pragma unroll
-        for (int u = 0; u < S::PER; ++u) {
-            const int q = tid + u * BS, lr = q / W, lc = q % W;
-            if (lr < wyn && lc < wxn) {
-                w[lr][lc] = {static_cast<double>(z[u].x), static_cast<double>(z[u].y)};
-                sph[lr][lc] = ph[u];
-                smk[lr][lc] = mk[u];
+            for (int v = 0; v < PG; ++v) {
+                const int q = ltid + (g * PG + v) * BS, lr = q / W, lc = q % W;
+                if (g * PG + v < S::PER && lr < wyn && lc < wxn) {
+                    w[lr][lc] = {static_cast<double>(z[v].x), static_cast<double>(z[v].y)};
+                    sph[lr][lc] = ph[v];
+                    smk[lr][lc] = mk[v];
+                }
             }
         }
         __syncthreads();
@@ -296,7 +305,7 @@ __global__ void __launch_bounds__(BS, 4) lattice_steps(LatticeArgs a, const void
                 x_group_tile<W, S::XB, BS>(a, w, xk, &kc, wx0, wxn, wy0, wyn, 1);
             }
         }
-        for (int q = tid; q < kTileW * kTileH; q += BS) {
+        for (int q = phase_tid(); q < kTileW * kTileH; q += BS) {
             const int r = r0 + q / kTileW, c = c0 + q % kTileW;
             if (r < a.ny && c < a.nx) {
                 const cx<double> v = w[r - wy0][c - wx0];
@@ -393,15 +402,12 @@ hipError_t launch_lattice_steps(bool fp64, const LatticeArgs& a, const void* src
                            last_stage);
     };
     constexpr int KL = kLatStepsPerLaunch;
-    if (fp64) {  // the FP64 window does not fit 512 threads in 128 VGPRs without spilling
-        if (units == 1) go(lattice_steps<double, 1, 1024>, 1024);
-        else go(lattice_steps<double, KL, 1024>, 1024);
-    } else if (tiles >= 512) {  // two tiles per CU on 256 CUs
-        if (units == 1) go(lattice_steps<float, 1, 512>, 512);
-        else go(lattice_steps<float, KL, 512>, 512);
+    if (tiles >= 512) {  // two tiles per CU on 256 CUs
+        if (units == 1) fp64 ? go(lattice_steps<double, 1, 512>, 512) : go(lattice_steps<float, 1, 512>, 512);
+        else fp64 ? go(lattice_steps<double, KL, 512>, 512) : go(lattice_steps<float, KL, 512>, 512);
     } else {
-        if (units == 1) go(lattice_steps<float, 1, 1024>, 1024);
-        else go(lattice_steps<float, KL, 1024>, 1024);
+        if (units == 1) fp64 ? go(lattice_steps<double, 1, 1024>, 1024) : go(lattice_steps<float, 1, 1024>, 1024);
+        else fp64 ? go(lattice_steps<double, KL, 1024>, 1024) : go(lattice_steps<float, KL, 1024>, 1024);
     }
     return hipGetLastError();
 }
