@@ -793,6 +793,16 @@ __device__ __forceinline__ void contig_tile(const SweepArgs& a, long long tile, 
     }
 }
 
+// threadIdx.x as a value the compiler cannot see through (the asm emits no
+// instruction): read once per tile, it keeps the tile's thread-indexed LDS
+// and global offsets inside the tile instead of hoisted out of the tile loop
+// and held in VGPRs across it.
+__device__ __forceinline__ int opaque_tid() {
+    int t = static_cast<int>(threadIdx.x);
+    asm volatile("" : "+v"(t));
+    return t;
+}
+
 // One tile of a strided sweep: TX consecutive lines.  `kp` holds the staged
 // line-axis kinetic table (times 1/N) for S_KIN.
 template <typename T, int N, int MODE, int V = 0>
@@ -924,7 +934,7 @@ contig_sweep(SweepArgs a) {
     SweepAcc ra;
     const long long ntiles = (a.nlines + S::LPB - 1) / S::LPB;
     for (long long tile = blockIdx.x; tile < ntiles; tile += gridDim.x)
-        contig_tile<T, N, MODE, V>(a, tile, lds, lds_tw, tid, ra);
+        contig_tile<T, N, MODE, V>(a, tile, lds, lds_tw, opaque_tid(), ra);
     if constexpr (mode_measures(MODE)) relax_block_done(a, ra);
     if constexpr (mode_observes(MODE)) obs_block_done(a, ra);
 }
@@ -947,7 +957,7 @@ strided_sweep(SweepArgs a) {
     SweepAcc ra;
     const long long ntiles = static_cast<long long>((a.nx + S::TX - 1) / S::TX) * a.nw;
     for (long long tile = blockIdx.x; tile < ntiles; tile += gridDim.x)
-        strided_tile<T, N, MODE, V>(a, tile, lds, lds_tw, lds_kp, tid, ra);
+        strided_tile<T, N, MODE, V>(a, tile, lds, lds_tw, lds_kp, opaque_tid(), ra);
     if constexpr (mode_measures(MODE)) relax_block_done(a, ra);
     if constexpr (mode_observes(MODE)) obs_block_done(a, ra);
 }
@@ -996,9 +1006,10 @@ contig_sweep_batch(BatchTable b) {
         if constexpr (MODE == C_SINGLE) {
             SweepArgs a = b.items[it];
             a.nsteps = b.nsteps;
-            contig_tile<T, N, MODE>(a, g - b.first_tile[it], lds, lds_tw, tid, unused);
+            contig_tile<T, N, MODE>(a, g - b.first_tile[it], lds, lds_tw, opaque_tid(), unused);
         } else {
-            contig_tile<T, N, MODE>(b.items[it], g - b.first_tile[it], lds, lds_tw, tid, unused);
+            contig_tile<T, N, MODE>(b.items[it], g - b.first_tile[it], lds, lds_tw, opaque_tid(),
+                                    unused);
         }
     }
 }
@@ -1032,8 +1043,8 @@ strided_sweep_batch(BatchTable b) {
             }
         }
         SweepAcc unused;
-        strided_tile<T, N, MODE>(b.items[it], g - b.first_tile[it], lds, lds_tw, lds_kp, tid,
-                                 unused);
+        strided_tile<T, N, MODE>(b.items[it], g - b.first_tile[it], lds, lds_tw, lds_kp,
+                                 opaque_tid(), unused);
     }
 }
 
