@@ -308,6 +308,18 @@ struct Last<Radices<R, R2, Rest...>> {
     static constexpr int value = Last<Radices<R2, Rest...>>::value;
 };
 
+template <class L>
+struct MinRadix;
+template <int R>
+struct MinRadix<Radices<R>> {
+    static constexpr int value = R;
+};
+template <int R, int R2, int... Rest>
+struct MinRadix<Radices<R, R2, Rest...>> {
+    static constexpr int value =
+        R < MinRadix<Radices<R2, Rest...>>::value ? R : MinRadix<Radices<R2, Rest...>>::value;
+};
+
 // ---- FP64 phase application ------------------------------------------------------
 
 // a * k with a widened to FP64 and the product rounded once to T.

@@ -98,6 +98,10 @@ struct Shape {
     static constexpr int TT = N / E;  // threads per line
     static constexpr int PASSES = P::Fwd::count;
     static constexpr bool EXCHANGE = PASSES > 1;
+    // Twiddle entries the passes read: a pass of radix R after Ns points reads
+    // W^{k N / (Ns R)}, k < Ns, all below N / R; the inverse runs the radix list
+    // reversed, so every radix has a pass after the first: N / (least radix).
+    static constexpr int TWN = EXCHANGE ? N / MinRadix<typename P::Fwd>::value : 1;
     static constexpr int CSZ = static_cast<int>(sizeof(cx<T>));
     static constexpr int LSTR = lds_len(N);
     // FP64 tables staged in LDS: twiddles and (strided kinetic sweep) K_p

@@ -921,10 +921,10 @@ __global__ void __launch_bounds__((Shape<T, N, V>::BLOCK_C), TDSE_MIN_WAVES)
 contig_sweep(SweepArgs a) {
     using S = Shape<T, N, V>;
     __shared__ cx<T> lds[S::LDS_C];
-    __shared__ cx<double> lds_tw[S::EXCHANGE ? N : 1];
+    __shared__ cx<double> lds_tw[S::TWN];
     const int tid = threadIdx.x;
     if constexpr (S::EXCHANGE) {
-        stage_table<N>(lds_tw, a.tw, tid, S::BLOCK_C);
+        stage_table<S::TWN>(lds_tw, a.tw, tid, S::BLOCK_C);
         __syncthreads();
     }
     if constexpr (MODE == C_RELAX_FIRST) {
@@ -945,10 +945,10 @@ strided_sweep(SweepArgs a) {
     using S = Shape<T, N, V>;
     constexpr bool KIN = MODE == S_KIN;
     __shared__ cx<T> lds[S::LDS_S];
-    __shared__ cx<double> lds_tw[S::EXCHANGE ? N : 1];
+    __shared__ cx<double> lds_tw[S::TWN];
     __shared__ cx<double> lds_kp[KIN ? N : 1];
     const int tid = threadIdx.x;
-    if constexpr (S::EXCHANGE) stage_table<N>(lds_tw, a.tw, tid, S::BLOCK_S);
+    if constexpr (S::EXCHANGE) stage_table<S::TWN>(lds_tw, a.tw, tid, S::BLOCK_S);
     if constexpr (KIN) stage_kinetic<T, N>(lds_kp, a, tid, S::BLOCK_S);
     if constexpr (S::EXCHANGE || KIN) __syncthreads();
     if constexpr (MODE == S_RELAX_FIRST_TW) {
@@ -989,10 +989,10 @@ __global__ void __launch_bounds__((Shape<T, N>::BLOCK_C), TDSE_MIN_WAVES)
 contig_sweep_batch(BatchTable b) {
     using S = Shape<T, N>;
     __shared__ cx<T> lds[S::LDS_C];
-    __shared__ cx<double> lds_tw[S::EXCHANGE ? N : 1];
+    __shared__ cx<double> lds_tw[S::TWN];
     const int tid = threadIdx.x;
     if constexpr (S::EXCHANGE) {
-        stage_table<N>(lds_tw, b.items[0].tw, tid, S::BLOCK_C);
+        stage_table<S::TWN>(lds_tw, b.items[0].tw, tid, S::BLOCK_C);
         __syncthreads();
     }
     // contiguous chunk of tiles per block: consecutive tiles mostly share a plan
@@ -1020,11 +1020,11 @@ strided_sweep_batch(BatchTable b) {
     using S = Shape<T, N>;
     constexpr bool KIN = MODE == S_KIN;
     __shared__ cx<T> lds[S::LDS_S];
-    __shared__ cx<double> lds_tw[S::EXCHANGE ? N : 1];
+    __shared__ cx<double> lds_tw[S::TWN];
     __shared__ cx<double> lds_kp[KIN ? N : 1];
     const int tid = threadIdx.x;
     if constexpr (S::EXCHANGE) {
-        stage_table<N>(lds_tw, b.items[0].tw, tid, S::BLOCK_S);
+        stage_table<S::TWN>(lds_tw, b.items[0].tw, tid, S::BLOCK_S);
         __syncthreads();
     }
     const long long total = b.first_tile[b.nitems];
