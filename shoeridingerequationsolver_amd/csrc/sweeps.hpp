@@ -1000,8 +1000,10 @@ contig_sweep_batch(BatchTable b) {
     const long long chunk = (total + gridDim.x - 1) / gridDim.x;
     const long long g0 = blockIdx.x * chunk;
     const long long g1 = g0 + chunk < total ? g0 + chunk : total;
+    // one search per block; consecutive tiles belong to the same or a later item
+    int it = g0 < g1 ? find_item(b, g0) : 0;
     for (long long g = g0; g < g1; ++g) {
-        const int it = find_item(b, g);
+        while (g >= b.first_tile[it + 1]) ++it;
         SweepAcc unused;
         if constexpr (MODE == C_SINGLE) {
             SweepArgs a = b.items[it];
@@ -1032,8 +1034,9 @@ strided_sweep_batch(BatchTable b) {
     const long long g0 = blockIdx.x * chunk;
     const long long g1 = g0 + chunk < total ? g0 + chunk : total;
     int staged = -1;
+    int it = g0 < g1 ? find_item(b, g0) : 0;
     for (long long g = g0; g < g1; ++g) {
-        const int it = find_item(b, g);
+        while (g >= b.first_tile[it + 1]) ++it;
         if constexpr (KIN) {
             if (it != staged) {  // block-uniform: every thread sees the same tile
                 __syncthreads();
