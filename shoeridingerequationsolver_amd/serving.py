@@ -162,6 +162,7 @@ class Server:
         max_active = 0
         peak = 0
         nxt = 0                         # next trace index whose inputs may be prebuilt
+        batch, members = None, ()       # the batch of the current active set, reused
 
         def now():
             return time.perf_counter() - t0
@@ -203,10 +204,14 @@ class Server:
                 continue
             max_active = max(max_active, len(active))
             s = min(self.quantum, min(a.remaining for a in active))
-            batch = Batch([a.prop for a in active], context=self.ctx)
+            # a new batch (device tables, streams) only when the active set changed
+            if tuple(id(a.prop) for a in active) != members:
+                if batch is not None:
+                    batch.close()
+                batch = Batch([a.prop for a in active], context=self.ctx)
+                members = tuple(id(a.prop) for a in active)
             batch.step(s)
             batch.sync()
-            batch.close()
             quanta += 1
             still = []
             for a in active:
@@ -216,10 +221,15 @@ class Server:
                     results.append(Result(a.req.name, a.arrival, a.admitted, now(), psi,
                                           a.req.steps, a.req.precision, a.req.n))
                     reserved -= self.footprint(a.req)
+                    if batch is not None:  # the batch holds the finished plan
+                        batch.close()
+                        batch, members = None, ()
                     a.prop.close()
                 else:
                     still.append(a)
             active = still
+        if batch is not None:
+            batch.close()
         return ServeReport(results, now(), quanta, max_active, peak)
 
 
