@@ -232,3 +232,29 @@ def test_large_kick_angles_match_oracle(orc, tdse, dims, prec):
     err = rel(p.download(), want)
     print(f"  rel err {err:.3e}", flush=True)
     assert err < (FP64_TOL if prec == "fp64" else FP32_TOL)
+
+
+def test_fp32_long_run_stays_near_fp64(tdse):
+    """20000 steps of the 128^3 hydrogen workload without a mask (a unitary
+    map): the FP32 state's norm drifts by ~1.5e-6 and its largest point error
+    against the FP64 state is ~5e-7 (measured); the bounds hold ~7x margin.
+    The FP64 state keeps its norm to ~1e-11."""
+    from shoeridingerequationsolver_amd import workloads
+    out = {}
+    for prec in ("fp64", "fp32"):
+        s = workloads.hydrogen(precision=prec)
+        r = s.recipe()
+        r.mask_width = 0.0
+        p = tdse.Propagator(s.grid(), None, s.dt, precision=prec)
+        p.build(r)
+        n0 = p.observe(1)["norm"]
+        p.step(20000)
+        n1 = p.observe(1)["norm"]
+        out[prec] = (p.download(), n1 / n0 - 1.0)
+        print(f"  {prec}: norm drift {n1 / n0 - 1.0:.3e}", flush=True)
+        p.close()
+    assert abs(out["fp64"][1]) < 1e-10
+    assert abs(out["fp32"][1]) < 1e-5
+    err = float(np.abs(out["fp32"][0] - out["fp64"][0]).max())
+    print(f"  max|dpsi| {err:.3e}", flush=True)
+    assert err < 3.5e-6
