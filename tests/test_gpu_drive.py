@@ -131,3 +131,39 @@ def test_driven_step_facade_matches_oracle(orc, tdse, prec):
     psi = psi0.copy()
     prop.step(psi, 9)
     assert rel(psi, orc.so3d_evolve(v, dims, lo, hi, 0.02, psi0, 9)) < tol
+
+
+def _radial_mask(dims, lo, hi, width):
+    axes = [lo[a] + np.arange(dims[a]) * (hi[a] - lo[a]) / dims[a] for a in range(3)]
+    z, y, x = np.meshgrid(axes[2], axes[1], axes[0], indexing="ij")
+    r = np.sqrt(x * x + y * y + z * z)
+    d = np.clip((min(hi) - r) / width, 0.0, 1.0)
+    return (np.sin(0.5 * np.pi * d) ** 2).ravel()
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("kind", ["ramps", "radial"])
+def test_driven_step_with_mask_matches_oracle(orc, tdse, kind, prec):
+    """Drive and absorbing mask together: each step is driven_step, then the
+    caller's mask (the oracle steps one at a time).  A product-of-ramps mask
+    takes the x sweeps' ramp path, a radial one the per-cell array."""
+    dims, lo, hi = (64, 32, 16), (-10.0, -6.0, -4.0), (10.0, 6.0, 4.0)
+    axis, amp, omega, steps, dt, t0 = (0.8, 0.0, 0.6), 0.3, 1.1, 12, 0.02, 0.21
+    print(f"case drive+mask {kind} prec={prec}", flush=True)
+    v = orc.harmonic_3d(dims, lo, hi, 0.3, (0.2, 0.0, 0.0))
+    m = orc.absorbing_mask_3d(dims, lo, hi, 2.5) if kind == "ramps" else _radial_mask(dims, lo, hi, 2.5)
+    psi0 = orc.gaussian_3d(dims, lo, hi, (5.0, 0.0, 0.0), (1.3, 1.1, 0.9),
+                           (1.4, 0.0, 0.0)).reshape(dims[::-1])
+    want = psi0.copy()
+    for s in range(steps):
+        want = orc.driven_evolve(v, dims, lo, hi, dt, want, 1, axis, amp, omega, t0=t0 + s * dt)
+        want = (want.ravel() * m).reshape(want.shape)
+    p = tdse.Propagator(grid3(tdse, dims, lo, hi), v, dt, precision=prec, mask=m)
+    p.set_drive(axis, amp, omega)
+    p.time = t0
+    p.upload(psi0)
+    p.step(5)
+    p.step(steps - 5)
+    err = rel(p.download(), want)
+    print(f"  rel err {err:.3e}", flush=True)
+    assert err < TOL[prec]
