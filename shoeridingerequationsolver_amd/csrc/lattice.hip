@@ -167,15 +167,6 @@ struct StepTile {
     static constexpr int XB = (H * (W / 2) + BS - 1) / BS;        // x bonds per thread
 };
 
-// threadIdx.x as a value the compiler cannot see through: each phase forms
-// its own LDS indices instead of hoisting all phases' indices out of the step
-// loop and keeping them live (which spills).  The asm emits no instruction.
-__device__ __forceinline__ int phase_tid() {
-    int t = static_cast<int>(threadIdx.x);
-    asm volatile("" : "+v"(t));
-    return t;
-}
-
 // y bonds (j, j + 1), j = parity mod 2, of the window rows [0, wyn) (row 0
 // is lattice row wy0) over columns [0, wxn)
 template <int W, int YB, int BS>
@@ -183,7 +174,7 @@ __device__ __forceinline__ void y_group_tile(cx<double> (*w)[W], int wxn, int wy
                                              int parity, const BondCoef& k) {
     const int first = (wy0 & 1) != parity ? 1 : 0;
     const int nb = (wyn - first) / 2;
-    const int tid = phase_tid();
+    const int tid = opaque_tid();
 #pragma unroll
     for (int u = 0; u < YB; ++u) {
         const int q = tid + u * BS;
@@ -201,7 +192,7 @@ __device__ __forceinline__ void x_group_tile(const LatticeArgs& a, cx<double> (*
                                              const BondCoef* xk, const BondCoef* kc, int wx0,
                                              int wxn, int wy0, int wyn, int parity) {
     constexpr int NB = W / 2;
-    const int tid = phase_tid();
+    const int tid = opaque_tid();
 #pragma unroll
     for (int u = 0; u < XB; ++u) {
         const int q = tid + u * BS;
@@ -250,7 +241,7 @@ __global__ void __launch_bounds__(BS, 4) lattice_steps(LatticeArgs a, const void
         // two rounds to stay within 128 VGPRs
         constexpr int G = sizeof(T) == 8 && BS == 512 ? 2 : 1;
         constexpr int PG = (S::PER + G - 1) / G;
-        const int ltid = phase_tid();
+        const int ltid = opaque_tid();
 #pragma unroll
         for (int g = 0; g < G; ++g) {
             cx<T> z[PG];
@@ -290,7 +281,7 @@ __global__ void __launch_bounds__(BS, 4) lattice_steps(LatticeArgs a, const void
             x_group_tile<W, S::XB, BS>(a, w, xk, &kc, wx0, wxn, wy0, wyn, 0);
             // end of one step (and its mask), start of the next: merged
             const double f = stage == 1 ? 2.0 : 1.0;
-            const int ptid = phase_tid();
+            const int ptid = opaque_tid();
 #pragma unroll
             for (int u = 0; u < S::PER; ++u) {
                 const int q = ptid + u * BS, lr = q / W, lc = q % W;
@@ -305,7 +296,7 @@ __global__ void __launch_bounds__(BS, 4) lattice_steps(LatticeArgs a, const void
                 x_group_tile<W, S::XB, BS>(a, w, xk, &kc, wx0, wxn, wy0, wyn, 1);
             }
         }
-        for (int q = phase_tid(); q < kTileW * kTileH; q += BS) {
+        for (int q = opaque_tid(); q < kTileW * kTileH; q += BS) {
             const int r = r0 + q / kTileW, c = c0 + q % kTileW;
             if (r < a.ny && c < a.nx) {
                 const cx<double> v = w[r - wy0][c - wx0];

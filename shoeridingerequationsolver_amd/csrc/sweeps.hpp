@@ -794,9 +794,9 @@ __device__ __forceinline__ void contig_tile(const SweepArgs& a, long long tile, 
 }
 
 // threadIdx.x as a value the compiler cannot see through (the asm emits no
-// instruction): read once per tile, it keeps the tile's thread-indexed LDS
-// and global offsets inside the tile instead of hoisted out of the tile loop
-// and held in VGPRs across it.
+// instruction): read once per tile (or per phase of a multi-phase tile), it
+// keeps thread-indexed LDS and global offsets where they are used instead of
+// hoisted out of the enclosing loop and held in VGPRs across it.
 __device__ __forceinline__ int opaque_tid() {
     int t = static_cast<int>(threadIdx.x);
     asm volatile("" : "+v"(t));
