@@ -1881,69 +1881,27 @@ int tdse_observe(tdse_plan* p, unsigned what, tdse_observables* out) {
     if ((rc = set_device(p->ctx))) return rc;
     std::memset(out, 0, sizeof(*out));
     tdse::ReduceArgs r = reduce_args(p);
-    double host[tdse::kReduceQ];
     TDSE_HIP(tdse::launch_reduce(p->fp64(), false, r, p->partials, p->red_out, p->stream));
-    TDSE_HIP(hipMemcpyAsync(host, p->red_out, sizeof(host), hipMemcpyDeviceToHost, p->stream));
-    TDSE_HIP(hipStreamSynchronize(p->stream));
-    const double cell = r.h[0] * r.h[1] * r.h[2];
     auto ratio = [](double num, double den) { return den > 0.0 ? num / den : 0.0; };
-    out->sum_abs2 = host[0];
-    out->norm = host[0] * cell;
-    out->position[0] = ratio(host[1], host[0]);
-    out->position[1] = ratio(host[2], host[0]);
-    out->position[2] = ratio(host[3], host[0]);
-    out->potential = ratio(host[4], host[0]);
-    for (int a = 0; a < 3; ++a) {
-        const double m = ratio(host[1 + a], host[0]);
-        out->sigma[a] = std::sqrt(std::max(0.0, ratio(host[6 + a], host[0]) - m * m));
-    }
-    if (!(what & (TDSE_OBS_ENERGY | TDSE_OBS_MOMENTUM))) return TDSE_OK;
-    if (p->kind == kKindLattice) {
-        // <H> of the lattice Hamiltonian (hops + onsite, core/src/lattice2d.ixx:251-274)
-        if (what & TDSE_OBS_MOMENTUM)
-            return fail(TDSE_E_UNSUPPORTED, "no momentum observable on the lattice");
-        double* dst = p->red_out + 24;
-        TDSE_HIP(tdse::launch_lattice_energy(p->fp64(), lattice_args(p), p->partials, dst, p->stream));
-        double hk[3];
-        if ((rc = d2h(p, hk, dst, sizeof(hk)))) return rc;
-        out->energy = ratio(hk[0], hk[1]);
-        out->kinetic = out->energy - out->potential;
-        return TDSE_OK;
-    }
+    const bool want_k = (what & (TDSE_OBS_ENERGY | TDSE_OBS_MOMENTUM)) != 0;
     // original axis of each squeezed axis
     int orig[3] = {0, 0, 0};
     for (int a = 0, q = 0; a < 3; ++a)
         if (p->n[a] > 1) orig[q++] = a;
-    if (p->kind == kKindFourStep) {
-        // the 1D transform of a four-step grid, in a scratch copy
-        if (!p->scratch && (rc = alloc(p, &p->scratch, p->total * p->csize()))) return rc;
-        TDSE_HIP(hipMemcpyAsync(p->scratch, p->psi, p->total * p->csize(), hipMemcpyDeviceToDevice,
-                                p->stream));
-        Geometry g = geometry(p, p->scratch);
-        std::vector<Launch> L;
-        L.push_back({tdse::S_FWD_TW, p->n2, g.col, launcher(p, p->n2, tdse::S_FWD_TW), 0});
-        L.push_back({tdse::C_FWD, p->n1, g.row, launcher(p, p->n1, tdse::C_FWD), 0});
-        if ((rc = run_launches(p, L))) return rc;
-        tdse::ReduceArgs rk = r;
-        rk.psi = p->scratch;
-        rk.v = nullptr;
-        rk.sq[0] = p->n1;
-        rk.sq[1] = p->n2;
-        rk.four_step_rows = p->n2;
-        double hk[tdse::kReduceQ];
-        TDSE_HIP(tdse::launch_reduce(p->fp64(), true, rk, p->partials, p->red_out, p->stream));
-        TDSE_HIP(hipMemcpyAsync(hk, p->red_out, sizeof(hk), hipMemcpyDeviceToHost, p->stream));
-        TDSE_HIP(hipStreamSynchronize(p->stream));
-        out->kinetic = ratio(hk[1], hk[0]);
-        out->momentum[orig[0]] = ratio(hk[2], hk[0]);
-    } else {
-        // Separable along axes: sum_k k_a^2 |psi_k|^2 / sum_k |psi_k|^2 equals
-        // the same ratio over the 1D spectra of the lines along axis a (the
-        // transforms along the other axes are unitary up to a constant), so
-        // one read-only line-FFT sweep per axis gives <k_a> and <k_a^2>.
+    // Grids swept by lines (not the lattice, not a four-step 1D grid): the
+    // k-space sweeps are queued behind the position-space reduction (its final
+    // pass has consumed the partials they reuse) and both results come back
+    // in one copy.  Separable along axes: sum_k k_a^2 |psi_k|^2 / sum_k
+    // |psi_k|^2 equals the same ratio over the 1D spectra of the lines along
+    // axis a (the transforms along the other axes are unitary up to a
+    // constant), so one read-only line-FFT sweep per axis gives <k_a> and
+    // <k_a^2>.
+    const bool line_k = want_k && p->kind != kKindLattice && p->kind != kKindFourStep;
+    const int axes = static_cast<int>(p->sq.size());
+    constexpr int kObsAt = 24;  // offset of the per-axis spectra sums in red_out
+    static_assert(tdse::kReduceQ <= kObsAt, "position sums and spectra sums overlap");
+    if (line_k) {
         const Geometry g = geometry(p, p->psi);
-        const int axes = static_cast<int>(p->sq.size());
-        double* dst = p->red_out + 24;
         for (int a = 0; a < axes; ++a) {
             SweepArgs args;
             int mode, n = p->sq[a].n;
@@ -1967,10 +1925,63 @@ int tdse_observe(tdse_plan* p, unsigned what, tdse_observables* out) {
             TDSE_HIP(fn(args, tdse::kObsMaxBlocks, p->stream));
             const long long tiles = tiles_of(p->precision, n, mode, args);
             const int blocks = static_cast<int>(std::min<long long>(tiles, tdse::kObsMaxBlocks));
-            TDSE_HIP(tdse::launch_reduce_final3(p->partials, blocks, dst + 3 * a, p->stream));
+            TDSE_HIP(tdse::launch_reduce_final3(p->partials, blocks, p->red_out + kObsAt + 3 * a,
+                                                p->stream));
         }
-        double hk[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-        if ((rc = d2h(p, hk, dst, sizeof(double) * 3 * axes))) return rc;
+    }
+    double host[kObsAt + 9] = {};
+    const size_t count = line_k ? static_cast<size_t>(kObsAt + 3 * axes) : tdse::kReduceQ;
+    TDSE_HIP(hipMemcpyAsync(host, p->red_out, count * sizeof(double), hipMemcpyDeviceToHost,
+                            p->stream));
+    TDSE_HIP(hipStreamSynchronize(p->stream));
+    const double cell = r.h[0] * r.h[1] * r.h[2];
+    out->sum_abs2 = host[0];
+    out->norm = host[0] * cell;
+    out->position[0] = ratio(host[1], host[0]);
+    out->position[1] = ratio(host[2], host[0]);
+    out->position[2] = ratio(host[3], host[0]);
+    out->potential = ratio(host[4], host[0]);
+    for (int a = 0; a < 3; ++a) {
+        const double m = ratio(host[1 + a], host[0]);
+        out->sigma[a] = std::sqrt(std::max(0.0, ratio(host[6 + a], host[0]) - m * m));
+    }
+    if (!want_k) return TDSE_OK;
+    if (p->kind == kKindLattice) {
+        // <H> of the lattice Hamiltonian (hops + onsite, core/src/lattice2d.ixx:251-274)
+        if (what & TDSE_OBS_MOMENTUM)
+            return fail(TDSE_E_UNSUPPORTED, "no momentum observable on the lattice");
+        double* dst = p->red_out + kObsAt;
+        TDSE_HIP(tdse::launch_lattice_energy(p->fp64(), lattice_args(p), p->partials, dst, p->stream));
+        double hk[3];
+        if ((rc = d2h(p, hk, dst, sizeof(hk)))) return rc;
+        out->energy = ratio(hk[0], hk[1]);
+        out->kinetic = out->energy - out->potential;
+        return TDSE_OK;
+    }
+    if (p->kind == kKindFourStep) {
+        // the 1D transform of a four-step grid, in a scratch copy
+        if (!p->scratch && (rc = alloc(p, &p->scratch, p->total * p->csize()))) return rc;
+        TDSE_HIP(hipMemcpyAsync(p->scratch, p->psi, p->total * p->csize(), hipMemcpyDeviceToDevice,
+                                p->stream));
+        Geometry g = geometry(p, p->scratch);
+        std::vector<Launch> L;
+        L.push_back({tdse::S_FWD_TW, p->n2, g.col, launcher(p, p->n2, tdse::S_FWD_TW), 0});
+        L.push_back({tdse::C_FWD, p->n1, g.row, launcher(p, p->n1, tdse::C_FWD), 0});
+        if ((rc = run_launches(p, L))) return rc;
+        tdse::ReduceArgs rk = r;
+        rk.psi = p->scratch;
+        rk.v = nullptr;
+        rk.sq[0] = p->n1;
+        rk.sq[1] = p->n2;
+        rk.four_step_rows = p->n2;
+        double hk[tdse::kReduceQ];
+        TDSE_HIP(tdse::launch_reduce(p->fp64(), true, rk, p->partials, p->red_out, p->stream));
+        TDSE_HIP(hipMemcpyAsync(hk, p->red_out, sizeof(hk), hipMemcpyDeviceToHost, p->stream));
+        TDSE_HIP(hipStreamSynchronize(p->stream));
+        out->kinetic = ratio(hk[1], hk[0]);
+        out->momentum[orig[0]] = ratio(hk[2], hk[0]);
+    } else {
+        const double* hk = host + kObsAt;
         double t = 0.0;
         for (int a = 0; a < axes; ++a) {
             t += 0.5 * ratio(hk[3 * a + 2], hk[3 * a]) / p->mass;
