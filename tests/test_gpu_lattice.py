@@ -53,6 +53,8 @@ CASES = [
     ((768, 704, 1), (-30.0, -28.0, -1.0), (30.0, 28.0, 1.0),
      dict(links=2, a=1.7, xs=0.9, ys=-1.1), 5),
     ((1024, 512, 1), (-40.0, -20.0, -1.0), (40.0, 20.0, 1.0), dict(links=1, a=0.15), 8),
+    # a strong field: x-link angles B hx y up to ~1.2e3 rad, reduced in the kernels
+    ((96, 80, 1), (-30.0, -40.0, -1.0), (30.0, 40.0, 1.0), dict(links=1, a=48.0), 10),
 ]
 
 
