@@ -382,20 +382,7 @@ hipError_t launch_scale(bool fp64, void* psi, long long total, double s, hipStre
 // (a zero field is left as it is, core/src/field.ixx:78-81), norm and peak.
 __global__ void __launch_bounds__(256) relax_finish(const double* partials, RelaxSlots* slots,
                                                     int parity, double cell) {
-    const unsigned nb = slots->nblocks;
-    double ts = 0.0, tm = 0.0;
-    for (unsigned b = threadIdx.x; b < nb; b += blockDim.x) {
-        ts += partials[2 * b];
-        const double q = partials[2 * b + 1];
-        tm = q > tm ? q : tm;
-    }
-    block_sum_max(ts, tm);
-    if (threadIdx.x == 0) {
-        const double norm = ts * cell;
-        slots->scale[parity ^ 1] = norm > 0.0 ? 1.0 / sqrt(norm) : 1.0;
-        slots->norm_sq = norm;
-        slots->peak = tm;
-    }
+    relax_finish_block(partials, slots, parity, cell);
 }
 
 hipError_t launch_relax_finish(const SweepArgs& a, int, hipStream_t stream) {

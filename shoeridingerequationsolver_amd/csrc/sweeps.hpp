@@ -174,6 +174,13 @@ struct SweepArgs {
     const double* mask_yz;
     uint32_t mask_bytes;  // byte extent of mask
     int mask_sep;
+    // the previous measuring sweep's norm, finished by block 0 of this sweep
+    // before its tiles (fin_slots null: nothing to finish): sum and max of
+    // fin_slots->nblocks partials -> scale[fin_parity ^ 1], norm_sq, peak
+    const double* fin_partials;
+    RelaxSlots* fin_slots;
+    double fin_cell;
+    int fin_parity;
 };
 
 // ---- buffer access -----------------------------------------------------------
@@ -581,6 +588,32 @@ __device__ __forceinline__ void block_sum_max(double& s, double& m) {
     }
 }
 
+// A measuring sweep's per-block (sum |psi|^2, max |psi|^2) partials -> the
+// step's norm, peak and the next measuring sweep's scale, in one block
+// (fixed order: the same partition and tree whichever block runs it).
+__device__ __forceinline__ void relax_finish_block(const double* partials, RelaxSlots* slots,
+                                                   int parity, double cell) {
+    const unsigned nb = slots->nblocks;
+    double ts = 0.0, tm = 0.0;
+    for (unsigned b = threadIdx.x; b < nb; b += blockDim.x) {
+        ts += partials[2 * b];
+        const double q = partials[2 * b + 1];
+        tm = q > tm ? q : tm;
+    }
+    block_sum_max(ts, tm);
+    if (threadIdx.x == 0) {
+        const double norm = ts * cell;
+        slots->scale[parity ^ 1] = norm > 0.0 ? 1.0 / sqrt(norm) : 1.0;
+        slots->norm_sq = norm;
+        slots->peak = tm;
+    }
+}
+
+// Sweeps that can follow a measuring sweep and finish its norm.
+constexpr bool mode_finishes(int mode) {
+    return mode == S_FWD || mode == S_KIN || mode == C_KIN_TW;
+}
+
 // End of a measuring sweep, per block: this block's (sum, max) of |psi|^2
 // (fixed-order block reduction) into partials[2 b], partials[2 b + 1]; block 0
 // records the grid size.  launch_relax_finish, queued right after the sweep,
@@ -923,6 +956,10 @@ contig_sweep(SweepArgs a) {
     __shared__ cx<T> lds[S::LDS_C];
     __shared__ cx<double> lds_tw[S::TWN];
     const int tid = threadIdx.x;
+    if constexpr (mode_finishes(MODE)) {
+        if (a.fin_slots && blockIdx.x == 0)
+            relax_finish_block(a.fin_partials, a.fin_slots, a.fin_parity, a.fin_cell);
+    }
     if constexpr (S::EXCHANGE) {
         stage_table<S::TWN>(lds_tw, a.tw, tid, S::BLOCK_C);
         __syncthreads();
@@ -948,6 +985,10 @@ strided_sweep(SweepArgs a) {
     __shared__ cx<double> lds_tw[S::TWN];
     __shared__ cx<double> lds_kp[KIN ? N : 1];
     const int tid = threadIdx.x;
+    if constexpr (mode_finishes(MODE)) {
+        if (a.fin_slots && blockIdx.x == 0)
+            relax_finish_block(a.fin_partials, a.fin_slots, a.fin_parity, a.fin_cell);
+    }
     if constexpr (S::EXCHANGE) stage_table<S::TWN>(lds_tw, a.tw, tid, S::BLOCK_S);
     if constexpr (KIN) stage_kinetic<T, N>(lds_kp, a, tid, S::BLOCK_S);
     if constexpr (S::EXCHANGE || KIN) __syncthreads();

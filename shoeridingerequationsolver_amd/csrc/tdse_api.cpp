@@ -800,7 +800,23 @@ std::vector<Launch> schedule(const tdse_plan* p, int nsteps, void* target = null
             add(s + 1 < nsteps ? 2 : 3, s);
         }
     }
-    return L;
+    // A finish followed by a sweep that can run it (the next step's first
+    // sweep after its measuring x sweep) becomes that sweep's block-0 preamble:
+    // one launch fewer per step.  The call's last finish stays a launch.
+    std::vector<Launch> F;
+    F.reserve(L.size());
+    for (size_t i = 0; i < L.size(); ++i) {
+        if (L[i].mode == kFinishMode && i + 1 < L.size() && tdse::mode_finishes(L[i + 1].mode)) {
+            SweepArgs& next = L[i + 1].args;
+            next.fin_partials = L[i].args.partials;
+            next.fin_slots = L[i].args.relax;
+            next.fin_cell = L[i].args.cell;
+            next.fin_parity = L[i].args.parity;
+            continue;
+        }
+        F.push_back(L[i]);
+    }
+    return F;
 }
 
 const char* const kSlots3D[] = {"x_first", "y_fwd", "z_kin", "y_inv", "x_mid", "x_last"};
