@@ -22,7 +22,7 @@ hipError_t launch_sweep(const SweepArgs& a, int max_blocks, hipStream_t stream) 
     }
     if (tiles <= 0) return hipSuccess;
     // measuring sweeps keep one partial per block
-    if (mode_measures(MODE) && max_blocks > kRelaxMaxBlocks) max_blocks = kRelaxMaxBlocks;
+    if (mode_measures(MODE) && max_blocks > relax_max_blocks<T>()) max_blocks = relax_max_blocks<T>();
     if (mode_observes(MODE) && max_blocks > kObsMaxBlocks) max_blocks = kObsMaxBlocks;
     const int grid = static_cast<int>(tiles < max_blocks ? tiles : max_blocks);
     if constexpr (mode_contiguous(MODE)) {
