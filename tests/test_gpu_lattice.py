@@ -117,6 +117,23 @@ def test_lattice_relax_matches_oracle(orc, tdse, prec):
 
 
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_lattice_relax_with_solenoid_matches_oracle(orc, tdse, prec):
+    """Imaginary time with a solenoid's branch cut (one x link per cut row),
+    on an odd-sized lattice of several step tiles."""
+    dims, lo, hi = (90, 70, 1), (-12.0, -9.0, -1.0), (12.0, 9.0, 1.0)
+    kw = dict(links=2, a=1.1, xs=0.4, ys=-0.6, cut_up=False)
+    v = orc.harmonic_3d(dims, lo, hi, 0.4, (0, 0, 0))
+    psi0 = packet(dims, lo, hi, 1.0, 0.5, 2.0, 0.3)
+    want, _ = orc.lat2d_evolve(v, dims, lo, hi, 0.02, psi0, 25, relax=True, **kw)
+    p = tdse.Propagator(grid3(tdse, dims, lo, hi), v, 0.02, precision=prec, lattice=True,
+                        imaginary_time=True)
+    _set_links(p, kw)
+    p.upload(psi0)
+    p.step(25)
+    assert rel(p.download(), want) < TOL[prec]
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
 def test_lattice_with_mask_matches_oracle(orc, tdse, prec):
     dims, lo, hi = (64, 48, 1), (-10.0, -8.0, -1.0), (10.0, 8.0, 1.0)
     v = np.zeros(64 * 48)
