@@ -193,13 +193,18 @@ bool four_step_split(long long n, int* n1, int* n2) {
     return true;
 }
 
-uint64_t plan_bytes_for(long long total, const std::vector<int>& lines, bool fp64, bool mask,
-                        long long kt_points) {
+// Device bytes of a plan: state, kick phase, potential, mask array, line
+// twiddles, kinetic tables, reduction partials and, for a masked grid whose x
+// sweeps can read per-axis ramps (3D, or 2D with x lines along the first
+// axis), the ramps (x in the plan's precision, y and z in FP64).
+uint64_t plan_bytes_for(const int n[3], long long total, const std::vector<int>& lines, bool fp64,
+                        bool mask, long long kt_points, bool ramps) {
     const uint64_t cs = fp64 ? 16 : 8, rs = fp64 ? 8 : 4;
     uint64_t b = total * cs + 2 * total * rs + (mask ? total * rs : 0);
-    for (int n : lines) b += static_cast<uint64_t>(n) * 16;
+    for (int len : lines) b += static_cast<uint64_t>(len) * 16;
     b += static_cast<uint64_t>(kt_points) * 16;
     b += static_cast<uint64_t>(tdse::kMaxReduceBlocks) * tdse::kReduceQ * 8 + 64 * 8;
+    if (mask && ramps) b += static_cast<uint64_t>(n[0]) * rs + static_cast<uint64_t>(n[1] + n[2]) * 8;
     return b;
 }
 
@@ -1364,7 +1369,9 @@ int tdse_plan_bytes(const int n[3], int precision, int with_mask, uint64_t* out)
     const long long total = static_cast<long long>(n[0]) * n[1] * n[2];
     long long ktp = 0;
     for (const Axis& ax : sq) ktp += ax.n;
-    *out = plan_bytes_for(total, lines, precision == TDSE_FP64, with_mask != 0, ktp);
+    // the x sweeps' mask ramps (mask_axes_apply of a plan of this shape)
+    const bool ramps = (kind == kKind3D || (kind == kKind2D && n[2] == 1)) && n[0] > 1;
+    *out = plan_bytes_for(n, total, lines, precision == TDSE_FP64, with_mask != 0, ktp, ramps);
     return TDSE_OK;
 }
 

@@ -97,3 +97,34 @@ def test_large_device_builds_are_complete(tdse):
         b.step(3)
         assert np.abs(b.download() - want).max() < 1e-11 * scale
         b.close()
+
+
+@pytest.mark.parametrize("shape", [(32, 32, 16), (64, 32, 1), (96, 96, 1), (4096, 1, 1)])
+@pytest.mark.parametrize("prec", ["fp32", "fp64"])
+@pytest.mark.parametrize("how", ["build", "set_mask"])
+def test_plan_bytes_equals_held_bytes(tdse, shape, prec, how):
+    """The admission estimate tdse_plan_bytes equals what a masked plan holds
+    once its inputs are built (device recipe) or set (host mask), mask ramps
+    included, so a server filling its budget to the byte never meets
+    TDSE_E_OOM mid-serve."""
+    from shoeridingerequationsolver_amd import serving, workloads
+    ctx = tdse.Context(0)
+    dims = sum(1 for v in shape if v > 1)
+    half = 8.0
+    lo = tuple(-half if v > 1 else 0.0 for v in shape)
+    hi = tuple(half if v > 1 else 1.0 for v in shape)
+    s = workloads.Solve("m", shape, lo, hi, "barrier", {"V0": 1.0, "x_lo": 0.0, "x_hi": 2.0},
+                        tuple(0.0 for _ in shape), tuple(1.5 if v > 1 else 1.0 for v in shape),
+                        tuple(0.0 for _ in shape), 0.01, 4, prec, mask_width=2.0)
+    want = serving.plan_bytes(s.grid(), prec, with_mask=True)
+    if how == "build":
+        p = tdse.Propagator(s.grid(), None, s.dt, precision=prec, context=ctx)
+        p.build(s.recipe())
+    else:
+        p = tdse.Propagator(s.grid(), s.build_potential(), s.dt, precision=prec,
+                            mask=s.build_mask(), context=ctx)
+    assert ctx.bytes_in_use() == want, f"{shape} {prec} {how} dims={dims}"
+    p.step(4)
+    p.close()
+    assert ctx.bytes_in_use() == 0
+    ctx.close()
