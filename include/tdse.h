@@ -156,6 +156,22 @@ typedef struct tdse_observables {
     double momentum[3];   /* <k_a> */
 } tdse_observables;
 int tdse_observe(tdse_plan* plan, unsigned what, tdse_observables* out);
+/* The position-space part of tdse_observe -- norm, sum |psi|^2, <x>, <y>, <z>,
+ * the widths and <V> (kinetic, energy and momentum are set to 0) -- over a
+ * state the caller holds in device memory (psi: complex64 or complex128 per
+ * point by `precision`, x-fastest; v: the potential in the same precision, or
+ * null), written to `out` in DEVICE memory, queued on `stream` (a
+ * hipStream_t; null = the null stream), with no synchronisation.  No
+ * counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc).  It allocates nothing: the reduction's partials live in a
+ * static 147,528-byte device array per GPU, held until the process ends.
+ * Calls that overlap on one GPU race on that array (order them on one
+ * stream).  States of 4 GiB or more return TDSE_E_UNSUPPORTED.  Same block
+ * count and summation order as tdse_observe on a plan of that shape. */
+int tdse_observe_noalloc(const void* psi, const void* v, int precision, const int n[3],
+                         const double lo[3], const double hi[3], tdse_observables* out,
+                         void* stream);
 /* Probability lo <= coord_axis < hi times the cell volume, not normalised
  * (core/src/observables.ixx:121-131, extended to one axis of a 3D grid). */
 int tdse_probability_in_slab(tdse_plan* plan, int axis, double lo, double hi, double* out);

@@ -38,6 +38,7 @@ UNITS = [
     ("observe.hip", DEVICE),
     ("device_build.hip", DEVICE),
     ("lattice.hip", DEVICE),
+    ("noalloc.hip", DEVICE),
     ("tdse_api.cpp", ["-x", "hip"] + DEVICE),
     ("builders.cpp", ["-ffp-contract=off"]),
 ]

@@ -48,7 +48,7 @@ EXPORTS = (
     "tdse_batch_stream", "tdse_batch_launches_per_round", "tdse_plan_build",
     "tdse_relax_deflated", "tdse_plan_relax_stats", "tdse_build_soft_coulomb",
     "tdse_plan_set_drive", "tdse_plan_set_time", "tdse_plan_get_time", "tdse_plan_set_magnetic",
-    "tdse_lattice_set_uniform_field", "tdse_lattice_set_solenoid",
+    "tdse_lattice_set_uniform_field", "tdse_lattice_set_solenoid", "tdse_observe_noalloc",
 )
 
 TDSE_POT = {"none": 0, "harmonic": 1, "regularized_coulomb": 2, "coulomb_floor": 3, "barrier": 4,
@@ -163,6 +163,8 @@ def lib() -> C.CDLL:
         L.tdse_lattice_set_uniform_field.argtypes = [C.c_void_p, C.c_double]
         L.tdse_lattice_set_solenoid.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double,
                                                 C.c_int]
+        L.tdse_observe_noalloc.argtypes = [C.c_void_p, C.c_void_p, C.c_int, _ip, _dp, _dp,
+                                           C.c_void_p, C.c_void_p]
         _LIB = L
     return _LIB
 

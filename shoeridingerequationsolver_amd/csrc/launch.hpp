@@ -57,4 +57,7 @@ SweepShape sweep_shape_f64(int n, int mode);
 
 inline bool mode_is_contiguous(int mode) { return mode_contiguous(mode); }
 
+// Set the calling thread's tdse_last_error text; returns status.
+int record_error(int status, const char* what);
+
 }  // namespace tdse

@@ -75,6 +75,8 @@ struct Launch {
 
 }  // namespace
 
+int tdse::record_error(int status, const char* what) { return fail(status, what); }
+
 // Two pinned host buffers per context for state uploads and downloads: the
 // host conversion of one chunk overlaps the DMA of the other.  Allocated on
 // the first transfer; without them transfers go through pageable memory.
