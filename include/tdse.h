@@ -140,11 +140,14 @@ int tdse_plan_stream(tdse_plan* plan, void** out);
  *                      widths sqrt(max(0, <x^2> - <x>^2))
  *   TDSE_OBS_ENERGY    <V> + <k^2/2m>, each term normalised  (core/src/observables.ixx:202-236)
  *   TDSE_OBS_MOMENTUM  <k_x>, <k_y>, <k_z> (ratio form)      (core/src/observables.ixx:52-65, 177-200)
+ *   TDSE_OBS_VARIANCE  Var(H) = ||(H - <H>) psi||^2 / ||psi||^2, residual form, 1D grids
+ *                      only (TDSE_E_UNSUPPORTED otherwise)    (core/src/observables.ixx:92-119)
  */
 #define TDSE_OBS_NORM 1u
 #define TDSE_OBS_POSITION 2u
 #define TDSE_OBS_ENERGY 4u
 #define TDSE_OBS_MOMENTUM 8u
+#define TDSE_OBS_VARIANCE 16u
 typedef struct tdse_observables {
     double norm;
     double sum_abs2;      /* sum |psi|^2 without the cell volume */
@@ -154,6 +157,7 @@ typedef struct tdse_observables {
     double energy;        /* potential + kinetic */
     double sigma[3];      /* sqrt(max(0, <x_a^2> - <x_a>^2)) */
     double momentum[3];   /* <k_a> */
+    double energy_variance; /* TDSE_OBS_VARIANCE */
 } tdse_observables;
 int tdse_observe(tdse_plan* plan, unsigned what, tdse_observables* out);
 /* The position-space part of tdse_observe -- norm, sum |psi|^2, <x>, <y>, <z>,
@@ -323,6 +327,17 @@ int tdse_build_absorbing_mask(const int n[3], const double lo[3], const double h
                               double width, double* m);
 int tdse_build_gaussian(const int n[3], const double lo[3], const double hi[3], const double r0[3],
                         const double sigma[3], const double k0[3], double* psi);
+/* The reference propagator's FP64 phase tables, interleaved complex128,
+ * x-fastest, bit-identical to SplitOperator1D/3D::half_potential_phase() and
+ * kinetic_phase() (core/src/propagator.ixx:23-40, 72-101, 107-108):
+ * half_v = cis(-0.5 V dt) per cell; kinetic = cis(-0.5 k^2 dt) on an n x 1 x 1
+ * grid, cis(-0.5 (kx^2 + ky^2 + kz^2) / mass dt) over the full grid otherwise.
+ * Either output may be null (v only needed for half_v).  The GPU path does
+ * not use these full tables (it keeps per-axis kinetic tables and reduced
+ * kick angles); they are for callers that read the reference's tables
+ * (core/src/drive.ixx:58-62, tests/propagator_tables_test.cpp:28-65). */
+int tdse_build_phase_tables(const int n[3], const double lo[3], const double hi[3], const double* v,
+                            double dt, double mass, double* half_v, double* kinetic);
 
 #ifdef __cplusplus
 }

@@ -165,12 +165,51 @@ private:
 
 inline Grid3D as_grid3(const Grid1D& g) { return Grid3D{g, Grid1D{0.0, 1.0, 1}, Grid1D{0.0, 1.0, 1}}; }
 
+// The reference propagator's FP64 phase tables (core/src/propagator.ixx:107-108),
+// built on the host on first use from the potential the propagator was given
+// (tdse_build_phase_tables, bit-identical to the reference's constructor).
+// The GPU path does not read them: it keeps reduced kick angles and per-axis
+// kinetic tables.
+class PhaseTables {
+public:
+    PhaseTables(const Grid3D& g, const std::vector<double>& v, double dt, double mass)
+        : g_(g), v_(v), dt_(dt), mass_(mass) {}
+    const std::vector<std::complex<double>>& half_potential_phase() const {
+        build();
+        return half_;
+    }
+    const std::vector<std::complex<double>>& kinetic_phase() const {
+        build();
+        return kin_;
+    }
+
+private:
+    void build() const {
+        if (built_) return;
+        const int n[3] = {g_.x.n, g_.y.n, g_.z.n};
+        const double lo[3] = {g_.x.xmin, g_.y.xmin, g_.z.xmin};
+        const double hi[3] = {g_.x.xmax, g_.y.xmax, g_.z.xmax};
+        half_.resize(v_.size());
+        kin_.resize(v_.size());
+        check(tdse_build_phase_tables(n, lo, hi, v_.data(), dt_, mass_,
+                                      reinterpret_cast<double*>(half_.data()),
+                                      reinterpret_cast<double*>(kin_.data())));
+        built_ = true;
+    }
+    Grid3D g_;
+    std::vector<double> v_;
+    double dt_, mass_;
+    mutable bool built_ = false;
+    mutable std::vector<std::complex<double>> half_, kin_;
+};
+
 // ses::SplitOperator1D (core/src/propagator.ixx:21-64)
 class SplitOperator1D {
 public:
     SplitOperator1D(const Grid1D& g, const std::vector<double>& potential, double dt,
                     Precision prec = Precision::FP64)
-        : dt_(dt), prop_(std::make_unique<Propagator>(as_grid3(g), potential, dt, 1.0, prec)) {}
+        : dt_(dt), prop_(std::make_unique<Propagator>(as_grid3(g), potential, dt, 1.0, prec)),
+          tables_(as_grid3(g), potential, dt, 1.0) {}
     double dt() const noexcept { return dt_; }
     void step(Field1D& psi, int nsteps = 1) const {
         if (nsteps <= 0) return;
@@ -178,10 +217,16 @@ public:
         prop_->step(nsteps);
         prop_->download(psi.data());
     }
+    const std::vector<std::complex<double>>& half_potential_phase() const {
+        return tables_.half_potential_phase();
+    }
+    const std::vector<std::complex<double>>& kinetic_phase() const { return tables_.kinetic_phase(); }
+    Propagator& device() const noexcept { return *prop_; }
 
 private:
     double dt_;
     std::unique_ptr<Propagator> prop_;
+    PhaseTables tables_;
 };
 
 // ses::SplitOperator3D (core/src/propagator.ixx:66-134); 2D is nx x ny x 1.
@@ -189,7 +234,8 @@ class SplitOperator3D {
 public:
     SplitOperator3D(const Grid3D& g, const std::vector<double>& potential, double dt,
                     double mass = 1.0, Precision prec = Precision::FP64)
-        : dt_(dt), prop_(std::make_unique<Propagator>(g, potential, dt, mass, prec)) {}
+        : dt_(dt), prop_(std::make_unique<Propagator>(g, potential, dt, mass, prec)),
+          tables_(g, potential, dt, mass) {}
     double dt() const noexcept { return dt_; }
     void step(Field3D& psi, int nsteps = 1) const {
         if (nsteps <= 0) return;
@@ -197,12 +243,18 @@ public:
         prop_->step(nsteps);
         prop_->download(psi.data());
     }
+    // core/src/propagator.ixx:107-108 (FP64 host tables, built on first use)
+    const std::vector<std::complex<double>>& half_potential_phase() const {
+        return tables_.half_potential_phase();
+    }
+    const std::vector<std::complex<double>>& kinetic_phase() const { return tables_.kinetic_phase(); }
     // the device plan behind this propagator (driven_step sets its drive)
     Propagator& device() const noexcept { return *prop_; }
 
 private:
     double dt_;
     std::unique_ptr<Propagator> prop_;
+    PhaseTables tables_;
 };
 
 // ses::DipoleDrive (core/src/drive.ixx:19-23): V = amplitude (axis . r) cos(omega t)
@@ -406,8 +458,8 @@ inline Field3D gaussian_wavepacket(const Grid3D& g, Vec3d r0, Vec3d sigma, Vec3d
     return f;
 }
 
-// ses::WavepacketSimulation (core/src/simulation.ixx:22-88): advance() keeps
-// the state on the GPU; psi() reads it back.
+// ses::WavepacketSimulation (core/src/simulation.ixx:22-88): advance() and
+// relax() keep the state on the GPU; psi() reads it back when it changed.
 class WavepacketSimulation {
 public:
     struct Config {
@@ -419,41 +471,69 @@ public:
         double dt;
     };
     explicit WavepacketSimulation(Config cfg, Precision prec = Precision::FP64)
-        : grid_(cfg.grid), dt_(cfg.dt), potential_(std::move(cfg.potential)),
-          prop_(grid_, potential_, cfg.dt, 1.0, prec), psi_(grid_) {
+        : grid_(cfg.grid), dt_(cfg.dt), prec_(prec), potential_(std::move(cfg.potential)),
+          propagator_(grid_, potential_, cfg.dt, 1.0, prec), psi_(grid_) {
         psi_ = gaussian_wavepacket(grid_, cfg.r0, cfg.sigma, cfg.k0);
-        prop_.upload(psi_.data());
+        propagator_.device().upload(psi_.data());
     }
     void advance(int nsteps) {
-        prop_.step(nsteps);
+        propagator_.device().step(nsteps);
         steps_ += nsteps;
+        stale_ = true;
+    }
+    // core/src/simulation.ixx:45-52: an imaginary-time relaxer keyed on dtau
+    // (rebuilt when dtau changes); tau is not t, so time() does not move.  The
+    // state goes to the relaxer's plan and back on the GPU.
+    void relax(int nsteps, double dtau) {
+        if (!relaxer_ || relaxer_dtau_ != dtau) {
+            relaxer_ = std::make_unique<Propagator>(grid_, potential_, dtau, 1.0, prec_,
+                                                    Context::instance(), TDSE_FLAG_IMAGINARY_TIME);
+            relaxer_dtau_ = dtau;
+        }
+        if (nsteps <= 0) return;
+        check(tdse_state_copy(relaxer_->get(), propagator_.device().get()));
+        relaxer_->step(nsteps);
+        check(tdse_state_copy(propagator_.device().get(), relaxer_->get()));
         stale_ = true;
     }
     double time() const noexcept { return steps_ * dt_; }
     double dt() const noexcept { return dt_; }
     const Grid3D& grid() const noexcept { return grid_; }
     const std::vector<double>& potential() const noexcept { return potential_; }
-    const Field3D& psi() {
+    // core/src/simulation.ixx:68: the state, read back from the GPU only when
+    // a step changed it since the last read
+    const Field3D& psi() const {
         if (stale_) {
-            prop_.download(psi_.data());
+            propagator_.device().download(psi_.data());
             stale_ = false;
         }
         return psi_;
     }
+    // core/src/simulation.ixx:70
+    const SplitOperator3D& propagator() const noexcept { return propagator_; }
     void set_psi(const Field3D& psi) {
         psi_ = psi;
-        prop_.upload(psi_.data());
+        propagator_.device().upload(psi_.data());
         stale_ = false;
+    }
+    std::vector<double> density() const {
+        const Field3D& f = psi();
+        std::vector<double> d(f.data().size());
+        for (std::size_t i = 0; i < d.size(); ++i) d[i] = std::norm(f.data()[i]);
+        return d;
     }
 
 private:
     Grid3D grid_;
     double dt_;
+    Precision prec_;
     std::vector<double> potential_;
-    Propagator prop_;
-    Field3D psi_;
+    SplitOperator3D propagator_;
+    mutable Field3D psi_;
     long long steps_ = 0;
-    bool stale_ = false;
+    mutable bool stale_ = false;
+    std::unique_ptr<Propagator> relaxer_;
+    double relaxer_dtau_ = 0.0;
 };
 
 }  // namespace tdse

@@ -56,6 +56,7 @@ def orc() -> C.CDLL:
             build()
         lib = C.CDLL(ORC_PATH)
         for name in ("orc_norm_sq", "orc_mean_position_1d", "orc_mean_energy_1d",
+                     "orc_energy_variance_1d",
                      "orc_probability_in_range_1d", "orc_mean_energy_3d",
                      "orc_probability_in_slab_3d", "orc_spacing", "orc_sigma_x_1d",
                      "orc_mean_momentum_1d", "orc_lat2d_energy"):
@@ -74,6 +75,7 @@ def ref() -> C.CDLL:
         lib = C.CDLL(REF_PATH)
         for name in ("ref_norm_sq_1d", "ref_norm_sq_3d", "ref_mean_position_1d",
                      "ref_mean_energy_1d", "ref_probability_in_range_1d", "ref_mean_energy_3d",
+                     "ref_energy_variance_1d",
                      "ref_sigma_x_1d", "ref_mean_momentum_1d"):
             getattr(lib, name).restype = C.c_double
         _REF = lib
@@ -468,6 +470,14 @@ def mean_energy_1d(psi, xmin, xmax, v):
     return orc().orc_mean_energy_1d(_p(psi), psi.size, C.c_double(xmin), C.c_double(xmax), _p(v))
 
 
+def energy_variance_1d(psi, xmin, xmax, v):
+    """energy_variance (core/src/observables.ixx:92-119), residual form."""
+    psi = _c128(psi)
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    return orc().orc_energy_variance_1d(_p(psi), psi.size, C.c_double(xmin), C.c_double(xmax),
+                                        _p(v))
+
+
 def probability_in_range_1d(psi, xmin, xmax, a, b):
     psi = _c128(psi)
     return orc().orc_probability_in_range_1d(_p(psi), psi.size, C.c_double(xmin),
@@ -653,6 +663,13 @@ class Ref:
         v = np.ascontiguousarray(v, dtype=np.float64)
         return ref().ref_mean_energy_1d(_p(psi), psi.size, C.c_double(xmin), C.c_double(xmax),
                                         _p(v))
+
+    @staticmethod
+    def energy_variance_1d(psi, xmin, xmax, v):
+        psi = _c128(psi)
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        return ref().ref_energy_variance_1d(_p(psi), psi.size, C.c_double(xmin),
+                                            C.c_double(xmax), _p(v))
 
     @staticmethod
     def probability_in_range_1d(psi, xmin, xmax, a, b):

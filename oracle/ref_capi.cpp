@@ -428,6 +428,12 @@ double ref_mean_energy_1d(const double* psi, int n, double xmin, double xmax, co
     load(f.data(), psi);
     return ses::mean_energy(f, std::vector<double>(v, v + n));
 }
+double ref_energy_variance_1d(const double* psi, int n, double xmin, double xmax,
+                              const double* v) {
+    ses::Field1D f{ses::Grid1D{xmin, xmax, n}};
+    load(f.data(), psi);
+    return ses::energy_variance(f, std::vector<double>(v, v + n));
+}
 double ref_probability_in_range_1d(const double* psi, int n, double xmin, double xmax, double a,
                                    double b) {
     ses::Field1D f{ses::Grid1D{xmin, xmax, n}};

@@ -1102,6 +1102,56 @@ double orc_mean_energy_1d(const double* psi, int n, double xmin, double xmax, co
     return obs_ratio(num_v, den_x) + obs_ratio(num_t, den_k);
 }
 
+/* core/src/observables.ixx:92-119: Var(H) in the residual form
+ * ||(H - <H>) psi||^2 / ||psi||^2 with H psi = ifft(0.5 k^2 fft(psi)) + V psi.
+ * A length that is not a power of two (no reference counterpart: the
+ * reference throws) takes the mixed-radix transform. */
+double orc_energy_variance_1d(const double* psi, int n, double xmin, double xmax, const double* v) {
+    double* h = (double*)malloc(sizeof(double) * 2 * (size_t)n);
+    memcpy(h, psi, sizeof(double) * 2 * (size_t)n);
+    const int pow2 = orc_fft1d(h, (size_t)n) == 0;
+    if (!pow2) orc_fft_any(h, (size_t)n, -1);
+    double* k = (double*)malloc(sizeof(double) * (size_t)n);
+    orc_wavenumbers(n, xmin, xmax, k);
+    for (int j = 0; j < n; ++j) {
+        const double s = 0.5 * k[j] * k[j];
+        h[2 * j] *= s;
+        h[2 * j + 1] *= s;
+    }
+    if (pow2) {
+        orc_ifft1d(h, (size_t)n);
+    } else {
+        /* the same conjugation identity as orc_ifft1d around the mixed-radix
+         * forward transform */
+        for (int i = 0; i < n; ++i) h[2 * i + 1] = -h[2 * i + 1];
+        orc_fft_any(h, (size_t)n, -1);
+        const double inv = 1.0 / (double)n;
+        for (int i = 0; i < n; ++i) {
+            h[2 * i] = inv * h[2 * i];
+            h[2 * i + 1] = inv * -h[2 * i + 1];
+        }
+    }
+    double num_h = 0.0, den = 0.0;
+    for (int i = 0; i < n; ++i) {
+        const double fr = psi[2 * i], fi = psi[2 * i + 1];
+        h[2 * i] += v[i] * fr;
+        h[2 * i + 1] += v[i] * fi;
+        den += fr * fr + fi * fi;
+        /* (conj(f) * h).real() */
+        num_h += fr * h[2 * i] - (-fi) * h[2 * i + 1];
+    }
+    const double mean = obs_ratio(num_h, den);
+    double num_var = 0.0;
+    for (int i = 0; i < n; ++i) {
+        const double dr = h[2 * i] - mean * psi[2 * i];
+        const double di = h[2 * i + 1] - mean * psi[2 * i + 1];
+        num_var += dr * dr + di * di;
+    }
+    free(k);
+    free(h);
+    return obs_ratio(num_var, den);
+}
+
 /* core/src/observables.ixx:121-131: not scale invariant, times dx */
 double orc_probability_in_range_1d(const double* psi, int n, double xmin, double xmax, double a,
                                    double b) {

@@ -136,6 +136,8 @@ void orc_soft_coulomb_3d(const int dims[3], const double lo[3], const double hi[
 
 /* Observables (core/src/observables.ixx). */
 double orc_mean_position_1d(const double* psi, int n, double xmin, double xmax);
+double orc_energy_variance_1d(const double* psi, int n, double xmin, double xmax,
+                              const double* v);
 double orc_mean_energy_1d(const double* psi, int n, double xmin, double xmax,
                           const double* v);
 double orc_probability_in_range_1d(const double* psi, int n, double xmin, double xmax,

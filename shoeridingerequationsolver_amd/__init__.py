@@ -33,7 +33,7 @@ import numpy as np
 
 from . import capi
 from .capi import (TDSE_FP32, TDSE_FP64, TDSE_OBS_ENERGY, TDSE_OBS_MOMENTUM, TDSE_OBS_NORM,
-                   TDSE_OBS_POSITION, TdseError)
+                   TDSE_OBS_POSITION, TDSE_OBS_VARIANCE, TdseError)
 
 __all__ = [
     "Grid1D", "Grid3D", "wavenumbers", "Context", "Propagator", "SplitOperator1D",
@@ -41,7 +41,7 @@ __all__ = [
     "regularized_coulomb_potential", "coulomb_floor_potential", "absorbing_mask",
     "gaussian_wavepacket", "TDSE_FP32", "TDSE_FP64", "TdseError", "Batch",
     "ImaginaryTimePropagator1D", "ImaginaryTimePropagator3D", "soft_coulomb_potential",
-    "MagneticPropagator3D", "PeierlsLattice2D", "DipoleDrive", "driven_step",
+    "MagneticPropagator3D", "PeierlsLattice2D", "DipoleDrive", "driven_step", "phase_tables",
 ]
 
 PRECISIONS = {"fp32": TDSE_FP32, "fp64": TDSE_FP64, TDSE_FP32: TDSE_FP32, TDSE_FP64: TDSE_FP64}
@@ -247,6 +247,13 @@ class Propagator:
     def step(self, nsteps=1):
         capi.check(capi.lib().tdse_step(self._h, int(nsteps)))
 
+    def copy_state_from(self, other: "Propagator"):
+        """This plan's state = ``other``'s (same shape and precision), on the GPU."""
+        try:
+            capi.check(capi.lib().tdse_state_copy(self._h, other.handle))
+        except TdseError as e:
+            raise _map_error(e) from None
+
     def sync(self, timeout_ms=-1):
         capi.check(capi.lib().tdse_plan_sync(self._h, int(timeout_ms)))
 
@@ -258,7 +265,8 @@ class Propagator:
         capi.check(capi.lib().tdse_observe(self._h, what, C.byref(o)))
         return {"norm": o.norm, "sum_abs2": o.sum_abs2, "position": tuple(o.position),
                 "potential": o.potential, "kinetic": o.kinetic, "energy": o.energy,
-                "sigma": tuple(o.sigma), "momentum": tuple(o.momentum)}
+                "sigma": tuple(o.sigma), "momentum": tuple(o.momentum),
+                "energy_variance": o.energy_variance}
 
     def probability_in_slab(self, axis, lo, hi):
         v = C.c_double()
@@ -459,15 +467,53 @@ class Batch:
 
 # ---- drop-in propagators (core/src/propagator.ixx) --------------------------
 
+def phase_tables(grid, potential, dt, mass=1.0):
+    """The reference propagator's FP64 tables (core/src/propagator.ixx:23-40,
+    72-101): (half_v, kinetic) as complex128 arrays in the field's shape,
+    bit-identical to ``half_potential_phase()`` / ``kinetic_phase()``."""
+    n, lo, hi = _axes(grid)
+    v = np.ascontiguousarray(potential, dtype=np.float64).ravel()
+    size = int(np.prod(n))
+    if v.size != size:
+        raise ValueError(f"potential has {v.size} entries, grid has {size}")
+    half = np.empty(size, np.complex128)
+    kin = np.empty(size, np.complex128)
+    capi.check(capi.lib().tdse_build_phase_tables(
+        capi.ivec(n), capi.dvec(lo), capi.dvec(hi), capi.dptr(v), C.c_double(dt),
+        C.c_double(mass), capi.dptr(half.view(np.float64)), capi.dptr(kin.view(np.float64))))
+    return half.reshape(_shape(grid)), kin.reshape(_shape(grid))
+
+
 class _SplitOperatorBase:
     def __init__(self, grid, potential, dt, mass, precision, context):
         self._grid = grid
         self._dt = float(dt)
-        self._prop = Propagator(grid, potential, dt, mass=mass, precision=precision,
+        self._mass = float(mass)
+        self._v = np.ascontiguousarray(potential, dtype=np.float64).ravel()
+        self._tables = None
+        self._prop = Propagator(grid, self._v, dt, mass=mass, precision=precision,
                                 context=context)
 
     def dt(self) -> float:
         return self._dt
+
+    def device(self) -> Propagator:
+        """The device plan behind this propagator."""
+        return self._prop
+
+    def _phase_tables(self):
+        if self._tables is None:
+            self._tables = phase_tables(self._grid, self._v, self._dt, self._mass)
+        return self._tables
+
+    def half_potential_phase(self) -> np.ndarray:
+        """cis(-0.5 V dt) per cell, FP64 (core/src/propagator.ixx:107); built on
+        first use on the host (the GPU path keeps reduced kick angles)."""
+        return self._phase_tables()[0]
+
+    def kinetic_phase(self) -> np.ndarray:
+        """cis(-0.5 |k|^2 / m dt) over the full grid, FP64 (core/src/propagator.ixx:108)."""
+        return self._phase_tables()[1]
 
     def step(self, psi: np.ndarray, nsteps: int = 1) -> None:
         """Advance ``psi`` in place by ``nsteps`` Strang steps."""
@@ -799,8 +845,9 @@ class SimulationConfig:
 
 class WavepacketSimulation:
     """``ses::WavepacketSimulation`` (core/src/simulation.ixx:22-88): owns the
-    grid, potential, propagator and state; ``advance(n)`` steps it.  The state
-    stays on the GPU between ``advance`` calls and is read back by ``psi()``."""
+    grid, potential, propagator and state; ``advance(n)`` steps it and
+    ``relax(n, dtau)`` relaxes it in imaginary time.  The state stays on the
+    GPU between calls and is read back by ``psi()``."""
 
     Config = SimulationConfig
 
@@ -808,14 +855,37 @@ class WavepacketSimulation:
         self._grid = cfg.grid
         self._dt = float(cfg.dt)
         self._potential = np.ascontiguousarray(cfg.potential, dtype=np.float64)
-        self._prop = Propagator(cfg.grid, self._potential, cfg.dt, precision=cfg.precision,
-                                context=context)
+        self._precision = cfg.precision
+        self._ctx = context
+        self._op = SplitOperator3D(cfg.grid, self._potential, cfg.dt, precision=cfg.precision,
+                                   context=context)
+        self._prop = self._op.device()
         self._prop.upload(gaussian_wavepacket(cfg.grid, cfg.r0, cfg.sigma, cfg.k0))
         self._steps = 0
+        self._relaxer = None
+        self._relaxer_dtau = 0.0
 
     def advance(self, nsteps: int) -> None:
         self._prop.step(nsteps)
         self._steps += nsteps
+
+    def relax(self, nsteps: int, dtau: float) -> None:
+        """``nsteps`` imaginary-time steps of the state (simulation.ixx:45-52):
+        the relaxer is rebuilt when dtau changes; the clock does not move.
+        The state moves between the two plans on the GPU."""
+        if self._relaxer is None or self._relaxer_dtau != dtau:
+            if self._relaxer is not None:
+                self._relaxer._prop.close()
+            self._relaxer = ImaginaryTimePropagator3D(self._grid, self._potential, dtau,
+                                                      precision=self._precision,
+                                                      context=self._ctx)
+            self._relaxer_dtau = dtau
+        if nsteps <= 0:
+            return
+        r = self._relaxer._prop
+        r.copy_state_from(self._prop)
+        r.step(nsteps)
+        self._prop.copy_state_from(r)
 
     def time(self) -> float:
         return self._steps * self._dt
@@ -838,6 +908,6 @@ class WavepacketSimulation:
     def density(self) -> np.ndarray:
         return np.abs(self.psi()) ** 2
 
-    @property
-    def propagator(self) -> Propagator:
-        return self._prop
+    def propagator(self) -> SplitOperator3D:
+        """The simulation's SplitOperator3D (its phase tables, its device plan)."""
+        return self._op

@@ -31,6 +31,7 @@ TDSE_OBS_NORM = 1
 TDSE_OBS_POSITION = 2
 TDSE_OBS_ENERGY = 4
 TDSE_OBS_MOMENTUM = 8
+TDSE_OBS_VARIANCE = 16
 
 # every symbol include/tdse.h declares (checked by the CPU test tier)
 EXPORTS = (
@@ -49,6 +50,7 @@ EXPORTS = (
     "tdse_relax_deflated", "tdse_plan_relax_stats", "tdse_build_soft_coulomb",
     "tdse_plan_set_drive", "tdse_plan_set_time", "tdse_plan_get_time", "tdse_plan_set_magnetic",
     "tdse_lattice_set_uniform_field", "tdse_lattice_set_solenoid", "tdse_observe_noalloc",
+    "tdse_build_phase_tables",
 )
 
 TDSE_POT = {"none": 0, "harmonic": 1, "regularized_coulomb": 2, "coulomb_floor": 3, "barrier": 4,
@@ -90,6 +92,7 @@ class Observables(C.Structure):
         ("energy", C.c_double),
         ("sigma", C.c_double * 3),
         ("momentum", C.c_double * 3),
+        ("energy_variance", C.c_double),
     ]
 
 
@@ -165,6 +168,7 @@ def lib() -> C.CDLL:
                                                 C.c_int]
         L.tdse_observe_noalloc.argtypes = [C.c_void_p, C.c_void_p, C.c_int, _ip, _dp, _dp,
                                            C.c_void_p, C.c_void_p]
+        L.tdse_build_phase_tables.argtypes = [_ip, _dp, _dp, _dp, C.c_double, C.c_double, _dp, _dp]
         _LIB = L
     return _LIB
 

@@ -216,4 +216,47 @@ int tdse_build_gaussian(const int n[3], const double lo[3], const double hi[3], 
     return TDSE_OK;
 }
 
+// The reference propagator's FP64 tables (core/src/propagator.ixx:23-40 for
+// an n x 1 x 1 grid, 72-101 otherwise): half_v[f] = cis(-0.5 V[f] dt) and the
+// full-grid kinetic[f] = cis(-0.5 (kx^2 + ky^2 + kz^2) / mass dt) with the
+// fftfreq wavenumbers of core/src/spectral.ixx:14-23, cos and sin as two libm
+// calls, interleaved complex128 out.  Either output may be null.
+int tdse_build_phase_tables(const int n[3], const double lo[3], const double hi[3], const double* v,
+                            double dt, double mass, double* half_v, double* kinetic) {
+    if (!ok_grid(n, lo, hi) || (half_v && !v) || !(mass != 0.0)) return TDSE_E_INVALID;
+    const long long total = static_cast<long long>(n[0]) * n[1] * n[2];
+    if (half_v)
+        for (long long f = 0; f < total; ++f) {
+            const double th = -0.5 * v[f] * dt;
+            half_v[2 * f] = std::cos(th);
+            half_v[2 * f + 1] = std::sin(th);
+        }
+    if (!kinetic) return TDSE_OK;
+    std::vector<double> k[3];
+    for (int a = 0; a < 3; ++a) {
+        const double dk = 2.0 * kPi / (hi[a] - lo[a]);
+        k[a].resize(static_cast<std::size_t>(n[a]));
+        for (int j = 0; j < n[a]; ++j) k[a][j] = dk * ((2 * j < n[a]) ? j : j - n[a]);
+    }
+    if (n[1] == 1 && n[2] == 1) {
+        // SplitOperator1D: no mass
+        for (int j = 0; j < n[0]; ++j) {
+            const double th = -0.5 * k[0][j] * k[0][j] * dt;
+            kinetic[2 * j] = std::cos(th);
+            kinetic[2 * j + 1] = std::sin(th);
+        }
+        return TDSE_OK;
+    }
+    for (int kk = 0; kk < n[2]; ++kk)
+        for (int j = 0; j < n[1]; ++j)
+            for (int i = 0; i < n[0]; ++i) {
+                const double kxx = k[0][i], kyy = k[1][j], kzz = k[2][kk];
+                const double th = -0.5 * (kxx * kxx + kyy * kyy + kzz * kzz) / mass * dt;
+                const long long f = flat(n, i, j, kk);
+                kinetic[2 * f] = std::cos(th);
+                kinetic[2 * f + 1] = std::sin(th);
+            }
+    return TDSE_OK;
+}
+
 }  // extern "C"

@@ -219,6 +219,39 @@ normalize_dev(void* psi_v, long long total, const double* sum, double dv, double
     }
 }
 
+// Energy variance ||(H - <H>) psi||^2 / ||psi||^2 (core/src/observables.ixx:
+// 92-119) from psi, T psi (kinetic part, already transformed back) and V:
+// pass 1 (mean == null) sums |psi|^2 and Re(conj(psi) H psi) into slots 0
+// and 1; pass 2 sums |H psi - <H> psi|^2 into slot 0, with <H> the ratio of
+// pass 1's final sums.  H psi = T psi + V psi is formed in FP64.
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+reduce_hvar(const void* psi_v, const void* tpsi_v, const void* v_v, long long total,
+            const double* mean_sums, double* partials) {
+    const cx<T>* psi = static_cast<const cx<T>*>(psi_v);
+    const cx<T>* tpsi = static_cast<const cx<T>*>(tpsi_v);
+    const T* v = static_cast<const T*>(v_v);
+    double mean = 0.0;
+    if (mean_sums) mean = mean_sums[0] > 0.0 ? mean_sums[1] / mean_sums[0] : 0.0;
+    double acc[kReduceQ] = {};
+    const long long stride = static_cast<long long>(gridDim.x) * kBlock;
+    for (long long f = blockIdx.x * static_cast<long long>(kBlock) + threadIdx.x; f < total;
+         f += stride) {
+        const double pr = psi[f].x, pi = psi[f].y;
+        const double vf = v ? static_cast<double>(v[f]) : 0.0;
+        const double hr = static_cast<double>(tpsi[f].x) + vf * pr;
+        const double hi = static_cast<double>(tpsi[f].y) + vf * pi;
+        if (!mean_sums) {
+            acc[0] += pr * pr + pi * pi;
+            acc[1] += pr * hr + pi * hi;
+        } else {
+            const double dr = hr - mean * pr, di = hi - mean * pi;
+            acc[0] += dr * dr + di * di;
+        }
+    }
+    block_reduce_store<kReduceQ>(acc, partials + blockIdx.x * kReduceQ);
+}
+
 // Unfused FP64 elementwise passes of the reference data flow (the baseline
 // the fused path is compared with): psi *= table, psi = conj(psi) * s,
 // psi *= mask.
@@ -312,6 +345,26 @@ long long scale_blocks(long long total) {
     return b;
 }
 }  // namespace
+
+hipError_t launch_energy_variance(bool fp64, const void* psi, const void* tpsi, const void* v,
+                                 long long total, double* partials, double* out,
+                                 hipStream_t stream) {
+    const int blocks = reduce_blocks(total);
+    double* first = out;             // pass 1 sums (|psi|^2, Re <psi|H psi>)
+    double* second = out + kReduceQ;  // pass 2 sum |H psi - <H> psi|^2
+    for (int pass = 0; pass < 2; ++pass) {
+        const double* mean = pass ? first : nullptr;
+        if (fp64)
+            hipLaunchKernelGGL(reduce_hvar<double>, dim3(blocks), dim3(kBlock), 0, stream, psi, tpsi,
+                               v, total, mean, partials);
+        else
+            hipLaunchKernelGGL(reduce_hvar<float>, dim3(blocks), dim3(kBlock), 0, stream, psi, tpsi,
+                               v, total, mean, partials);
+        hipLaunchKernelGGL(reduce_final, dim3(1), dim3(kBlock), 0, stream, partials, blocks,
+                           pass ? second : first);
+    }
+    return hipGetLastError();
+}
 
 hipError_t launch_reduce_final3(const double* partials, int nblocks, double* out,
                                 hipStream_t stream) {

@@ -32,6 +32,12 @@ hipError_t launch_scale(bool fp64, void* psi, long long total, double s, hipStre
 // out[q] = sum over blocks b < nblocks of partials[3 b + q], q < 3, fixed order
 hipError_t launch_reduce_final3(const double* partials, int nblocks, double* out,
                                 hipStream_t stream);
+// Energy variance sums from psi, its kinetic part T psi and V (or null):
+// out[0], out[1] = sum |psi|^2, sum Re(conj(psi) H psi); out[kReduceQ] =
+// sum |H psi - <H> psi|^2 (2 kReduceQ doubles of device memory)
+hipError_t launch_energy_variance(bool fp64, const void* psi, const void* tpsi, const void* v,
+                                 long long total, double* partials, double* out,
+                                 hipStream_t stream);
 // psi *= *s (device scalar)
 hipError_t launch_scale_dev(bool fp64, void* psi, long long total, const double* s,
                             hipStream_t stream);

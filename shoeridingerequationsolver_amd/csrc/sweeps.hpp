@@ -593,18 +593,27 @@ __device__ __forceinline__ void block_sum_max(double& s, double& m) {
 }
 
 // A measuring sweep's per-block (sum |psi|^2, max |psi|^2) partials -> the
-// step's norm, peak and the next measuring sweep's scale, in one block
-// (fixed order: the same partition and tree whichever block runs it).
+// step's norm, peak and the next measuring sweep's scale.  Wave 0 of the
+// block does it with a fixed partition (lane l adds partials l, l + 64, ...
+// in order) and a fixed xor-shuffle tree, so the result is the same bits
+// whatever kernel runs it and whatever its block size (every block holds at
+// least one full wave); the other waves skip it.
 __device__ __forceinline__ void relax_finish_block(const double* partials, RelaxSlots* slots,
                                                    int parity, double cell) {
+    if (threadIdx.x >= 64) return;
     const unsigned nb = slots->nblocks;
     double ts = 0.0, tm = 0.0;
-    for (unsigned b = threadIdx.x; b < nb; b += blockDim.x) {
+#pragma unroll 4
+    for (unsigned b = threadIdx.x; b < nb; b += 64) {
         ts += partials[2 * b];
         const double q = partials[2 * b + 1];
         tm = q > tm ? q : tm;
     }
-    block_sum_max(ts, tm);
+    for (int off = 32; off > 0; off >>= 1) {
+        ts += __shfl_xor(ts, off, 64);
+        const double o = __shfl_xor(tm, off, 64);
+        tm = o > tm ? o : tm;
+    }
     if (threadIdx.x == 0) {
         const double norm = ts * cell;
         slots->scale[parity ^ 1] = norm > 0.0 ? 1.0 / sqrt(norm) : 1.0;
@@ -957,6 +966,7 @@ template <typename T, int N, int MODE, int V = 0>
 __global__ void __launch_bounds__((Shape<T, N, V>::BLOCK_C), TDSE_MIN_WAVES)
 contig_sweep(SweepArgs a) {
     using S = Shape<T, N, V>;
+    static_assert(!mode_finishes(MODE) || S::BLOCK_C >= 64, "a finishing sweep holds a full wave");
     __shared__ cx<T> lds[S::LDS_C];
     __shared__ cx<double> lds_tw[S::TWN];
     const int tid = threadIdx.x;
@@ -984,6 +994,7 @@ template <typename T, int N, int MODE, int V = 0>
 __global__ void __launch_bounds__((Shape<T, N, V>::BLOCK_S), TDSE_MIN_WAVES)
 strided_sweep(SweepArgs a) {
     using S = Shape<T, N, V>;
+    static_assert(!mode_finishes(MODE) || S::BLOCK_S >= 64, "a finishing sweep holds a full wave");
     constexpr bool KIN = MODE == S_KIN;
     __shared__ cx<T> lds[S::LDS_S];
     __shared__ cx<double> lds_tw[S::TWN];

@@ -117,7 +117,8 @@ struct tdse_plan {
     // the x sweeps instead of the full array (which the other sweeps keep)
     void* mask_x = nullptr;
     double* mask_yz = nullptr;
-    void* scratch = nullptr;  // forward transform for <T>
+    void* scratch = nullptr;  // forward transform for <T>; T psi for Var(H)
+    void* kvar = nullptr;     // 1D: (0.5 k_j^2 / m, 0) per bin, FP64 (Var(H))
     double* partials = nullptr;
     double* red_out = nullptr;
     std::map<int, void*> tw;  // line length -> W_N table
@@ -1492,6 +1493,7 @@ int tdse_plan_destroy(tdse_plan* p) {
     release(p, p->mask, p->total * rs);
     release_mask_axes(p);
     release(p, p->scratch, p->total * cs);
+    release(p, p->kvar, p->total * 16);
     release(p, p->base_half, p->total * 16);
     release(p, p->base_kin, p->total * 16);
     void* part = p->partials;
@@ -1899,10 +1901,72 @@ int tdse_plan_stream(tdse_plan* p, void** out) {
     return TDSE_OK;
 }
 
+namespace {
+
+// Var(H) = ||(H - <H>) psi||^2 / ||psi||^2 of a 1D plan's state
+// (core/src/observables.ixx:92-119): T psi = IFFT(0.5 k^2 / m FFT(psi)) in the
+// plan's scratch array with the step path's sweep kernels (a line <= 1024:
+// one FFT-K-IFFT sweep; a four-step line: column FFT and twiddle, row
+// FFT-K-IFFT and conjugate twiddle, column IFFT), then H psi = T psi + V psi
+// and the two fixed-order reductions of the residual form.
+int energy_variance(tdse_plan* p, tdse_observables* out) {
+    if (p->kind != kKindSingle && p->kind != kKindFourStep)
+        return fail(TDSE_E_UNSUPPORTED, "the energy variance covers 1D grids (Field1D)");
+    int rc;
+    if (!p->scratch && (rc = alloc(p, &p->scratch, p->total * p->csize()))) return rc;
+    if (!p->kvar) {
+        if ((rc = alloc(p, &p->kvar, p->total * 16))) return rc;
+        const Axis& ax = p->sq[0];
+        const double dk = dk_of(ax);
+        std::vector<double> t(2 * static_cast<size_t>(ax.n));
+        for (int j = 0; j < ax.n; ++j) {
+            const double k = dk * ((2 * j < ax.n) ? j : j - ax.n);
+            t[2 * j] = 0.5 * k * k / p->mass;
+            t[2 * j + 1] = 0.0;
+        }
+        if ((rc = h2d(p, p->kvar, t.data(), t.size() * 8))) {
+            release(p, p->kvar, p->total * 16);
+            return rc;
+        }
+    }
+    TDSE_HIP(hipMemcpyAsync(p->scratch, p->psi, p->total * p->csize(), hipMemcpyDeviceToDevice,
+                            p->stream));
+    Geometry g = geometry(p, p->scratch);
+    std::vector<Launch> L;
+    if (p->kind == kKindSingle) {
+        SweepArgs a = g.single;
+        a.kt[0] = a.kt[1] = a.kt[2] = nullptr;
+        a.kt[2] = p->kvar;
+        a.n_rows = 1;
+        a.n_total = p->sq[0].n;
+        L.push_back({tdse::C_KIN_TW, p->sq[0].n, a, launcher(p, p->sq[0].n, tdse::C_KIN_TW), 0});
+    } else {
+        SweepArgs row = g.row;
+        row.kt[2] = p->kvar;
+        L.push_back({tdse::S_FWD_TW, p->n2, g.col, launcher(p, p->n2, tdse::S_FWD_TW), 0});
+        L.push_back({tdse::C_KIN_TW, p->n1, row, launcher(p, p->n1, tdse::C_KIN_TW), 0});
+        L.push_back({tdse::S_INV, p->n2, g.col, launcher(p, p->n2, tdse::S_INV), 0});
+    }
+    if ((rc = run_launches(p, L))) return rc;
+    double* sums = p->red_out + 40;  // 2 kReduceQ doubles
+    static_assert(40 + 2 * tdse::kReduceQ <= 64, "variance sums fit the reduction output");
+    TDSE_HIP(tdse::launch_energy_variance(p->fp64(), p->psi, p->scratch,
+                                          p->has_potential ? p->v : nullptr, p->total, p->partials,
+                                          sums, p->stream));
+    double h[2 * tdse::kReduceQ];
+    if ((rc = d2h(p, h, sums, sizeof(h)))) return rc;
+    out->energy_variance = h[0] > 0.0 ? h[tdse::kReduceQ] / h[0] : 0.0;
+    return TDSE_OK;
+}
+
+}  // namespace
+
 int tdse_observe(tdse_plan* p, unsigned what, tdse_observables* out) {
     int rc = check_plan(p);
     if (rc) return rc;
     if (!out) return fail(TDSE_E_INVALID, "null out");
+    if ((what & TDSE_OBS_VARIANCE) && p->kind != kKindSingle && p->kind != kKindFourStep)
+        return fail(TDSE_E_UNSUPPORTED, "the energy variance covers 1D grids (Field1D)");
     if ((rc = set_device(p->ctx))) return rc;
     std::memset(out, 0, sizeof(*out));
     tdse::ReduceArgs r = reduce_args(p);
@@ -1970,7 +2034,7 @@ int tdse_observe(tdse_plan* p, unsigned what, tdse_observables* out) {
         const double m = ratio(host[1 + a], host[0]);
         out->sigma[a] = std::sqrt(std::max(0.0, ratio(host[6 + a], host[0]) - m * m));
     }
-    if (!want_k) return TDSE_OK;
+    if (!want_k) return (what & TDSE_OBS_VARIANCE) ? energy_variance(p, out) : TDSE_OK;
     if (p->kind == kKindLattice) {
         // <H> of the lattice Hamiltonian (hops + onsite, core/src/lattice2d.ixx:251-274)
         if (what & TDSE_OBS_MOMENTUM)
@@ -2015,6 +2079,7 @@ int tdse_observe(tdse_plan* p, unsigned what, tdse_observables* out) {
         out->kinetic = t;
     }
     out->energy = out->potential + out->kinetic;
+    if (what & TDSE_OBS_VARIANCE) return energy_variance(p, out);
     return TDSE_OK;
 }
 

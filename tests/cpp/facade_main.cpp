@@ -130,6 +130,80 @@ int main(int argc, char** argv) {
                 std::abs(n2 - 1.0) > 1e-12)
                 return 1;
         }
+        // SplitOperator3D::half_potential_phase / kinetic_phase, the shape of
+        // tests/propagator_tables_test.cpp:28-65: entries equal the defining
+        // formulas bit for bit
+        {
+            const tdse::Grid1D axis{0.0, 4.0, 4};
+            const tdse::Grid3D g4{axis, axis, axis};
+            std::vector<double> v4(static_cast<std::size_t>(g4.size()));
+            const int n4[3] = {4, 4, 4};
+            const double lo4[3] = {0.0, 0.0, 0.0}, hi4[3] = {4.0, 4.0, 4.0}, c4[3] = {1.0, 2.0, 3.0};
+            tdse::check(tdse_build_harmonic(n4, lo4, hi4, 2.0, c4, v4.data()));
+            const double dt4 = 0.1;
+            const tdse::SplitOperator3D prop{g4, v4, dt4};
+            const auto& half = prop.half_potential_phase();
+            const auto& kin = prop.kinetic_phase();
+            bool ok = half.size() == v4.size() && kin.size() == v4.size();
+            for (const std::size_t idx : {std::size_t{0}, std::size_t{17}, v4.size() - 1}) {
+                const double th = -0.5 * v4[idx] * dt4;
+                ok = ok && half[idx].real() == std::cos(th) && half[idx].imag() == std::sin(th);
+            }
+            const double dk = 2.0 * 3.14159265358979323846 / 4.0;
+            auto kw = [&](int j) { return dk * ((2 * j < 4) ? j : j - 4); };
+            const int ijk[3][3] = {{1, 0, 0}, {3, 2, 1}, {0, 0, 2}};
+            for (const auto& t : ijk) {
+                const double kxx = kw(t[0]), kyy = kw(t[1]), kzz = kw(t[2]);
+                const double th = -0.5 * (kxx * kxx + kyy * kyy + kzz * kzz) * dt4;
+                const std::size_t idx = static_cast<std::size_t>(g4.flat(t[0], t[1], t[2]));
+                ok = ok && kin[idx].real() == std::cos(th) && kin[idx].imag() == std::sin(th);
+            }
+            std::printf("facade: phase tables match their formulas: %s\n", ok ? "yes" : "no");
+            if (!ok) return 1;
+        }
+        // WavepacketSimulation::propagator() (tests/simulation_state_test.cpp:
+        // 71-88 shape) on a const simulation, const psi(), and relax(n, dtau)
+        // (core/src/simulation.ixx:45-52): the displaced packet relaxes onto the
+        // centred harmonic ground state, the clock does not move
+        {
+            const tdse::Grid1D axis{-8.0, 8.0, 16};
+            const tdse::Grid3D g16{axis, axis, axis};
+            std::vector<double> v16(static_cast<std::size_t>(g16.size()));
+            const int n16[3] = {16, 16, 16};
+            const double lo16[3] = {-8.0, -8.0, -8.0}, hi16[3] = {8.0, 8.0, 8.0}, c0[3] = {0, 0, 0};
+            tdse::check(tdse_build_harmonic(n16, lo16, hi16, 1.0, c0, v16.data()));
+            const tdse::WavepacketSimulation::Config cfg{g16, v16, {1.0, 0.0, 0.0}, {1.0, 1.0, 1.0},
+                                                         {0.0, 0.0, 0.0}, 0.02};
+            const tdse::WavepacketSimulation sim{cfg};
+            const tdse::SplitOperator3D reference{cfg.grid, cfg.potential, cfg.dt};
+            const auto& h = sim.propagator().half_potential_phase();
+            const auto& rh = reference.half_potential_phase();
+            const auto& k = sim.propagator().kinetic_phase();
+            const auto& rk = reference.kinetic_phase();
+            const bool same = h.size() == rh.size() && h[17] == rh[17] && k.size() == rk.size() &&
+                              k[123] == rk[123];
+            const tdse::Field3D& p0 = sim.psi();
+            tdse::WavepacketSimulation rs{cfg};
+            rs.advance(3);
+            rs.relax(1000, 0.01);
+            const tdse::Field3D& pr = rs.psi();
+            double nrm = 0.0, mx = 0.0;
+            for (int kk = 0; kk < 16; ++kk)
+                for (int j = 0; j < 16; ++j)
+                    for (int i = 0; i < 16; ++i) {
+                        const double w = std::norm(pr(i, j, kk));
+                        nrm += w;
+                        mx += g16.x.coord(i) * w;
+                    }
+            mx /= nrm;
+            nrm *= g16.cell_volume();
+            std::printf("facade: simulation tables shared: %s, psi()[0] %.3e, relaxed <x> %.3e, "
+                        "norm %.15f, t %.3f\n", same ? "yes" : "no", std::abs(p0.data()[0]), mx, nrm,
+                        rs.time());
+            if (!same || std::abs(mx) > 1e-3 || std::abs(nrm - 1.0) > 1e-12 ||
+                std::abs(rs.time() - 0.06) > 1e-15)
+                return 1;
+        }
         if (argc > 1) {
             std::ofstream out(argv[1], std::ios::binary);
             out.write(reinterpret_cast<const char*>(psi.data().data()),

@@ -3,6 +3,7 @@ and its host-side parts (input builders, admission arithmetic, argument
 checks, no-device path) behave like the reference without a GPU."""
 import ctypes as C
 import os
+import math
 import re
 
 import numpy as np
@@ -150,3 +151,41 @@ def test_request_stream_is_seeded_and_poisson(tdse):
     assert all(100 <= s.steps <= 2000 for _, s in a)
     assert {s.precision for _, s in a} == {"fp32", "fp64"}
     assert {s.n for _, s in a} == set(serving.STREAM_SHAPES)
+
+
+@pytest.mark.parametrize("dims,mass", [((8, 16, 4), 1.0), ((16, 8, 1), 0.067), ((6, 10, 12), 2.5)])
+def test_phase_tables_equal_reference_tables(tdse, orc, dims, mass):
+    """SplitOperator3D::half_potential_phase()/kinetic_phase() through the C
+    ABI (core/src/propagator.ixx:72-108): bit for bit the oracle's tables, the
+    reference core's own where it is built here, and the defining formulas at
+    sampled entries (tests/propagator_tables_test.cpp:28-65)."""
+    lo, hi = (-3.0, -5.0, -2.0), (4.0, 5.0, 2.5)
+    v = orc.harmonic_3d(dims, lo, hi, 1.3, (0.5, -0.25, 0.0))
+    grid = tdse.Grid3D(*[tdse.Grid1D(lo[a], hi[a], dims[a]) for a in range(3)])
+    dt = 0.037
+    half, kin = tdse.phase_tables(grid, v, dt, mass)
+    ohalf, okin = orc.so3d_tables(v, dims, lo, hi, dt, mass)
+    assert np.array_equal(half.ravel(), ohalf) and np.array_equal(kin.ravel(), okin)
+    if orc.ref_available():
+        rhalf, rkin = orc.Ref.so3d_tables(v, dims, lo, hi, dt, mass)
+        assert np.array_equal(half.ravel(), rhalf) and np.array_equal(kin.ravel(), rkin)
+    k = [tdse.wavenumbers(tdse.Grid1D(lo[a], hi[a], dims[a])) for a in range(3)]
+    for (i, j, kk) in [(1, 0, 0), (dims[0] - 1, dims[1] // 2, dims[2] - 1), (0, 3, min(1, dims[2] - 1))]:
+        f = i + dims[0] * (j + dims[1] * kk)
+        th = -0.5 * (k[0][i] * k[0][i] + k[1][j] * k[1][j] + k[2][kk] * k[2][kk]) / mass * dt
+        assert kin.ravel()[f] == complex(math.cos(th), math.sin(th))
+        th = -0.5 * v[f] * dt
+        assert half.ravel()[f] == complex(math.cos(th), math.sin(th))
+    op = tdse.SplitOperator3D.__new__(tdse.SplitOperator3D)  # tables only, no device
+    op._grid, op._v, op._dt, op._mass, op._tables = grid, v, dt, mass, None
+    assert np.array_equal(op.half_potential_phase(), half)
+    assert np.array_equal(op.kinetic_phase(), kin)
+
+
+def test_phase_tables_1d_equal_reference(tdse, orc):
+    """SplitOperator1D tables (core/src/propagator.ixx:23-40): no mass."""
+    n, xmin, xmax, dt = 96, -12.0, 12.0, 0.013
+    v = orc.harmonic_1d(xmin, xmax, n, 0.4, 1.5)
+    half, kin = tdse.phase_tables(tdse.Grid1D(xmin, xmax, n), v, dt)
+    ohalf, okin = orc.so1d_tables(v, xmin, xmax, dt)
+    assert np.array_equal(half, ohalf) and np.array_equal(kin, okin)

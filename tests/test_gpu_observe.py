@@ -87,3 +87,63 @@ def test_observables_of_relaxed_ground_state(orc, tdse):
     np.testing.assert_allclose(o["sigma"], [2 ** -0.5] * 3, atol=1e-4)
     np.testing.assert_allclose(o["momentum"], [0, 0, 0], atol=1e-8)
     np.testing.assert_allclose(o["position"], [0, 0, 0], atol=1e-8)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("n,xmin,xmax,omega", [(256, -20.0, 20.0, 0.25), (96, -15.0, 15.0, 0.5),
+                                               (1024, -30.0, 30.0, 0.2),
+                                               (65536, -100.0, 100.0, 4.0)])
+def test_energy_variance_1d(tdse, orc, prec, n, xmin, xmax, omega):
+    """TDSE_OBS_VARIANCE (core/src/observables.ixx:92-119) on the one-block
+    line (256, 96, 1024) and the four-step line (65536): the oscillator's
+    eigenstates give ~0, their equal mix (E1 - E0)^2 / 4 = w^2 / 4, a displaced
+    packet the oracle's value (FP64 to 1e-11 relative, FP32 to 1e-5).  An FP32
+    state carries rounding noise of relative size 2^-24 in every bin, on
+    which H acts with up to 0.5 (pi/h)^2: FP32 results are checked within
+    that floor, 4 (0.5 (pi/h)^2)^2 2^-48, on top."""
+    from ho_states import ho_eigenstate
+    grid = tdse.Grid1D(xmin, xmax, n)
+    v = orc.harmonic_1d(xmin, xmax, n, omega)
+    p = tdse.Propagator(grid, v, 0.01, precision=prec)
+    states = {"ground": ho_eigenstate(xmin, xmax, n, omega, 0),
+              "first": ho_eigenstate(xmin, xmax, n, omega, 1)}
+    states["mix"] = states["ground"] + states["first"]
+    states["packet"] = orc.gaussian_1d(xmin, xmax, n, 0.3 * xmax / 4, 1.2, 0.7)
+    h = (xmax - xmin) / n
+    floor = 0.0 if prec == "fp64" else 4.0 * (0.5 * (np.pi / h) ** 2) ** 2 * 2.0 ** -48
+    for name, psi in states.items():
+        p.upload(psi)
+        got = p.observe(tdse.TDSE_OBS_NORM | tdse.TDSE_OBS_VARIANCE)["energy_variance"]
+        want = orc.energy_variance_1d(psi, xmin, xmax, v)
+        print(f"case {prec} n={n} {name}: Var(H) {got:.12e} oracle {want:.12e}", flush=True)
+        if name in ("ground", "first"):
+            assert abs(got) < (1e-10 if prec == "fp64" else 1e-9 * omega * omega) + floor
+        elif name == "mix":
+            assert abs(got - omega * omega / 4.0) < (1e-10 if prec == "fp64" else 1e-6) * omega ** 2 \
+                + floor
+        tol = 1e-11 if prec == "fp64" else 1e-5
+        assert abs(got - want) <= tol * max(abs(want), omega * omega / 4.0) + floor
+    p.close()
+
+
+def test_energy_variance_high_level_eigenstate(tdse, orc):
+    """The reference's lock (tests/observables_test.cpp:113-121): the level
+    2500 eigenstate of w = 4 on the 65536 scene grid (E = 10002) has Var(H)
+    at round-off, < 1e-8, through the four-step sweeps."""
+    from ho_states import ho_eigenstate
+    v = orc.harmonic_1d(-100.0, 100.0, 65536, 4.0)
+    psi = ho_eigenstate(-100.0, 100.0, 65536, 4.0, 2500)
+    p = tdse.Propagator(tdse.Grid1D(-100.0, 100.0, 65536), v, 0.001, precision="fp64")
+    p.upload(psi)
+    o = p.observe(tdse.TDSE_OBS_NORM | tdse.TDSE_OBS_ENERGY | tdse.TDSE_OBS_VARIANCE)
+    print(f"case level 2500: Var(H) {o['energy_variance']:.3e}, <E> {o['energy']:.9f}", flush=True)
+    assert o["energy_variance"] < 1e-8
+    assert abs(o["energy"] - 10002.0) < 1e-2
+
+
+def test_energy_variance_rejects_3d(tdse):
+    g = tdse.Grid3D(tdse.Grid1D(-4, 4, 16), tdse.Grid1D(-4, 4, 16), tdse.Grid1D(-4, 4, 16))
+    p = tdse.Propagator(g, np.zeros(16 ** 3), 0.01)
+    with pytest.raises(tdse.TdseError):
+        p.observe(tdse.TDSE_OBS_VARIANCE)
+    p.close()

@@ -357,3 +357,34 @@ def test_golden_step_small(orc):
     o1 = orc.so1d_evolve(g["v1"], float(g["lo1"]), float(g["hi1"]), float(g["dt1"]), g["psi1_0"],
                          int(g["steps1"]), mask=g["mask1"])
     assert np.array_equal(o1, g["psi1"])
+
+
+def test_energy_variance_restatement(orc):
+    """energy_variance (core/src/observables.ixx:92-119): the C restatement
+    against the reference core where it is built here (bit for bit), and the
+    closed forms of tests/observables_test.cpp:55-93 and 113-121: zero on the
+    oscillator's eigenstates, (E1 - E0)^2 / 4 on their equal mix, scale
+    invariance, positive on a displaced packet, and round-off level on an
+    eigenstate of energy 10002."""
+    from ho_states import ho_eigenstate
+    omega, xmin, xmax, n = 0.25, -20.0, 20.0, 256
+    v = orc.harmonic_1d(xmin, xmax, n, omega)
+    p0 = ho_eigenstate(xmin, xmax, n, omega, 0)
+    p1 = ho_eigenstate(xmin, xmax, n, omega, 1)
+    mix = p0 + p1
+    assert abs(orc.energy_variance_1d(p0, xmin, xmax, v)) < 1e-10
+    assert abs(orc.energy_variance_1d(p1, xmin, xmax, v)) < 1e-10
+    assert abs(orc.energy_variance_1d(mix, xmin, xmax, v) - omega * omega / 4.0) < 1e-10
+    g = orc.gaussian_1d(-16.0, 16.0, 128, 1.0, 1.3, 0.4)
+    vg = orc.harmonic_1d(-16.0, 16.0, 128, 0.5)
+    var1 = orc.energy_variance_1d(g, -16.0, 16.0, vg)
+    assert var1 > 0.0
+    assert abs(orc.energy_variance_1d(3.0 * g, -16.0, 16.0, vg) - var1) < 1e-9 * (1.0 + var1)
+    if orc.ref_available():
+        for psi, a, b, vv in ((p0, xmin, xmax, v), (mix, xmin, xmax, v), (g, -16.0, 16.0, vg)):
+            assert orc.energy_variance_1d(psi, a, b, vv) == orc.Ref.energy_variance_1d(psi, a, b, vv)
+    # the reference's high-rung lock on the 65536 scene grid
+    vh = orc.harmonic_1d(-100.0, 100.0, 65536, 4.0)
+    ph = ho_eigenstate(-100.0, 100.0, 65536, 4.0, 2500)
+    assert orc.energy_variance_1d(ph, -100.0, 100.0, vh) < 1e-8
+    assert abs(orc.mean_energy_1d(ph, -100.0, 100.0, vh) - 2500.5 * 4.0) < 1e-6 * 10002.0
