@@ -83,6 +83,84 @@ def time_steps(hip, prop, steps, warmup):
     return hip.elapsed_ms(e0, e1) / 1e3, wall
 
 
+def cpu_info():
+    """Host CPUs: count, the share this process may use, the model."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except AttributeError:
+        affinity = os.cpu_count()
+    return {"nproc": os.cpu_count(), "affinity": affinity, "model": model,
+            "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}
+
+
+def default_cpu_threads():
+    """Every core this job may use: OMP_NUM_THREADS where the host sets the
+    job's share, else the CPU affinity."""
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit() and int(env) > 0:
+        return int(env)
+    return cpu_info()["affinity"] or 1
+
+
+def cpu_workload_rate(name, sample_steps):
+    """The reference's own CPU FP64 propagator (oracle/_ref) on `sample_steps`
+    steps of a named workload after one warm-up step (the policy of
+    bench/bench_main.cpp:21-29); the table build is timed by a one-step call
+    and subtracted.  Runs in the calling process (the reference's worker pool
+    is sized once per process from SES_PARALLEL_WORKERS)."""
+    from oracle import oracle as orc
+    from shoeridingerequationsolver_amd import workloads
+    s = {"ho1d": lambda: workloads.ho1d(seed=1),
+         "hydrogen128": lambda: workloads.hydrogen(seed=2),
+         "barrier256": lambda: workloads.barrier(seed=3)}[name]()
+    v, mask, psi0 = s.build_potential(), s.build_mask(), s.build_psi0()
+    if orc.ref_available():
+        kind, workers = "reference", orc.ref().ref_parallel_workers()
+        if s.n[1] == s.n[2] == 1:
+            run = lambda k: orc.Ref.so1d_evolve(v, s.lo[0], s.hi[0], s.dt, psi0, k, mask=mask)
+            workers = 1  # SplitOperator1D::step is serial (core/src/propagator.ixx:44-53)
+        else:
+            p3 = psi0.reshape(s.n[::-1])
+            run = lambda k: orc.Ref.so3d_evolve(v, s.n, s.lo, s.hi, s.dt, p3, k, mask=mask)
+    else:
+        kind, workers = "port", int(os.environ.get("OMP_NUM_THREADS", "1"))
+        if s.n[1] == s.n[2] == 1:
+            run = lambda k: orc.so1d_evolve(v, s.lo[0], s.hi[0], s.dt, psi0, k, mask=mask)
+        else:
+            p3 = psi0.reshape(s.n[::-1])
+            run = lambda k: orc.so3d_evolve(v, s.n, s.lo, s.hi, s.dt, p3, k, mask=mask)
+    t0 = time.perf_counter()
+    run(1)
+    t1 = time.perf_counter()
+    run(1 + sample_steps)
+    t2 = time.perf_counter()
+    per_step = ((t2 - t1) - (t1 - t0)) / sample_steps
+    return {"value": round(1.0 / per_step, 3), "unit": "steps/s", "cores": workers, "kind": kind,
+            "sample": f"{sample_steps} FP64 steps of {name} after 1 warm-up step; table build "
+                      f"subtracted"}
+
+
+def cpu_workload_rate_subprocess(name, sample_steps, threads):
+    import subprocess
+    env = dict(os.environ, SES_PARALLEL_WORKERS=str(threads), OMP_NUM_THREADS=str(threads))
+    code = ("import json, sys; sys.path.insert(0, %r); import bench; "
+            "print(json.dumps(bench.cpu_workload_rate(%r, %d)))") % (ROOT, name, sample_steps)
+    res = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True,
+                         timeout=900)
+    if res.returncode != 0:
+        raise RuntimeError(res.stderr[-500:])
+    return json.loads(res.stdout.strip().splitlines()[-1])
+
+
 def cpu_baseline(solve, v, mask, psi0, sample_steps, threads):
     """Reference CPU FP64 propagator (oracle/_ref) on `sample_steps` steps."""
     os.environ["SES_PARALLEL_WORKERS"] = str(threads)
@@ -221,6 +299,126 @@ def batch_leg(t, workloads, steps, small=False):
     return out
 
 
+GOLDEN_DIR = os.path.join(ROOT, "tests", "golden")
+
+
+def golden_file(name):
+    """A fixture of the reference's CPU FP64 run (tests/golden/, written by
+    tests/golden/make_golden.py from the reference core): data only."""
+    path = os.path.join(GOLDEN_DIR, name)
+    return np.load(path, allow_pickle=False) if os.path.exists(path) else None
+
+
+def vs_reference(t, p, g):
+    """max|dpsi|, norm drift and <E> error of the plan's state against a
+    reference fixture (sub-lattice samples and full-state observables)."""
+    psi = p.download().ravel()
+    idx = g["idx"]
+    d = float(np.abs(psi[idx] - g["psi"]).max())
+    if "line_idx" in g:
+        d = max(d, float(np.abs(psi[g["line_idx"]] - g["line"]).max()))
+    o = p.observe(t.TDSE_OBS_NORM | t.TDSE_OBS_POSITION | t.TDSE_OBS_ENERGY)
+    out = {"max_abs_dpsi": d, "max_rel_dpsi": d / float(g["max_abs"]),
+           "norm_drift": o["norm"] - float(g["norm"]),
+           "energy_err": o["energy"] - float(g["energy"])}
+    if "transmitted" in g:
+        out["transmitted_err"] = p.probability_in_slab(0, 2.0, 1e300) - float(g["transmitted"])
+    return out
+
+
+def configs_leg(t, hip, workloads, cpu_threads, with_cpu):
+    """The other single-solve workloads at their stated sizes: the 1D
+    oscillator (65536 points, 2000 steps) and the hydrogen-like atom (128^3,
+    1000 steps), FP64 (the workloads' precision) and FP32; each plan
+    autotuned, then one full run timed with HIP events on its stream and
+    compared with the reference CPU run stored in tests/golden/.  Beside
+    them, the reference CPU propagator on all job cores and on one core."""
+    out = {}
+    cases = (("ho1d_65536", workloads.ho1d(seed=1), "ho1d.npz", "ho1d", (200, 200)),
+             ("hydrogen_128", workloads.hydrogen(seed=2), "hydrogen128.npz", "hydrogen128", (20, 4)))
+    for name, s, gname, cpu_name, (k_all, k_one) in cases:
+        rec = {}
+        g = golden_file(gname)
+        v, mask, psi0 = s.build_potential(), s.build_mask(), s.build_psi0()
+        for prec in ("fp64", "fp32"):
+            p = t.Propagator(s.grid(), v, s.dt, precision=prec, mask=mask)
+            p.upload(psi0)
+            variants = p.autotune(reps=6)
+            p.upload(psi0)
+            st = p.stream()
+            e0, e1 = hip.event(), hip.event()
+            hip.record(e0, st)
+            p.step(s.steps)
+            hip.record(e1, st)
+            sec = hip.elapsed_ms(e0, e1) / 1e3
+            r = {"steps": s.steps, "steps_per_s": round(s.steps / sec, 1),
+                 "ms_per_step": round(1e3 * sec / s.steps, 5), "step_bytes": p.step_bytes(),
+                 "achieved_gbs": round(p.step_bytes() * s.steps / sec / 1e9, 1),
+                 "variants": variants}
+            if g is not None:
+                r["vs_reference_cpu"] = vs_reference(t, p, g)
+            rec[prec] = r
+            log(f"{name} {prec}: {r['steps_per_s']} steps/s, vs reference {r.get('vs_reference_cpu')}")
+            p.close()
+        if with_cpu:
+            try:
+                rec["cpu_reference"] = cpu_workload_rate_subprocess(cpu_name, k_all, cpu_threads)
+                rec["cpu_reference_one_core"] = cpu_workload_rate_subprocess(cpu_name, k_one, 1)
+                log(f"{name} reference CPU: {rec['cpu_reference']}, one core "
+                    f"{rec['cpu_reference_one_core']}")
+            except Exception as e:
+                log(f"{name} reference CPU unavailable: {e!r}")
+        out[name] = rec
+    return out
+
+
+def big_leg(t, hip, workloads, steps):
+    """512^3 barrier workload in FP32 with the absorbing mask (box 160 a.u.,
+    same spacing as the headline): a 1.07 GB state, four times the 256 MB
+    Infinity Cache, so the sweeps' bandwidth is HBM's.  Inputs built on the
+    GPU from the recipe.  Per-sweep launch times (HIP events) give each
+    sweep's TB/s; a device-to-device copy of the same 1.07 GB (one state into
+    another plan's, 5 times) gives the bandwidth a plain copy reaches."""
+    s = workloads.barrier(seed=3, n=512)
+    p = t.Propagator(s.grid(), None, s.dt, precision="fp32")
+    p.build(s.recipe())
+    variants = p.autotune(reps=4)
+    gpu_s, _ = time_steps(hip, p, steps, 5)
+    prof = p.profile(20)
+    N = s.points
+    mask_bytes = p.step_bytes() - N * (4 * 2 * 8 + 4)
+    sweep_bytes = {"x_mid": N * (2 * 8 + 4) + mask_bytes, "y_fwd": N * 16, "z_kin": N * 16,
+                   "y_inv": N * 16}
+    sweeps = {}
+    for k, b in sweep_bytes.items():
+        ms, c = prof[k]
+        avg = ms / max(c, 1) / 1e3
+        sweeps[k] = {"us": round(avg * 1e6, 1), "bytes": b, "tbs": round(b / avg / 1e12, 3),
+                     "frac": round(b / avg / 1e9 / HBM_PEAK_GBS, 4)}
+    q = t.Propagator(s.grid(), None, s.dt, precision="fp32")
+    st = q.stream()
+    q.copy_state_from(p)
+    q.sync()
+    e0, e1 = hip.event(), hip.event()
+    hip.record(e0, st)
+    for _ in range(5):
+        q.copy_state_from(p)
+    hip.record(e1, st)
+    copy_s = hip.elapsed_ms(e0, e1) / 1e3 / 5
+    copy_tbs = 2 * N * 8 / copy_s / 1e12
+    for k in sweeps:
+        sweeps[k]["frac_of_copy"] = round(sweeps[k]["tbs"] / copy_tbs, 4)
+    out = {"grid": list(s.n), "steps": steps, "steps_per_s": round(steps / gpu_s, 1),
+           "ms_per_step": round(1e3 * gpu_s / steps, 4), "step_bytes": p.step_bytes(),
+           "step_tbs": round(p.step_bytes() * steps / gpu_s / 1e12, 3),
+           "step_frac": round(p.step_bytes() * steps / gpu_s / 1e9 / HBM_PEAK_GBS, 4),
+           "copy_bytes": 2 * N * 8, "copy_tbs": round(copy_tbs, 3), "sweeps": sweeps,
+           "variants": variants}
+    q.close()
+    p.close()
+    return out
+
+
 def paths_leg(t, hip, workloads, steps):
     """Throughput of the other propagator paths on the headline grid (256^3,
     inputs resident, each timed over `steps` steps with HIP events): imaginary
@@ -335,12 +533,17 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=30)
-    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--cpu-threads", type=int, default=None,
+                    help="reference CPU threads (default: every core of the job, OMP_NUM_THREADS "
+                         "or the CPU affinity)")
     ap.add_argument("--cpu-steps-1core", type=int, default=4)
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-paths", action="store_true")
     ap.add_argument("--path-steps", type=int, default=200)
     ap.add_argument("--batch-steps", type=int, default=500)
+    ap.add_argument("--no-configs", action="store_true")
+    ap.add_argument("--no-big", action="store_true")
+    ap.add_argument("--big-steps", type=int, default=100)
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=30)
     ap.add_argument("--serve-budget-gib", type=float, default=6.0)
@@ -363,6 +566,8 @@ def main():
     import shoeridingerequationsolver_amd as t
     from shoeridingerequationsolver_amd import workloads
 
+    if args.cpu_threads is None:
+        args.cpu_threads = default_cpu_threads()
     hip = Hip()
     solve = workloads.barrier(seed=args.seed, n=args.n)
     log(f"workload {solve.name} n={solve.n} V0={solve.params['V0']:.6f} "
@@ -433,24 +638,42 @@ def main():
                                    / HBM_PEAK_GBS, 4),
                 "sweeps_ms": {k: round(ms_ / max(c, 1), 5) for k, (ms_, c) in prof.items()}}
 
+    # accuracy after the workload's full 2000 steps: FP32 against the FP64
+    # GPU path on the whole state, and both against the reference's own CPU
+    # FP64 run (tests/golden/barrier256.npz: a sub-lattice around the packet,
+    # a full x line and the full-state observables)
     accuracy = None
+    accuracy_ok = None
     if not args.no_accuracy and "fp64" in props:
         full = solve.steps
         outs = {}
+        ref = golden_file("barrier256.npz") if args.n == 256 and args.seed == 3 else None
+        vs_ref = {}
         for prec in ("fp32", "fp64"):
             p = props[prec]
             p.upload(psi0)
             p.step(full)
-            obs = p.observe(t.TDSE_OBS_NORM)
-            outs[prec] = (p.download(), obs["norm"], p.probability_in_slab(0, 2.0, 1e300))
+            obs = p.observe(t.TDSE_OBS_NORM | t.TDSE_OBS_ENERGY)
+            outs[prec] = (p.download(), obs["norm"], p.probability_in_slab(0, 2.0, 1e300),
+                          obs["energy"])
+            if ref is not None:
+                vs_ref[prec] = vs_reference(t, p, ref)
         d = np.abs(outs["fp32"][0] - outs["fp64"][0]).max()
         accuracy = {"steps": full, "max_abs_dpsi_fp32_vs_fp64": float(d),
                     "max_abs_psi": float(np.abs(outs["fp64"][0]).max()),
                     "norm_fp64": outs["fp64"][1], "norm_fp32": outs["fp32"][1],
                     "norm_drift_fp32_vs_fp64": outs["fp32"][1] - outs["fp64"][1],
+                    "energy_fp64": outs["fp64"][3],
+                    "energy_err_fp32_vs_fp64": outs["fp32"][3] - outs["fp64"][3],
                     "transmitted_fp64": outs["fp64"][2],
-                    "transmitted_err_fp32": outs["fp32"][2] - outs["fp64"][2]}
-        log(f"accuracy after {full} steps: max|dpsi| {d:.3e}, transmitted {outs['fp64'][2]:.6f}")
+                    "transmitted_err_fp32": outs["fp32"][2] - outs["fp64"][2],
+                    "vs_reference_cpu_fp64": vs_ref or None}
+        # the headline's stated bound: max|dpsi| <= 1e-5 against FP64 (both
+        # the FP64 GPU state and, where the fixture is present, the reference
+        # CPU run)
+        accuracy_ok = bool(d <= 1e-5 and (not vs_ref or vs_ref["fp32"]["max_abs_dpsi"] <= 1e-5))
+        log(f"accuracy after {full} steps: max|dpsi| {d:.3e}, transmitted {outs['fp64'][2]:.6f}, "
+            f"vs reference CPU {vs_ref}")
 
     # the secondary legs are reported when they run; a failure in one is
     # logged and never costs the headline line
@@ -461,6 +684,19 @@ def main():
             log(f"paths: {paths}")
         except Exception as e:
             log(f"paths leg failed: {e!r}")
+    configs = None
+    if not args.no_configs:
+        try:
+            configs = configs_leg(t, hip, workloads, args.cpu_threads, not args.no_cpu)
+        except Exception as e:
+            log(f"configs leg failed: {e!r}")
+    big = None
+    if not args.no_big:
+        try:
+            big = big_leg(t, hip, workloads, args.big_steps)
+            log(f"512^3: {big}")
+        except Exception as e:
+            log(f"512^3 leg failed: {e!r}")
     batch = None
     if not args.no_batch:
         try:
@@ -482,6 +718,7 @@ def main():
     if not args.no_cpu:
         try:
             cpu = cpu_baseline(solve, v, mask, psi0, args.cpu_steps, args.cpu_threads)
+            cpu["host"] = cpu_info()
             log(f"cpu baseline: {cpu}")
             one = cpu_baseline_subprocess(args, 1, args.cpu_steps_1core)
             cpu["one_core"] = {k: one[k] for k in ("value", "unit", "cores", "kind", "sample")}
@@ -520,13 +757,19 @@ def main():
                                    if "fp64_unfused" in result else None),
         "wall_steps_per_s": round(result["fp32"]["wall_steps_per_s"], 2),
         "accuracy": accuracy,
+        "accuracy_ok": accuracy_ok,
         "roofline": roofline,
         "cpu_baseline": cpu,
         "batch": batch,
         "serving": serve,
         "paths": paths,
+        "configs": configs,
+        "hbm_512": big,
     }
     print(json.dumps(line), flush=True)
+    if accuracy_ok is False:
+        log("headline INVALID: FP32 max|dpsi| above 1e-5 against FP64")
+        sys.exit(3)
 
 
 if __name__ == "__main__":

@@ -199,7 +199,9 @@ int tdse_plan_step_bytes(const tdse_plan* plan, uint64_t* out);
  * the potential's, so they merge into the same kick sweeps (at no extra
  * memory traffic); amplitude 0 removes the drive (the static step, bit for
  * bit).  The clock starts at 0, advances by dt per step of every tdse_step
- * call and can be set.  Driven plans do not batch. */
+ * call and can be set.  Driven plans do not batch.  A drive whose kick angle
+ * |amplitude| dt |axis . r| reaches 1e5 rad anywhere in the box returns
+ * TDSE_E_UNSUPPORTED (the kernels reduce it with a short Cody-Waite step). */
 int tdse_plan_set_drive(tdse_plan* plan, const double axis[3], double amplitude, double omega);
 int tdse_plan_set_time(tdse_plan* plan, double t);
 /* Uniform magnetic field `bfield` along `axis` (0 = x, 1 = y, 2 = z) in the

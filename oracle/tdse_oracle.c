@@ -224,9 +224,44 @@ void orc_dft_brute(const double* in, double* out, size_t n, int sign) {
 }
 
 /* Recursive decimation in time by the smallest prime factor p:
- *   X[k] = sum_q W_n^{q k} FFT_{n/p}(x[q::p])[k mod n/p]. */
-static void fft_rec(const double* in, size_t istride, double* out, size_t n, int sign,
-                    double* scratch) {
+ *   X[k] = sum_q W_n^{q k} FFT_{n/p}(x[q::p])[k mod n/p].
+ * The factor of level `depth` (size n) is W_n^j with j = (q k) mod n, taken
+ * from that level's table tab[depth][j] = (cos th, sin th),
+ * th = sign 2 pi j / n: the same expression, so the same bits, as forming
+ * cos and sin of it term by term. */
+typedef struct {
+    int levels;
+    size_t n[64];
+    double* tab[64];
+    int sign;
+} any_plan;
+
+static void any_plan_make(any_plan* pl, size_t n, int sign) {
+    pl->levels = 0;
+    pl->sign = sign;
+    while (n > 1) {
+        size_t p = 2;
+        while (n % p) ++p;
+        double* t = (double*)malloc(sizeof(double) * 2 * n);
+        for (size_t j = 0; j < n; ++j) {
+            const double th = sign * 2.0 * ORC_PI * (double)j / (double)n;
+            t[2 * j] = cos(th);
+            t[2 * j + 1] = sin(th);
+        }
+        pl->n[pl->levels] = n;
+        pl->tab[pl->levels] = t;
+        ++pl->levels;
+        n /= p;
+    }
+}
+
+static void any_plan_free(any_plan* pl) {
+    for (int l = 0; l < pl->levels; ++l) free(pl->tab[l]);
+    pl->levels = 0;
+}
+
+static void fft_rec(const double* in, size_t istride, double* out, size_t n, const any_plan* pl,
+                    int depth, double* scratch) {
     if (n == 1) {
         out[0] = in[0];
         out[1] = in[1];
@@ -235,16 +270,18 @@ static void fft_rec(const double* in, size_t istride, double* out, size_t n, int
     size_t p = 2;
     while (n % p) ++p;
     const size_t m = n / p;
+    const double* w = pl->tab[depth];
     /* sub-transforms into out[q*m .. q*m+m) */
     for (size_t q = 0; q < p; ++q)
-        fft_rec(in + 2 * q * istride, istride * p, out + 2 * q * m, m, sign, scratch + 2 * n);
+        fft_rec(in + 2 * q * istride, istride * p, out + 2 * q * m, m, pl, depth + 1,
+                scratch + 2 * n);
     for (size_t k = 0; k < n; ++k) {
         double sr = 0.0, si = 0.0;
         for (size_t q = 0; q < p; ++q) {
-            const double th = sign * 2.0 * ORC_PI * (double)((q * k) % n) / (double)n;
+            const size_t j = (q * k) % n;
             double pr, pi;
             const double* s = out + 2 * (q * m + (k % m));
-            cmul(s[0], s[1], cos(th), sin(th), &pr, &pi);
+            cmul(s[0], s[1], w[2 * j], w[2 * j + 1], &pr, &pi);
             sr += pr;
             si += pi;
         }
@@ -254,26 +291,48 @@ static void fft_rec(const double* in, size_t istride, double* out, size_t n, int
     memcpy(out, scratch, sizeof(double) * 2 * n);
 }
 
+static void fft_any_planned(double* a, size_t n, const any_plan* pl, double* tmp,
+                            double* scratch) {
+    memcpy(tmp, a, sizeof(double) * 2 * n);
+    fft_rec(tmp, 1, a, n, pl, 0, scratch);
+}
+
 int orc_fft_any(double* a, size_t n, int sign) {
     if (n < 2) return 0;
+    any_plan pl;
+    any_plan_make(&pl, n, sign);
     /* scratch depth: sum over recursion levels of 2*n_level <= 4n */
     double* tmp = (double*)malloc(sizeof(double) * 2 * n);
     double* scratch = (double*)malloc(sizeof(double) * 8 * n + 16);
-    memcpy(tmp, a, sizeof(double) * 2 * n);
-    fft_rec(tmp, 1, a, n, sign, scratch);
+    fft_any_planned(a, n, &pl, tmp, scratch);
+    free(scratch);
+    free(tmp);
+    any_plan_free(&pl);
+    return 0;
+}
+
+/* A 3D pass of the mixed-radix transform: one plan per axis (passed as the
+ * line function's table pointer), per-thread work buffers. */
+static int any_line(double* line, size_t n, const double* w, int sign) {
+    (void)sign;
+    const any_plan* pl = (const any_plan*)(const void*)w;
+    double* tmp = (double*)malloc(sizeof(double) * 2 * n);
+    double* scratch = (double*)malloc(sizeof(double) * 8 * n + 16);
+    fft_any_planned(line, n, pl, tmp, scratch);
     free(scratch);
     free(tmp);
     return 0;
 }
 
-static int any_line(double* line, size_t n, const double* w, int sign) {
-    (void)w;
-    return orc_fft_any(line, n, sign);
-}
-
 int orc_fft3d_any(double* a, int nx, int ny, int nz, int sign) {
     const int d[3] = {nx, ny, nz};
-    for (int axis = 0; axis < 3; ++axis) axis_pass(a, d, axis, any_line, NULL, sign);
+    for (int axis = 0; axis < 3; ++axis) {
+        if (d[axis] < 2) continue;
+        any_plan pl;
+        any_plan_make(&pl, (size_t)d[axis], sign);
+        axis_pass(a, d, axis, any_line, (const double*)(const void*)&pl, sign);
+        any_plan_free(&pl);
+    }
     return 0;
 }
 

@@ -953,6 +953,33 @@ __device__ __forceinline__ void strided_tile(const SweepArgs& a, long long tile,
     }
 }
 
+// XCD-aware tile order of strided sweeps whose tiles are narrower than a
+// 128-byte line (TX * sizeof(cx<T>) < 128: long lines in FP32, FP64 lines of
+// 512 and more).  The G = 128 / (TX * sizeof(cx<T>)) x-tiles that share the
+// same 128-byte lines of every row should meet in one L2: workgroups are
+// dispatched round-robin over the 8 XCDs (workgroup i on XCD i mod 8), so
+// launch index L = 8 G q + 8 j + x (x the XCD, j < G) sweeps tile
+// 8 G q + G x + j: the G tiles of a line group go to consecutive workgroups
+// of one XCD.  A bijection on every whole chunk of 8 G; the last partial
+// chunk keeps the identity order.
+constexpr int kXcds = 8;
+template <typename T, int TX>
+constexpr int line_group() {
+    constexpr int w = TX * static_cast<int>(sizeof(cx<T>));
+    return w >= 128 ? 1 : 128 / w;
+}
+template <int G>
+__device__ __forceinline__ long long xcd_tile(long long l, long long ntiles) {
+    if constexpr (G == 1) {
+        return l;
+    } else {
+        constexpr long long C = static_cast<long long>(kXcds) * G;
+        if (l >= (ntiles / C) * C) return l;
+        const long long q = l / C, r = l % C;
+        return q * C + (r % kXcds) * G + r / kXcds;
+    }
+}
+
 template <typename T, int N>
 __device__ __forceinline__ void stage_kinetic(cx<double>* kp, const SweepArgs& a, int tid,
                                               int nthreads) {
@@ -1012,8 +1039,10 @@ strided_sweep(SweepArgs a) {
     }
     SweepAcc ra;
     const long long ntiles = static_cast<long long>((a.nx + S::TX - 1) / S::TX) * a.nw;
-    for (long long tile = blockIdx.x; tile < ntiles; tile += gridDim.x)
-        strided_tile<T, N, MODE, V>(a, tile, lds, lds_tw, lds_kp, opaque_tid(), ra);
+    constexpr int G = line_group<T, S::TX>();
+    for (long long l = blockIdx.x; l < ntiles; l += gridDim.x)
+        strided_tile<T, N, MODE, V>(a, xcd_tile<G>(l, ntiles), lds, lds_tw, lds_kp, opaque_tid(),
+                                    ra);
     if constexpr (mode_measures(MODE)) relax_block_done(a, ra);
     if constexpr (mode_observes(MODE)) obs_block_done(a, ra);
 }
@@ -1087,7 +1116,10 @@ strided_sweep_batch(BatchTable b) {
     }
     const long long total = b.first_tile[b.nitems];
     const long long chunk = (total + gridDim.x - 1) / gridDim.x;
-    const long long g0 = blockIdx.x * chunk;
+    // one tile per workgroup: the XCD-aware order (a block of several
+    // consecutive tiles already keeps a line group in one L2)
+    const long long g0 = chunk == 1 ? xcd_tile<line_group<T, S::TX>()>(blockIdx.x, total)
+                                    : blockIdx.x * chunk;
     const long long g1 = g0 + chunk < total ? g0 + chunk : total;
     int staged = -1;
     int it = g0 < g1 ? find_item(b, g0) : 0;

@@ -17,7 +17,17 @@ the observables of the full state, so the files stay small.  Inputs are
 rebuilt from the seed by shoeridingerequationsolver_amd.workloads, whose
 builders equal the reference builders bit for bit (tests/test_abi.py).
 
-usage: python tests/golden/make_golden.py [small|ho1d|hydrogen|barrier|all]
+  batch500.npz      the uneven batch workload (seed 4, 500 steps): one solve per
+                    grid shape (64^3, 96^3, 128^3, 160^3, 512^2, 1024^2, 65536)
+  served.npz        the kept requests of a seeded served stream (1024^2, 192^3,
+                    256^3 in both precisions and three smaller ones)
+
+Power-of-two grids run the reference core itself; 96^3, 160^3 and 192^3 (the
+reference throws for them, core/src/fft.ixx:42-44) run the oracle's
+restatement with its mixed-radix transform, pinned by the brute-force DFT.
+The solves of batch500.npz and served.npz are chosen by tests/golden/traces.py.
+
+usage: python tests/golden/make_golden.py [small|ho1d|hydrogen|barrier|batch|served|all]
 """
 import os
 import sys
@@ -118,11 +128,69 @@ def barrier():
          line_idx=line, line=psi.ravel()[line], **_observe3(s, psi, v))
 
 
+def _evolve(s):
+    """FP64 reference run of one solve: (psi in the field's shape, v)."""
+    v, mask, psi0 = s.build_potential(), s.build_mask(), s.build_psi0()
+    dims, lo, hi = s.n, s.lo, s.hi
+    if dims[1] == dims[2] == 1:
+        return R.so1d_evolve(v, lo[0], hi[0], s.dt, psi0, s.steps, mask=mask), v
+    pow2 = all(d & (d - 1) == 0 for d in dims)
+    psi0 = psi0.reshape(dims[::-1])
+    if pow2:
+        return R.so3d_evolve(v, dims, lo, hi, s.dt, psi0, s.steps, mask=mask), v
+    return orc.so3d_evolve(v, dims, lo, hi, s.dt, psi0, s.steps, mask=mask, any_size=True), v
+
+
+def _record(out, key, s, psi, v):
+    from traces import sample_indices
+    dims, lo, hi = s.n, s.lo, s.hi
+    flat = psi.ravel()
+    idx = sample_indices(dims)
+    cell = np.prod([(h - l) / n for l, h, n in zip(lo, hi, dims)])
+    out[f"{key}_idx"] = idx
+    out[f"{key}_psi"] = flat[idx]
+    out[f"{key}_norm"] = orc.norm_sq(flat, cell)
+    out[f"{key}_max_abs"] = np.abs(flat).max()
+    if dims[1] == dims[2] == 1:
+        out[f"{key}_position"] = np.array([R.mean_position_1d(flat, lo[0], hi[0]), 0.0, 0.0])
+        out[f"{key}_energy"] = R.mean_energy_1d(flat, lo[0], hi[0], v)
+    else:
+        f3 = flat.reshape(dims[::-1])
+        out[f"{key}_position"] = np.asarray(orc.mean_position_3d(f3, dims, lo, hi))
+        out[f"{key}_energy"] = orc.mean_energy_3d(f3, dims, lo, hi, v)
+
+
+def batch():
+    from traces import BATCH_COUNT, BATCH_SEED, BATCH_STEPS, batch_golden, tag
+    out = {"seed": BATCH_SEED, "count": BATCH_COUNT, "steps": BATCH_STEPS}
+    for i, s in batch_golden():
+        t0 = time.time()
+        psi, v = _evolve(s)
+        print(f"batch {s.name} {s.n} {s.potential} reference run {time.time() - t0:.1f} s",
+              flush=True)
+        _record(out, tag(s.n), s, psi, v)
+        out[f"{tag(s.n)}_index"] = i
+    save("batch500.npz", **out)
+
+
+def served():
+    from traces import SERVED_SEED, served_golden
+    out = {"seed": SERVED_SEED}
+    for t, s in served_golden():
+        t0 = time.time()
+        psi, v = _evolve(s)
+        print(f"served {s.name} {s.n} {s.precision} {s.potential} mask={s.mask_width} "
+              f"steps={s.steps} reference run {time.time() - t0:.1f} s", flush=True)
+        _record(out, s.name, s, psi, v)
+    save("served.npz", **out)
+
+
 if __name__ == "__main__":
     what = sys.argv[1] if len(sys.argv) > 1 else "small"
     if not orc.ref_available():
         sys.exit("oracle/_ref/libsesref.so is missing: run `make -C oracle ref`")
+    sys.path.insert(0, HERE)
     for name, fn in (("small", small), ("ho1d", ho1d), ("hydrogen", hydrogen),
-                     ("barrier", barrier)):
+                     ("barrier", barrier), ("batch", batch), ("served", served)):
         if what in (name, "all"):
             fn()

@@ -1,0 +1,127 @@
+"""The two multi-solve workloads against the reference's CPU propagator on
+their stated shapes.
+
+* Batched uneven solves (seed 4, 500 steps): one solve of every grid shape of
+  the workload (64^3, 96^3, 128^3, 160^3, 512^2, 1024^2, 65536), each in FP32
+  and in FP64, swept together in ONE tdse_batch with other uneven members of
+  the same seeded list, against tests/golden/batch500.npz.
+* A served stream: requests of 1024^2, 192^3 and 256^3 in both precisions
+  (and three smaller ones), built on the GPU from their recipes, admitted
+  under an HBM budget and stepped in quanta together with the other active
+  requests, against tests/golden/served.npz.
+
+The fixtures come from the reference core compiled from its sources
+(power-of-two grids) or the oracle's mixed-radix restatement (96^3, 160^3,
+192^3, which the reference rejects), written by tests/golden/make_golden.py
+from inputs equal bit for bit to the reference builders.  Each stores about
+2000 points on a sub-lattice of the whole grid, the x line through the centre
+and the observables of the full state.
+
+Tolerances: FP64 max|dpsi| / max|psi| <= 1e-11 (device-built requests: their
+inputs equal the host builders to a few ulps) and norm within 1e-11 (the
+fixture's serial sum); FP32 max|dpsi| <= 1e-5 and norm within 1e-6.  <x> and
+<E> are checked to 1e-9 (FP64) and 1e-5 (FP32, relative to the scale of the
+quantity)."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(HERE, "golden"))
+
+
+def golden(name):
+    path = os.path.join(HERE, "golden", name)
+    if not os.path.exists(path):
+        pytest.skip(f"{name} missing (tests/golden/make_golden.py)")
+    return np.load(path, allow_pickle=False)
+
+
+def check(g, key, got_psi, obs, precision, label):
+    idx = g[f"{key}_idx"]
+    want = g[f"{key}_psi"]
+    scale = float(g[f"{key}_max_abs"])
+    d = float(np.abs(got_psi.ravel()[idx] - want).max())
+    dn = obs["norm"] - float(g[f"{key}_norm"])
+    dx = np.abs(np.asarray(obs["position"]) - g[f"{key}_position"]).max()
+    de = obs["energy"] - float(g[f"{key}_energy"])
+    print(f"case {label} {precision}: max|dpsi| {d:.3e} (rel {d / scale:.3e}), dnorm {dn:.2e}, "
+          f"d<x> {dx:.2e}, dE {de:.2e}", flush=True)
+    e_scale = max(1.0, abs(float(g[f"{key}_energy"])))
+    if precision == "fp64":
+        assert d / scale < 1e-11
+        # the fixture's norm is the reference's serial sum over up to 16.8M
+        # points (core/src/field.ixx:70-76), itself rounded at the 1e-12 level
+        assert abs(dn) < 1e-11
+        assert dx < 1e-9 and abs(de) < 1e-9 * e_scale
+    else:
+        assert d < 1e-5
+        assert abs(dn) < 1e-6
+        assert dx < 1e-5 and abs(de) < 1e-5 * e_scale
+
+
+def test_uneven_batch_matches_reference(tdse):
+    from traces import BATCH_COUNT, BATCH_SEED, BATCH_STEPS, batch_golden, tag
+    from shoeridingerequationsolver_amd import workloads
+    g = golden("batch500.npz")
+    assert int(g["seed"]) == BATCH_SEED and int(g["steps"]) == BATCH_STEPS
+    picked = batch_golden()
+    solves = workloads.batch_instances(seed=BATCH_SEED, count=BATCH_COUNT, steps=BATCH_STEPS)
+    checked = []
+    members = []
+    for i, s in picked:
+        assert int(g[f"{tag(s.n)}_index"]) == i
+        for prec in ("fp32", "fp64"):
+            p = tdse.Propagator(s.grid(), s.build_potential(), s.dt, precision=prec)
+            p.upload(s.build_psi0())
+            checked.append((s, prec, p))
+            members.append(p)
+    # other uneven members of the same seeded list share the batch
+    used = {i for i, _ in picked}
+    extra = [s for j, s in enumerate(solves) if j not in used][:10]
+    for s in extra:
+        p = tdse.Propagator(s.grid(), None, s.dt, precision=s.precision)
+        p.build(s.recipe())
+        members.append(p)
+    b = tdse.Batch(members)
+    print(f"{len(members)} members, {b.launches_per_round()} batched launches per round",
+          flush=True)
+    b.step(BATCH_STEPS)
+    b.sync()
+    for s, prec, p in checked:
+        obs = p.observe(tdse.TDSE_OBS_NORM | tdse.TDSE_OBS_POSITION | tdse.TDSE_OBS_ENERGY)
+        check(g, tag(s.n), p.download(), obs, prec, f"batch {s.name} {s.n} {s.potential}")
+    b.close()
+    for p in members:
+        p.close()
+
+
+def test_served_stream_matches_reference(tdse):
+    from traces import SERVED_SEED, served_golden
+    from shoeridingerequationsolver_amd import serving
+    g = golden("served.npz")
+    assert int(g["seed"]) == SERVED_SEED
+    trace = served_golden()
+    # room for about three of the largest requests at once: admission waits
+    need = sorted(serving.plan_bytes(s.grid(), s.precision, s.mask_width > 0) for _, s in trace)
+    budget = need[-1] + need[-2] + need[-3]
+    srv = serving.Server(budget, quantum=40)
+    rep = srv.run(trace)
+    srv.close()
+    summ = rep.summary()
+    print(summ, flush=True)
+    assert summ["solves"] == len(trace) and rep.max_active > 1 and rep.quanta > 1
+    by_name = {r.name: r for r in rep.results}
+    for _, s in trace:
+        r = by_name[s.name]
+        # observables of the served result, on a plan holding it
+        p = tdse.Propagator(s.grid(), s.build_potential(), s.dt, precision="fp64")
+        p.upload(r.psi)
+        obs = p.observe(tdse.TDSE_OBS_NORM | tdse.TDSE_OBS_POSITION | tdse.TDSE_OBS_ENERGY)
+        p.close()
+        check(g, s.name, r.psi, obs, s.precision,
+              f"served {s.name} {s.n} {s.potential} steps={s.steps}")

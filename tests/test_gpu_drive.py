@@ -167,3 +167,34 @@ def test_driven_step_with_mask_matches_oracle(orc, tdse, kind, prec):
     err = rel(p.download(), want)
     print(f"  rel err {err:.3e}", flush=True)
     assert err < TOL[prec]
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_drive_near_the_angle_bound(orc, tdse, prec):
+    """A drive whose kick angles reach 2e4 rad (a fifth of the 1e5 rad bound)
+    still matches the oracle; the angle's own FP64 rounding (ulp 3.6e-12)
+    enters every kick, so the FP64 bound here is 1e-10.  Beyond the
+    bound set_drive refuses (TDSE_E_UNSUPPORTED -> ValueError)."""
+    dims, lo, hi = (64, 32, 16), (-10, -6, -4), (10, 6, 4)
+    axis, dt, steps = (1.0, 0.5, 0.25), 0.02, 6
+    max_u = sum(abs(a) * max(abs(l), abs(h)) for a, l, h in zip(axis, lo, hi))
+    amp = 2.0e4 / (dt * max_u)
+    print(f"case drive near bound {prec}: amplitude {amp:.1f}", flush=True)
+    v = orc.harmonic_3d(dims, lo, hi, 0.3, (0.2, 0.0, 0.0))
+    psi0 = orc.gaussian_3d(dims, lo, hi, (1.0, 0.0, 0.0), (1.3, 1.1, 0.9),
+                           (0.4, 0.0, 0.0)).reshape(dims[::-1])
+    want = orc.driven_evolve(v, dims, lo, hi, dt, psi0, steps, axis, amp, 0.7, t0=0.1)
+    p = tdse.Propagator(grid3(tdse, dims, lo, hi), v, dt, precision=prec)
+    p.set_drive(axis, amp, 0.7)
+    p.time = 0.1
+    p.upload(psi0)
+    p.step(steps)
+    err = rel(p.download().ravel(), want.ravel())
+    print(f"  rel err {err:.3e}", flush=True)
+    assert err < (1e-10 if prec == "fp64" else TOL[prec])
+    with pytest.raises(ValueError):
+        p.set_drive(axis, 1.01e5 / (dt * max_u), 0.7)
+    with pytest.raises(ValueError):
+        p.set_drive(axis, -3.0e5 / (dt * max_u), 0.7)
+    p.set_drive(axis, 0.9e5 / (dt * max_u), 0.7)  # below the bound: accepted
+    p.close()
