@@ -257,6 +257,14 @@ class Propagator:
     def sync(self, timeout_ms=-1):
         capi.check(capi.lib().tdse_plan_sync(self._h, int(timeout_ms)))
 
+    def done(self) -> bool:
+        """True once the plan's queued work has finished (non-blocking)."""
+        st = capi.lib().tdse_plan_sync(self._h, 0)
+        if st == capi.TDSE_E_TIMEOUT:
+            return False
+        capi.check(st)
+        return True
+
     def scale(self, s):
         capi.check(capi.lib().tdse_state_scale(self._h, C.c_double(s)))
 

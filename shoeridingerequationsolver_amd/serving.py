@@ -8,15 +8,24 @@ precisions) arrive over time.  The server
   its potential, mask and packet directly in HBM (``tdse_plan_build``; or,
   with ``device_build=False``, on host threads while the GPU works),
 * keeps every admitted state resident in HBM (one plan each),
-* advances all active requests together in quanta: one ``tdse_batch_step``
-  over the active set, of ``min(quantum, fewest remaining steps)`` steps, so
-  plans that share a sweep shape share launches and different shapes overlap,
-* between quanta retires finished requests (download, free) and admits new
-  ones, so arrivals join while others are mid-run.
+* overlaps the work of all active requests on the device, by one of two
+  policies:
+
+  - ``"streams"`` (default): each request advances on its own stream in
+    chunks of about ``chunk_ms`` of device time; the loop polls the streams,
+    queues the next chunk of a request as soon as its last one finished,
+    retires finished requests and admits arrivals at once.  No request waits
+    for another's chunk boundary, and nothing synchronises the device as a
+    whole.
+  - ``"batch"``: all active requests advance together in quanta, one
+    ``tdse_batch_step`` of ``min(quantum, fewest remaining steps)`` steps, so
+    requests that share a sweep shape share launches; retiring and admitting
+    happen between quanta.
 
 The reference has no serving layer; its CPU loop advances one scene per frame
 (scenario/src/base_director.ixx:339-357).  The baseline ``run_sequential``
-solves the same trace one request at a time in plain FP64.
+solves the same trace one request at a time, in plain FP64 or at each
+request's own precision.
 """
 from __future__ import annotations
 
@@ -89,6 +98,7 @@ class Result:
     steps: int
     precision: str
     shape: tuple
+    obs: dict | None = None  # norm and <E> of the final state, when asked for
 
     @property
     def latency(self) -> float:
@@ -102,6 +112,8 @@ class _Active:
     admitted: float
     prop: Propagator
     remaining: int
+    inflight: int = 0       # steps queued and not yet seen finished
+    chunk: int = 1          # steps per chunk (streams policy)
 
 
 @dataclass
@@ -123,21 +135,48 @@ class ServeReport:
                 "peak_bytes": self.peak_bytes}
 
 
+_OBS = 1 | 4  # TDSE_OBS_NORM | TDSE_OBS_ENERGY
+
+
+def _finish(a_req, prop, keep, observe, on_result, arrival, admitted, now):
+    """Read a finished request back (and its observables), free its plan."""
+    obs = None
+    if observe:
+        o = prop.observe(_OBS)
+        obs = {"norm": o["norm"], "energy": o["energy"]}
+    psi = prop.download() if (keep or on_result) else None
+    r = Result(a_req.name, arrival, admitted, now(), psi, a_req.steps, a_req.precision, a_req.n,
+               obs)
+    prop.close()
+    if on_result:
+        on_result(r)
+        if not keep:
+            r.psi = None
+    return r
+
+
 def _build(req: Solve):
     return req.build_potential(), req.build_mask(), req.build_psi0()
 
 
 class Server:
-    """Continuous-batching server over one device context with an HBM budget."""
+    """Continuous batching server over one device context with an HBM budget."""
 
     def __init__(self, hbm_budget_bytes: int, quantum: int = 200, device: int = 0,
                  keep_results: bool = True, builder_threads: int = 8,
-                 device_build: bool = True):
+                 device_build: bool = True, policy: str = "streams", chunk_ms: float = 4.0,
+                 observe: bool = False, on_result=None):
+        if policy not in ("streams", "batch"):
+            raise ValueError("policy must be 'streams' or 'batch'")
         self.ctx = Context(device, hbm_budget_bytes)
         self.budget = hbm_budget_bytes
         self.quantum = quantum
         self.keep = keep_results
         self.device_build = device_build
+        self.policy = policy
+        self.chunk_ms = chunk_ms
+        self.observe = observe
+        self.on_result = on_result
         self.pool = cf.ThreadPoolExecutor(max_workers=builder_threads)
 
     def close(self):
@@ -146,6 +185,13 @@ class Server:
 
     def footprint(self, req: Solve) -> int:
         return plan_bytes(req.grid(), req.precision, with_mask=req.mask_width > 0)
+
+    def _chunk(self, prop: Propagator) -> int:
+        """Steps per chunk for about chunk_ms of device time: the step's bytes
+        at 4 TB/s, but no less than 12 us per step (the latency of a small
+        grid's launches)."""
+        est_s = max(prop.step_bytes() / 4.0e12, 12e-6)
+        return int(min(500, max(4, self.chunk_ms * 1e-3 / est_s)))
 
     def run(self, trace: list) -> ServeReport:
         """Serve [(arrival_s, Solve)] in real time; returns per-request results."""
@@ -162,10 +208,17 @@ class Server:
         max_active = 0
         peak = 0
         nxt = 0                         # next trace index whose inputs may be prebuilt
-        batch, members = None, ()       # the batch of the current active set, reused
+        batch, members = None, ()       # batch policy: the batch of the current active set
+        streams = self.policy == "streams"
 
         def now():
             return time.perf_counter() - t0
+
+        def retire(a):
+            nonlocal reserved
+            results.append(_finish(a.req, a.prop, self.keep, self.observe, self.on_result,
+                                   a.arrival, a.admitted, now))
+            reserved -= self.footprint(a.req)
 
         while pending or active:
             clock = now()
@@ -192,7 +245,13 @@ class Server:
                                       context=self.ctx)
                     prop.upload(psi0)
                 reserved += need
-                active.append(_Active(req, arr, now(), prop, req.steps))
+                a = _Active(req, arr, now(), prop, req.steps)
+                if streams:
+                    a.chunk = self._chunk(prop)
+                    a.inflight = min(a.chunk, a.remaining)
+                    prop.step(a.inflight)
+                    quanta += 1
+                active.append(a)
                 pending.pop(0)
             peak = max(peak, reserved)
             if not active:
@@ -203,6 +262,25 @@ class Server:
                         time.sleep(min(wait, 0.01))
                 continue
             max_active = max(max_active, len(active))
+            if streams:
+                progressed = False
+                still = []
+                for a in active:
+                    if a.inflight and a.prop.done():
+                        a.remaining -= a.inflight
+                        a.inflight = 0
+                        progressed = True
+                        if a.remaining == 0:
+                            retire(a)
+                            continue
+                        a.inflight = min(a.chunk, a.remaining)
+                        a.prop.step(a.inflight)
+                        quanta += 1
+                    still.append(a)
+                active = still
+                if not progressed:
+                    time.sleep(0.0002)
+                continue
             s = min(self.quantum, min(a.remaining for a in active))
             # a new batch (device tables, streams) only when the active set changed
             if tuple(id(a.prop) for a in active) != members:
@@ -217,14 +295,10 @@ class Server:
             for a in active:
                 a.remaining -= s
                 if a.remaining == 0:
-                    psi = a.prop.download() if self.keep else None
-                    results.append(Result(a.req.name, a.arrival, a.admitted, now(), psi,
-                                          a.req.steps, a.req.precision, a.req.n))
-                    reserved -= self.footprint(a.req)
                     if batch is not None:  # the batch holds the finished plan
                         batch.close()
                         batch, members = None, ()
-                    a.prop.close()
+                    retire(a)
                 else:
                     still.append(a)
             active = still
@@ -233,11 +307,12 @@ class Server:
         return ServeReport(results, now(), quanta, max_active, peak)
 
 
-def run_sequential(trace: list, precision: str = "fp64", device: int = 0,
-                   keep_results: bool = False, device_build: bool = True) -> ServeReport:
-    """Baseline: the same trace solved one request at a time, plain FP64 (or
-    `precision`), each plan created, stepped to the end and read back before
-    the next request starts."""
+def run_sequential(trace: list, precision: str | None = "fp64", device: int = 0,
+                   keep_results: bool = False, device_build: bool = True, observe: bool = False,
+                   on_result=None) -> ServeReport:
+    """Baseline: the same trace solved one request at a time, in plain FP64
+    (or `precision`; None: each request's own precision), each plan created,
+    stepped to the end and read back before the next request starts."""
     ctx = Context(device)
     t0 = time.perf_counter()
     results = []
@@ -246,18 +321,28 @@ def run_sequential(trace: list, precision: str = "fp64", device: int = 0,
         if wait > 0:
             time.sleep(wait)
         start = time.perf_counter() - t0
+        prec = precision or req.precision
         if device_build:
-            p = Propagator(req.grid(), None, req.dt, precision=precision, context=ctx)
+            p = Propagator(req.grid(), None, req.dt, precision=prec, context=ctx)
             p.build(req.recipe())
         else:
             v, m, psi0 = _build(req)
-            p = Propagator(req.grid(), v, req.dt, precision=precision, mask=m, context=ctx)
+            p = Propagator(req.grid(), v, req.dt, precision=prec, mask=m, context=ctx)
             p.upload(psi0)
         p.step(req.steps)
-        psi = p.download()
+        obs = None
+        if observe:
+            o = p.observe(_OBS)
+            obs = {"norm": o["norm"], "energy": o["energy"]}
+        psi = p.download() if (keep_results or on_result) else None
         p.close()
-        results.append(Result(req.name, arr, start, time.perf_counter() - t0,
-                              psi if keep_results else None, req.steps, precision, req.n))
+        r = Result(req.name, arr, start, time.perf_counter() - t0, psi, req.steps, prec, req.n,
+                   obs)
+        if on_result:
+            on_result(r)
+            if not keep_results:
+                r.psi = None
+        results.append(r)
     report = ServeReport(results, time.perf_counter() - t0, len(trace), 1, 0)
     ctx.close()
     return report

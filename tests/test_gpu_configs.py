@@ -100,7 +100,8 @@ def test_uneven_batch_matches_reference(tdse):
         p.close()
 
 
-def test_served_stream_matches_reference(tdse):
+@pytest.mark.parametrize("policy", ["streams", "batch"])
+def test_served_stream_matches_reference(tdse, policy):
     from traces import SERVED_SEED, served_golden
     from shoeridingerequationsolver_amd import serving
     g = golden("served.npz")
@@ -109,7 +110,7 @@ def test_served_stream_matches_reference(tdse):
     # room for about three of the largest requests at once: admission waits
     need = sorted(serving.plan_bytes(s.grid(), s.precision, s.mask_width > 0) for _, s in trace)
     budget = need[-1] + need[-2] + need[-3]
-    srv = serving.Server(budget, quantum=40)
+    srv = serving.Server(budget, quantum=40, policy=policy, chunk_ms=1.0)
     rep = srv.run(trace)
     srv.close()
     summ = rep.summary()
@@ -124,4 +125,4 @@ def test_served_stream_matches_reference(tdse):
         obs = p.observe(tdse.TDSE_OBS_NORM | tdse.TDSE_OBS_POSITION | tdse.TDSE_OBS_ENERGY)
         p.close()
         check(g, s.name, r.psi, obs, s.precision,
-              f"served {s.name} {s.n} {s.potential} steps={s.steps}")
+              f"served ({policy}) {s.name} {s.n} {s.potential} steps={s.steps}")

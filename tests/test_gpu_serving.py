@@ -9,13 +9,14 @@ pytestmark = pytest.mark.gpu
 SHAPES = [(65536, 1, 1), (256, 256, 1), (32, 32, 32), (64, 64, 64), (96, 96, 96), (1024, 1, 1)]
 
 
-def test_mix_equals_alone(tdse):
+@pytest.mark.parametrize("policy", ["streams", "batch"])
+def test_mix_equals_alone(tdse, policy):
     from shoeridingerequationsolver_amd import serving
     trace = serving.request_stream(seed=21, count=14, rate=400.0, max_steps=160, shapes=SHAPES)
     # a budget that holds only a few requests at once, so admission waits
     need = sorted(serving.plan_bytes(s.grid(), s.precision, s.mask_width > 0) for _, s in trace)
     budget = need[-1] + need[-2] + need[-3]
-    srv = serving.Server(budget, quantum=40)
+    srv = serving.Server(budget, quantum=40, policy=policy, chunk_ms=0.5)
     rep = srv.run(trace)
     srv.close()
     summ = rep.summary()
