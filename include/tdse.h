@@ -257,30 +257,36 @@ int tdse_plan_relax_stats(tdse_plan* plan, tdse_relax_stats* out);
  * measurement, one pass of coordinate descent over the slots), the sweeps run
  * once per call by the time of `reps` launches of their own.  A change from
  * the current choice must be at least 2% faster.  variants[] receives the
- * choice per slot.  Lengths 64-1024 and 96 have variants; the others, the
- * one-block 1D kernel and the
- * batched kernels run variant 0.  No reference counterpart (the reference
- * has one FFT plan per length, core/src/fft.ixx:64-100). */
+ * choice per slot.  Lengths 64-1024 and 96 have variants; the others and the
+ * one-block 1D kernel run variant 0.  A batch sweeps each member with the
+ * variants its plan holds when the batch is created (plans of one line
+ * length, mode and variant share a launch).  No reference counterpart (the
+ * reference has one FFT plan per length, core/src/fft.ixx:64-100). */
 int tdse_plan_autotune(tdse_plan* plan, int reps, int max_slots, int* variants, int* n_slots);
 int tdse_plan_get_variants(const tdse_plan* plan, int max_slots, int* variants, int* n_slots);
 /* TDSE_E_UNSUPPORTED when a slot's variant is not compiled for its sweep. */
 int tdse_plan_set_variants(tdse_plan* plan, const int* variants, int n_slots);
 
 /* Batches: many plans (any shapes, both precisions) advanced together.
- * Plans that share a line length and sweep mode are swept by one batched
- * launch; the launches of different plans overlap (each launch waits only for the sweeps of its
- * own plans).  use_graph must be 0: replay of stream-captured hipGraphs is
- * switched off (TDSE_E_UNSUPPORTED).  The batch snapshots each plan's buffers at
+ * Members of one class (precision, geometry, line lengths) form units whose
+ * one-step working set stays within 64 MB (a quarter of the Infinity Cache);
+ * a unit's members share one launch per sweep (per kernel variant), and each
+ * of four streams runs its units' steps back to back, so a unit's state stays
+ * cached between its sweeps while four units overlap on the device.
+ * use_graph must be 0: replay of stream-captured hipGraphs is switched off
+ * (TDSE_E_UNSUPPORTED).  The batch snapshots each plan's buffers at
  * creation: re-create it after tdse_plan_set_mask changes a plan's mask from
  * null to non-null.  tdse_batch_step(b, n) == tdse_step(p, n) for every plan
- * p of the batch (no reference counterpart: the reference steps one scene at a time). */
+ * p of the batch (no reference counterpart: the reference steps one scene at a time).
+ * Each member is swept in the kernel variants its plan holds at creation
+ * (tdse_plan_autotune / tdse_plan_set_variants). */
 typedef struct tdse_batch tdse_batch;
 int tdse_batch_create(tdse_ctx* ctx, tdse_plan* const* plans, int count, tdse_batch** out);
 int tdse_batch_destroy(tdse_batch* batch);
 int tdse_batch_step(tdse_batch* batch, int nsteps, int use_graph);
 int tdse_batch_sync(tdse_batch* batch, int timeout_ms);
 int tdse_batch_stream(tdse_batch* batch, void** out);
-/* Number of batched launches in one interior round (all sub-phases). */
+/* Number of batched launches in one interior round of every unit. */
 int tdse_batch_launches_per_round(const tdse_batch* batch, int* out);
 
 /* Build a plan's inputs on the GPU from a recipe: potential (and its kick

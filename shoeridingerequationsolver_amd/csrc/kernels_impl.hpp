@@ -33,35 +33,39 @@ hipError_t launch_sweep(const SweepArgs& a, int max_blocks, hipStream_t stream) 
     return hipGetLastError();
 }
 
-template <typename T, int N, int MODE>
+template <typename T, int N, int MODE, int V = 0>
 hipError_t launch_batch(const BatchTable& b, long long tiles, int max_blocks, hipStream_t stream) {
-    using S = Shape<T, N>;
+    using S = Shape<T, N, V>;
     if (tiles <= 0) return hipSuccess;
     const int grid = static_cast<int>(tiles < max_blocks ? tiles : max_blocks);
     if constexpr (mode_contiguous(MODE)) {
-        hipLaunchKernelGGL((contig_sweep_batch<T, N, MODE>), dim3(grid), dim3(S::BLOCK_C), 0,
+        hipLaunchKernelGGL((contig_sweep_batch<T, N, MODE, V>), dim3(grid), dim3(S::BLOCK_C), 0,
                            stream, b);
     } else {
-        hipLaunchKernelGGL((strided_sweep_batch<T, N, MODE>), dim3(grid), dim3(S::BLOCK_S), 0,
+        hipLaunchKernelGGL((strided_sweep_batch<T, N, MODE, V>), dim3(grid), dim3(S::BLOCK_S), 0,
                            stream, b);
     }
     return hipGetLastError();
 }
 
-template <typename T, int N>
+// The batched modes; variants 1..3 for the modes that have them (C_SINGLE:
+// variant 0 only).
+template <typename T, int N, int V = 0>
 BatchLaunchFn pick_batch_mode(int mode) {
     switch (mode) {
-        case C_KICK_FIRST: return &launch_batch<T, N, C_KICK_FIRST>;
-        case C_KICK_MID: return &launch_batch<T, N, C_KICK_MID>;
-        case C_KICK_LAST: return &launch_batch<T, N, C_KICK_LAST>;
-        case C_KIN_TW: return &launch_batch<T, N, C_KIN_TW>;
-        case C_SINGLE: return &launch_batch<T, N, C_SINGLE>;
-        case S_FWD: return &launch_batch<T, N, S_FWD>;
-        case S_INV: return &launch_batch<T, N, S_INV>;
-        case S_KIN: return &launch_batch<T, N, S_KIN>;
-        case S_KICK_FIRST_TW: return &launch_batch<T, N, S_KICK_FIRST_TW>;
-        case S_KICK_MID_TW: return &launch_batch<T, N, S_KICK_MID_TW>;
-        case S_KICK_LAST: return &launch_batch<T, N, S_KICK_LAST>;
+        case C_KICK_FIRST: return &launch_batch<T, N, C_KICK_FIRST, V>;
+        case C_KICK_MID: return &launch_batch<T, N, C_KICK_MID, V>;
+        case C_KICK_LAST: return &launch_batch<T, N, C_KICK_LAST, V>;
+        case C_KIN_TW: return &launch_batch<T, N, C_KIN_TW, V>;
+        case C_SINGLE:
+            if constexpr (V == 0) return &launch_batch<T, N, C_SINGLE>;
+            else return nullptr;
+        case S_FWD: return &launch_batch<T, N, S_FWD, V>;
+        case S_INV: return &launch_batch<T, N, S_INV, V>;
+        case S_KIN: return &launch_batch<T, N, S_KIN, V>;
+        case S_KICK_FIRST_TW: return &launch_batch<T, N, S_KICK_FIRST_TW, V>;
+        case S_KICK_MID_TW: return &launch_batch<T, N, S_KICK_MID_TW, V>;
+        case S_KICK_LAST: return &launch_batch<T, N, S_KICK_LAST, V>;
         default: return nullptr;
     }
 }
@@ -106,9 +110,9 @@ LaunchFn pick_mode(int mode) {
     }
 }
 
-template <typename T, int N>
+template <typename T, int N, int V = 0>
 SweepShape shape_of(int mode) {
-    using S = Shape<T, N>;
+    using S = Shape<T, N, V>;
     if (mode_is_contiguous(mode)) return {S::BLOCK_C, S::LPB};
     return {S::BLOCK_S, S::TX};
 }
@@ -145,6 +149,30 @@ LaunchFn TDSE_CAT(TDSE_CAT(find_variant_launcher_, TDSE_SUFFIX), TDSE_CAT(_v, TD
 #define TDSE_CASE(N_) \
     case N_:          \
         return pick_variant_mode<TDSE_REAL, N_, TDSE_VARIANT_UNIT>(mode);
+        TDSE_FOR_EACH_TUNED_N(TDSE_CASE)
+#undef TDSE_CASE
+        default:
+            return nullptr;
+    }
+}
+SweepShape TDSE_CAT(TDSE_CAT(sweep_shape_, TDSE_SUFFIX), TDSE_CAT(_v, TDSE_VARIANT_UNIT))(int n,
+                                                                                   int mode) {
+    switch (n) {
+#define TDSE_CASE(N_) \
+    case N_:          \
+        return shape_of<TDSE_REAL, N_, TDSE_VARIANT_UNIT>(mode);
+        TDSE_FOR_EACH_TUNED_N(TDSE_CASE)
+#undef TDSE_CASE
+        default:
+            return {0, 0};
+    }
+}
+BatchLaunchFn TDSE_CAT(TDSE_CAT(find_batch_variant_launcher_, TDSE_SUFFIX),
+                       TDSE_CAT(_v, TDSE_VARIANT_UNIT))(int n, int mode) {
+    switch (n) {
+#define TDSE_CASE(N_) \
+    case N_:          \
+        return pick_batch_mode<TDSE_REAL, N_, TDSE_VARIANT_UNIT>(mode);
         TDSE_FOR_EACH_TUNED_N(TDSE_CASE)
 #undef TDSE_CASE
         default:

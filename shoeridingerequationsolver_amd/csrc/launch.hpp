@@ -32,6 +32,48 @@ LaunchFn find_variant_launcher_f64_v1(int n, int mode);
 LaunchFn find_variant_launcher_f64_v2(int n, int mode);
 LaunchFn find_variant_launcher_f64_v3(int n, int mode);
 
+BatchLaunchFn find_batch_variant_launcher_f32_v1(int n, int mode);
+BatchLaunchFn find_batch_variant_launcher_f32_v2(int n, int mode);
+BatchLaunchFn find_batch_variant_launcher_f32_v3(int n, int mode);
+BatchLaunchFn find_batch_variant_launcher_f64_v1(int n, int mode);
+BatchLaunchFn find_batch_variant_launcher_f64_v2(int n, int mode);
+BatchLaunchFn find_batch_variant_launcher_f64_v3(int n, int mode);
+
+SweepShape sweep_shape_f32_v1(int n, int mode);
+SweepShape sweep_shape_f32_v2(int n, int mode);
+SweepShape sweep_shape_f32_v3(int n, int mode);
+SweepShape sweep_shape_f64_v1(int n, int mode);
+SweepShape sweep_shape_f64_v2(int n, int mode);
+SweepShape sweep_shape_f64_v3(int n, int mode);
+
+// Batched launcher of (line length, mode) in kernel variant `variant`
+// (0: the base kernels); null if not compiled.
+inline BatchLaunchFn find_batch_launcher(bool fp64, int n, int mode, int variant) {
+    switch (variant) {
+        case 0: return fp64 ? find_batch_launcher_f64(n, mode) : find_batch_launcher_f32(n, mode);
+        case 1: return fp64 ? find_batch_variant_launcher_f64_v1(n, mode)
+                            : find_batch_variant_launcher_f32_v1(n, mode);
+        case 2: return fp64 ? find_batch_variant_launcher_f64_v2(n, mode)
+                            : find_batch_variant_launcher_f32_v2(n, mode);
+        case 3: return fp64 ? find_batch_variant_launcher_f64_v3(n, mode)
+                            : find_batch_variant_launcher_f32_v3(n, mode);
+        default: return nullptr;
+    }
+}
+
+SweepShape sweep_shape_f32(int n, int mode);
+SweepShape sweep_shape_f64(int n, int mode);
+// Block size and tile width of (line length, mode) in kernel variant `variant`.
+inline SweepShape sweep_shape(bool fp64, int n, int mode, int variant) {
+    switch (variant) {
+        case 0: return fp64 ? sweep_shape_f64(n, mode) : sweep_shape_f32(n, mode);
+        case 1: return fp64 ? sweep_shape_f64_v1(n, mode) : sweep_shape_f32_v1(n, mode);
+        case 2: return fp64 ? sweep_shape_f64_v2(n, mode) : sweep_shape_f32_v2(n, mode);
+        case 3: return fp64 ? sweep_shape_f64_v3(n, mode) : sweep_shape_f32_v3(n, mode);
+        default: return {0, 0};
+    }
+}
+
 inline LaunchFn find_variant_launcher_f32(int n, int mode, int variant) {
     switch (variant) {
         case 1: return find_variant_launcher_f32_v1(n, mode);

@@ -1069,10 +1069,10 @@ __device__ __forceinline__ int find_item(const BatchTable& b, long long tile) {
     return lo;
 }
 
-template <typename T, int N, int MODE>
-__global__ void __launch_bounds__((Shape<T, N>::BLOCK_C), TDSE_MIN_WAVES)
+template <typename T, int N, int MODE, int V = 0>
+__global__ void __launch_bounds__((Shape<T, N, V>::BLOCK_C), TDSE_MIN_WAVES)
 contig_sweep_batch(BatchTable b) {
-    using S = Shape<T, N>;
+    using S = Shape<T, N, V>;
     __shared__ cx<T> lds[S::LDS_C];
     __shared__ cx<double> lds_tw[S::TWN];
     const int tid = threadIdx.x;
@@ -1093,18 +1093,18 @@ contig_sweep_batch(BatchTable b) {
         if constexpr (MODE == C_SINGLE) {
             SweepArgs a = b.items[it];
             a.nsteps = b.nsteps;
-            contig_tile<T, N, MODE>(a, g - b.first_tile[it], lds, lds_tw, opaque_tid(), unused);
+            contig_tile<T, N, MODE, V>(a, g - b.first_tile[it], lds, lds_tw, opaque_tid(), unused);
         } else {
-            contig_tile<T, N, MODE>(b.items[it], g - b.first_tile[it], lds, lds_tw, opaque_tid(),
-                                    unused);
+            contig_tile<T, N, MODE, V>(b.items[it], g - b.first_tile[it], lds, lds_tw,
+                                       opaque_tid(), unused);
         }
     }
 }
 
-template <typename T, int N, int MODE>
-__global__ void __launch_bounds__((Shape<T, N>::BLOCK_S), TDSE_MIN_WAVES)
+template <typename T, int N, int MODE, int V = 0>
+__global__ void __launch_bounds__((Shape<T, N, V>::BLOCK_S), TDSE_MIN_WAVES)
 strided_sweep_batch(BatchTable b) {
-    using S = Shape<T, N>;
+    using S = Shape<T, N, V>;
     constexpr bool KIN = MODE == S_KIN;
     __shared__ cx<T> lds[S::LDS_S];
     __shared__ cx<double> lds_tw[S::TWN];
@@ -1134,8 +1134,8 @@ strided_sweep_batch(BatchTable b) {
             }
         }
         SweepAcc unused;
-        strided_tile<T, N, MODE>(b.items[it], g - b.first_tile[it], lds, lds_tw, lds_kp,
-                                 opaque_tid(), unused);
+        strided_tile<T, N, MODE, V>(b.items[it], g - b.first_tile[it], lds, lds_tw, lds_kp,
+                                    opaque_tid(), unused);
     }
 }
 

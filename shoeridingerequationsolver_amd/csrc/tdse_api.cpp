@@ -1058,14 +1058,22 @@ namespace {
 #endif
 constexpr int kBatchMaxBlocks = TDSE_BATCH_BLOCKS;
 
+// Streams a batch spreads its launches over (the main one included).
+#ifndef TDSE_BATCH_LANES
+#define TDSE_BATCH_LANES 4
+#endif
+// Working-set cap of a unit (bytes of state, kick phase and mask that one
+// step touches): a quarter of the 256 MB Infinity Cache, one lane's share.
+#ifndef TDSE_BATCH_UNIT_BYTES
+#define TDSE_BATCH_UNIT_BYTES (64LL << 20)
+#endif
+
 // Plans that share (precision, line length, mode) in one sub-phase of a round
 // are swept by one batched launch.
 struct BatchGroup {
-    int precision = 0, mode = 0, n = 0;
+    int precision = 0, mode = 0, n = 0, variant = 0;
     tdse::BatchLaunchFn fn = nullptr;
     std::vector<int> plans;       // indices into the batch's plan list
-    hipEvent_t done = nullptr;    // recorded after each launch of this group
-    int lane = 0;                 // stream the group is launched on
     void* d_items = nullptr;  // SweepArgs[nitems]
     void* d_first = nullptr;  // long long[nitems + 1]
     int nitems = 0;
@@ -1077,32 +1085,36 @@ struct BatchGroup {
 };
 
 // One sweep of a plan inside a round: sub-phase 0..3, modes for an interior
-// round and for the last round of a call.
+// round and for the last round of a call, and the plan's kernel variants of
+// those sweeps (the schedule slots they stand for).
 struct RoundSweep {
     int sub, n, mode_mid, mode_last;
     SweepArgs args;
+    int var_mid, var_last;
 };
 
 std::vector<RoundSweep> plan_round(const tdse_plan* p) {
     Geometry g = geometry(p, p->psi);
+    const int* v = p->variant;
     std::vector<RoundSweep> r;
     if (p->kind == kKind3D) {
-        r.push_back({0, p->sq[1].n, tdse::S_FWD, tdse::S_FWD, g.yf});
-        r.push_back({1, p->sq[2].n, tdse::S_KIN, tdse::S_KIN, g.z});
-        r.push_back({2, p->sq[1].n, tdse::S_INV, tdse::S_INV, g.yi});
-        r.push_back({3, p->sq[0].n, tdse::C_KICK_MID, tdse::C_KICK_LAST, g.x});
+        r.push_back({0, p->sq[1].n, tdse::S_FWD, tdse::S_FWD, g.yf, v[1], v[1]});
+        r.push_back({1, p->sq[2].n, tdse::S_KIN, tdse::S_KIN, g.z, v[2], v[2]});
+        r.push_back({2, p->sq[1].n, tdse::S_INV, tdse::S_INV, g.yi, v[3], v[3]});
+        r.push_back({3, p->sq[0].n, tdse::C_KICK_MID, tdse::C_KICK_LAST, g.x, v[4], v[5]});
     } else if (p->kind == kKind2D) {
-        r.push_back({1, p->sq[1].n, tdse::S_KIN, tdse::S_KIN, g.yf});
-        r.push_back({3, p->sq[0].n, tdse::C_KICK_MID, tdse::C_KICK_LAST, g.x});
+        r.push_back({1, p->sq[1].n, tdse::S_KIN, tdse::S_KIN, g.yf, v[1], v[1]});
+        r.push_back({3, p->sq[0].n, tdse::C_KICK_MID, tdse::C_KICK_LAST, g.x, v[2], v[3]});
     } else if (p->kind == kKindFourStep) {
-        r.push_back({1, p->n1, tdse::C_KIN_TW, tdse::C_KIN_TW, g.row});
-        r.push_back({3, p->n2, tdse::S_KICK_MID_TW, tdse::S_KICK_LAST, g.col});
+        r.push_back({1, p->n1, tdse::C_KIN_TW, tdse::C_KIN_TW, g.row, v[1], v[1]});
+        r.push_back({3, p->n2, tdse::S_KICK_MID_TW, tdse::S_KICK_LAST, g.col, v[2], v[3]});
     }
     return r;
 }
 
-bool plan_init(const tdse_plan* p, int* n, int* mode, SweepArgs* a) {
+bool plan_init(const tdse_plan* p, int* n, int* mode, SweepArgs* a, int* variant) {
     Geometry g = geometry(p, p->psi);
+    *variant = p->variant[0];
     if (p->kind == kKind3D || p->kind == kKind2D) {
         *n = p->sq[0].n;
         *mode = tdse::C_KICK_FIRST;
@@ -1118,9 +1130,8 @@ bool plan_init(const tdse_plan* p, int* n, int* mode, SweepArgs* a) {
     return false;
 }
 
-long long tiles_of(int precision, int n, int mode, const SweepArgs& a) {
-    const tdse::SweepShape sh =
-        precision == TDSE_FP64 ? tdse::sweep_shape_f64(n, mode) : tdse::sweep_shape_f32(n, mode);
+long long tiles_of(int precision, int n, int mode, const SweepArgs& a, int variant = 0) {
+    const tdse::SweepShape sh = tdse::sweep_shape(precision == TDSE_FP64, n, mode, variant);
     if (sh.per_block <= 0) return 0;
     if (tdse::mode_is_contiguous(mode)) return (a.nlines + sh.per_block - 1) / sh.per_block;
     return static_cast<long long>((a.nx + sh.per_block - 1) / sh.per_block) * a.nw;
@@ -1128,24 +1139,26 @@ long long tiles_of(int precision, int n, int mode, const SweepArgs& a) {
 
 }  // namespace
 
+// A unit of a batch: members that advance together, a call's steps at a
+// time, on one lane.  Its groups sweep all of its members per launch.
+struct BatchUnit {
+    std::vector<int> members;  // indices into the batch's plan list
+    std::vector<BatchGroup> init, mid[4], last[4];
+    int lane = 0;
+    uint64_t bytes = 0;        // working set of one step (state, kick phase, mask)
+};
+
 struct tdse_batch {
     tdse_ctx* ctx = nullptr;
     std::vector<tdse_plan*> plans;
-    std::vector<tdse_plan*> singles;  // 1D lines run whole calls in one block
     hipStream_t stream = nullptr;
-    static constexpr int kSide = 3;
+    static constexpr int kSide = TDSE_BATCH_LANES - 1;
     hipStream_t side[kSide] = {};
     hipEvent_t fork = nullptr;
     hipEvent_t done = nullptr;  // end of the last call, awaited by the plans' streams
     hipEvent_t join[kSide] = {};
-    std::vector<BatchGroup> init;
-    std::vector<BatchGroup> mid[4], last[4];
-    BatchGroup single_group;
-    void* d_single_items = nullptr;
-    void* d_single_first = nullptr;
+    std::vector<BatchUnit> units;
     uint64_t bytes = 0;
-    // dependency tracking: the event after which each plan's state is ready
-    std::vector<hipEvent_t> ready;
 };
 
 namespace {
@@ -1164,7 +1177,7 @@ int upload_group(tdse_batch* b, BatchGroup& g, const std::vector<SweepArgs>& ite
     g.nitems = static_cast<int>(items.size());
     std::vector<long long> first(items.size() + 1, 0);
     for (size_t i = 0; i < items.size(); ++i)
-        first[i + 1] = first[i] + tiles_of(g.precision, g.n, g.mode, items[i]);
+        first[i + 1] = first[i] + tiles_of(g.precision, g.n, g.mode, items[i], g.variant);
     g.tiles = first.back();
     int rc;
     if ((rc = batch_alloc(b, &g.d_items, items.size() * sizeof(SweepArgs)))) return rc;
@@ -1175,138 +1188,121 @@ int upload_group(tdse_batch* b, BatchGroup& g, const std::vector<SweepArgs>& ite
     TDSE_HIP(hipMemcpyAsync(g.d_first, first.data(), first.size() * sizeof(long long),
                             hipMemcpyHostToDevice, b->stream));
     TDSE_HIP(hipStreamSynchronize(b->stream));
-    g.fn = g.precision == TDSE_FP64 ? tdse::find_batch_launcher_f64(g.n, g.mode)
-                                    : tdse::find_batch_launcher_f32(g.n, g.mode);
+    g.fn = tdse::find_batch_launcher(g.precision == TDSE_FP64, g.n, g.mode, g.variant);
     if (!g.fn) return fail(TDSE_E_UNSUPPORTED, "no batched kernel for line length " + std::to_string(g.n));
     return TDSE_OK;
 }
 
-// Group (precision, n, mode) -> items, in first-seen order (deterministic).
-int build_groups(tdse_batch* b, std::vector<BatchGroup>& out,
-                 const std::vector<std::pair<const tdse_plan*, std::pair<int, std::pair<int, SweepArgs>>>>& rows) {
+// One plan's sweep in a sub-phase: line length, mode, kernel variant, descriptor.
+struct BatchRow {
+    const tdse_plan* plan;
+    int n, mode, variant;
+    SweepArgs args;
+};
+
+// Group (precision, n, mode, variant) -> items, in first-seen order
+// (deterministic).  A variant the batched kernels lack falls back to 0.
+int build_groups(tdse_batch* b, std::vector<BatchGroup>& out, const std::vector<BatchRow>& rows) {
     std::vector<BatchGroup> groups;
     std::vector<std::vector<SweepArgs>> items;
-    for (const auto& row : rows) {
-        const int prec = row.first->precision, n = row.second.first, mode = row.second.second.first;
+    for (const BatchRow& row : rows) {
+        const int prec = row.plan->precision, n = row.n, mode = row.mode;
+        const int var = tdse::find_batch_launcher(prec == TDSE_FP64, n, mode, row.variant)
+                            ? row.variant
+                            : 0;
         size_t gi = 0;
         for (; gi < groups.size(); ++gi)
-            if (groups[gi].precision == prec && groups[gi].n == n && groups[gi].mode == mode) break;
+            if (groups[gi].precision == prec && groups[gi].n == n && groups[gi].mode == mode &&
+                groups[gi].variant == var)
+                break;
         if (gi == groups.size()) {
             BatchGroup g;
             g.precision = prec;
             g.n = n;
             g.mode = mode;
+            g.variant = var;
             groups.push_back(g);
             items.emplace_back();
         }
-        items[gi].push_back(row.second.second.second);
-        const auto it = std::find(b->plans.begin(), b->plans.end(), row.first);
+        items[gi].push_back(row.args);
+        const auto it = std::find(b->plans.begin(), b->plans.end(), row.plan);
         groups[gi].plans.push_back(static_cast<int>(it - b->plans.begin()));
     }
     for (size_t gi = 0; gi < groups.size(); ++gi) {
         int rc = upload_group(b, groups[gi], items[gi]);
         if (rc) return rc;
-        TDSE_HIP(hipEventCreateWithFlags(&groups[gi].done, hipEventDisableTiming));
         out.push_back(groups[gi]);
     }
     return TDSE_OK;
 }
 
-// Launch the groups of one sub-phase.  Each group waits only for the events
-// after which its own plans are ready (their previous sweeps), so groups of
-// different plans overlap freely across lanes: a dependency DAG, not a
-// barrier per sub-phase.
-int launch_phase(tdse_batch* b, const std::vector<BatchGroup>& gs, int nsteps = 1) {
-    const int max_blocks = kBatchMaxBlocks;
-    for (const BatchGroup& g : gs) {
-        hipStream_t st = g.lane == 0 ? b->stream : b->side[g.lane - 1];
-        hipEvent_t waited[8];
-        int nw = 0;
-        for (int pi : g.plans) {
-            hipEvent_t e = b->ready[pi];
-            bool seen = false;
-            for (int k = 0; k < nw; ++k) seen = seen || waited[k] == e;
-            if (seen) continue;
-            TDSE_HIP(hipStreamWaitEvent(st, e, 0));
-            if (nw < 8) waited[nw++] = e;
-        }
-        TDSE_HIP(g.fn(g.table(nsteps), g.tiles, max_blocks, st));
-        TDSE_HIP(hipEventRecord(g.done, st));
-        for (int pi : g.plans) b->ready[pi] = g.done;
-    }
+hipStream_t lane_stream(const tdse_batch* b, int lane) {
+    return lane == 0 ? b->stream : b->side[lane - 1];
+}
+
+// Launch groups one after another on `st` (stream order is the dependency:
+// a unit's sweeps of one sub-phase never touch the same plan twice, and the
+// next sub-phase follows them on the same stream).
+int launch_groups(const std::vector<BatchGroup>& gs, hipStream_t st, int nsteps = 1) {
+    for (const BatchGroup& g : gs) TDSE_HIP(g.fn(g.table(nsteps), g.tiles, kBatchMaxBlocks, st));
     return TDSE_OK;
 }
 
-// Groups with the same plan set form a dependency chain (the same plans sweep
-// after sweep); a chain stays on one lane so consecutive sweeps need no
-// cross-stream wait, and chains are spread over the lanes by their work
-// (tiles per round), largest first.
-void assign_lanes(tdse_batch* b) {
-    std::vector<BatchGroup*> all;
-    for (auto& g : b->init) all.push_back(&g);
-    for (int sub = 0; sub < 4; ++sub) {
-        for (auto& g : b->mid[sub]) all.push_back(&g);
-        for (auto& g : b->last[sub]) all.push_back(&g);
-    }
-    std::vector<std::vector<int>> keys;
-    std::vector<long long> work;
-    std::vector<int> chain_of(all.size());
-    for (size_t i = 0; i < all.size(); ++i) {
-        std::vector<int> k = all[i]->plans;
-        std::sort(k.begin(), k.end());
-        size_t c = 0;
-        for (; c < keys.size(); ++c)
-            if (keys[c] == k) break;
-        if (c == keys.size()) {
-            keys.push_back(k);
-            work.push_back(0);
-        }
-        chain_of[i] = static_cast<int>(c);
-        work[c] += all[i]->tiles;
-    }
-    std::vector<int> order(keys.size());
-    for (size_t c = 0; c < order.size(); ++c) order[c] = static_cast<int>(c);
-    std::sort(order.begin(), order.end(), [&](int x, int y) { return work[x] > work[y]; });
+// A call of `nsteps` steps: each lane runs its units one after another, each
+// unit's opening sweeps, interior rounds and closing round.  The host queues
+// the lanes' work in slices of a few rounds, lane by lane, so every lane has
+// work queued from the start of the call.
+int launch_units(const tdse_batch* b, int nsteps) {
+    constexpr int kSliceRounds = 8;
     const int lanes = tdse_batch::kSide + 1;
-    std::vector<long long> load(lanes, 0);
-    std::vector<int> lane_of(keys.size(), 0);
-    for (int c : order) {
-        int best = 0;
-        for (int l = 1; l < lanes; ++l)
-            if (load[l] < load[best]) best = l;
-        lane_of[c] = best;
-        load[best] += work[c];
-    }
-    for (size_t i = 0; i < all.size(); ++i) all[i]->lane = lane_of[chain_of[i]];
-}
-
-// Open a call: every lane waits for `start` recorded on the main stream, and
-// every plan is ready at `start`.  Close it: the main stream waits for all
-// lanes.
-int open_call(tdse_batch* b) {
-    TDSE_HIP(hipEventRecord(b->fork, b->stream));
-    for (int s = 0; s < tdse_batch::kSide; ++s) TDSE_HIP(hipStreamWaitEvent(b->side[s], b->fork, 0));
-    std::fill(b->ready.begin(), b->ready.end(), b->fork);
-    return TDSE_OK;
-}
-
-int close_call(tdse_batch* b) {
-    for (int s = 0; s < tdse_batch::kSide; ++s) {
-        TDSE_HIP(hipEventRecord(b->join[s], b->side[s]));
-        TDSE_HIP(hipStreamWaitEvent(b->stream, b->join[s], 0));
-    }
-    return TDSE_OK;
-}
-
-int launch_rounds(tdse_batch* b, int rounds, bool last_round) {
-    for (int r = 0; r < rounds; ++r) {
-        const bool last = last_round && r + 1 == rounds;
-        for (int sub = 0; sub < 4; ++sub) {
-            int rc = launch_phase(b, last ? b->last[sub] : b->mid[sub]);
-            if (rc) return rc;
+    std::vector<std::vector<const BatchUnit*>> per_lane(static_cast<size_t>(lanes));
+    for (const BatchUnit& u : b->units) per_lane[static_cast<size_t>(u.lane)].push_back(&u);
+    std::vector<size_t> unit(static_cast<size_t>(lanes), 0);
+    std::vector<int> round(static_cast<size_t>(lanes), -1);  // -1: opening sweeps not queued
+    for (bool queued = true; queued;) {
+        queued = false;
+        for (int l = 0; l < lanes; ++l) {
+            if (unit[l] >= per_lane[l].size()) continue;
+            queued = true;
+            const BatchUnit& u = *per_lane[l][unit[l]];
+            const hipStream_t st = lane_stream(b, l);
+            int rc;
+            if (round[l] < 0) {
+                // the one-block 1D groups take nsteps and run the whole call
+                if ((rc = launch_groups(u.init, st, nsteps))) return rc;
+                round[l] = 0;
+            }
+            const int end = std::min(nsteps, round[l] + kSliceRounds);
+            for (int r = round[l]; r < end; ++r) {
+                const bool last = r + 1 == nsteps;
+                for (int sub = 0; sub < 4; ++sub)
+                    if ((rc = launch_groups(last ? u.last[sub] : u.mid[sub], st))) return rc;
+            }
+            round[l] = end;
+            if (end == nsteps) {
+                ++unit[l];
+                round[l] = -1;
+            }
         }
     }
     return TDSE_OK;
+}
+
+// Working set of one step of a plan: state, kick phase, mask array.
+uint64_t step_working_set(const tdse_plan* p) {
+    uint64_t b = static_cast<uint64_t>(p->total) * (p->csize() + p->rsize());
+    if (p->mask && !(p->mask_x && mask_axes_apply(p))) b += static_cast<uint64_t>(p->total) * p->rsize();
+    return b;
+}
+
+// Same unit class: members that can share launches (precision, geometry kind,
+// line lengths).
+bool same_class(const tdse_plan* a, const tdse_plan* b) {
+    if (a->precision != b->precision || a->kind != b->kind || a->sq.size() != b->sq.size())
+        return false;
+    for (size_t i = 0; i < a->sq.size(); ++i)
+        if (a->sq[i].n != b->sq[i].n) return false;
+    return a->n1 == b->n1 && a->n2 == b->n2;
 }
 
 }  // namespace
@@ -2473,7 +2469,6 @@ int tdse_batch_create(tdse_ctx* ctx, tdse_plan* const* plans, int count, tdse_ba
     tdse_batch* b = new tdse_batch();
     b->ctx = ctx;
     b->plans.assign(plans, plans + count);
-    b->ready.assign(b->plans.size(), nullptr);
     auto bail = [&](int code) {
         tdse_batch_destroy(b);
         return code;
@@ -2491,38 +2486,54 @@ int tdse_batch_create(tdse_ctx* ctx, tdse_plan* const* plans, int count, tdse_ba
             return bail(fail(TDSE_E_DEVICE, "stream/event creation failed"));
         }
     }
-    using Row = std::pair<const tdse_plan*, std::pair<int, std::pair<int, SweepArgs>>>;
-    std::vector<Row> init_rows, mid_rows[4], last_rows[4];
-    for (tdse_plan* p : b->plans) {
-        if (p->kind == kKindSingle) {
-            b->singles.push_back(p);
-            continue;
+    // Units: members of one class (precision, geometry, line lengths) join a
+    // unit while its one-step working set stays within a quarter of the
+    // Infinity Cache (one lane's share), so a unit's state stays cached
+    // between its sweeps; a call runs each unit's steps back to back on one
+    // lane.  Units go to the lanes in the order of their first member.
+    for (int i = 0; i < count; ++i) {
+        tdse_plan* p = b->plans[i];
+        const uint64_t ws = step_working_set(p);
+        BatchUnit* u = nullptr;
+        for (BatchUnit& cand : b->units)
+            if (same_class(b->plans[cand.members.front()], p) &&
+                cand.bytes + ws <= static_cast<uint64_t>(TDSE_BATCH_UNIT_BYTES)) {
+                u = &cand;
+                break;
+            }
+        if (!u) {
+            b->units.emplace_back();
+            u = &b->units.back();
         }
-        int n, mode;
-        SweepArgs a;
-        if (plan_init(p, &n, &mode, &a)) init_rows.push_back({p, {n, {mode, a}}});
-        for (const RoundSweep& r : plan_round(p)) {
-            mid_rows[r.sub].push_back({p, {r.n, {r.mode_mid, r.args}}});
-            last_rows[r.sub].push_back({p, {r.n, {r.mode_last, r.args}}});
+        u->members.push_back(i);
+        u->bytes += ws;
+    }
+    for (size_t k = 0; k < b->units.size(); ++k) {
+        BatchUnit& u = b->units[k];
+        u.lane = static_cast<int>(k % (tdse_batch::kSide + 1));
+        std::vector<BatchRow> init_rows, mid_rows[4], last_rows[4];
+        for (int i : u.members) {
+            tdse_plan* p = b->plans[i];
+            if (p->kind == kKindSingle) {
+                // every step of a call in one block: nsteps is set per call
+                Geometry g = geometry(p, p->psi);
+                init_rows.push_back({p, p->sq[0].n, tdse::C_SINGLE, 0, g.single});
+                continue;
+            }
+            int n, mode, var;
+            SweepArgs a;
+            if (plan_init(p, &n, &mode, &a, &var)) init_rows.push_back({p, n, mode, var, a});
+            for (const RoundSweep& r : plan_round(p)) {
+                mid_rows[r.sub].push_back({p, r.n, r.mode_mid, r.var_mid, r.args});
+                last_rows[r.sub].push_back({p, r.n, r.mode_last, r.var_last, r.args});
+            }
+        }
+        if ((rc = build_groups(b, u.init, init_rows))) return bail(rc);
+        for (int sub = 0; sub < 4; ++sub) {
+            if ((rc = build_groups(b, u.mid[sub], mid_rows[sub]))) return bail(rc);
+            if ((rc = build_groups(b, u.last[sub], last_rows[sub]))) return bail(rc);
         }
     }
-    if ((rc = build_groups(b, b->init, init_rows))) return bail(rc);
-    for (int sub = 0; sub < 4; ++sub) {
-        if ((rc = build_groups(b, b->mid[sub], mid_rows[sub]))) return bail(rc);
-        if ((rc = build_groups(b, b->last[sub], last_rows[sub]))) return bail(rc);
-    }
-    if (!b->singles.empty()) {
-        // one group per (precision, n) of single-line plans; nsteps is set per call
-        std::vector<Row> rows;
-        for (tdse_plan* p : b->singles) {
-            Geometry g = geometry(p, p->psi);
-            rows.push_back({p, {p->sq[0].n, {tdse::C_SINGLE, g.single}}});
-        }
-        std::vector<BatchGroup> sg;
-        if ((rc = build_groups(b, sg, rows))) return bail(rc);
-        for (auto& g : sg) b->init.push_back(g);  // refreshed with nsteps before each call
-    }
-    assign_lanes(b);
     *out = b;
     return TDSE_OK;
 }
@@ -2535,14 +2546,15 @@ int tdse_batch_destroy(tdse_batch* b) {
         for (auto& g : gs) {
             if (g.d_items) (void)hipFree(g.d_items);
             if (g.d_first) (void)hipFree(g.d_first);
-            if (g.done) (void)hipEventDestroy(g.done);
         }
         gs.clear();
     };
-    free_groups(b->init);
-    for (int sub = 0; sub < 4; ++sub) {
-        free_groups(b->mid[sub]);
-        free_groups(b->last[sub]);
+    for (BatchUnit& u : b->units) {
+        free_groups(u.init);
+        for (int sub = 0; sub < 4; ++sub) {
+            free_groups(u.mid[sub]);
+            free_groups(u.last[sub]);
+        }
     }
     for (int s = 0; s < tdse_batch::kSide; ++s) {
         if (b->side[s]) (void)hipStreamDestroy(b->side[s]);
@@ -2567,15 +2579,15 @@ int tdse_batch_step(tdse_batch* b, int nsteps, int use_graph) {
     if (rc) return rc;
     // the plans' own queues must be drained before the batch touches their state
     for (tdse_plan* p : b->plans) TDSE_HIP(hipStreamSynchronize(p->stream));
-    if ((rc = open_call(b))) return rc;
-    // (the one-block 1D groups of the first phase take nsteps as an argument)
-    if ((rc = launch_phase(b, b->init, nsteps))) return rc;
-    if ((rc = close_call(b))) return rc;
-    const int mid_rounds = nsteps - 1;
-    if ((rc = open_call(b))) return rc;
-    if ((rc = launch_rounds(b, mid_rounds, false))) return rc;
-    if ((rc = launch_rounds(b, 1, true))) return rc;
-    if ((rc = close_call(b))) return rc;
+    // fork: every lane starts after the main stream's earlier work
+    TDSE_HIP(hipEventRecord(b->fork, b->stream));
+    for (int s = 0; s < tdse_batch::kSide; ++s) TDSE_HIP(hipStreamWaitEvent(b->side[s], b->fork, 0));
+    if ((rc = launch_units(b, nsteps))) return rc;
+    // join: the main stream waits for every lane
+    for (int s = 0; s < tdse_batch::kSide; ++s) {
+        TDSE_HIP(hipEventRecord(b->join[s], b->side[s]));
+        TDSE_HIP(hipStreamWaitEvent(b->stream, b->join[s], 0));
+    }
     // later work queued on a plan (download, observe, step) waits for the batch
     TDSE_HIP(hipEventRecord(b->done, b->stream));
     for (tdse_plan* p : b->plans) TDSE_HIP(hipStreamWaitEvent(p->stream, b->done, 0));
@@ -2611,7 +2623,8 @@ int tdse_batch_stream(tdse_batch* b, void** out) {
 int tdse_batch_launches_per_round(const tdse_batch* b, int* out) {
     if (!b || !out) return fail(TDSE_E_INVALID, "null argument");
     int n = 0;
-    for (int sub = 0; sub < 4; ++sub) n += static_cast<int>(b->mid[sub].size());
+    for (const BatchUnit& u : b->units)
+        for (int sub = 0; sub < 4; ++sub) n += static_cast<int>(u.mid[sub].size());
     *out = n;
     return TDSE_OK;
 }

@@ -69,3 +69,28 @@ def test_batch_rejects_bad_members(tdse):
     p = make(tdse, s)
     with pytest.raises(ValueError):
         tdse.Batch([p, p])
+
+
+def test_batch_runs_each_plans_variants(tdse):
+    """Members swept in the kernel variants their plans hold (set here, as
+    the autotuner would) give exactly the solo plans' states in the same
+    variants; members of one shape in different variants share no launch."""
+    from shoeridingerequationsolver_amd import workloads
+    shapes = [(64, 64, 64), (128, 64, 1), (96, 64, 64), (65536, 1, 1)]  # lengths with variants
+    solves = workloads.batch_instances(seed=17, count=8, steps=15, shapes=shapes)
+    solo, members = [], []
+    for i, s in enumerate(solves):
+        for group in (solo, members):
+            p = make(tdse, s)
+            names = list(p.variants)
+            p.set_variants({k: (i + j) % 4 for j, k in enumerate(names)})
+            group.append(p)
+    for p in solo:
+        p.step(15)
+    b = tdse.Batch(members)
+    print(f"{len(members)} plans with variants, {b.launches_per_round()} launches per round",
+          flush=True)
+    b.step(15)
+    for s, p, q in zip(solves, solo, members):
+        assert p.variants == q.variants
+        assert np.array_equal(p.download(), q.download()), f"{s.name} {s.n} {p.variants}"
