@@ -260,7 +260,9 @@ def batch_leg(t, workloads, steps, small=False):
     """48 uneven instances (seeded shapes 64^3..160^3, 512^2, 1024^2, 65536),
     complex64, inputs built in HBM; one batch vs the same plans stepped
     independently (one stream each).  small: 256 instances of 32^3 instead
-    (launch-bound when stepped one plan at a time)."""
+    (launch-bound when stepped one plan at a time).  host_enqueue_s is the
+    time the batch call takes to return (all launches queued), beside the
+    device time of the call."""
     if small:
         solves = workloads.batch_instances(seed=5, count=256, steps=steps, shapes=[(32, 32, 32)])
     else:
@@ -286,11 +288,13 @@ def batch_leg(t, workloads, steps, small=False):
     bt.sync()
     t0 = time.perf_counter()
     bt.step(steps)
+    enqueue = time.perf_counter() - t0
     bt.sync()
     batched = time.perf_counter() - t0
     out = {"instances": len(solves), "steps": steps, "cells": cells,
            "launches_per_round": bt.launches_per_round(),
            "batched_s": round(batched, 4), "independent_streams_s": round(indep, 4),
+           "host_enqueue_s": round(enqueue, 4),
            "instance_steps_per_s": round(len(solves) * steps / batched, 1),
            "gcell_steps_per_s": round(cells * steps / batched / 1e9, 2)}
     bt.close()
@@ -378,7 +382,9 @@ def big_leg(t, hip, workloads, steps):
     Infinity Cache, so the sweeps' bandwidth is HBM's.  Inputs built on the
     GPU from the recipe.  Per-sweep launch times (HIP events) give each
     sweep's TB/s; a device-to-device copy of the same 1.07 GB (one state into
-    another plan's, 5 times) gives the bandwidth a plain copy reaches."""
+    another plan's, 5 times, by the runtime's copy and by a streaming copy
+    kernel; the faster is the reference) gives the bandwidth a plain copy
+    reaches."""
     s = workloads.barrier(seed=3, n=512)
     p = t.Propagator(s.grid(), None, s.dt, precision="fp32")
     p.build(s.recipe())
@@ -397,22 +403,25 @@ def big_leg(t, hip, workloads, steps):
                      "frac": round(b / avg / 1e9 / HBM_PEAK_GBS, 4)}
     q = t.Propagator(s.grid(), None, s.dt, precision="fp32")
     st = q.stream()
-    q.copy_state_from(p)
-    q.sync()
-    e0, e1 = hip.event(), hip.event()
-    hip.record(e0, st)
-    for _ in range(5):
-        q.copy_state_from(p)
-    hip.record(e1, st)
-    copy_s = hip.elapsed_ms(e0, e1) / 1e3 / 5
-    copy_tbs = 2 * N * 8 / copy_s / 1e12
+    copies = {}
+    for kind in ("runtime", "kernel"):
+        q.copy_state_from(p, kernel=kind == "kernel")
+        q.sync()
+        e0, e1 = hip.event(), hip.event()
+        hip.record(e0, st)
+        for _ in range(5):
+            q.copy_state_from(p, kernel=kind == "kernel")
+        hip.record(e1, st)
+        copies[kind] = 2 * N * 8 / (hip.elapsed_ms(e0, e1) / 1e3 / 5) / 1e12
+    copy_tbs = max(copies.values())
     for k in sweeps:
         sweeps[k]["frac_of_copy"] = round(sweeps[k]["tbs"] / copy_tbs, 4)
     out = {"grid": list(s.n), "steps": steps, "steps_per_s": round(steps / gpu_s, 1),
            "ms_per_step": round(1e3 * gpu_s / steps, 4), "step_bytes": p.step_bytes(),
            "step_tbs": round(p.step_bytes() * steps / gpu_s / 1e12, 3),
            "step_frac": round(p.step_bytes() * steps / gpu_s / 1e9 / HBM_PEAK_GBS, 4),
-           "copy_bytes": 2 * N * 8, "copy_tbs": round(copy_tbs, 3), "sweeps": sweeps,
+           "copy_bytes": 2 * N * 8, "copy_tbs": round(copy_tbs, 3),
+           "copy_tbs_by_kind": {k: round(v, 3) for k, v in copies.items()}, "sweeps": sweeps,
            "variants": variants}
     q.close()
     p.close()

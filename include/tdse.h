@@ -119,8 +119,12 @@ int tdse_plan_set_mask(tdse_plan* plan, const double* mask);
 /* State transfer, interleaved complex128 on the host. */
 int tdse_state_upload(tdse_plan* plan, const double* psi);
 int tdse_state_download(tdse_plan* plan, double* psi);
-/* Copy the state of `src` into `dst` (same shape and precision), on the GPU. */
+/* Copy the state of `src` into `dst` (same shape and precision), on the GPU:
+ * by the runtime's device copy, or by a plain streaming copy kernel (16 bytes
+ * per lane), the bandwidth reference the sweeps are measured against.  Both
+ * queue on dst's stream after src's queued work. */
 int tdse_state_copy(tdse_plan* dst, const tdse_plan* src);
+int tdse_state_copy_kernel(tdse_plan* dst, const tdse_plan* src);
 /* Device address of the resident state (complex64 or complex128 per point). */
 int tdse_state_device_ptr(tdse_plan* plan, void** out);
 

@@ -247,10 +247,12 @@ class Propagator:
     def step(self, nsteps=1):
         capi.check(capi.lib().tdse_step(self._h, int(nsteps)))
 
-    def copy_state_from(self, other: "Propagator"):
-        """This plan's state = ``other``'s (same shape and precision), on the GPU."""
+    def copy_state_from(self, other: "Propagator", kernel: bool = False):
+        """This plan's state = ``other``'s (same shape and precision), on the GPU
+        (``kernel``: by the library's streaming copy kernel)."""
+        fn = capi.lib().tdse_state_copy_kernel if kernel else capi.lib().tdse_state_copy
         try:
-            capi.check(capi.lib().tdse_state_copy(self._h, other.handle))
+            capi.check(fn(self._h, other.handle))
         except TdseError as e:
             raise _map_error(e) from None
 

@@ -50,7 +50,7 @@ EXPORTS = (
     "tdse_relax_deflated", "tdse_plan_relax_stats", "tdse_build_soft_coulomb",
     "tdse_plan_set_drive", "tdse_plan_set_time", "tdse_plan_get_time", "tdse_plan_set_magnetic",
     "tdse_lattice_set_uniform_field", "tdse_lattice_set_solenoid", "tdse_observe_noalloc",
-    "tdse_build_phase_tables",
+    "tdse_build_phase_tables", "tdse_state_copy_kernel",
 )
 
 TDSE_POT = {"none": 0, "harmonic": 1, "regularized_coulomb": 2, "coulomb_floor": 3, "barrier": 4,
@@ -127,6 +127,7 @@ def lib() -> C.CDLL:
         L.tdse_state_upload.argtypes = [C.c_void_p, _dp]
         L.tdse_state_download.argtypes = [C.c_void_p, _dp]
         L.tdse_state_copy.argtypes = [C.c_void_p, C.c_void_p]
+        L.tdse_state_copy_kernel.argtypes = [C.c_void_p, C.c_void_p]
         L.tdse_state_device_ptr.argtypes = [C.c_void_p, C.POINTER(C.c_void_p)]
         L.tdse_step.argtypes = [C.c_void_p, C.c_int]
         L.tdse_plan_sync.argtypes = [C.c_void_p, C.c_int]

@@ -97,6 +97,14 @@ constexpr int kLdsCap = 80 * 1024;
 #ifndef TDSE_TILE_LDS
 #define TDSE_TILE_LDS 40960
 #endif
+// Narrow strided tiles (rows under 128 bytes: lines of 512 and more) may take
+// up to this much LDS, tables included, to reach 128-byte rows (two blocks
+// per CU).  Measured (variant 0, one box): 512^3 FP32 2287 -> 2086 us per
+// step (z sweep 745 -> 609 us), 512^3 FP64 4537 -> 4013; 256^3 unchanged.
+// 0 = off.
+#ifndef TDSE_NARROW_LDS
+#define TDSE_NARROW_LDS 81920
+#endif
 
 template <typename T, int N, int V = 0>
 struct Shape {
@@ -127,7 +135,18 @@ struct Shape {
     // strided sweeps
     static constexpr int TX_WANT = cmax(1, cmin(64, WIDE * 256 / TT));
     static constexpr int TX_LDS = EXCHANGE ? cmax(1, DATA_CAP / LSTR) : TX_WANT;
-    static constexpr int TX = pow2_floor(cmin(cmin(TX_WANT, TX_LDS), OCC_LINES));
+    static constexpr int TX_BASE = pow2_floor(cmin(cmin(TX_WANT, TX_LDS), OCC_LINES));
+    // Narrow tiles (rows under 128 bytes: long lines) read rows of a few
+    // dozen bytes far apart; they may take up to TDSE_NARROW_LDS of LDS
+    // (tables included) to reach 128-byte rows.
+    static constexpr int TX_ROW = 128 / CSZ;
+    static constexpr int NARROW_TX =
+        TDSE_NARROW_LDS > 0 && EXCHANGE
+            ? cmax(1, (TDSE_NARROW_LDS - TWN * 16 - N * 16) / (LSTR * CSZ))
+            : 0;
+    static constexpr int TX = TX_BASE * CSZ >= 128 || NARROW_TX <= TX_BASE
+                                  ? TX_BASE
+                                  : pow2_floor(cmin(cmin(cmax(TX_WANT, TX_ROW), 1024 / TT), NARROW_TX));
     static constexpr int BLOCK_S = TX * TT;
     static constexpr int LDS_S = EXCHANGE ? TX * LSTR : 1;
 };

@@ -285,6 +285,23 @@ mask_mul(void* psi_v, const double* mask, long long total) {
         psi[f] = cscale(psi[f], mask[f]);
 }
 
+// A plain streaming copy, 16 bytes per lane per load and store, grid-stride:
+// the bandwidth reference the sweeps are compared with (a kernel like theirs,
+// reading and writing each byte once).
+__global__ void __launch_bounds__(kBlock) copy16(uint4* __restrict__ dst,
+                                                 const uint4* __restrict__ src, long long n16) {
+    const long long stride = static_cast<long long>(gridDim.x) * kBlock;
+    long long i = blockIdx.x * static_cast<long long>(kBlock) + threadIdx.x;
+    for (; i + 3 * stride < n16; i += 4 * stride) {
+        const uint4 a = src[i], b = src[i + stride], c = src[i + 2 * stride], d = src[i + 3 * stride];
+        dst[i] = a;
+        dst[i + stride] = b;
+        dst[i + 2 * stride] = c;
+        dst[i + 3 * stride] = d;
+    }
+    for (; i < n16; i += stride) dst[i] = src[i];
+}
+
 int elementwise_blocks(long long total) {
     long long b = (total + kBlock * 4 - 1) / (kBlock * 4);
     if (b < 1) b = 1;
@@ -309,6 +326,22 @@ hipError_t launch_conj_scale(void* psi, long long total, double s, hipStream_t s
 hipError_t launch_mask_mul(void* psi, const double* mask, long long total, hipStream_t stream) {
     hipLaunchKernelGGL(mask_mul, dim3(elementwise_blocks(total)), dim3(kBlock), 0, stream, psi,
                        mask, total);
+    return hipGetLastError();
+}
+
+hipError_t launch_copy(void* dst, const void* src, size_t bytes, hipStream_t stream) {
+    const long long n16 = static_cast<long long>(bytes / 16);
+    if (n16 > 0) {
+        long long blocks = (n16 + kBlock * 8 - 1) / (kBlock * 8);
+        if (blocks > 8192) blocks = 8192;
+        hipLaunchKernelGGL(copy16, dim3(static_cast<unsigned>(blocks)), dim3(kBlock), 0, stream,
+                           static_cast<uint4*>(dst), static_cast<const uint4*>(src), n16);
+    }
+    const size_t tail = bytes - static_cast<size_t>(n16) * 16;
+    if (tail)
+        return hipMemcpyAsync(static_cast<char*>(dst) + n16 * 16,
+                              static_cast<const char*>(src) + n16 * 16, tail,
+                              hipMemcpyDeviceToDevice, stream);
     return hipGetLastError();
 }
 

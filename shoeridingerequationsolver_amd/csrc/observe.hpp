@@ -54,6 +54,8 @@ hipError_t launch_normalize_dev(bool fp64, void* psi, long long total, const dou
 // partials into the next scale, the norm and the peak (a LaunchFn)
 struct SweepArgs;
 hipError_t launch_relax_finish(const SweepArgs& a, int max_blocks, hipStream_t stream);
+// Streaming copy kernel (16-byte lanes), the bandwidth reference
+hipError_t launch_copy(void* dst, const void* src, size_t bytes, hipStream_t stream);
 // FP64 elementwise passes of the unfused baseline
 hipError_t launch_table_mul(void* psi, const void* table, long long total, hipStream_t stream);
 hipError_t launch_conj_scale(void* psi, long long total, double s, hipStream_t stream);

@@ -1689,6 +1689,17 @@ int tdse_state_copy(tdse_plan* dst, const tdse_plan* src) {
     return TDSE_OK;
 }
 
+int tdse_state_copy_kernel(tdse_plan* dst, const tdse_plan* src) {
+    if (!dst || !src) return fail(TDSE_E_INVALID, "null plan");
+    if (dst->total != src->total || dst->precision != src->precision)
+        return fail(TDSE_E_INVALID, "shape or precision mismatch");
+    int rc = set_device(dst->ctx);
+    if (rc) return rc;
+    TDSE_HIP(hipStreamSynchronize(src->stream));
+    TDSE_HIP(tdse::launch_copy(dst->psi, src->psi, dst->total * dst->csize(), dst->stream));
+    return TDSE_OK;
+}
+
 int tdse_state_device_ptr(tdse_plan* p, void** out) {
     int rc = check_plan(p);
     if (rc) return rc;
