@@ -279,6 +279,8 @@ constexpr T kMaskOne = sizeof(T) == 8 ? T(1) : T(0);  // stored form of m = 1
 // phi: exp(i phi) = ((1 - t^2) + 2 i t) / (1 + t^2), of modulus 1 up to a few
 // FP64 roundings and with no sine or cosine (the reciprocal is the hardware
 // estimate refined by two Newton steps); f = 2 squares it.
+// (One Newton step instead of two measured 1% faster but raised the 2000-step
+// FP32 norm drift from 4e-8 to 2.3e-7: the one-step reciprocal undershoots.)
 __device__ __forceinline__ cx<double> tan_rotation(double t, double f) {
     const double t2 = t * t;
     const double den = 1.0 + t2;
