@@ -590,7 +590,7 @@ def main():
     ap.add_argument("--big-steps", type=int, default=100)
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=200)
-    ap.add_argument("--serve-rates", type=str, default="5,20,50,200")
+    ap.add_argument("--serve-rates", type=str, default="5,10,20,50,100,200")
     ap.add_argument("--serve-budget-gib", type=float, default=6.0)
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 512 MiB device copy (torch) 5 times: the "

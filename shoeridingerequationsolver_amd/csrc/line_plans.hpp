@@ -101,7 +101,8 @@ constexpr int kLdsCap = 80 * 1024;
 // up to this much LDS, tables included, to reach 128-byte rows (two blocks
 // per CU).  Measured (variant 0, one box): 512^3 FP32 2287 -> 2086 us per
 // step (z sweep 745 -> 609 us), 512^3 FP64 4537 -> 4013; 256^3 unchanged.
-// 0 = off.
+// Rows of 256 bytes (160 KB of LDS, one block per CU) measured slower:
+// 512^3 FP32 z sweep 610 -> 797 us, FP64 step 4009 -> 3979 us.  0 = off.
 #ifndef TDSE_NARROW_LDS
 #define TDSE_NARROW_LDS 81920
 #endif
