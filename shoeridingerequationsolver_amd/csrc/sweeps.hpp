@@ -955,20 +955,33 @@ __device__ __forceinline__ void strided_tile(const SweepArgs& a, long long tile,
     }
 }
 
-// XCD-aware tile order of strided sweeps whose tiles are narrower than a
-// 128-byte line (TX * sizeof(cx<T>) < 128: long lines in FP32, FP64 lines of
-// 512 and more).  The G = 128 / (TX * sizeof(cx<T>)) x-tiles that share the
-// same 128-byte lines of every row should meet in one L2: workgroups are
-// dispatched round-robin over the 8 XCDs (workgroup i on XCD i mod 8), so
-// launch index L = 8 G q + 8 j + x (x the XCD, j < G) sweeps tile
-// 8 G q + G x + j: the G tiles of a line group go to consecutive workgroups
-// of one XCD.  A bijection on every whole chunk of 8 G; the last partial
-// chunk keeps the identity order.
+// XCD-aware tile order of strided sweeps.  Workgroups are dispatched
+// round-robin over the 8 XCDs (workgroup i on XCD i mod 8), each with its own
+// L2.  Consecutive x-tiles read neighbouring pieces of the same rows; the
+// G = ROW / (TX * sizeof(cx<T>)) x-tiles that together cover ROW bytes of
+// every row go to consecutive workgroups of one XCD: launch index
+// L = 8 G q + 8 j + x (x the XCD, j < G) sweeps tile 8 G q + G x + j.  A
+// bijection on every whole chunk of 8 G; the last partial chunk keeps the
+// identity order.
+//
+// ROW is 128 bytes (one cache line: tiles narrower than a line, 1024-point
+// lines, share it in one L2; against the identity order 1024^2 FP32 34.1 ->
+// 28.2 us per step, 1024 x 1024 x 16 FP32 381 -> 232 us, FP64 745 -> 505 us)
+// except where a wider run measured faster on one
+// box (whole steps, two alternated runs): 512-point lines 1024 bytes (512^3
+// FP32 2095 -> 2053 us, FP64 4009 -> 3813 us), 128-point FP32 lines and
+// 256-point FP64 lines 512 bytes (128^3 FP32 47.4 -> 44.4 us, 256^3 FP64
+// 369 -> 364 us).  Wider runs for other lengths were neutral or slower (128^3
+// FP64 +10% at 512 bytes, 192^3 +1-3%).
 constexpr int kXcds = 8;
-template <typename T, int TX>
+template <typename T, int N, int TX>
 constexpr int line_group() {
+    constexpr bool f64 = sizeof(T) == 8;
+    constexpr int row = N == 512                               ? 1024
+                        : (N == 128 && !f64) || (N == 256 && f64) ? 512
+                                                                 : 128;
     constexpr int w = TX * static_cast<int>(sizeof(cx<T>));
-    return w >= 128 ? 1 : 128 / w;
+    return w >= row ? 1 : row / w;
 }
 template <int G>
 __device__ __forceinline__ long long xcd_tile(long long l, long long ntiles) {
@@ -1041,7 +1054,7 @@ strided_sweep(SweepArgs a) {
     }
     SweepAcc ra;
     const long long ntiles = static_cast<long long>((a.nx + S::TX - 1) / S::TX) * a.nw;
-    constexpr int G = line_group<T, S::TX>();
+    constexpr int G = line_group<T, N, S::TX>();
     for (long long l = blockIdx.x; l < ntiles; l += gridDim.x)
         strided_tile<T, N, MODE, V>(a, xcd_tile<G>(l, ntiles), lds, lds_tw, lds_kp, opaque_tid(),
                                     ra);
@@ -1120,7 +1133,7 @@ strided_sweep_batch(BatchTable b) {
     const long long chunk = (total + gridDim.x - 1) / gridDim.x;
     // one tile per workgroup: the XCD-aware order (a block of several
     // consecutive tiles already keeps a line group in one L2)
-    const long long g0 = chunk == 1 ? xcd_tile<line_group<T, S::TX>()>(blockIdx.x, total)
+    const long long g0 = chunk == 1 ? xcd_tile<line_group<T, N, S::TX>()>(blockIdx.x, total)
                                     : blockIdx.x * chunk;
     const long long g1 = g0 + chunk < total ? g0 + chunk : total;
     int staged = -1;
