@@ -139,6 +139,20 @@ __device__ __forceinline__ cx<T> mul_cw(cx<T> a) {
     return {mulk(a.x, c) - mulk(a.y, s), mulk(a.x, s) + mulk(a.y, c)};
 }
 
+// a * W_R^m at an odd multiple of pi/4 (|cos| = |sin| = 1/sqrt 2) in FP32:
+// ((+-x -+ y) + i (+-x +- y)) / sqrt 2, the sums rounded once and the
+// constant applied compensated (mulk): 6 operations instead of 10, one
+// rounding fewer per component.
+template <typename T, int DIR, int M, int R>
+__device__ __forceinline__ cx<T> mul_octant(cx<T> a) {
+    constexpr SinCos sc = sincos_2pi(M, R);
+    constexpr bool cpos = sc.c > 0, spos = DIR * sc.s > 0;
+    constexpr double h = 0.70710678118654752440;
+    const T u = cpos ? (spos ? a.x - a.y : a.x + a.y) : (spos ? -a.x - a.y : a.y - a.x);
+    const T w = cpos ? (spos ? a.x + a.y : a.y - a.x) : (spos ? a.x - a.y : -a.x - a.y);
+    return {mulk(u, h), mulk(w, h)};
+}
+
 template <typename T, int DIR, int M, int R>
 __device__ __forceinline__ cx<T> twiddle_const(cx<T> a) {
     constexpr int m = ((M % R) + R) % R;
@@ -150,6 +164,8 @@ __device__ __forceinline__ cx<T> twiddle_const(cx<T> a) {
         return cx<T>{-a.x, -a.y};
     } else if constexpr (4 * m == 3 * R) {
         return mul_i<-DIR>(a);
+    } else if constexpr (sizeof(T) == 4 && (8 * m) % R == 0) {
+        return mul_octant<T, DIR, m, R>(a);
     } else {
         return mul_cw<T, DIR, m, R>(a);
     }
@@ -337,18 +353,36 @@ __device__ __forceinline__ cx<T> mul_d(cx<T> a, cx<double> k) {
 // ---- one Stockham pass in registers -------------------------------------------
 
 // Twiddle then DFT for every butterfly this thread owns.  `tw` holds
-// W_N^m = exp(-2 pi i m / N), m < N, in FP64 (forward sign).  The butterfly's
-// twiddles W^{r k} (r = 1..R-1) are generated in FP64 from the one table
-// entry W^k by repeated multiplication (relative error below 2e-15 for
-// R <= 16) and applied in FP64, so no float-rounded twiddle ever scales the
-// state.
-template <typename T, int N, int E, int R, int NS, int DIR>
+// W_N^m = exp(-2 pi i m / N) in FP64 (forward sign).  The butterfly's
+// twiddles W^{r k} (r = 1..R-1) are applied in FP64, so no float-rounded
+// twiddle ever scales the state.  With TABLE, `tw` holds all N entries and
+// each twiddle is read from it (r k step < N); otherwise `tw` holds the
+// first N / (least radix) and the twiddles are generated in FP64 from the
+// one entry W^k by repeated multiplication (relative error below 2e-15 for
+// R <= 16).
+template <typename T, int N, int E, int R, int NS, int DIR, bool TABLE = false>
 __device__ __forceinline__ void pass_compute(cx<T> (&v)[E], int t,
                                              const cx<double>* __restrict__ tw) {
     constexpr int TT = N / E;
 #pragma unroll
     for (int b = 0; b < E / R; ++b) {
-        if constexpr (NS > 1) {
+        if constexpr (NS > 1 && TABLE) {
+            // W^{r k step}: the offset is stepped through an empty asm so the
+            // compiler forms each read's address in turn instead of holding
+            // R - 1 offsets live
+            const int j = t + b * TT;
+            const int ks = (j % NS) * (N / (NS * R));
+            int m = 0;
+#pragma unroll
+            for (int r = 1; r < R; ++r) {
+                m += ks;
+                asm volatile("" : "+v"(m));
+                cx<double> w = tw[m];
+                if constexpr (DIR > 0) w.y = -w.y;
+                v[b * R + r] = mul_d(v[b * R + r], w);
+                __builtin_amdgcn_sched_barrier(0);
+            }
+        } else if constexpr (NS > 1) {
             const int j = t + b * TT;
             const int k = j % NS;
             constexpr int step = N / (NS * R);
@@ -408,7 +442,8 @@ __device__ __forceinline__ void for_each_point(int t, Fn&& fn) {
 }
 
 // Full transform: passes over the radix list with LDS exchanges in between.
-// `x` is the exchange buffer: x.sync(), x.put(i, v), x.get(i).
+// `x` is the exchange buffer: x.sync(), x.put(i, v), x.get(i); X::kTwTable
+// says whether `tw` is the full twiddle table (see pass_compute).
 template <typename T, int N, int E, int DIR, int NS, class L>
 struct Transform;
 
@@ -416,7 +451,7 @@ template <typename T, int N, int E, int DIR, int NS, int R>
 struct Transform<T, N, E, DIR, NS, Radices<R>> {
     template <class X>
     __device__ __forceinline__ static void run(cx<T> (&v)[E], int t, const cx<double>* tw, X& x) {
-        pass_compute<T, N, E, R, NS, DIR>(v, t, tw);
+        pass_compute<T, N, E, R, NS, DIR, X::kTwTable>(v, t, tw);
     }
 };
 
@@ -424,7 +459,7 @@ template <typename T, int N, int E, int DIR, int NS, int R, int R2, int... Rest>
 struct Transform<T, N, E, DIR, NS, Radices<R, R2, Rest...>> {
     template <class X>
     __device__ __forceinline__ static void run(cx<T> (&v)[E], int t, const cx<double>* tw, X& x) {
-        pass_compute<T, N, E, R, NS, DIR>(v, t, tw);
+        pass_compute<T, N, E, R, NS, DIR, X::kTwTable>(v, t, tw);
         x.sync();
         pass_scatter<T, N, E, R, NS>(v, t, [&](int i, cx<T> z) { x.put(i, z); });
         x.sync();

@@ -119,6 +119,11 @@ struct Shape {
     // W^{k N / (Ns R)}, k < Ns, all below N / R; the inverse runs the radix list
     // reversed, so every radix has a pass after the first: N / (least radix).
     static constexpr int TWN = EXCHANGE ? N / MinRadix<typename P::Fwd>::value : 1;
+    // Contiguous sweeps of lines up to 256 points may stage the whole table
+    // (4 KB at most: the tiles still fit four blocks per CU) and read every
+    // twiddle from it, four FP64 operations fewer per twiddle than the
+    // recurrence (which sweeps do: see contig_tw_table).
+    static constexpr bool TW_TABLE_C = EXCHANGE && N <= 256;
     static constexpr int CSZ = static_cast<int>(sizeof(cx<T>));
     static constexpr int LSTR = lds_len(N);
     // FP64 tables staged in LDS: twiddles and (strided kinetic sweep) K_p

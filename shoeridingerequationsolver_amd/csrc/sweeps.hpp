@@ -277,16 +277,17 @@ constexpr T kMaskOne = sizeof(T) == 8 ? T(1) : T(0);  // stored form of m = 1
 
 // exp(i f phi) from t = tan(phi / 2), which FP32 plans store in place of
 // phi: exp(i phi) = ((1 - t^2) + 2 i t) / (1 + t^2), of modulus 1 up to a few
-// FP64 roundings and with no sine or cosine (the reciprocal is the hardware
-// estimate refined by two Newton steps); f = 2 squares it.
-// (One Newton step instead of two measured 1% faster but raised the 2000-step
-// FP32 norm drift from 4e-8 to 2.3e-7: the one-step reciprocal undershoots.)
+// FP64 roundings and with no sine or cosine; f = 2 squares it.  The
+// reciprocal is the hardware estimate r0 (about 17 bits) refined by one
+// third-order step, r0 (1 + e + e^2) with e = 1 - den r0 (relative error
+// about e^3, below 2^-50).  (One Newton step instead, error e^2 and always
+// short, raised the 2000-step FP32 norm drift from 4e-8 to 2.3e-7.)
 __device__ __forceinline__ cx<double> tan_rotation(double t, double f) {
     const double t2 = t * t;
     const double den = 1.0 + t2;
     double r = __builtin_amdgcn_rcp(den);
-    r = fma(r, fma(-den, r, 1.0), r);
-    r = fma(r, fma(-den, r, 1.0), r);
+    const double e = fma(-den, r, 1.0);
+    r = fma(r, fma(e, e, e), r);
     const double c = (1.0 - t2) * r, s = (t + t) * r;
     if (f == 2.0) return {fma(c, c, -s * s), (c + c) * s};
     return {c, s};
@@ -329,16 +330,31 @@ __device__ __forceinline__ cx<double> step_twiddle(long long a, long long b, int
 
 // ---- LDS exchange buffers -------------------------------------------------------
 
-template <typename T>
+// Contiguous tiles: every line lies inside one wave (its N / E threads are
+// consecutive and N / E divides 64), so a line's exchange only has to order
+// the wave's own LDS accesses, which execute in order: the barrier fences the
+// compiler, not the block.  Waves of a block then run their tiles
+// independently and drift apart, so one wave's loads and stores overlap
+// another's arithmetic (x sweep at 256^3 FP32: 59.1 -> 58.0 us against block
+// barriers, same box).
+template <typename T, bool TW_TABLE = false>
 struct ContigX {
+    static constexpr bool kTwTable = TW_TABLE;
     cx<T>* base;  // this thread's line
-    __device__ __forceinline__ void sync() const { __syncthreads(); }
+    __device__ __forceinline__ void sync() const {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
     __device__ __forceinline__ void put(int i, cx<T> z) const { base[lds_pad(i)] = z; }
     __device__ __forceinline__ cx<T> get(int i) const { return base[lds_pad(i)]; }
 };
 
+// Strided tiles: a line's threads span the block's waves (the unit-stride
+// index is fastest across lanes, for whole cache lines per access).
 template <typename T, int TX>
 struct StridedX {
+    static constexpr bool kTwTable = false;
     cx<T>* base;  // lds + xi
     __device__ __forceinline__ void sync() const { __syncthreads(); }
     __device__ __forceinline__ void put(int i, cx<T> z) const { base[lds_pad(i) * TX] = z; }
@@ -400,12 +416,25 @@ struct Line {
     T km[E];  // mask, stored form (see mask_value)
     __device__ __forceinline__ void load_kick(bool with_mask) {
         if (!valid) return;
-        const bool m = with_mask && a.mask;
-        for_each_point<N, E, R1>(t, [&](int s, int, int j, int ro) {
-            const uint32_t vo = (base + j * es) * RS, so = ro * es * RS;
-            kp[s] = F.ld(vo, so);
-            km[s] = m ? M.ld(a.mask_sep ? j * es * RS : vo, so) : kMaskOne<T>;
-        });
+        // the branches are uniform: each case issues its loads back to back
+        if (with_mask && a.mask && a.mask_sep) {
+            for_each_point<N, E, R1>(t, [&](int s, int, int j, int ro) {
+                const uint32_t vo = (base + j * es) * RS, so = ro * es * RS;
+                kp[s] = F.ld(vo, so);
+                km[s] = M.ld(j * es * RS, so);
+            });
+        } else if (with_mask && a.mask) {
+            for_each_point<N, E, R1>(t, [&](int s, int, int j, int ro) {
+                const uint32_t vo = (base + j * es) * RS, so = ro * es * RS;
+                kp[s] = F.ld(vo, so);
+                km[s] = M.ld(vo, so);
+            });
+        } else {
+            for_each_point<N, E, R1>(t, [&](int s, int, int j, int ro) {
+                kp[s] = F.ld((base + j * es) * RS, ro * es * RS);
+                km[s] = kMaskOne<T>;
+            });
+        }
     }
     // v *= m exp(i (f phi + g u)), the exponential in FP64; u = axis . r of
     // the point (du0 + du1 * position along the line), g the drive angle of
@@ -414,6 +443,14 @@ struct Line {
     double du0 = 0.0, du1 = 0.0;
     // separable mask: the ramps of the line's other two axes (1 otherwise)
     double my = 1.0, mz = 1.0;
+    // mask value of a point from its stored line-ramp entry: (m_x m_y) m_z,
+    // the reference's product order, for FP64 states; m_x (m_y m_z) with the
+    // line's factor formed once for FP32 states (an FP64 rounding apart, far
+    // below the state's precision)
+    __device__ __forceinline__ double mask_of(T stored) const {
+        if constexpr (sizeof(T) == 8) return (mask_value(stored) * my) * mz;
+        else return mask_value(stored) * (my * mz);
+    }
     template <bool DRIVE>
     __device__ __forceinline__ void kick_points(double f, bool use_mask, double g) {
         if constexpr (DRIVE && sizeof(T) == 4) {
@@ -443,7 +480,7 @@ struct Line {
         } else {
             for_each_point<N, E, R1>(t, [&](int s, int, int j, int ro) {
                 const double extra = DRIVE ? g * (du0 + du1 * (j + ro)) : 0.0;
-                const double m = use_mask ? (mask_value(km[s]) * my) * mz : 1.0;
+                const double m = use_mask ? mask_of(km[s]) : 1.0;
                 v[s] = mul_d(v[s], kick_factor<T>(kp[s], f, m, extra, DRIVE));
             });
         }
@@ -457,7 +494,7 @@ struct Line {
             kick_points<DRIVE>(f, use_mask, g);
         } else if (use_mask && a.mask && a.mask_sep) {
             for (int s = 0; s < E; ++s) {
-                const double m = (mask_value(km[s]) * my) * mz;
+                const double m = mask_of(km[s]);
                 v[s] = {static_cast<T>(m * static_cast<double>(v[s].x)),
                         static_cast<T>(m * static_cast<double>(v[s].y))};
             }
@@ -665,6 +702,21 @@ __device__ __forceinline__ void obs_block_done(const SweepArgs& a, SweepAcc acc)
     }
 }
 
+// Contiguous sweeps that read every twiddle from the full staged table: the
+// FP32 two-transform kick sweep between steps (with the wave-local exchange:
+// x sweep at 256^3 FP32 61.2 -> 57.1 us, same box).  Elsewhere the recurrence
+// stays: in other sweeps the table's reads raised the register count past an
+// occupancy step, and the FP64 x sweep measured slower with it (115.7 ->
+// 120.2 us).
+template <typename T, int N, int MODE, int V>
+constexpr bool contig_tw_table() {
+    return sizeof(T) == 4 && Shape<T, N, V>::TW_TABLE_C && base_mode(MODE) == C_KICK_MID;
+}
+template <typename T, int N, int MODE, int V>
+constexpr int contig_twn() {
+    return contig_tw_table<T, N, MODE, V>() ? N : Shape<T, N, V>::TWN;
+}
+
 // ---- per-tile bodies ---------------------------------------------------------------
 
 // One tile of a contiguous sweep: LPB whole lines.  `lds` holds LPB padded
@@ -675,10 +727,12 @@ __device__ __forceinline__ void contig_tile(const SweepArgs& a, long long tile, 
     using S = Shape<T, N, V>;
     constexpr int BM = base_mode(MODE);
     constexpr bool DRV = mode_driven(MODE);
-    using LT = Line<T, N, ContigX<T>, V>;
+    using XC = ContigX<T, contig_tw_table<T, N, MODE, V>()>;
+    using LT = Line<T, N, XC, V>;
     constexpr int TT = S::TT, LPB = S::LPB, E = S::E;
+    static_assert(64 % TT == 0, "a contiguous line must lie inside one wave");
     const int l = tid / TT, t = tid % TT;
-    ContigX<T> x{lds + l * S::LSTR};
+    XC x{lds + l * S::LSTR};
     const Buf<cx<T>> P(a.psi, a.psi_bytes);
     const Buf<T> F(a.phi, a.real_bytes);
     const Buf<T> M(a.mask, a.mask ? a.mask_bytes : 0u);
@@ -1010,14 +1064,15 @@ contig_sweep(SweepArgs a) {
     using S = Shape<T, N, V>;
     static_assert(!mode_finishes(MODE) || S::BLOCK_C >= 64, "a finishing sweep holds a full wave");
     __shared__ cx<T> lds[S::LDS_C];
-    __shared__ cx<double> lds_tw[S::TWN];
+    constexpr int TWN = contig_twn<T, N, MODE, V>();
+    __shared__ cx<double> lds_tw[TWN];
     const int tid = threadIdx.x;
     if constexpr (mode_finishes(MODE)) {
         if (a.fin_slots && blockIdx.x == 0)
             relax_finish_block(a.fin_partials, a.fin_slots, a.fin_parity, a.fin_cell);
     }
     if constexpr (S::EXCHANGE) {
-        stage_table<S::TWN>(lds_tw, a.tw, tid, S::BLOCK_C);
+        stage_table<TWN>(lds_tw, a.tw, tid, S::BLOCK_C);
         __syncthreads();
     }
     if constexpr (MODE == C_RELAX_FIRST) {
@@ -1089,10 +1144,11 @@ __global__ void __launch_bounds__((Shape<T, N, V>::BLOCK_C), TDSE_MIN_WAVES)
 contig_sweep_batch(BatchTable b) {
     using S = Shape<T, N, V>;
     __shared__ cx<T> lds[S::LDS_C];
-    __shared__ cx<double> lds_tw[S::TWN];
+    constexpr int TWN = contig_twn<T, N, MODE, V>();
+    __shared__ cx<double> lds_tw[TWN];
     const int tid = threadIdx.x;
     if constexpr (S::EXCHANGE) {
-        stage_table<S::TWN>(lds_tw, b.items[0].tw, tid, S::BLOCK_C);
+        stage_table<TWN>(lds_tw, b.items[0].tw, tid, S::BLOCK_C);
         __syncthreads();
     }
     // contiguous chunk of tiles per block: consecutive tiles mostly share a plan
