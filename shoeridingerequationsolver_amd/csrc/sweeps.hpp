@@ -438,8 +438,7 @@ struct Line {
     }
     // v *= m exp(i (f phi + g u)), the exponential in FP64; u = axis . r of
     // the point (du0 + du1 * position along the line), g the drive angle of
-    // this kick (0: none).  Waves with no angle at all skip the exponential
-    // and multiply by the real m alone, rounding each component once.
+    // this kick (0: none).
     double du0 = 0.0, du1 = 0.0;
     // separable mask: the ramps of the line's other two axes (1 otherwise)
     double my = 1.0, mz = 1.0;
@@ -488,23 +487,23 @@ struct Line {
     template <bool DRIVE>
     __device__ __forceinline__ void apply_kick(double f, bool use_mask, double g) {
         if (!valid) return;
-        bool nz = DRIVE;
-        for (int s = 0; s < E; ++s) nz |= kp[s] != T(0);
-        if (__any(nz)) {
+        if constexpr (sizeof(T) == 4) {
+            // FP32 states always take the exponential (its test for a wave
+            // of zero phases cost an instruction per point and, in 3D,
+            // never passes: every x line crosses the potential)
+            __builtin_amdgcn_sched_barrier(0);
             kick_points<DRIVE>(f, use_mask, g);
-        } else if (use_mask && a.mask && a.mask_sep) {
-            for (int s = 0; s < E; ++s) {
-                const double m = mask_of(km[s]);
-                v[s] = {static_cast<T>(m * static_cast<double>(v[s].x)),
-                        static_cast<T>(m * static_cast<double>(v[s].y))};
-            }
-        } else if (use_mask && a.mask) {
-            if constexpr (sizeof(T) == 4) {
-                // v (1 - c) = v - v c with one rounding per component
-                for (int s = 0; s < E; ++s)
-                    v[s] = {__builtin_fmaf(-v[s].x, km[s], v[s].x), __builtin_fmaf(-v[s].y, km[s], v[s].y)};
-            } else {
-                for (int s = 0; s < E; ++s) v[s] = cscale(v[s], km[s]);
+        } else {
+            // FP64 states: waves with no angle at all multiply by the real
+            // m alone.  (Taking the exponential unconditionally here let the
+            // compiler overlap it with the transform before it, at a wave
+            // of occupancy in the x sweeps.)
+            bool nz = DRIVE;
+            for (int s = 0; s < E; ++s) nz |= kp[s] != T(0);
+            if (__any(nz)) {
+                kick_points<DRIVE>(f, use_mask, g);
+            } else if (use_mask && a.mask) {
+                for (int s = 0; s < E; ++s) v[s] = cscale(v[s], mask_of(km[s]));
             }
         }
     }
