@@ -25,7 +25,17 @@ hipError_t launch_sweep(const SweepArgs& a, int max_blocks, hipStream_t stream) 
     if (mode_measures(MODE) && max_blocks > relax_max_blocks<T>()) max_blocks = relax_max_blocks<T>();
     if (mode_observes(MODE) && max_blocks > kObsMaxBlocks) max_blocks = kObsMaxBlocks;
     const int grid = static_cast<int>(tiles < max_blocks ? tiles : max_blocks);
-    if constexpr (mode_contiguous(MODE)) {
+    const LaunchTiming tm = g_launch_timing;
+    if (tm.start) {
+        g_launch_timing = LaunchTiming{};
+        if constexpr (mode_contiguous(MODE)) {
+            hipExtLaunchKernelGGL((contig_sweep<T, N, MODE, V>), dim3(grid), dim3(block), 0, stream,
+                                  tm.start, tm.stop, 0, a);
+        } else {
+            hipExtLaunchKernelGGL((strided_sweep<T, N, MODE, V>), dim3(grid), dim3(block), 0, stream,
+                                  tm.start, tm.stop, 0, a);
+        }
+    } else if constexpr (mode_contiguous(MODE)) {
         hipLaunchKernelGGL((contig_sweep<T, N, MODE, V>), dim3(grid), dim3(block), 0, stream, a);
     } else {
         hipLaunchKernelGGL((strided_sweep<T, N, MODE, V>), dim3(grid), dim3(block), 0, stream, a);

@@ -1,6 +1,7 @@
 // launch.hpp -- host-side view of the compiled sweep kernels.
 #pragma once
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include "sweeps.hpp"
@@ -9,6 +10,16 @@ namespace tdse {
 
 // Launch one sweep over the whole array described by `a` on `stream`.
 using LaunchFn = hipError_t (*)(const SweepArgs& a, int max_blocks, hipStream_t stream);
+
+// Kernel-boundary timing (per-sweep profiling): when `start` is set, the next
+// single-plan sweep launched from this thread records `start` and `stop` at
+// its kernel's own begin and end (hipExtLaunchKernelGGL, no marker packets
+// around it) and clears both.  A sweep with nothing to launch leaves them set.
+struct LaunchTiming {
+    hipEvent_t start = nullptr;
+    hipEvent_t stop = nullptr;
+};
+extern thread_local LaunchTiming g_launch_timing;
 
 struct SweepShape {
     int block;      // threads per block

@@ -77,6 +77,8 @@ struct Launch {
 
 int tdse::record_error(int status, const char* what) { return fail(status, what); }
 
+thread_local tdse::LaunchTiming tdse::g_launch_timing;
+
 // Two pinned host buffers per context for state uploads and downloads: the
 // host conversion of one chunk overlaps the DMA of the other.  Allocated on
 // the first transfer; without them transfers go through pageable memory.
@@ -2155,15 +2157,25 @@ int tdse_profile_step(tdse_plan* p, int nsteps, int max_slots, double* ms_per_sw
     std::vector<hipEvent_t> ev(2 * L.size());
     for (auto& e : ev) TDSE_HIP(hipEventCreate(&e));
     int status = TDSE_OK;
+    // Sweeps record their events at their kernel's own start and end
+    // (tdse::g_launch_timing); the norm finish of a relax step, launched
+    // elsewhere, between events recorded around it.
     for (size_t i = 0; i < L.size() && status == TDSE_OK; ++i) {
         if (!L[i].fn) {
             status = fail(TDSE_E_UNSUPPORTED, "no kernel");
             break;
         }
-        if (hipEventRecord(ev[2 * i], p->stream) != hipSuccess ||
-            L[i].fn(L[i].args, p->ctx->max_blocks, p->stream) != hipSuccess ||
-            hipEventRecord(ev[2 * i + 1], p->stream) != hipSuccess)
-            status = fail(TDSE_E_DEVICE, "launch failed while profiling");
+        hipError_t e = hipSuccess;
+        if (L[i].mode == kFinishMode) {
+            if ((e = hipEventRecord(ev[2 * i], p->stream)) == hipSuccess &&
+                (e = L[i].fn(L[i].args, p->ctx->max_blocks, p->stream)) == hipSuccess)
+                e = hipEventRecord(ev[2 * i + 1], p->stream);
+        } else {
+            tdse::g_launch_timing = tdse::LaunchTiming{ev[2 * i], ev[2 * i + 1]};
+            e = L[i].fn(L[i].args, p->ctx->max_blocks, p->stream);
+            tdse::g_launch_timing = tdse::LaunchTiming{};
+        }
+        if (e != hipSuccess) status = fail(TDSE_E_DEVICE, "launch failed while profiling");
     }
     if (status == TDSE_OK && hipStreamSynchronize(p->stream) != hipSuccess)
         status = fail(TDSE_E_DEVICE, "sync failed while profiling");
