@@ -186,8 +186,11 @@ int tdse_probability_in_slab(tdse_plan* plan, int axis, double lo, double hi, do
 /* Multiply the state by s on the GPU (renormalisation; solver/shaders/scale.comp.slang). */
 int tdse_state_scale(tdse_plan* plan, double s);
 
-/* Per-launch timing of `nsteps` steps with HIP events around every sweep
- * kernel on the plan's stream.  ms_per_sweep[i] / launches[i] describe sweep
+/* Per-launch timing of `nsteps` steps: every sweep kernel on the plan's
+ * stream is timed by HIP events recorded at its own start and end (the norm
+ * finish of a relax step by events around its launch).  No reference
+ * counterpart (the reference times whole steps, bench/bench_main.cpp:21-29).
+ * ms_per_sweep[i] / launches[i] describe sweep
  * slot i of the step schedule (slots listed by tdse_plan_sweep_name); at most
  * max_slots are written; *n_slots receives the number of slots. */
 int tdse_profile_step(tdse_plan* plan, int nsteps, int max_slots, double* ms_per_sweep,
