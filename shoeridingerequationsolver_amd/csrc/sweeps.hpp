@@ -294,24 +294,18 @@ __device__ __forceinline__ cx<double> tan_rotation(double t, double f) {
 }
 
 // kick factor m * exp(i * (f * phi + extra)); f = 1 (half kick) or 2 (merged
-// full kick), extra = the drive's linear angle (0 without one).  FP32 plans
-// store tan(phi / 2) (tan_rotation), FP64 plans phi.
+// full kick), extra = the drive's linear angle (0 without one).  Real-time
+// plans of both precisions store tan(phi / 2) (tan_rotation).
 template <typename T>
 __device__ __forceinline__ cx<double> kick_factor(T stored, double f, double m, double extra,
                                                   bool drive) {
-    if constexpr (sizeof(T) == 4) {
-        cx<double> k = tan_rotation(static_cast<double>(stored), f);
-        if (drive) {
-            double s, c;
-            sincos_small(extra, &s, &c);
-            k = {k.x * c - k.y * s, k.x * s + k.y * c};
-        }
-        return {k.x * m, k.y * m};
-    } else {
+    cx<double> k = tan_rotation(static_cast<double>(stored), f);
+    if (drive) {
         double s, c;
-        sincos_small(f * static_cast<double>(stored) + extra, &s, &c);
-        return {c * m, s * m};
+        sincos_small(extra, &s, &c);
+        k = {k.x * c - k.y * s, k.x * s + k.y * c};
     }
+    return {k.x * m, k.y * m};
 }
 
 // theta(t) = E0 cos(w t) dt / 2 of a dipole drive (core/src/drive.ixx:25-26)

@@ -158,10 +158,10 @@ struct tdse_plan {
     bool imag() const { return (flags & TDSE_FLAG_IMAGINARY_TIME) != 0; }
     // imaginary time: device slots inside the reduction buffer (after red_out)
     tdse::RelaxSlots* relax() const { return reinterpret_cast<tdse::RelaxSlots*>(red_out + 16); }
-    // FP32 real-time sweep plans store the kick phase phi as tan(phi / 2):
-    // the kernels form exp(i phi) = (1 - t^2 + 2 i t) / (1 + t^2) without a
-    // sine or cosine (the lattice keeps phi)
-    bool tan_kick() const { return !fp64() && !imag() && kind != kKindLattice; }
+    // Real-time sweep plans store the kick phase phi as tan(phi / 2): the
+    // kernels form exp(i phi) = (1 - t^2 + 2 i t) / (1 + t^2) without a sine
+    // or cosine (the lattice keeps phi)
+    bool tan_kick() const { return !imag() && kind != kKindLattice; }
 };
 
 namespace {
