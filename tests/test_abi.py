@@ -119,21 +119,24 @@ def test_wavenumbers_match_oracle(tdse, orc):
                                    orc.wavenumbers(n, lo, hi), rtol=1e-15, atol=0)
 
 
-def build_facade(tmp_path):
-    """Compile the reference-style C++ call site against the facade."""
+def build_facade(tmp_path, std="c++20"):
+    """Compile the reference-style C++ call site against the facade (C++20 by
+    default, the reference's own standard)."""
     import subprocess
     lib_dir = os.path.join(ROOT, "shoeridingerequationsolver_amd", "_lib")
-    exe = str(tmp_path / "facade_main")
-    cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-I", os.path.join(ROOT, "include"),
+    exe = str(tmp_path / f"facade_main_{std}")
+    cmd = ["g++", f"-std={std}", "-O2", "-Wall", "-Wextra", "-Werror",
+           "-I", os.path.join(ROOT, "include"),
            os.path.join(ROOT, "tests", "cpp", "facade_main.cpp"), "-L", lib_dir, "-ltdse",
            f"-Wl,-rpath,{lib_dir}", "-o", exe]
     subprocess.run(cmd, check=True)
     return exe
 
 
-def test_cpp_facade_compiles_and_reports_no_device(tdse, tmp_path):
+@pytest.mark.parametrize("std", ["c++17", "c++20"])
+def test_cpp_facade_compiles_and_reports_no_device(tdse, tmp_path, std):
     import subprocess
-    exe = build_facade(tmp_path)
+    exe = build_facade(tmp_path, std)
     if os.path.exists("/dev/kfd") and os.access("/dev/kfd", os.R_OK):
         pytest.skip("a GPU is visible here; the GPU tier runs the facade")
     r = subprocess.run([exe], capture_output=True, text=True)
