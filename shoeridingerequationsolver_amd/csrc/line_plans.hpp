@@ -44,7 +44,7 @@ TDSE_LINE_PLAN(512, 0, 16, 8, 8, 8)
 TDSE_LINE_PLAN(1024, 0, 16, 16, 16, 4)
 TDSE_LINE_PLAN(96, 0, 12, 6, 4, 4)
 TDSE_LINE_PLAN(160, 0, 20, 10, 4, 4)
-TDSE_LINE_PLAN(192, 0, 24, 8, 6, 4)
+TDSE_LINE_PLAN(192, 0, 12, 12, 4, 4)
 // alternatives
 TDSE_LINE_PLAN(64, 1, 4, 4, 4, 4)
 TDSE_LINE_PLAN(128, 1, 8, 8, 4, 4)
@@ -52,6 +52,7 @@ TDSE_LINE_PLAN(256, 1, 8, 8, 8, 4)
 TDSE_LINE_PLAN(512, 1, 8, 8, 8, 8)
 TDSE_LINE_PLAN(1024, 1, 32, 32, 32)
 TDSE_LINE_PLAN(96, 1, 24, 12, 8)
+TDSE_LINE_PLAN(192, 1, 24, 8, 6, 4)
 #undef TDSE_LINE_PLAN
 
 template <int N>
@@ -61,7 +62,7 @@ struct LinePlan<N, 1> : LinePlan<N, 0> {};
 #define TDSE_FOR_EACH_N(X) \
     X(2) X(4) X(8) X(16) X(32) X(64) X(128) X(256) X(512) X(1024) X(96) X(160) X(192)
 // Lengths whose variants 1..3 are compiled (the others run variant 0 only).
-#define TDSE_FOR_EACH_TUNED_N(X) X(64) X(128) X(256) X(512) X(1024) X(96)
+#define TDSE_FOR_EACH_TUNED_N(X) X(64) X(128) X(256) X(512) X(1024) X(96) X(192)
 constexpr int kVariants = 4;
 
 constexpr bool line_supported(int n) {

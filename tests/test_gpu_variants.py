@@ -46,6 +46,8 @@ CASES = [
     # forward/inverse, z strided kinetic, 2D kinetic, four-step 1D)
     ((64, 96, 128), (-10, -12, -14), (10, 12, 14), 6),
     ((96, 64, 256), (-12, -8, -16), (12, 8, 16), 4),
+    ((192, 192, 16), (-12, -12, -8), (12, 12, 8), 4),
+    ((32, 16, 192), (-8, -8, -12), (8, 8, 12), 4),
     ((256, 128, 1), (-30, -15, 0), (30, 15, 1), 6),
     ((1024, 512, 1), (-60, -40, 0), (60, 40, 1), 3),
     ((65536, 1, 1), (-100, 0, 0), (100, 1, 1), 5),
