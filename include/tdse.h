@@ -264,7 +264,7 @@ int tdse_plan_relax_stats(tdse_plan* plan, tdse_relax_stats* out);
  * measurement, one pass of coordinate descent over the slots), the sweeps run
  * once per call by the time of `reps` launches of their own.  A change from
  * the current choice must be at least 2% faster.  variants[] receives the
- * choice per slot.  Lengths 64-1024 and 96 have variants; the others and the
+ * choice per slot.  Lengths 64-1024, 96 and 192 have variants; the others and the
  * one-block 1D kernel run variant 0.  A batch sweeps each member with the
  * variants its plan holds when the batch is created (plans of one line
  * length, mode and variant share a launch).  No reference counterpart (the
