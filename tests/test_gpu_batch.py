@@ -76,8 +76,9 @@ def test_batch_runs_each_plans_variants(tdse):
     the autotuner would) give exactly the solo plans' states in the same
     variants; members of one shape in different variants share no launch."""
     from shoeridingerequationsolver_amd import workloads
-    shapes = [(64, 64, 64), (128, 64, 1), (96, 64, 64), (65536, 1, 1)]  # lengths with variants
-    solves = workloads.batch_instances(seed=17, count=8, steps=15, shapes=shapes)
+    shapes = [(64, 64, 64), (128, 64, 1), (96, 64, 64), (65536, 1, 1),  # lengths with variants
+              (192, 64, 64), (64, 192, 96)]
+    solves = workloads.batch_instances(seed=17, count=12, steps=15, shapes=shapes)
     solo, members = [], []
     for i, s in enumerate(solves):
         for group in (solo, members):
