@@ -296,6 +296,23 @@ int tdse_batch_stream(tdse_batch* batch, void** out);
 /* Number of batched launches in one interior round of every unit. */
 int tdse_batch_launches_per_round(const tdse_batch* batch, int* out);
 
+/* Uneven submissions: advance plans[i] by nsteps[i] steps (tdse_step), each
+ * plan on its own stream, so solves of different shapes, precisions and step
+ * counts overlap on the GPU (the policy the served stream measured fastest).
+ * Every plan must belong to ctx and have a potential; all arguments are
+ * checked before anything is queued (a device error while queuing leaves the
+ * earlier plans' steps queued).  Asynchronous.  No reference counterpart:
+ * the reference GPU engine queues one scene's steps at a time
+ * (solver/src/vk_engine.ixx:645, Engine::step_async). */
+int tdse_batch_submit(tdse_ctx* ctx, tdse_plan* const* plans, const int* nsteps, int count);
+/* Wait for the queued work of every live plan and batch of ctx;
+ * timeout_ms < 0 waits forever, otherwise TDSE_E_TIMEOUT once timeout_ms
+ * have passed with work still queued (0: a non-blocking poll).  Plans and
+ * batches of ctx are not created or destroyed on other threads while it
+ * waits (those calls wait for it).  Behavioural precedent: the engine's
+ * bounded wait, solver/src/vk_engine.ixx:681 (Engine::wait_async). */
+int tdse_sync(tdse_ctx* ctx, int timeout_ms);
+
 /* Build a plan's inputs on the GPU from a recipe: potential (and its kick
  * phase), optional absorbing mask, and the normalised Gaussian packet as the
  * state.  Same formulas as the host builders below, evaluated in FP64 on the

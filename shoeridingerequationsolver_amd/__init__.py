@@ -126,6 +126,25 @@ class Context:
         capi.check(capi.lib().tdse_ctx_bytes_in_use(self._h, C.byref(v)))
         return v.value
 
+    def submit(self, props, nsteps):
+        """Queue nsteps[i] steps on props[i] (each on its own stream; all
+        arguments are checked first).  ``tdse_batch_submit``."""
+        props, nsteps = list(props), [int(k) for k in nsteps]
+        if len(props) != len(nsteps):
+            raise ValueError("one step count per propagator")
+        hs = (C.c_void_p * len(props))(*[p.handle for p in props])
+        ks = (C.c_int * len(nsteps))(*nsteps)
+        try:
+            capi.check(capi.lib().tdse_batch_submit(self._h, hs, ks, len(props)))
+        except TdseError as e:
+            raise _map_error(e) from None
+
+    def sync(self, timeout_ms: int = -1):
+        """Wait for every plan and batch of the context (``tdse_sync``);
+        TdseError with TDSE_E_TIMEOUT when work is still queued after
+        timeout_ms (0: a poll)."""
+        capi.check(capi.lib().tdse_sync(self._h, int(timeout_ms)))
+
     def close(self):
         if self._h:
             capi.lib().tdse_ctx_destroy(self._h)

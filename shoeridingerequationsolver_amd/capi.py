@@ -50,7 +50,7 @@ EXPORTS = (
     "tdse_relax_deflated", "tdse_plan_relax_stats", "tdse_build_soft_coulomb",
     "tdse_plan_set_drive", "tdse_plan_set_time", "tdse_plan_get_time", "tdse_plan_set_magnetic",
     "tdse_lattice_set_uniform_field", "tdse_lattice_set_solenoid", "tdse_observe_noalloc",
-    "tdse_build_phase_tables", "tdse_state_copy_kernel",
+    "tdse_build_phase_tables", "tdse_state_copy_kernel", "tdse_batch_submit", "tdse_sync",
 )
 
 TDSE_POT = {"none": 0, "harmonic": 1, "regularized_coulomb": 2, "coulomb_floor": 3, "barrier": 4,
@@ -154,6 +154,9 @@ def lib() -> C.CDLL:
         L.tdse_batch_destroy.argtypes = [C.c_void_p]
         L.tdse_batch_step.argtypes = [C.c_void_p, C.c_int, C.c_int]
         L.tdse_batch_sync.argtypes = [C.c_void_p, C.c_int]
+        L.tdse_batch_submit.argtypes = [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_int),
+                                        C.c_int]
+        L.tdse_sync.argtypes = [C.c_void_p, C.c_int]
         L.tdse_batch_stream.argtypes = [C.c_void_p, C.POINTER(C.c_void_p)]
         L.tdse_batch_launches_per_round.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
         L.tdse_plan_build.argtypes = [C.c_void_p, C.POINTER(Recipe)]

@@ -97,6 +97,11 @@ struct tdse_ctx {
     int max_blocks = 1 << 20;
     std::mutex mu;
     PinnedStage pin;
+    // streams of the context's live plans and batches (tdse_sync waits on
+    // them); reg_mu is held for a whole tdse_sync, so a plan or batch is not
+    // destroyed while its stream is waited on
+    std::mutex reg_mu;
+    std::vector<hipStream_t> streams;
 };
 
 struct tdse_plan {
@@ -176,6 +181,17 @@ tdse::LaunchFn launcher(const tdse_plan* p, int n, int mode, int variant = 0) {
 int set_device(const tdse_ctx* ctx) {
     TDSE_HIP(hipSetDevice(ctx->device));
     return TDSE_OK;
+}
+
+void register_stream(tdse_ctx* ctx, hipStream_t s) {
+    std::lock_guard<std::mutex> lk(ctx->reg_mu);
+    ctx->streams.push_back(s);
+}
+
+void unregister_stream(tdse_ctx* ctx, hipStream_t s) {
+    std::lock_guard<std::mutex> lk(ctx->reg_mu);
+    ctx->streams.erase(std::remove(ctx->streams.begin(), ctx->streams.end(), s),
+                       ctx->streams.end());
 }
 
 // Four-step split N = n1 * n2 (n1 contiguous row length, n2 rows), both
@@ -1450,6 +1466,7 @@ int tdse_plan_create(tdse_ctx* ctx, const int n[3], const double lo[3], const do
         delete p;
         return fail(TDSE_E_DEVICE, "hipStreamCreate failed");
     }
+    register_stream(ctx, p->stream);
     if ((rc = alloc(p, &p->psi, total * p->csize()))) return bail(rc);
     if ((rc = alloc(p, &p->phi, total * p->rsize()))) return bail(rc);
     if ((rc = alloc(p, &p->v, total * p->rsize()))) return bail(rc);
@@ -1504,7 +1521,10 @@ int tdse_plan_destroy(tdse_plan* p) {
     release(p, part, static_cast<uint64_t>(tdse::kMaxReduceBlocks) * tdse::kReduceQ * 8 + 64 * 8);
     for (auto& kv : p->tw) release(p, kv.second, static_cast<uint64_t>(kv.first) * 16);
     for (int a = 0; a < 3; ++a) release(p, p->kt[a], static_cast<uint64_t>(p->kt_len[a]) * 16);
-    if (p->stream) (void)hipStreamDestroy(p->stream);
+    if (p->stream) {
+        unregister_stream(p->ctx, p->stream);
+        (void)hipStreamDestroy(p->stream);
+    }
     delete p;
     return TDSE_OK;
 }
@@ -2496,11 +2516,16 @@ int tdse_batch_create(tdse_ctx* ctx, tdse_plan* const* plans, int count, tdse_ba
         tdse_batch_destroy(b);
         return code;
     };
-    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess ||
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+        (void)hipGetLastError();
+        b->stream = nullptr;
+        return bail(fail(TDSE_E_DEVICE, "stream creation failed"));
+    }
+    register_stream(ctx, b->stream);
+    if (hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&b->done, hipEventDisableTiming) != hipSuccess) {
         (void)hipGetLastError();
-        return bail(fail(TDSE_E_DEVICE, "stream/event creation failed"));
+        return bail(fail(TDSE_E_DEVICE, "event creation failed"));
     }
     for (int s = 0; s < tdse_batch::kSide; ++s) {
         if (hipStreamCreateWithFlags(&b->side[s], hipStreamNonBlocking) != hipSuccess ||
@@ -2585,7 +2610,10 @@ int tdse_batch_destroy(tdse_batch* b) {
     }
     if (b->fork) (void)hipEventDestroy(b->fork);
     if (b->done) (void)hipEventDestroy(b->done);
-    if (b->stream) (void)hipStreamDestroy(b->stream);
+    if (b->stream) {
+        unregister_stream(b->ctx, b->stream);
+        (void)hipStreamDestroy(b->stream);
+    }
     delete b;
     return TDSE_OK;
 }
@@ -2635,6 +2663,51 @@ int tdse_batch_sync(tdse_batch* b, int timeout_ms) {
             return fail(TDSE_E_TIMEOUT, "batch wait timed out");
         std::this_thread::yield();
     }
+}
+
+int tdse_batch_submit(tdse_ctx* ctx, tdse_plan* const* plans, const int* nsteps, int count) {
+    if (!ctx) return fail(TDSE_E_INVALID, "null ctx");
+    if (count < 0) return fail(TDSE_E_INVALID, "count < 0");
+    if (count > 0 && (!plans || !nsteps)) return fail(TDSE_E_INVALID, "null plans or nsteps");
+    // every argument is checked before anything is queued
+    for (int i = 0; i < count; ++i) {
+        const std::string at = "plan " + std::to_string(i) + ": ";
+        if (!plans[i]) return fail(TDSE_E_INVALID, at + "null plan");
+        if (plans[i]->ctx != ctx) return fail(TDSE_E_INVALID, at + "belongs to another context");
+        if (nsteps[i] < 0) return fail(TDSE_E_INVALID, at + "nsteps < 0");
+        if (nsteps[i] > 0 && !plans[i]->has_potential)
+            return fail(TDSE_E_STATE, at + "tdse_plan_set_potential not called");
+    }
+    for (int i = 0; i < count; ++i) {
+        const int rc = tdse_step(plans[i], nsteps[i]);
+        if (rc) return rc;
+    }
+    return TDSE_OK;
+}
+
+int tdse_sync(tdse_ctx* ctx, int timeout_ms) {
+    if (!ctx) return fail(TDSE_E_INVALID, "null ctx");
+    int rc = set_device(ctx);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> lk(ctx->reg_mu);
+    if (timeout_ms < 0) {
+        for (hipStream_t s : ctx->streams) TDSE_HIP(hipStreamSynchronize(s));
+        return TDSE_OK;
+    }
+    const auto t0 = std::chrono::steady_clock::now();
+    for (hipStream_t s : ctx->streams) {
+        for (;;) {
+            const hipError_t e = hipStreamQuery(s);
+            if (e == hipSuccess) break;
+            if (e != hipErrorNotReady) return fail(TDSE_E_DEVICE, hipGetErrorString(e));
+            if (std::chrono::duration_cast<std::chrono::milliseconds>(
+                    std::chrono::steady_clock::now() - t0)
+                    .count() >= timeout_ms)
+                return fail(TDSE_E_TIMEOUT, "context wait timed out");
+            std::this_thread::yield();
+        }
+    }
+    return TDSE_OK;
 }
 
 int tdse_batch_stream(tdse_batch* b, void** out) {
