@@ -183,6 +183,12 @@ int tdse_observe_noalloc(const void* psi, const void* v, int precision, const in
 /* Probability lo <= coord_axis < hi times the cell volume, not normalised
  * (core/src/observables.ixx:121-131, extended to one axis of a 3D grid). */
 int tdse_probability_in_slab(tdse_plan* plan, int axis, double lo, double hi, double* out);
+/* Probability lo[a] <= coord_a < hi[a] on every axis of extent above 1 (axes
+ * of extent 1 are not restricted; infinite bounds leave an axis open) times
+ * the cell volume, not normalised: a box region, the 3D form of
+ * core/src/observables.ixx:121-131.  One restricted axis gives exactly
+ * tdse_probability_in_slab. */
+int tdse_probability_in_box(tdse_plan* plan, const double lo[3], const double hi[3], double* out);
 /* Multiply the state by s on the GPU (renormalisation; solver/shaders/scale.comp.slang). */
 int tdse_state_scale(tdse_plan* plan, double s);
 

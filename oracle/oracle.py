@@ -58,7 +58,8 @@ def orc() -> C.CDLL:
         for name in ("orc_norm_sq", "orc_mean_position_1d", "orc_mean_energy_1d",
                      "orc_energy_variance_1d",
                      "orc_probability_in_range_1d", "orc_mean_energy_3d",
-                     "orc_probability_in_slab_3d", "orc_spacing", "orc_sigma_x_1d",
+                     "orc_probability_in_slab_3d", "orc_probability_in_box_3d",
+                     "orc_spacing", "orc_sigma_x_1d",
                      "orc_mean_momentum_1d", "orc_lat2d_energy"):
             getattr(lib, name).restype = C.c_double
         _ORC = lib
@@ -502,6 +503,12 @@ def probability_in_slab_3d(psi, dims, lo, hi, axis, a, b):
     psi = _c128(psi)
     return orc().orc_probability_in_slab_3d(_p(psi), _ivec(dims), _dvec(lo), _dvec(hi),
                                             C.c_int(axis), C.c_double(a), C.c_double(b))
+
+
+def probability_in_box_3d(psi, dims, lo, hi, a, b):
+    psi = _c128(psi)
+    return orc().orc_probability_in_box_3d(_p(psi), _ivec(dims), _dvec(lo), _dvec(hi),
+                                           _dvec(a), _dvec(b))
 
 
 # ---- the reference core (oracle/_ref) --------------------------------------

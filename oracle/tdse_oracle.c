@@ -1387,3 +1387,29 @@ double orc_probability_in_slab_3d(const double* psi, const int d[3], const doubl
             }
     return acc * cell_volume3(lo, hi, d);
 }
+
+/* No counterpart in the reference (extension of the slab above to a box):
+ * the sum over cells with a[ax] <= coord_ax < b[ax] on every axis of extent
+ * above 1 (axes of extent 1 are not restricted), times the cell volume.
+ * Pinned in the oracle tests by the slab (one restricted axis), by the
+ * product of the three slabs for a separable state, and by the Gaussian
+ * closed form. */
+double orc_probability_in_box_3d(const double* psi, const int d[3], const double lo[3],
+                                 const double hi[3], const double a[3], const double b[3]) {
+    double acc = 0.0;
+    for (int k = 0; k < d[2]; ++k)
+        for (int j = 0; j < d[1]; ++j)
+            for (int i = 0; i < d[0]; ++i) {
+                const int idx[3] = {i, j, k};
+                int in = 1;
+                for (int ax = 0; ax < 3; ++ax) {
+                    if (d[ax] == 1) continue;
+                    const double x = coord(lo[ax], hi[ax], d[ax], idx[ax]);
+                    if (!(x >= a[ax] && x < b[ax])) in = 0;
+                }
+                if (!in) continue;
+                const long f = i + (long)d[0] * (j + (long)d[1] * k);
+                acc += psi[2 * f] * psi[2 * f] + psi[2 * f + 1] * psi[2 * f + 1];
+            }
+    return acc * cell_volume3(lo, hi, d);
+}

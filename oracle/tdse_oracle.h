@@ -12,8 +12,8 @@
  *     (core/src/grid.ixx:15-17).
  *
  * Every function names the reference lines it restates.  Functions with no
- * reference counterpart (mixed-radix FFT, 3D probability in a slab, the
- * -1/max(r, h/2) Coulomb builder) say so and how they are pinned.
+ * reference counterpart (mixed-radix FFT, 3D probability in a slab or a box,
+ * the -1/max(r, h/2) Coulomb builder) say so and how they are pinned.
  */
 #ifndef TDSE_ORACLE_H
 #define TDSE_ORACLE_H
@@ -154,6 +154,8 @@ void orc_mean_momentum_3d(const double* psi, const int dims[3], const double lo[
                           const double hi[3], double out[3]);
 double orc_probability_in_slab_3d(const double* psi, const int dims[3], const double lo[3],
                                   const double hi[3], int axis, double a, double b);
+double orc_probability_in_box_3d(const double* psi, const int dims[3], const double lo[3],
+                                 const double hi[3], const double a[3], const double b[3]);
 
 #ifdef __cplusplus
 }

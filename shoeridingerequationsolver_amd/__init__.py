@@ -302,6 +302,14 @@ class Propagator:
         capi.check(capi.lib().tdse_probability_in_slab(self._h, axis, lo, hi, C.byref(v)))
         return v.value
 
+    def probability_in_box(self, lo, hi):
+        """Probability lo[a] <= x_a < hi[a] on every axis of extent above 1,
+        times the cell volume (``tdse_probability_in_box``)."""
+        v = C.c_double()
+        capi.check(capi.lib().tdse_probability_in_box(self._h, capi.dvec(lo), capi.dvec(hi),
+                                                       C.byref(v)))
+        return v.value
+
     def stream(self):
         s = C.c_void_p()
         capi.check(capi.lib().tdse_plan_stream(self._h, C.byref(s)))

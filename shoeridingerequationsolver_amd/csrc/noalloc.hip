@@ -115,9 +115,7 @@ int tdse_observe_noalloc(const void* psi, const void* v, int precision, const in
         r.sq[a] = 1;
         r.dk[a] = 0.0;
     }
-    r.slab_axis = 0;
-    r.slab_lo = 1.0;
-    r.slab_hi = 0.0;  // empty slab
+    tdse::empty_box(r);
     r.mass = 1.0;
     // same block count, partial layout and final tree as tdse_observe
     static_assert(kScratchDoubles >= tdse::kMaxReduceBlocks * tdse::kReduceQ + tdse::kReduceQ,

@@ -44,7 +44,15 @@ __device__ __forceinline__ void block_reduce_store(double (&acc)[Q], double* out
 
 __device__ __forceinline__ int shift_k(int j, int n) { return (2 * j < n) ? j : j - n; }
 
-// x-space sums: |psi|^2, x|psi|^2, y|psi|^2, z|psi|^2, V|psi|^2, slab |psi|^2,
+// Grid coordinate lo + i h with the product and the sum each rounded (no
+// fused multiply-add), as core/src/grid.ixx:17 computes it: a cell exactly on
+// a region's edge then falls on the same side as in the reference.
+__device__ __forceinline__ double grid_coord(double lo, int i, double h) {
+#pragma clang fp contract(off)
+    return lo + static_cast<double>(i) * h;
+}
+
+// x-space sums: |psi|^2, x|psi|^2, y|psi|^2, z|psi|^2, V|psi|^2, box |psi|^2,
 // x^2|psi|^2, y^2|psi|^2, z^2|psi|^2
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
@@ -61,16 +69,17 @@ reduce_x(const ReduceArgs a, double* partials) {
         const long long r = f / a.n[0];
         const int iy = static_cast<int>(r % a.n[1]);
         const int iz = static_cast<int>(r / a.n[1]);
-        const double cxv = a.lo[0] + ix * a.h[0];
-        const double cyv = a.lo[1] + iy * a.h[1];
-        const double czv = a.lo[2] + iz * a.h[2];
+        const double cxv = grid_coord(a.lo[0], ix, a.h[0]);
+        const double cyv = grid_coord(a.lo[1], iy, a.h[1]);
+        const double czv = grid_coord(a.lo[2], iz, a.h[2]);
         acc[0] += w;
         acc[1] += cxv * w;
         acc[2] += cyv * w;
         acc[3] += czv * w;
         if (v) acc[4] += static_cast<double>(v[f]) * w;
-        const double c = a.slab_axis == 0 ? cxv : (a.slab_axis == 1 ? cyv : czv);
-        if (c >= a.slab_lo && c < a.slab_hi) acc[5] += w;
+        if (cxv >= a.box_lo[0] && cxv < a.box_hi[0] && cyv >= a.box_lo[1] && cyv < a.box_hi[1] &&
+            czv >= a.box_lo[2] && czv < a.box_hi[2])
+            acc[5] += w;
         acc[6] += cxv * cxv * w;
         acc[7] += cyv * cyv * w;
         acc[8] += czv * czv * w;

@@ -16,13 +16,29 @@ struct ReduceArgs {
     long long total;   // points
     int n[3];          // original dims (x space coordinates)
     double lo[3], h[3];
-    int slab_axis;
-    double slab_lo, slab_hi;
+    // box of the region probability (slot 5): box_lo <= coordinate < box_hi
+    // on every axis (infinite bounds leave an axis unrestricted)
+    double box_lo[3], box_hi[3];
     int sq[3];         // squeezed dims (k space)
     double dk[3];      // 2 pi / L per squeezed axis
     double mass;
     long long four_step_rows;  // N2 of a four-step 1D grid, else 0
 };
+
+// the region probability's box: empty (no region asked for), or every axis
+// unrestricted before the caller bounds some
+inline void empty_box(ReduceArgs& r) {
+    for (int a = 0; a < 3; ++a) {
+        r.box_lo[a] = 1.0;
+        r.box_hi[a] = 0.0;
+    }
+}
+inline void open_box(ReduceArgs& r) {
+    for (int a = 0; a < 3; ++a) {
+        r.box_lo[a] = -__builtin_huge_val();
+        r.box_hi[a] = __builtin_huge_val();
+    }
+}
 
 int reduce_blocks(long long total);
 // partials: reduce_blocks(total) * kReduceQ doubles; out: kReduceQ doubles (device)

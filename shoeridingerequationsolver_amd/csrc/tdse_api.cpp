@@ -588,7 +588,7 @@ SweepArgs base_args(const tdse_plan* p) {
 
 double dk_of(const Axis& ax) { return 2.0 * 3.14159265358979323846 / ax.length(); }
 
-// Reduction arguments over the plan's state (x space, empty slab).
+// Reduction arguments over the plan's state (x space, empty box).
 tdse::ReduceArgs reduce_args(const tdse_plan* p) {
     tdse::ReduceArgs r{};
     r.psi = p->psi;
@@ -601,9 +601,7 @@ tdse::ReduceArgs reduce_args(const tdse_plan* p) {
         r.sq[a] = a < static_cast<int>(p->sq.size()) ? p->sq[a].n : 1;
         r.dk[a] = a < static_cast<int>(p->sq.size()) ? dk_of(p->sq[a]) : 0.0;
     }
-    r.slab_axis = 0;
-    r.slab_lo = 1.0;
-    r.slab_hi = 0.0;  // empty slab
+    tdse::empty_box(r);
     r.mass = p->mass;
     return r;
 }
@@ -2128,12 +2126,13 @@ int tdse_observe(tdse_plan* p, unsigned what, tdse_observables* out) {
     return TDSE_OK;
 }
 
-int tdse_probability_in_slab(tdse_plan* p, int axis, double lo, double hi, double* out) {
-    int rc = check_plan(p);
+namespace {
+
+// |psi|^2 summed over the box r.box_lo <= coordinate < r.box_hi, times the
+// cell volume (slot 5 of the x-space reduction).
+int region_probability(tdse_plan* p, tdse::ReduceArgs& r, double* out) {
+    int rc = set_device(p->ctx);
     if (rc) return rc;
-    if (!out || axis < 0 || axis > 2) return fail(TDSE_E_INVALID, "bad axis or null out");
-    if ((rc = set_device(p->ctx))) return rc;
-    tdse::ReduceArgs r{};
     r.psi = p->psi;
     r.v = nullptr;
     r.total = p->total;
@@ -2142,9 +2141,6 @@ int tdse_probability_in_slab(tdse_plan* p, int axis, double lo, double hi, doubl
         r.lo[a] = p->lo[a];
         r.h[a] = (p->hi[a] - p->lo[a]) / p->n[a];
     }
-    r.slab_axis = axis;
-    r.slab_lo = lo;
-    r.slab_hi = hi;
     r.mass = p->mass;
     double host[tdse::kReduceQ];
     TDSE_HIP(tdse::launch_reduce(p->fp64(), false, r, p->partials, p->red_out, p->stream));
@@ -2152,6 +2148,35 @@ int tdse_probability_in_slab(tdse_plan* p, int axis, double lo, double hi, doubl
     TDSE_HIP(hipStreamSynchronize(p->stream));
     *out = host[5] * (r.h[0] * r.h[1] * r.h[2]);
     return TDSE_OK;
+}
+
+}  // namespace
+
+int tdse_probability_in_slab(tdse_plan* p, int axis, double lo, double hi, double* out) {
+    int rc = check_plan(p);
+    if (rc) return rc;
+    if (!out || axis < 0 || axis > 2) return fail(TDSE_E_INVALID, "bad axis or null out");
+    tdse::ReduceArgs r{};
+    tdse::open_box(r);
+    r.box_lo[axis] = lo;
+    r.box_hi[axis] = hi;
+    return region_probability(p, r, out);
+}
+
+int tdse_probability_in_box(tdse_plan* p, const double lo[3], const double hi[3], double* out) {
+    int rc = check_plan(p);
+    if (rc) return rc;
+    if (!out || !lo || !hi) return fail(TDSE_E_INVALID, "null argument");
+    tdse::ReduceArgs r{};
+    tdse::open_box(r);
+    for (int a = 0; a < 3; ++a) {
+        if (std::isnan(lo[a]) || std::isnan(hi[a])) return fail(TDSE_E_INVALID, "NaN box bound");
+        if (p->n[a] > 1) {  // axes of extent 1 are not restricted
+            r.box_lo[a] = lo[a];
+            r.box_hi[a] = hi[a];
+        }
+    }
+    return region_probability(p, r, out);
 }
 
 int tdse_state_scale(tdse_plan* p, double s) {

@@ -290,6 +290,46 @@ def test_slab_probability_gaussian_tail(orc):
     assert abs(orc.probability_in_slab_3d(psi, dims, lo, hi, 0, -1e300, 1e300) - 1) < 1e-12
 
 
+def test_box_probability_pins(orc):
+    """The box: one restricted axis is the slab bit for bit; for a separable
+    (product) state the sum over a box factorises, so box * N^2 equals the
+    product of the three slabs, N the whole grid's sum (the norm), up to the
+    rounding of sequential sums over millions of cells; the Gaussian closed
+    form per axis as in the slab test; an inverted box is empty; axes of
+    extent 1 are not restricted."""
+    dims, lo, hi = (256, 192, 128), (-12.0, -12.0, -8.0), (12.0, 12.0, 8.0)
+    x0, sig = (0.3, -0.7, 0.2), (1.4, 1.6, 1.5)
+    psi = orc.gaussian_3d(dims, lo, hi, x0, sig, (0.4, 0.0, -0.3))
+    inf = 1e300
+    a, b = (-1.0, -2.5, -0.5), (2.0, 0.5, 1.5)
+    for ax in range(3):
+        lo_b, hi_b = [-inf] * 3, [inf] * 3
+        lo_b[ax], hi_b[ax] = a[ax], b[ax]
+        assert (orc.probability_in_box_3d(psi, dims, lo, hi, lo_b, hi_b)
+                == orc.probability_in_slab_3d(psi, dims, lo, hi, ax, a[ax], b[ax]))
+    box = orc.probability_in_box_3d(psi, dims, lo, hi, a, b)
+    norm = orc.probability_in_box_3d(psi, dims, lo, hi, [-inf] * 3, [inf] * 3)
+    assert norm == orc.probability_in_slab_3d(psi, dims, lo, hi, 0, -inf, inf)
+    assert abs(norm - 1) < 1e-10
+    slabs = [orc.probability_in_slab_3d(psi, dims, lo, hi, ax, a[ax], b[ax]) for ax in range(3)]
+    assert abs(box * norm ** 2 - slabs[0] * slabs[1] * slabs[2]) < 1e-10
+    want = 1.0
+    for ax in range(3):
+        h = (hi[ax] - lo[ax]) / dims[ax]
+        first = lo[ax] + math.ceil((a[ax] - lo[ax]) / h - 1e-12) * h
+        past = lo[ax] + math.ceil((b[ax] - lo[ax]) / h - 1e-12) * h
+        s2 = math.sqrt(2) * sig[ax]
+        want *= 0.5 * (math.erfc((first - 0.5 * h - x0[ax]) / s2)
+                       - math.erfc((past - 0.5 * h - x0[ax]) / s2))
+    assert abs(box - want) < 5e-4
+    assert orc.probability_in_box_3d(psi, dims, lo, hi, b, a) == 0.0
+    d2, lo2, hi2 = (64, 48, 1), (-16.0, -12.0, 0.0), (16.0, 12.0, 1.0)
+    psi2 = orc.gaussian_3d(d2, lo2, hi2, (0.5, -0.5, 0.0), (1.5, 1.2, 1.0), (0.0, 0.0, 0.0))
+    z_excluded = orc.probability_in_box_3d(psi2, d2, lo2, hi2, (-1, -1, 5.0), (2, 1, 6.0))
+    z_open = orc.probability_in_box_3d(psi2, d2, lo2, hi2, (-1, -1, -inf), (2, 1, inf))
+    assert z_excluded == z_open > 0
+
+
 # ---- bitwise against the reference build -------------------------------------
 
 needs_ref = pytest.mark.skipif(
