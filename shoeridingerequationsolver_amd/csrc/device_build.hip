@@ -10,6 +10,7 @@
 #include "../../include/tdse.h"
 #include "device_build.hpp"
 #include "fft_engine.hpp"
+#include "grid.hpp"
 
 namespace tdse {
 namespace {
@@ -52,7 +53,8 @@ build_cells(BuildArgs a) {
         const long long q = f / a.n[0];
         const int j = static_cast<int>(q % a.n[1]);
         const int k = static_cast<int>(q / a.n[1]);
-        const double x = a.lo[0] + i * a.h[0], y = a.lo[1] + j * a.h[1], z = a.lo[2] + k * a.h[2];
+        const double x = grid_coord(a.lo[0], i, a.h[0]), y = grid_coord(a.lo[1], j, a.h[1]),
+                     z = grid_coord(a.lo[2], k, a.h[2]);
         const double dx = x - r.center[0], dy = y - r.center[1], dz = z - r.center[2];
         double pot = 0.0;
         switch (r.potential) {

@@ -9,6 +9,7 @@
 // (core/src/parallel.ixx:14-16, 211-230).
 #include <hip/hip_runtime.h>
 
+#include "grid.hpp"
 #include "observe.hpp"
 #include "sweeps.hpp"
 
@@ -43,14 +44,6 @@ __device__ __forceinline__ void block_reduce_store(double (&acc)[Q], double* out
 }
 
 __device__ __forceinline__ int shift_k(int j, int n) { return (2 * j < n) ? j : j - n; }
-
-// Grid coordinate lo + i h with the product and the sum each rounded (no
-// fused multiply-add), as core/src/grid.ixx:17 computes it: a cell exactly on
-// a region's edge then falls on the same side as in the reference.
-__device__ __forceinline__ double grid_coord(double lo, int i, double h) {
-#pragma clang fp contract(off)
-    return lo + static_cast<double>(i) * h;
-}
 
 // x-space sums: |psi|^2, x|psi|^2, y|psi|^2, z|psi|^2, V|psi|^2, box |psi|^2,
 // x^2|psi|^2, y^2|psi|^2, z^2|psi|^2
