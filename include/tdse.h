@@ -146,12 +146,15 @@ int tdse_plan_stream(tdse_plan* plan, void** out);
  *   TDSE_OBS_MOMENTUM  <k_x>, <k_y>, <k_z> (ratio form)      (core/src/observables.ixx:52-65, 177-200)
  *   TDSE_OBS_VARIANCE  Var(H) = ||(H - <H>) psi||^2 / ||psi||^2, residual form, 1D grids
  *                      only (TDSE_E_UNSUPPORTED otherwise)    (core/src/observables.ixx:92-119)
+ *   TDSE_OBS_PEAK      peak = max |psi|^2 over the cells       (solver/src/vk_engine.ixx:1665-1691,
+ *                      (no cell volume)                        Engine::norm_and_peak)
  */
 #define TDSE_OBS_NORM 1u
 #define TDSE_OBS_POSITION 2u
 #define TDSE_OBS_ENERGY 4u
 #define TDSE_OBS_MOMENTUM 8u
 #define TDSE_OBS_VARIANCE 16u
+#define TDSE_OBS_PEAK 32u
 typedef struct tdse_observables {
     double norm;
     double sum_abs2;      /* sum |psi|^2 without the cell volume */
@@ -162,6 +165,7 @@ typedef struct tdse_observables {
     double sigma[3];      /* sqrt(max(0, <x_a^2> - <x_a>^2)) */
     double momentum[3];   /* <k_a> */
     double energy_variance; /* TDSE_OBS_VARIANCE */
+    double peak;          /* TDSE_OBS_PEAK */
 } tdse_observables;
 int tdse_observe(tdse_plan* plan, unsigned what, tdse_observables* out);
 /* The position-space part of tdse_observe -- norm, sum |psi|^2, <x>, <y>, <z>,

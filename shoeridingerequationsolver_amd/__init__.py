@@ -295,7 +295,7 @@ class Propagator:
         return {"norm": o.norm, "sum_abs2": o.sum_abs2, "position": tuple(o.position),
                 "potential": o.potential, "kinetic": o.kinetic, "energy": o.energy,
                 "sigma": tuple(o.sigma), "momentum": tuple(o.momentum),
-                "energy_variance": o.energy_variance}
+                "energy_variance": o.energy_variance, "peak": o.peak}
 
     def probability_in_slab(self, axis, lo, hi):
         v = C.c_double()

@@ -32,6 +32,7 @@ TDSE_OBS_POSITION = 2
 TDSE_OBS_ENERGY = 4
 TDSE_OBS_MOMENTUM = 8
 TDSE_OBS_VARIANCE = 16
+TDSE_OBS_PEAK = 32
 
 # every symbol include/tdse.h declares (checked by the CPU test tier)
 EXPORTS = (
@@ -94,6 +95,7 @@ class Observables(C.Structure):
         ("sigma", C.c_double * 3),
         ("momentum", C.c_double * 3),
         ("energy_variance", C.c_double),
+        ("peak", C.c_double),
     ]
 
 

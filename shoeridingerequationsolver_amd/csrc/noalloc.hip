@@ -55,6 +55,7 @@ __global__ void __launch_bounds__(64) finish_observables(const double* sums, dou
     o.kinetic = 0.0;
     o.energy = 0.0;
     o.energy_variance = 0.0;
+    o.peak = 0.0;
     *out = o;
 }
 

@@ -144,6 +144,50 @@ scale_state(void* psi_v, long long total, double s) {
     }
 }
 
+// max |psi|^2 per block, then over the blocks (a max: the same bits in any
+// order).
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+reduce_peak(const void* psi_v, long long total, double* partials) {
+    const cx<T>* psi = static_cast<const cx<T>*>(psi_v);
+    double m = 0.0;
+    const long long stride = static_cast<long long>(gridDim.x) * kBlock;
+    for (long long f = blockIdx.x * static_cast<long long>(kBlock) + threadIdx.x; f < total;
+         f += stride) {
+        const cx<T> z = psi[f];
+        const double w = static_cast<double>(z.x) * z.x + static_cast<double>(z.y) * z.y;
+        m = w > m ? w : m;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        const double o = __shfl_xor(m, off, 64);
+        m = o > m ? o : m;
+    }
+    __shared__ double part[kWaves];
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < kWaves; ++w) m = part[w] > m ? part[w] : m;
+        partials[blockIdx.x] = part[0] > m ? part[0] : m;
+    }
+}
+
+__global__ void __launch_bounds__(kBlock)
+reduce_peak_final(const double* partials, int nparts, double* out) {
+    double m = 0.0;
+    for (int p = threadIdx.x; p < nparts; p += kBlock) m = partials[p] > m ? partials[p] : m;
+    for (int off = 32; off > 0; off >>= 1) {
+        const double o = __shfl_xor(m, off, 64);
+        m = o > m ? o : m;
+    }
+    __shared__ double part[kWaves];
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 0; w < kWaves; ++w) m = part[w] > m ? part[w] : m;
+        out[0] = m;
+    }
+}
+
 // Sum of per-block triples (observing sweeps), fixed order.
 __global__ void __launch_bounds__(kBlock)
 reduce_final3(const double* partials, int nparts, double* out) {
@@ -344,6 +388,19 @@ hipError_t launch_copy(void* dst, const void* src, size_t bytes, hipStream_t str
         return hipMemcpyAsync(static_cast<char*>(dst) + n16 * 16,
                               static_cast<const char*>(src) + n16 * 16, tail,
                               hipMemcpyDeviceToDevice, stream);
+    return hipGetLastError();
+}
+
+hipError_t launch_peak(bool fp64, const void* psi, long long total, double* partials, double* out,
+                       hipStream_t stream) {
+    const int blocks = reduce_blocks(total);
+    if (fp64)
+        hipLaunchKernelGGL(reduce_peak<double>, dim3(blocks), dim3(kBlock), 0, stream, psi, total,
+                           partials);
+    else
+        hipLaunchKernelGGL(reduce_peak<float>, dim3(blocks), dim3(kBlock), 0, stream, psi, total,
+                           partials);
+    hipLaunchKernelGGL(reduce_peak_final, dim3(1), dim3(kBlock), 0, stream, partials, blocks, out);
     return hipGetLastError();
 }
 

@@ -46,6 +46,9 @@ hipError_t launch_reduce(bool fp64, bool kspace, const ReduceArgs& a, double* pa
                          double* out, hipStream_t stream);
 hipError_t launch_scale(bool fp64, void* psi, long long total, double s, hipStream_t stream);
 // out[q] = sum over blocks b < nblocks of partials[3 b + q], q < 3, fixed order
+// max |psi|^2 over `total` points into out[0] (partials: reduce_blocks(total) doubles)
+hipError_t launch_peak(bool fp64, const void* psi, long long total, double* partials, double* out,
+                       hipStream_t stream);
 hipError_t launch_reduce_final3(const double* partials, int nblocks, double* out,
                                 hipStream_t stream);
 // Energy variance sums from psi, its kinetic part T psi and V (or null):

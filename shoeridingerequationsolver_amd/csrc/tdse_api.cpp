@@ -2077,6 +2077,11 @@ int tdse_observe(tdse_plan* p, unsigned what, tdse_observables* out) {
         const double m = ratio(host[1 + a], host[0]);
         out->sigma[a] = std::sqrt(std::max(0.0, ratio(host[6 + a], host[0]) - m * m));
     }
+    if (what & TDSE_OBS_PEAK) {
+        // the sums above are on the host: the peak reuses their scratch
+        TDSE_HIP(tdse::launch_peak(p->fp64(), p->psi, p->total, p->partials, p->red_out, p->stream));
+        if ((rc = d2h(p, &out->peak, p->red_out, sizeof(double)))) return rc;
+    }
     if (!want_k) return (what & TDSE_OBS_VARIANCE) ? energy_variance(p, out) : TDSE_OK;
     if (p->kind == kKindLattice) {
         // <H> of the lattice Hamiltonian (hops + onsite, core/src/lattice2d.ixx:251-274)

@@ -92,6 +92,41 @@ def test_box_probability_matches_oracle(orc, tdse, case, prec):
 
 
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("dims", [(64, 64, 64), (96, 32, 16), (65536, 1, 1), (160, 192, 1)],
+                         ids=lambda d: "x".join(map(str, d)))
+def test_peak_density_matches_the_state(tdse, dims, prec):
+    """TDSE_OBS_PEAK: max |psi|^2 over the cells, without the cell volume (the
+    reference engine's norm_and_peak().peak, solver/src/vk_engine.ixx:1665-1691),
+    against the downloaded state (each term may round once differently from
+    numpy's, so within 4 ulps); a max, so a second call gives the same bits;
+    without the flag the field stays 0."""
+    from shoeridingerequationsolver_amd import workloads
+    print(f"case peak dims={dims} prec={prec}", flush=True)
+    one = [n == 1 for n in dims]
+    half = 100.0 if dims[1] == 1 else 16.0
+    lo = tuple(0.0 if o else -half for o in one)
+    hi = tuple(1.0 if o else half for o in one)
+    pot = "harmonic" if dims[1] == 1 else "regularized_coulomb"
+    params = {"omega": 0.1} if pot == "harmonic" else {"Z": 1.0}
+    s = workloads.Solve("peak", dims, lo, hi, pot, params,
+                        tuple(0.0 if o else c for o, c in zip(one, (0.5, -0.3, 0.2))),
+                        tuple(1.0 if o else c for o, c in zip(one, (1.3, 1.1, 0.9))),
+                        tuple(0.0 if o else c for o, c in zip(one, (0.4, 0.2, -0.3))),
+                        0.01, 5, prec)
+    p = tdse.Propagator(s.grid(), None, s.dt, precision=prec)
+    p.build(s.recipe())
+    p.step(5)
+    psi = p.download()
+    want = float((psi.real ** 2 + psi.imag ** 2).max())
+    o = p.observe(tdse.capi.TDSE_OBS_NORM | tdse.capi.TDSE_OBS_PEAK)
+    print(f"  peak {o['peak']:.17g} vs {want:.17g}", flush=True)
+    assert abs(o["peak"] - want) <= 4 * np.finfo(np.float64).eps * want
+    assert p.observe(tdse.capi.TDSE_OBS_PEAK)["peak"] == o["peak"]
+    assert p.observe(tdse.capi.TDSE_OBS_NORM)["peak"] == 0.0
+    p.close()
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
 @pytest.mark.parametrize("n,lo,hi", [(512, -16.0, 16.0), (1024, -40.0, 40.0),
                                      (65536, -100.0, 100.0)])
 def test_observables_1d_match_oracle(orc, tdse, n, lo, hi, prec):
