@@ -84,7 +84,8 @@ def test_step_3d_with_mask_matches_oracle(orc, tdse, prec):
 
 
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
-@pytest.mark.parametrize("n,steps", [(1024, 200), (256, 50), (65536, 6), (96 * 512, 4)])
+@pytest.mark.parametrize("n,steps", [(1024, 200), (256, 50), (65536, 6), (96 * 512, 4),
+                                     (160 * 160, 4), (1024 * 1024, 2)])
 def test_step_1d_matches_oracle(orc, tdse, n, steps, prec):
     lo, hi = (-40.0, 40.0) if n <= 1024 else (-100.0, 100.0)
     print(f"case 1d n={n} prec={prec}", flush=True)
