@@ -119,6 +119,11 @@ int tdse_plan_set_mask(tdse_plan* plan, const double* mask);
 /* State transfer, interleaved complex128 on the host. */
 int tdse_state_upload(tdse_plan* plan, const double* psi);
 int tdse_state_download(tdse_plan* plan, double* psi);
+/* The state as interleaved complex64 (two floats per point, x-fastest): the
+ * FP32 state as stored, the FP64 state rounded to nearest.  Waits for the
+ * plan's queued work.  The reference engine's readback(std::vector<float>&),
+ * solver/src/vk_engine.ixx:2136-2155 (half the bytes of tdse_state_download). */
+int tdse_state_download_f32(tdse_plan* plan, float* out);
 /* Copy the state of `src` into `dst` (same shape and precision), on the GPU:
  * by the runtime's device copy, or by a plain streaming copy kernel (16 bytes
  * per lane), the bandwidth reference the sweeps are measured against.  Both

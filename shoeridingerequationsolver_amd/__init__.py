@@ -263,6 +263,14 @@ class Propagator:
         capi.check(capi.lib().tdse_state_download(self._h, capi.dptr(out)))
         return out
 
+    def download_f32(self) -> np.ndarray:
+        """The state as complex64 (``tdse_state_download_f32``): as stored for
+        FP32 plans, rounded to nearest for FP64 plans."""
+        out = np.empty(self.shape, np.complex64)
+        capi.check(capi.lib().tdse_state_download_f32(
+            self._h, out.ctypes.data_as(C.POINTER(C.c_float))))
+        return out
+
     def step(self, nsteps=1):
         capi.check(capi.lib().tdse_step(self._h, int(nsteps)))
 

@@ -52,7 +52,7 @@ EXPORTS = (
     "tdse_plan_set_drive", "tdse_plan_set_time", "tdse_plan_get_time", "tdse_plan_set_magnetic",
     "tdse_lattice_set_uniform_field", "tdse_lattice_set_solenoid", "tdse_observe_noalloc",
     "tdse_build_phase_tables", "tdse_state_copy_kernel", "tdse_batch_submit", "tdse_sync",
-    "tdse_probability_in_box",
+    "tdse_probability_in_box", "tdse_state_download_f32",
 )
 
 TDSE_POT = {"none": 0, "harmonic": 1, "regularized_coulomb": 2, "coulomb_floor": 3, "barrier": 4,
@@ -138,6 +138,7 @@ def lib() -> C.CDLL:
         L.tdse_observe.argtypes = [C.c_void_p, C.c_uint, C.POINTER(Observables)]
         L.tdse_probability_in_slab.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double, _dp]
         L.tdse_probability_in_box.argtypes = [C.c_void_p, _dp, _dp, _dp]
+        L.tdse_state_download_f32.argtypes = [C.c_void_p, C.POINTER(C.c_float)]
         L.tdse_state_scale.argtypes = [C.c_void_p, C.c_double]
         L.tdse_profile_step.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, _ip, _ip]
         L.tdse_plan_sweep_name.argtypes = [C.c_void_p, C.c_int]

@@ -371,8 +371,8 @@ int upload_staged(tdse_plan* p, void* dst, const double* src, long long n) {
 
 // Device array of T (n values) -> FP64 host values: the DMA of chunk k + 1 is
 // in flight while chunk k is converted.
-template <typename T>
-int download_staged(tdse_plan* p, double* dst, const void* src, long long n) {
+template <typename T, typename D = double>
+int download_staged(tdse_plan* p, D* dst, const void* src, long long n) {
     PinnedStage& st = p->ctx->pin;
     const long long per = static_cast<long long>(PinnedStage::kBytes / sizeof(T));
     const long long chunks = (n + per - 1) / per;
@@ -391,7 +391,7 @@ int download_staged(tdse_plan* p, double* dst, const void* src, long long n) {
         const long long off = k * per, len = std::min(per, n - off);
         const T* h = static_cast<const T*>(st.buf[k & 1]);
         parallel_chunks(len, [&](long long i0, long long i1) {
-            for (long long i = i0; i < i1; ++i) dst[off + i] = static_cast<double>(h[i]);
+            for (long long i = i0; i < i1; ++i) dst[off + i] = static_cast<D>(h[i]);
         });
     }
     return rc;
@@ -1693,6 +1693,24 @@ int tdse_state_download(tdse_plan* p, double* psi) {
     if ((rc = d2h(p, f.get(), p->psi, 2 * static_cast<size_t>(p->total) * 4))) return rc;
     parallel_chunks(2 * p->total, [&](long long i0, long long i1) {
         for (long long i = i0; i < i1; ++i) psi[i] = f[i];
+    });
+    return TDSE_OK;
+}
+
+int tdse_state_download_f32(tdse_plan* p, float* psi) {
+    int rc = check_plan(p);
+    if (rc) return rc;
+    if (!psi) return fail(TDSE_E_INVALID, "null state");
+    if ((rc = set_device(p->ctx))) return rc;
+    if (!p->fp64()) return d2h(p, psi, p->psi, 2 * static_cast<size_t>(p->total) * 4);
+    {
+        std::lock_guard<std::mutex> lk(p->ctx->pin.mu);
+        if (pinned_stage(p->ctx)) return download_staged<double, float>(p, psi, p->psi, 2 * p->total);
+    }
+    std::unique_ptr<double[]> d(new double[2 * static_cast<size_t>(p->total)]);
+    if ((rc = d2h(p, d.get(), p->psi, 2 * static_cast<size_t>(p->total) * 8))) return rc;
+    parallel_chunks(2 * p->total, [&](long long i0, long long i1) {
+        for (long long i = i0; i < i1; ++i) psi[i] = static_cast<float>(d[i]);
     });
     return TDSE_OK;
 }

@@ -1,4 +1,5 @@
-"""State uploads and downloads (tdse_state_upload / tdse_state_download).
+"""State uploads and downloads (tdse_state_upload / tdse_state_download /
+tdse_state_download_f32).
 
 Transfers go through two pinned buffers of the context in chunks (8 Mi FP32
 or 4 Mi FP64 values each); the host threads convert one chunk while the
@@ -51,6 +52,23 @@ def test_round_trip(tdse, n, prec):
     p.download(out)
     want2 = psi2 if prec == "fp64" else psi2.astype(np.complex64).astype(np.complex128)
     assert np.array_equal(out.view(np.float64), want2.view(np.float64))
+    p.close()
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("n", SHAPES, ids=lambda n: "x".join(map(str, n)))
+def test_download_f32(tdse, n, prec):
+    """tdse_state_download_f32 (the reference engine's float readback): the
+    FP32 state as stored, the FP64 state rounded to nearest, i.e. bit for bit
+    the FP64 download rounded to complex64, over one or several chunks."""
+    print(f"case download f32 n={n} prec={prec}", flush=True)
+    p = tdse.Propagator(grid3(tdse, n), np.zeros(int(np.prod(n))), 0.01, precision=prec)
+    p.upload(state(n, 9))
+    p.step(2)
+    got = p.download_f32()
+    want = p.download().astype(np.complex64)
+    assert got.dtype == np.complex64 and got.shape == want.shape
+    assert np.array_equal(got.view(np.float32), want.view(np.float32))
     p.close()
 
 
